@@ -1,0 +1,32 @@
+"""Small input builders shared by the tests (no reference code involved)."""
+import numpy as np
+
+import oracle as orc
+
+
+def lattice(nx, ny, nz, h, jitter, seed, origin=(0.0, 0.0, 0.0)):
+    """x-fastest simple-cubic lattice with uniform jitter in +-jitter*h."""
+    rng = np.random.default_rng(seed)
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    g = np.stack([i, j, k], -1).reshape(-1, 3).astype(np.float64) * h
+    g += rng.uniform(-1, 1, g.shape) * jitter * h
+    g += np.asarray(origin)
+    x4 = np.zeros((g.shape[0], 4))
+    x4[:, :3] = g
+    return x4
+
+
+def material(h, horizon_factor=3.0, stiffness=1e4, mass=1e-3):
+    """(horizon, radius, c_bond, mass) for spacing h: r = h/2, c = k h."""
+    return (horizon_factor * h, 0.5 * h, stiffness * h, mass)
+
+
+def make_system(x4, materials, body=None, s0=0.02, seed=1973, v4=None, **kw):
+    n = x4.shape[0]
+    hs = [m[0] for m in materials]
+    counts, lists, total = orc.bonds_brute(x4, hs, body=body)
+    assert total >= 0
+    rs, col = orc.lists_to_csr(counts, lists)
+    s00 = orc.stretch_init(n, s0, seed)
+    v4 = np.zeros((n, 4)) if v4 is None else v4
+    return orc.OracleSystem(x4, v4, s00, rs, col, materials, body=body, **kw)
