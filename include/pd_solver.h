@@ -1,0 +1,231 @@
+/*
+ * pd_solver.h -- C ABI of the MI355X implicit bond-based peridynamics
+ * engine.  Plain C types only; the library owns device memory and one HIP
+ * stream per context.  Every call returns a pd_status; pd_last_error()
+ * returns the message of the most recent failure.
+ *
+ * Each entry point replaces a launch site or host routine of the reference
+ * (SCIInstitute/SCI-Solver_Peridynamic, src/); the replaced file:line is
+ * given on each declaration.
+ *
+ * Units: whatever the caller uses consistently (the reference's scaled
+ * units or SI); the engine adds no scale factor.
+ * Vectors: real4 layout, 4 doubles per particle with the w lane ignored
+ * on input and written as 0 (core/definitions.h:37-39).
+ */
+#ifndef PD_SOLVER_H
+#define PD_SOLVER_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PD_ABI_VERSION 1
+#define PD_MAX_BODIES 8
+#define PD_MAX_BONDS_LIMIT 1024 /* upper bound accepted for max_bonds */
+
+typedef enum pd_status {
+  PD_OK = 0,
+  PD_ERR_HIP = 1,            /* HIP runtime failure (message has details) */
+  PD_ERR_ARG = 2,            /* invalid argument or size mismatch */
+  PD_ERR_NOT_CONVERGED = 3,  /* CG reached max_iter before rtol */
+  PD_ERR_BOND_OVERFLOW = 4,  /* a particle has more than max_bonds bonds */
+  PD_ERR_STATE = 5,          /* call out of order (e.g. step before bonds) */
+  PD_ERR_NO_DEVICE = 6       /* no usable HIP device */
+} pd_status;
+
+/* Material of one body (the reference has one body; these are its
+ * pd_horizon, pd_particle_radius, pd_C_times_V*PD_K and pd_particle_mass,
+ * core/parameters.cu:351-370). */
+typedef struct pd_material {
+  double horizon; /* bond exists iff dist < horizon (strict) */
+  double radius;  /* volume-correction width r (cg_solver.cuh:281-284) */
+  double c_bond;  /* force per unit stretch, C*K */
+  double mass;    /* particle mass */
+} pd_material;
+
+typedef struct pd_config {
+  int32_t num_particles;
+  int32_t device;     /* HIP device ordinal (reference gpu_device) */
+  int32_t max_bonds;  /* per-particle cap, 528 in the reference */
+  int32_t num_bodies; /* 1..PD_MAX_BODIES */
+  pd_material material[PD_MAX_BODIES];
+} pd_config;
+
+/* Fields for pd_upload / pd_download */
+typedef enum pd_field {
+  PD_FIELD_POSITION = 0,          /* double4[N]  pd_position */
+  PD_FIELD_VELOCITY = 1,          /* double4[N]  pd_velocity */
+  PD_FIELD_ORIGINAL_POSITION = 2, /* double4[N]  pd_original_pos */
+  PD_FIELD_STRETCH = 3,           /* double[N]   pd_stretch */
+  PD_FIELD_ORIGINAL_STRETCH = 4,  /* double[N]   pd_original_stretch */
+  PD_FIELD_ACTIVITY = 5,          /* int32[N]    pd_activity */
+  PD_FIELD_BODY = 6,              /* int32[N]    body index */
+  PD_FIELD_BC_FLAG = 7,           /* int32[N]    1 = prescribed velocity */
+  PD_FIELD_BC_VELOCITY = 8,       /* double4[N]  prescribed velocity */
+  PD_FIELD_SOLUTION = 9,          /* double4[N]  pd_system_solution */
+  PD_FIELD_FORCE = 10,            /* double4[N]  pd_force (download) */
+  PD_FIELD_RHS = 11,              /* double4[N]  pd_system_vector */
+  PD_FIELD_NEW_STRETCH = 12,      /* double[N]   pd_new_stretch */
+  PD_FIELD_BOND_TOP = 13,         /* int32[N]    pd_bond_list_top */
+  PD_FIELD_CONTACT_FORCE = 14,    /* double4[N]  contact force */
+  PD_FIELD_COUNT = 15
+} pd_field;
+
+typedef struct pd_step_params {
+  double dt;          /* seconds; reference: pd_time_step * 2^-23 */
+  double clock_scale; /* phase-in factor (simulator.cu:1163-1167) */
+  double gravity;     /* 9.8 in the reference, applied along -y */
+  int32_t integrator; /* 0 implicit Euler, 1 Newmark-beta */
+  int32_t precond;    /* 0 identity (reference CUSP), 1 block Jacobi */
+  double rtol;        /* stop when ||r|| <= rtol * ||S b|| */
+  int32_t max_iter;   /* CG iteration limit (reference 100000) */
+  int32_t walls;      /* 1: clamp positions to [bmin+r, bmax-r] */
+  double bmin[3];
+  double bmax[3];
+  double wall_radius;
+  double restitution;       /* 0.001 in the reference */
+  double contact_radius;    /* Rc; 0 disables contact */
+  double contact_stiffness; /* Kc */
+  double contact_volume;    /* PD_V divisor */
+  double contact_cell;      /* grid cell edge for contact, >= Rc */
+  int32_t track_gap;        /* 1: report the bond closest to breaking */
+  int32_t timing;           /* 1: time phases and SpMV launches */
+} pd_step_params;
+
+typedef struct pd_step_stats {
+  int32_t cg_iterations;
+  int32_t converged;
+  double residual_norm;
+  double b_norm;
+  int64_t broken_bonds; /* directed bonds removed this step */
+  int64_t live_bonds;   /* directed bonds alive after the step */
+  double min_gap;       /* min |stretch - threshold| (track_gap) */
+  int32_t gap_p;
+  int32_t gap_q;
+  /* filled when timing = 1 (HIP events on the context stream) */
+  double ms_contact;
+  double ms_assemble;
+  double ms_solve;
+  double ms_update;
+  double ms_spmv_total; /* sum over the CG SpMV launches that ran */
+  int32_t spmv_launches;
+  int32_t host_syncs;
+} pd_step_stats;
+
+typedef struct pd_bond_stats {
+  int64_t total_bonds; /* directed bonds */
+  int32_t min_bonds;
+  int32_t max_bonds;
+  int64_t slots;       /* padded slot count of the sliced layout */
+  double ms_build;
+} pd_bond_stats;
+
+typedef struct pd_ctx pd_ctx;
+
+/* Simulator::Simulator + MemoryManager::MemoryManager
+ * (core/simulator.cu:34-54, core/memory_manager.cu:16-217): select
+ * cfg->device, allocate every per-particle buffer, create the stream. */
+int pd_ctx_create(const pd_config* cfg, pd_ctx** out);
+
+/* Simulator::~Simulator / MemoryManager::~MemoryManager
+ * (core/memory_manager.cu:219-233). */
+int pd_ctx_destroy(pd_ctx* ctx);
+
+/* checkCudaErrors / TRUE_OR_DIE messages (core/helper_cuda.h:892-922,
+ * core/utilities.h:27-35), returned instead of exiting.  ctx may be NULL
+ * for failures of pd_ctx_create. */
+const char* pd_last_error(const pd_ctx* ctx);
+
+/* MemoryManager::uploadToDevice (core/memory_manager.cu:236-260):
+ * synchronous host-to-device copy of one field; bytes must equal the
+ * field size for num_particles. */
+int pd_upload(pd_ctx* ctx, int32_t field, const void* src, size_t bytes);
+
+/* MemoryManager::downloadFromDevice (core/memory_manager.cu:274-298),
+ * without the scale factor (the caller owns units). */
+int pd_download(pd_ctx* ctx, int32_t field, void* dst, size_t bytes);
+
+/* Simulator::initPeridynamicsBonds bond part (core/simulator.cu:825-943,
+ * core/kdtree.h:141-353): GPU uniform-grid hash of the current positions,
+ * LDS radix sort, cell start/end, then for every particle the sorted list
+ * of q != p in the same body with dist(p,q) < horizon[body].  Sets
+ * pd_bond_list_top = count - 1.  Fails with PD_ERR_BOND_OVERFLOW if a
+ * particle exceeds max_bonds (the reference silently truncated). */
+int pd_build_bonds(pd_ctx* ctx, pd_bond_stats* stats);
+
+/* Load a host-built bond list instead (e.g. a restored state):
+ * counts[N], lists row-major with row stride `stride`, each row sorted. */
+int pd_set_bonds(pd_ctx* ctx, const int32_t* counts, const int32_t* lists,
+                 int32_t stride);
+
+/* Current bond lists in list order (live bonds first, then the broken ones
+ * that swap-remove moved past the top): capacity[N] and lists row-major
+ * with row stride `stride` >= max capacity. */
+int pd_get_bonds(pd_ctx* ctx, int32_t* capacity, int32_t* lists,
+                 int32_t stride);
+
+/* Simulator::printDebugInfoLinearSystem (core/simulator.cu:656-732) as
+ * data: the assembled system in the reference's row-major Mat3x3 layout,
+ * diag9[N*9] = A_pp and vals9[(p*stride + k)*9] = A_{p,list[p][k]} for
+ * the live slots k <= top[p] (the remaining entries are left untouched). */
+int pd_get_matrix(pd_ctx* ctx, double* diag9, double* vals9, int32_t stride);
+
+/* Simulator::integratePeridynamicsImplicitEuler (core/simulator.cu:
+ * 297-467) and integratePeridynamicsNewmarkBeta (core/simulator.cu:
+ * 470-652) without the adaptive rollback branch: contact (if enabled),
+ * fused calculatePDForceDerivative + fill (core/cg_solver.cuh:200,
+ * core/implicit_euler.cuh:19,62), stretch update (core/simulator.cuh:
+ * 2601), CG solve (core/simulator.cu:735), velocity / position update
+ * (core/implicit_euler.cuh:90, core/newmark_beta.cuh:89,113).
+ * Returns PD_ERR_NOT_CONVERGED (state already advanced) when CG hits
+ * max_iter, matching CG_convergent_ = false (core/simulator.cu:780-790). */
+int pd_step(pd_ctx* ctx, const pd_step_params* prm, pd_step_stats* stats);
+
+/* Test hook, first half of pd_step: contact + force / Jacobian / breaking
+ * + fill of A and b (launch sites core/simulator.cu:306, 430, 438).  Leaves
+ * the stretch limits unswapped and does not solve. */
+int pd_assemble(pd_ctx* ctx, const pd_step_params* prm, pd_step_stats* stats);
+
+/* updatePDStretchNeighborList (core/simulator.cuh:2601-2620): s <- s_new */
+int pd_commit_stretch(pd_ctx* ctx);
+
+/* PeridynamicsMatrix::operator() / computeAx (core/cg_solver.cuh:343-428):
+ * y = A x for host real4 vectors x, y of num_particles entries. */
+int pd_spmv(pd_ctx* ctx, const double* x4, double* y4);
+
+/* Simulator::solveLinearSystem (core/simulator.cu:735-799): CG on the
+ * assembled A and b, warm-started from the current solution. */
+int pd_solve(pd_ctx* ctx, const pd_step_params* prm, pd_step_stats* stats);
+
+/* MemoryManager::transferData(PD_VELOCITY, PD_SYSTEM_SOLUTION) +
+ * updatePDPosition* (core/memory_manager.cu:541, core/implicit_euler.cuh:
+ * 90-176, core/newmark_beta.cuh:89-199). */
+int pd_update(pd_ctx* ctx, const pd_step_params* prm);
+
+/* MemoryManager::backup{BondListTopIndex,PDPosition,PDVelocity} /
+ * restore* (core/memory_manager.cu:491-538) for adaptive-dt rollback. */
+int pd_backup(pd_ctx* ctx);
+int pd_restore(pd_ctx* ctx);
+
+/* Wait for all work on the context stream. */
+int pd_synchronize(pd_ctx* ctx);
+
+/* Times `iters` launches of the plain SpMV kernel y = A x on the resident
+ * matrix with HIP events on the context stream; returns the mean kernel
+ * time in ms and the algorithmic bytes one launch moves. */
+int pd_bench_spmv(pd_ctx* ctx, int32_t iters, double* ms_per_launch,
+                  double* bytes_per_launch);
+
+/* Algorithmic HBM bytes of one CG SpMV launch on the current bonds
+ * (live blocks * (6*8 + 4) + N * (6*8 + 4 + 2*24)). */
+int pd_spmv_bytes(pd_ctx* ctx, double* bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PD_SOLVER_H */

@@ -1,0 +1,926 @@
+// pd_api.hip -- the C ABI declared in include/pd_solver.h.
+//
+// A context owns every device buffer of one particle system and one HIP
+// stream (default priority).  Inside a step the host waits only at the CG
+// chunk boundaries (a pinned copy of the device-side CG state) and, when
+// contact is on, once for the grid bounds.
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "pd_internal.h"
+#include "../../include/pd_solver.h"
+
+static thread_local std::string g_last_error;
+
+struct pd_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int32_t n = 0;
+  int32_t nslices = 0;
+  int32_t max_bonds = 528;
+  int32_t nbody = 1;
+  pd_material mat[PD_MAX_BODIES];
+  std::string err;
+
+  // particle state
+  double4 *x = nullptr, *v = nullptr, *x0 = nullptr, *sol = nullptr, *f = nullptr,
+          *b = nullptr, *fc = nullptr, *vbc = nullptr;
+  double *s = nullptr, *s_new = nullptr, *s00 = nullptr;
+  int32_t *act = nullptr, *body = nullptr, *bc = nullptr, *top = nullptr, *cap = nullptr;
+
+  // bonds and matrix (sliced layout, see pd_device.h)
+  int64_t* slice_off = nullptr;
+  std::vector<int64_t> h_slice_off;
+  int64_t slots = 0;      // slot rows (each 64 lanes)
+  int64_t slots_cap = 0;  // allocated slot rows
+  int32_t* col = nullptr;
+  double* val = nullptr;
+  double* diag = nullptr;
+  double* minv = nullptr;
+  bool bonds_ready = false;
+  int64_t live_bonds = 0;
+
+  // CG
+  double4 *r = nullptr, *z = nullptr, *p = nullptr, *y = nullptr;
+  double* part = nullptr;
+  pd_cg_state* st = nullptr;
+  pd_step_counters* counters = nullptr;
+  pd_mat_dev* dmat = nullptr;
+  double* dhorizon = nullptr;
+  int last_iters = 0;
+
+  // bond-gap tracking
+  double* gap_val = nullptr;
+  int32_t* gap_q = nullptr;
+  double* gpart_v = nullptr;
+  int32_t* gpart_p = nullptr;
+  int32_t* gpart_q = nullptr;
+  double* gap_out = nullptr;
+  int32_t* gap_pq = nullptr;
+
+  // grid, sort
+  uint32_t *keys_a = nullptr, *keys_b = nullptr;
+  int32_t *idx_a = nullptr, *idx_b = nullptr;
+  uint32_t* hist = nullptr;
+  double* bbox_part = nullptr;
+  double* bbox = nullptr;
+  int32_t *cell_start = nullptr, *cell_end = nullptr;
+  int64_t cell_cap = 0;
+  int32_t* counts = nullptr;
+  int32_t* widths = nullptr;
+
+  // rollback copies
+  double4 *x_bak = nullptr, *v_bak = nullptr;
+  int32_t* top_bak = nullptr;
+
+  // pinned host mirrors
+  pd_cg_state* h_st = nullptr;
+  pd_step_counters* h_cnt = nullptr;
+  double* h_bbox = nullptr;
+  double* h_gap = nullptr;
+  int32_t* h_gap_pq = nullptr;
+
+  std::vector<hipEvent_t> ev;  // event pool for timing
+};
+
+namespace {
+
+int fail(pd_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  g_last_error = msg;
+  return code;
+}
+
+#define HIPCHK(ctx, expr)                                                                \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return fail((ctx), PD_ERR_HIP,                                                     \
+                  std::string(#expr) + ": " + hipGetErrorString(e_));                    \
+  } while (0)
+
+#define CHECK_LAUNCH(ctx, what)                                                          \
+  do {                                                                                   \
+    hipError_t e_ = hipGetLastError();                                                   \
+    if (e_ != hipSuccess)                                                                \
+      return fail((ctx), PD_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e_));  \
+  } while (0)
+
+template <typename T>
+int dalloc(pd_ctx* c, T** p, size_t count) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+  if (e != hipSuccess)
+    return fail(c, PD_ERR_HIP, std::string("hipMalloc(") + std::to_string(count * sizeof(T)) +
+                                   " B): " + hipGetErrorString(e));
+  e = hipMemsetAsync(*p, 0, count * sizeof(T), c->stream);
+  if (e != hipSuccess) return fail(c, PD_ERR_HIP, std::string("hipMemsetAsync: ") + hipGetErrorString(e));
+  return PD_OK;
+}
+
+template <typename T>
+void dfree(T*& p) {
+  if (p) (void)hipFree((void*)p);
+  p = nullptr;
+}
+
+struct FieldInfo {
+  void** dptr;
+  size_t elem;  // bytes per particle
+  bool vec4;
+  bool writable;
+};
+
+FieldInfo field_info(pd_ctx* c, int32_t f) {
+  switch (f) {
+    case PD_FIELD_POSITION: return {(void**)&c->x, 32, true, true};
+    case PD_FIELD_VELOCITY: return {(void**)&c->v, 32, true, true};
+    case PD_FIELD_ORIGINAL_POSITION: return {(void**)&c->x0, 32, true, true};
+    case PD_FIELD_STRETCH: return {(void**)&c->s, 8, false, true};
+    case PD_FIELD_ORIGINAL_STRETCH: return {(void**)&c->s00, 8, false, true};
+    case PD_FIELD_ACTIVITY: return {(void**)&c->act, 4, false, true};
+    case PD_FIELD_BODY: return {(void**)&c->body, 4, false, true};
+    case PD_FIELD_BC_FLAG: return {(void**)&c->bc, 4, false, true};
+    case PD_FIELD_BC_VELOCITY: return {(void**)&c->vbc, 32, true, true};
+    case PD_FIELD_SOLUTION: return {(void**)&c->sol, 32, true, true};
+    case PD_FIELD_FORCE: return {(void**)&c->f, 32, true, false};
+    case PD_FIELD_RHS: return {(void**)&c->b, 32, true, false};
+    case PD_FIELD_NEW_STRETCH: return {(void**)&c->s_new, 8, false, false};
+    case PD_FIELD_BOND_TOP: return {(void**)&c->top, 4, false, false};
+    case PD_FIELD_CONTACT_FORCE: return {(void**)&c->fc, 32, true, false};
+    default: return {nullptr, 0, false, false};
+  }
+}
+
+int ensure_events(pd_ctx* c, size_t count) {
+  while (c->ev.size() < count) {
+    hipEvent_t e;
+    HIPCHK(c, hipEventCreate(&e));
+    c->ev.push_back(e);
+  }
+  return PD_OK;
+}
+
+float elapsed(hipEvent_t a, hipEvent_t b) {
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.f;
+  return ms;
+}
+
+int check_params(pd_ctx* c, const pd_step_params* prm) {
+  if (!prm) return fail(c, PD_ERR_ARG, "null step parameters");
+  if (!(prm->dt > 0.0)) return fail(c, PD_ERR_ARG, "dt must be > 0");
+  if (prm->integrator != 0 && prm->integrator != 1)
+    return fail(c, PD_ERR_ARG, "integrator must be 0 (implicit Euler) or 1 (Newmark-beta)");
+  if (prm->precond != 0 && prm->precond != 1)
+    return fail(c, PD_ERR_ARG, "precond must be 0 (identity) or 1 (block Jacobi)");
+  if (prm->contact_radius > 0.0 && !(prm->contact_cell >= prm->contact_radius))
+    return fail(c, PD_ERR_ARG, "contact_cell must be >= contact_radius");
+  if (prm->contact_radius > 0.0 && !(prm->contact_volume != 0.0))
+    return fail(c, PD_ERR_ARG, "contact_volume must be non-zero");
+  if (!c->bonds_ready) return fail(c, PD_ERR_STATE, "bonds not built (pd_build_bonds / pd_set_bonds)");
+  return PD_OK;
+}
+
+// Uniform grid over the current positions: bounds, keys, sort, ranges.
+int build_grid(pd_ctx* c, double cell, pd::GridView* g) {
+  const int32_t n = c->n;
+  pd::launch_bbox(c->x, n, c->bbox_part, c->bbox, c->stream);
+  CHECK_LAUNCH(c, "bbox");
+  HIPCHK(c, hipMemcpyAsync(c->h_bbox, c->bbox, 6 * sizeof(double), hipMemcpyDeviceToHost,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  int64_t ncell = 1;
+  for (int k = 0; k < 3; ++k) {
+    const double lo = c->h_bbox[k], hi = c->h_bbox[3 + k];
+    if (!(isfinite(lo) && isfinite(hi)))
+      return fail(c, PD_ERR_ARG, "non-finite particle position");
+    const double span = floor((hi - lo) / cell) + 1.0;
+    if (span > 2.0e9) return fail(c, PD_ERR_ARG, "grid too large for the cell size");
+    g->dims[k] = (int32_t)span;
+    g->mn[k] = lo;
+    ncell *= g->dims[k];
+  }
+  if (ncell > (int64_t)1 << 31) return fail(c, PD_ERR_ARG, "grid has more than 2^31 cells");
+  g->cell = cell;
+  if (ncell > c->cell_cap) {
+    dfree(c->cell_start);
+    dfree(c->cell_end);
+    const int64_t cap = std::max<int64_t>(ncell, c->cell_cap * 3 / 2);
+    if (int e = dalloc(c, &c->cell_start, (size_t)cap)) return e;
+    if (int e = dalloc(c, &c->cell_end, (size_t)cap)) return e;
+    c->cell_cap = cap;
+  }
+  pd::launch_cell_keys(c->x, n, g->mn, cell, g->dims, c->keys_a, c->idx_a, c->stream);
+  CHECK_LAUNCH(c, "cell keys");
+  int bits = 1;
+  while (bits < 32 && ((int64_t)1 << bits) < ncell) ++bits;
+  const int where = pd::radix_sort(c->keys_a, c->idx_a, c->keys_b, c->idx_b, n, bits, c->hist,
+                                   c->stream);
+  CHECK_LAUNCH(c, "radix sort");
+  const uint32_t* keys = where ? c->keys_b : c->keys_a;
+  const int32_t* idx = where ? c->idx_b : c->idx_a;
+  pd::launch_cell_ranges(keys, n, c->cell_start, c->cell_end, ncell, c->stream);
+  CHECK_LAUNCH(c, "cell ranges");
+  g->sorted_key = keys;
+  g->sorted_idx = idx;
+  g->cell_start = c->cell_start;
+  g->cell_end = c->cell_end;
+  return PD_OK;
+}
+
+int ensure_slots(pd_ctx* c, int64_t slots) {
+  if (slots <= c->slots_cap) return PD_OK;
+  dfree(c->col);
+  dfree(c->val);
+  if (int e = dalloc(c, &c->col, (size_t)slots * 64)) return e;
+  if (int e = dalloc(c, &c->val, (size_t)slots * 64 * 6)) return e;
+  c->slots_cap = slots;
+  return PD_OK;
+}
+
+int set_slice_offsets(pd_ctx* c, const std::vector<int32_t>& widths) {
+  c->h_slice_off.assign(c->nslices + 1, 0);
+  for (int s = 0; s < c->nslices; ++s) c->h_slice_off[s + 1] = c->h_slice_off[s] + widths[s];
+  c->slots = c->h_slice_off[c->nslices];
+  if (int e = ensure_slots(c, c->slots)) return e;
+  HIPCHK(c, hipMemcpyAsync(c->slice_off, c->h_slice_off.data(),
+                           sizeof(int64_t) * (c->nslices + 1), hipMemcpyHostToDevice, c->stream));
+  return PD_OK;
+}
+
+pd::SpmvView spmv_view(pd_ctx* c) {
+  pd::SpmvView A;
+  A.n = c->n;
+  A.slice_off = c->slice_off;
+  A.top = c->top;
+  A.col = c->col;
+  A.val = c->val;
+  A.diag = c->diag;
+  return A;
+}
+
+pd::CgArgs cg_args(pd_ctx* c, int precond) {
+  pd::CgArgs a;
+  a.A = spmv_view(c);
+  a.minv = c->minv;
+  a.bc = c->bc;
+  a.b = c->b;
+  a.x = c->sol;
+  a.r = c->r;
+  a.z = c->z;
+  a.p = c->p;
+  a.y = c->y;
+  a.part = c->part;
+  a.st = c->st;
+  a.precond = precond;
+  return a;
+}
+
+int enqueue_contact(pd_ctx* c, const pd_step_params* prm) {
+  pd::GridView g;
+  if (int e = build_grid(c, prm->contact_cell, &g)) return e;
+  pd::launch_contact(c->n, c->x, c->body, c->act, g, prm->contact_radius, prm->contact_stiffness,
+                     prm->contact_volume, c->fc, c->stream);
+  CHECK_LAUNCH(c, "contact");
+  return PD_OK;
+}
+
+int enqueue_assemble(pd_ctx* c, const pd_step_params* prm) {
+  pd::launch_reset_counters(c->counters, c->stream);
+  pd::AsmArgs a;
+  a.n = c->n;
+  a.slice_off = c->slice_off;
+  a.top = c->top;
+  a.col = c->col;
+  a.val = c->val;
+  a.diag = c->diag;
+  a.minv = c->minv;
+  a.x = c->x;
+  a.x0 = c->x0;
+  a.v = c->v;
+  a.fc = c->fc;
+  a.vbc = c->vbc;
+  a.f = c->f;
+  a.b = c->b;
+  a.sol = c->sol;
+  a.s = c->s;
+  a.s_new = c->s_new;
+  a.s00 = c->s00;
+  a.act = c->act;
+  a.body = c->body;
+  a.bc = c->bc;
+  a.mat = c->dmat;
+  a.dt = prm->dt;
+  a.dt2 = prm->integrator == 1 ? (prm->dt * prm->dt) / 4.0 : prm->dt * prm->dt;
+  a.clock = prm->clock_scale;
+  a.gravity = prm->gravity;
+  a.integrator = prm->integrator;
+  a.contact = prm->contact_radius > 0.0 ? 1 : 0;
+  a.track_gap = prm->track_gap ? 1 : 0;
+  a.gap_val = c->gap_val;
+  a.gap_q = c->gap_q;
+  a.counters = c->counters;
+  pd::launch_assemble(a, c->stream);
+  CHECK_LAUNCH(c, "assemble");
+  if (prm->track_gap) {
+    pd::launch_gap_reduce(c->gap_val, c->gap_q, c->n, c->gpart_v, c->gpart_p, c->gpart_q,
+                          c->gap_out, c->gap_pq, c->stream);
+    CHECK_LAUNCH(c, "gap reduce");
+  }
+  HIPCHK(c, hipMemcpyAsync(c->h_cnt, c->counters, sizeof(pd_step_counters),
+                           hipMemcpyDeviceToHost, c->stream));
+  if (prm->track_gap) {
+    HIPCHK(c, hipMemcpyAsync(c->h_gap, c->gap_out, sizeof(double), hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->h_gap_pq, c->gap_pq, 2 * sizeof(int32_t), hipMemcpyDeviceToHost,
+                             c->stream));
+  }
+  return PD_OK;
+}
+
+void read_assemble_stats(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st) {
+  c->live_bonds = (int64_t)c->h_cnt->live;
+  if (!st) return;
+  st->broken_bonds = (int64_t)c->h_cnt->broken;
+  st->live_bonds = (int64_t)c->h_cnt->live;
+  if (prm->track_gap) {
+    st->min_gap = *c->h_gap;
+    st->gap_p = c->h_gap_pq[0];
+    st->gap_q = c->h_gap_pq[1];
+  } else {
+    st->min_gap = INFINITY;
+    st->gap_p = -1;
+    st->gap_q = -1;
+  }
+}
+
+// CG on the device; the host waits once per chunk of iterations.
+int run_solve(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st) {
+  const pd::CgArgs a = cg_args(c, prm->precond);
+  pd::launch_cg_setup(c->st, prm->rtol, prm->max_iter, c->stream);
+  pd::launch_cg_init(a, c->stream);
+  CHECK_LAUNCH(c, "cg init");
+  int launched = 0;
+  int syncs = 0;
+  int chunk = std::max(1, c->last_iters);
+  int next = 1;
+  const bool timing = prm->timing != 0;
+  for (;;) {
+    if (timing)
+      if (int e = ensure_events(c, 2 * (size_t)(launched + chunk) + 8)) return e;
+    for (int i = 0; i < chunk; ++i) {
+      if (timing) HIPCHK(c, hipEventRecord(c->ev[8 + 2 * (launched + i)], c->stream));
+      pd::launch_cg_spmv(a, c->stream);
+      if (timing) HIPCHK(c, hipEventRecord(c->ev[8 + 2 * (launched + i) + 1], c->stream));
+      pd::launch_cg_update(a, c->stream);
+      pd::launch_cg_dir(a, c->stream);
+    }
+    CHECK_LAUNCH(c, "cg iteration");
+    launched += chunk;
+    HIPCHK(c, hipMemcpyAsync(c->h_st, c->st, sizeof(pd_cg_state), hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    ++syncs;
+    if (c->h_st->done) break;
+    if (launched > prm->max_iter + 1)
+      return fail(c, PD_ERR_HIP, "CG state did not terminate on the device");
+    chunk = next;
+    next = std::min(next * 2, 32);
+  }
+  c->last_iters = c->h_st->iter;
+  if (st) {
+    st->cg_iterations = c->h_st->iter;
+    st->converged = c->h_st->converged;
+    st->residual_norm = c->h_st->rnorm;
+    st->b_norm = c->h_st->bnorm;
+    st->host_syncs += syncs;
+    if (timing) {
+      double tot = 0.0;
+      const int ran = std::min(c->h_st->iter, launched);
+      for (int k = 0; k < ran; ++k) tot += elapsed(c->ev[8 + 2 * k], c->ev[8 + 2 * k + 1]);
+      st->ms_spmv_total = tot;
+      st->spmv_launches = ran;
+    }
+  }
+  return PD_OK;
+}
+
+int enqueue_update(pd_ctx* c, const pd_step_params* prm) {
+  double lo[3], hi[3];
+  for (int k = 0; k < 3; ++k) {
+    lo[k] = prm->bmin[k] + prm->wall_radius;
+    hi[k] = prm->bmax[k] - prm->wall_radius;
+  }
+  pd::launch_update(c->n, c->x, c->v, c->sol, c->act, prm->integrator, prm->dt, prm->walls, lo, hi,
+                    prm->restitution, c->stream);
+  CHECK_LAUNCH(c, "update");
+  return PD_OK;
+}
+
+void zero_timing(pd_step_stats* st) {
+  if (!st) return;
+  st->ms_contact = st->ms_assemble = st->ms_solve = st->ms_update = 0.0;
+  st->ms_spmv_total = 0.0;
+  st->spmv_launches = 0;
+  st->host_syncs = 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* pd_last_error(const pd_ctx* ctx) {
+  if (ctx) return ctx->err.c_str();
+  return g_last_error.c_str();
+}
+
+int pd_ctx_create(const pd_config* cfg, pd_ctx** out) {
+  if (!out) return fail(nullptr, PD_ERR_ARG, "null output pointer");
+  *out = nullptr;
+  if (!cfg) return fail(nullptr, PD_ERR_ARG, "null config");
+  if (cfg->num_particles < 0) return fail(nullptr, PD_ERR_ARG, "num_particles < 0");
+  if (cfg->max_bonds < 1 || cfg->max_bonds > PD_MAX_BONDS_LIMIT)
+    return fail(nullptr, PD_ERR_ARG, "max_bonds outside [1, 1024]");
+  if (cfg->num_bodies < 1 || cfg->num_bodies > PD_MAX_BODIES)
+    return fail(nullptr, PD_ERR_ARG, "num_bodies outside [1, 8]");
+  for (int i = 0; i < cfg->num_bodies; ++i) {
+    const pd_material& m = cfg->material[i];
+    if (!(m.horizon > 0.0) || !(m.radius > 0.0) || !(m.mass > 0.0))
+      return fail(nullptr, PD_ERR_ARG, "material horizon, radius and mass must be > 0");
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return fail(nullptr, PD_ERR_NO_DEVICE, "no HIP device available");
+  if (cfg->device < 0 || cfg->device >= ndev)
+    return fail(nullptr, PD_ERR_ARG, "device ordinal out of range");
+  pd_ctx* c = new pd_ctx();
+  c->device = cfg->device;
+  c->n = cfg->num_particles;
+  c->nslices = (c->n + 63) / 64;
+  c->max_bonds = cfg->max_bonds;
+  c->nbody = cfg->num_bodies;
+  for (int i = 0; i < PD_MAX_BODIES; ++i) c->mat[i] = cfg->material[i];
+  auto bail = [&](int code) {
+    std::string msg = c->err;
+    pd_ctx_destroy(c);
+    g_last_error = msg;
+    return code;
+  };
+  if (hipSetDevice(c->device) != hipSuccess) return bail(fail(c, PD_ERR_HIP, "hipSetDevice failed"));
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+    return bail(fail(c, PD_ERR_HIP, "hipStreamCreate failed"));
+  const size_t n = (size_t)c->n;
+  const size_t npad = (size_t)c->nslices * 64;
+  const size_t ntiles = (n + PD_BLOCK - 1) / PD_BLOCK;
+  int e = PD_OK;
+#define ALLOC(ptr, count) \
+  if ((e = dalloc(c, &(ptr), (count))) != PD_OK) return bail(e)
+  ALLOC(c->x, n);
+  ALLOC(c->v, n);
+  ALLOC(c->x0, n);
+  ALLOC(c->sol, n);
+  ALLOC(c->f, n);
+  ALLOC(c->b, n);
+  ALLOC(c->fc, n);
+  ALLOC(c->vbc, n);
+  ALLOC(c->s, n);
+  ALLOC(c->s_new, n);
+  ALLOC(c->s00, n);
+  ALLOC(c->act, n);
+  ALLOC(c->body, n);
+  ALLOC(c->bc, n);
+  ALLOC(c->top, n);
+  ALLOC(c->cap, n);
+  ALLOC(c->slice_off, (size_t)c->nslices + 1);
+  ALLOC(c->diag, npad * 6);
+  ALLOC(c->minv, npad * 6);
+  ALLOC(c->r, n);
+  ALLOC(c->z, n);
+  ALLOC(c->p, n);
+  ALLOC(c->y, n);
+  ALLOC(c->part, 3 * ntiles);
+  ALLOC(c->st, 1);
+  ALLOC(c->counters, 1);
+  ALLOC(c->dmat, PD_MAX_BODIES);
+  ALLOC(c->dhorizon, PD_MAX_BODIES);
+  ALLOC(c->gap_val, n);
+  ALLOC(c->gap_q, n);
+  ALLOC(c->gpart_v, ntiles);
+  ALLOC(c->gpart_p, ntiles);
+  ALLOC(c->gpart_q, ntiles);
+  ALLOC(c->gap_out, 1);
+  ALLOC(c->gap_pq, 2);
+  ALLOC(c->keys_a, n);
+  ALLOC(c->keys_b, n);
+  ALLOC(c->idx_a, n);
+  ALLOC(c->idx_b, n);
+  ALLOC(c->hist, pd::radix_hist_words(c->n));
+  ALLOC(c->bbox_part, 256 * 6);
+  ALLOC(c->bbox, 6);
+  ALLOC(c->counts, n);
+  ALLOC(c->widths, (size_t)c->nslices);
+  ALLOC(c->x_bak, n);
+  ALLOC(c->v_bak, n);
+  ALLOC(c->top_bak, n);
+#undef ALLOC
+  if (hipHostMalloc((void**)&c->h_st, sizeof(pd_cg_state), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&c->h_cnt, sizeof(pd_step_counters), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&c->h_bbox, 6 * sizeof(double), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&c->h_gap, sizeof(double), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&c->h_gap_pq, 2 * sizeof(int32_t), hipHostMallocDefault) != hipSuccess)
+    return bail(fail(c, PD_ERR_HIP, "hipHostMalloc failed"));
+  pd_mat_dev hm[PD_MAX_BODIES];
+  double hh[PD_MAX_BODIES];
+  for (int i = 0; i < PD_MAX_BODIES; ++i) {
+    hm[i].horizon = c->mat[i].horizon;
+    hm[i].radius = c->mat[i].radius;
+    hm[i].c_bond = c->mat[i].c_bond;
+    hm[i].mass = c->mat[i].mass;
+    hh[i] = c->mat[i].horizon;
+  }
+  if (hipMemcpyAsync(c->dmat, hm, sizeof(hm), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+      hipMemcpyAsync(c->dhorizon, hh, sizeof(hh), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess)
+    return bail(fail(c, PD_ERR_HIP, "material upload failed"));
+  *out = c;
+  return PD_OK;
+}
+
+int pd_ctx_destroy(pd_ctx* c) {
+  if (!c) return PD_OK;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  dfree(c->x); dfree(c->v); dfree(c->x0); dfree(c->sol); dfree(c->f); dfree(c->b);
+  dfree(c->fc); dfree(c->vbc); dfree(c->s); dfree(c->s_new); dfree(c->s00);
+  dfree(c->act); dfree(c->body); dfree(c->bc); dfree(c->top); dfree(c->cap);
+  dfree(c->slice_off); dfree(c->col); dfree(c->val); dfree(c->diag); dfree(c->minv);
+  dfree(c->r); dfree(c->z); dfree(c->p); dfree(c->y); dfree(c->part); dfree(c->st);
+  dfree(c->counters); dfree(c->dmat); dfree(c->dhorizon);
+  dfree(c->gap_val); dfree(c->gap_q); dfree(c->gpart_v); dfree(c->gpart_p); dfree(c->gpart_q);
+  dfree(c->gap_out); dfree(c->gap_pq);
+  dfree(c->keys_a); dfree(c->keys_b); dfree(c->idx_a); dfree(c->idx_b); dfree(c->hist);
+  dfree(c->bbox_part); dfree(c->bbox); dfree(c->cell_start); dfree(c->cell_end);
+  dfree(c->counts); dfree(c->widths); dfree(c->x_bak); dfree(c->v_bak); dfree(c->top_bak);
+  if (c->h_st) (void)hipHostFree(c->h_st);
+  if (c->h_cnt) (void)hipHostFree(c->h_cnt);
+  if (c->h_bbox) (void)hipHostFree(c->h_bbox);
+  if (c->h_gap) (void)hipHostFree(c->h_gap);
+  if (c->h_gap_pq) (void)hipHostFree(c->h_gap_pq);
+  for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return PD_OK;
+}
+
+int pd_upload(pd_ctx* c, int32_t field, const void* src, size_t bytes) {
+  if (!c) return fail(nullptr, PD_ERR_ARG, "null context");
+  FieldInfo fi = field_info(c, field);
+  if (!fi.dptr || !fi.writable) return fail(c, PD_ERR_ARG, "field cannot be uploaded");
+  if (bytes != fi.elem * (size_t)c->n) return fail(c, PD_ERR_ARG, "upload size mismatch");
+  if (c->n == 0) return PD_OK;
+  if (!src) return fail(c, PD_ERR_ARG, "null source");
+  if (field == PD_FIELD_BODY) {
+    const int32_t* bt = (const int32_t*)src;
+    for (int32_t i = 0; i < c->n; ++i)
+      if (bt[i] < 0 || bt[i] >= c->nbody)
+        return fail(c, PD_ERR_ARG, "body index outside [0, num_bodies)");
+  }
+  HIPCHK(c, hipSetDevice(c->device));
+  if (fi.vec4) {
+    // the w lane is not part of the state: store it as 0
+    std::vector<double> tmp((const double*)src, (const double*)src + 4 * (size_t)c->n);
+    for (size_t i = 0; i < (size_t)c->n; ++i) tmp[4 * i + 3] = 0.0;
+    HIPCHK(c, hipMemcpyAsync(*fi.dptr, tmp.data(), bytes, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  } else {
+    HIPCHK(c, hipMemcpyAsync(*fi.dptr, src, bytes, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
+  return PD_OK;
+}
+
+int pd_download(pd_ctx* c, int32_t field, void* dst, size_t bytes) {
+  if (!c) return fail(nullptr, PD_ERR_ARG, "null context");
+  FieldInfo fi = field_info(c, field);
+  if (!fi.dptr) return fail(c, PD_ERR_ARG, "unknown field");
+  if (bytes != fi.elem * (size_t)c->n) return fail(c, PD_ERR_ARG, "download size mismatch");
+  if (c->n == 0) return PD_OK;
+  if (!dst) return fail(c, PD_ERR_ARG, "null destination");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipMemcpyAsync(dst, *fi.dptr, bytes, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return PD_OK;
+}
+
+int pd_build_bonds(pd_ctx* c, pd_bond_stats* stats) {
+  if (!c) return fail(nullptr, PD_ERR_ARG, "null context");
+  HIPCHK(c, hipSetDevice(c->device));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIPCHK(c, hipEventCreate(&e0));
+  HIPCHK(c, hipEventCreate(&e1));
+  HIPCHK(c, hipEventRecord(e0, c->stream));
+  const int32_t n = c->n;
+  std::vector<int32_t> widths(c->nslices, 0);
+  std::vector<int32_t> counts(n, 0);
+  if (n > 0) {
+    double hmax = 0.0;
+    for (int i = 0; i < c->nbody; ++i) hmax = std::max(hmax, c->mat[i].horizon);
+    // a cell slightly wider than the horizon keeps every bonded pair within
+    // one cell of each other despite rounding in the cell index
+    pd::GridView g;
+    if (int e = build_grid(c, hmax * (1.0 + 1e-7), &g)) return e;
+    pd::launch_bond_count(n, c->x, c->body, c->dhorizon, g, c->counts, c->stream);
+    CHECK_LAUNCH(c, "bond count");
+    pd::launch_slice_width(c->counts, n, c->widths, c->stream);
+    CHECK_LAUNCH(c, "slice width");
+    HIPCHK(c, hipMemcpyAsync(widths.data(), c->widths, sizeof(int32_t) * c->nslices,
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(counts.data(), c->counts, sizeof(int32_t) * n,
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const int32_t mx = *std::max_element(widths.begin(), widths.end());
+    if (mx > c->max_bonds) {
+      c->bonds_ready = false;
+      return fail(c, PD_ERR_BOND_OVERFLOW,
+                  "a particle has " + std::to_string(mx) + " bonds, more than max_bonds = " +
+                      std::to_string(c->max_bonds));
+    }
+    if (int e = set_slice_offsets(c, widths)) return e;
+    pd::launch_bond_fill(n, c->x, c->body, c->dhorizon, g, c->counts, c->slice_off, c->col,
+                         c->top, c->cap, c->stream);
+    CHECK_LAUNCH(c, "bond fill");
+  } else {
+    if (int e = set_slice_offsets(c, widths)) return e;
+  }
+  HIPCHK(c, hipEventRecord(e1, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  int64_t total = 0;
+  int32_t mn = n ? counts[0] : 0, mxc = 0;
+  for (int32_t k : counts) {
+    total += k;
+    mn = std::min(mn, k);
+    mxc = std::max(mxc, k);
+  }
+  c->live_bonds = total;
+  c->bonds_ready = true;
+  c->last_iters = 0;
+  if (stats) {
+    stats->total_bonds = total;
+    stats->min_bonds = mn;
+    stats->max_bonds = mxc;
+    stats->slots = c->slots * 64;
+    stats->ms_build = elapsed(e0, e1);
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return PD_OK;
+}
+
+int pd_set_bonds(pd_ctx* c, const int32_t* counts, const int32_t* lists, int32_t stride) {
+  if (!c) return fail(nullptr, PD_ERR_ARG, "null context");
+  const int32_t n = c->n;
+  if (n > 0 && (!counts || !lists)) return fail(c, PD_ERR_ARG, "null bond arrays");
+  if (stride < 0) return fail(c, PD_ERR_ARG, "negative stride");
+  HIPCHK(c, hipSetDevice(c->device));
+  std::vector<int32_t> widths(c->nslices, 0);
+  for (int32_t p = 0; p < n; ++p) {
+    if (counts[p] < 0 || counts[p] > stride) return fail(c, PD_ERR_ARG, "count outside [0, stride]");
+    if (counts[p] > c->max_bonds) return fail(c, PD_ERR_BOND_OVERFLOW, "row exceeds max_bonds");
+    for (int32_t k = 0; k < counts[p]; ++k) {
+      const int32_t q = lists[(int64_t)p * stride + k];
+      if (q < 0 || q >= n || q == p) return fail(c, PD_ERR_ARG, "bond index out of range");
+    }
+    widths[p >> 6] = std::max(widths[p >> 6], counts[p]);
+  }
+  if (int e = set_slice_offsets(c, widths)) return e;
+  std::vector<int32_t> col((size_t)c->slots * 64, 0), top(n), cap(n);
+  int64_t total = 0;
+  for (int32_t p = 0; p < n; ++p) {
+    const int64_t base = c->h_slice_off[p >> 6];
+    for (int32_t k = 0; k < counts[p]; ++k)
+      col[(size_t)(base + k) * 64 + (p & 63)] = lists[(int64_t)p * stride + k];
+    top[p] = counts[p] - 1;
+    cap[p] = counts[p];
+    total += counts[p];
+  }
+  if (n > 0) {
+    HIPCHK(c, hipMemcpyAsync(c->col, col.data(), col.size() * sizeof(int32_t),
+                             hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->top, top.data(), n * sizeof(int32_t), hipMemcpyHostToDevice,
+                             c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->cap, cap.data(), n * sizeof(int32_t), hipMemcpyHostToDevice,
+                             c->stream));
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->live_bonds = total;
+  c->bonds_ready = true;
+  c->last_iters = 0;
+  return PD_OK;
+}
+
+int pd_get_bonds(pd_ctx* c, int32_t* capacity, int32_t* lists, int32_t stride) {
+  if (!c) return fail(nullptr, PD_ERR_ARG, "null context");
+  if (!c->bonds_ready) return fail(c, PD_ERR_STATE, "bonds not built");
+  const int32_t n = c->n;
+  if (n == 0) return PD_OK;
+  if (!capacity || !lists) return fail(c, PD_ERR_ARG, "null output");
+  HIPCHK(c, hipSetDevice(c->device));
+  std::vector<int32_t> col((size_t)c->slots * 64);
+  HIPCHK(c, hipMemcpyAsync(capacity, c->cap, n * sizeof(int32_t), hipMemcpyDeviceToHost,
+                           c->stream));
+  if (!col.empty())
+    HIPCHK(c, hipMemcpyAsync(col.data(), c->col, col.size() * sizeof(int32_t),
+                             hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int32_t p = 0; p < n; ++p) {
+    if (capacity[p] > stride) return fail(c, PD_ERR_ARG, "stride smaller than a row");
+    const int64_t base = c->h_slice_off[p >> 6];
+    for (int32_t k = 0; k < capacity[p]; ++k)
+      lists[(int64_t)p * stride + k] = col[(size_t)(base + k) * 64 + (p & 63)];
+  }
+  return PD_OK;
+}
+
+int pd_get_matrix(pd_ctx* c, double* diag9, double* vals9, int32_t stride) {
+  if (!c) return fail(nullptr, PD_ERR_ARG, "null context");
+  if (!c->bonds_ready) return fail(c, PD_ERR_STATE, "bonds not built");
+  const int32_t n = c->n;
+  if (n == 0) return PD_OK;
+  if (!diag9 || !vals9) return fail(c, PD_ERR_ARG, "null output");
+  HIPCHK(c, hipSetDevice(c->device));
+  std::vector<double> dg((size_t)c->nslices * 64 * 6), vv((size_t)c->slots * 64 * 6);
+  std::vector<int32_t> top(n);
+  HIPCHK(c, hipMemcpyAsync(dg.data(), c->diag, dg.size() * sizeof(double), hipMemcpyDeviceToHost,
+                           c->stream));
+  if (!vv.empty())
+    HIPCHK(c, hipMemcpyAsync(vv.data(), c->val, vv.size() * sizeof(double),
+                             hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(top.data(), c->top, n * sizeof(int32_t), hipMemcpyDeviceToHost,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  auto unpack = [](const double* six, size_t stridev, double* out9) {
+    const double a00 = six[0], a01 = six[stridev], a02 = six[2 * stridev],
+                 a11 = six[3 * stridev], a12 = six[4 * stridev], a22 = six[5 * stridev];
+    const double m[9] = {a00, a01, a02, a01, a11, a12, a02, a12, a22};
+    memcpy(out9, m, sizeof(m));
+  };
+  for (int32_t p = 0; p < n; ++p) {
+    const int s = p >> 6, lane = p & 63;
+    unpack(dg.data() + (size_t)s * 384 + lane, 64, diag9 + 9 * (size_t)p);
+    if (top[p] + 1 > stride) return fail(c, PD_ERR_ARG, "stride smaller than a row");
+    const int64_t base = c->h_slice_off[s];
+    for (int32_t k = 0; k <= top[p]; ++k)
+      unpack(vv.data() + (size_t)(base + k) * 384 + lane, 64,
+             vals9 + 9 * ((size_t)p * stride + k));
+  }
+  return PD_OK;
+}
+
+int pd_assemble(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st) {
+  if (!c) return fail(nullptr, PD_ERR_ARG, "null context");
+  if (int e = check_params(c, prm)) return e;
+  HIPCHK(c, hipSetDevice(c->device));
+  zero_timing(st);
+  if (prm->contact_radius > 0.0)
+    if (int e = enqueue_contact(c, prm)) return e;
+  if (int e = enqueue_assemble(c, prm)) return e;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  read_assemble_stats(c, prm, st);
+  return PD_OK;
+}
+
+int pd_commit_stretch(pd_ctx* c) {
+  if (!c) return fail(nullptr, PD_ERR_ARG, "null context");
+  std::swap(c->s, c->s_new);
+  return PD_OK;
+}
+
+int pd_solve(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st) {
+  if (!c) return fail(nullptr, PD_ERR_ARG, "null context");
+  if (int e = check_params(c, prm)) return e;
+  HIPCHK(c, hipSetDevice(c->device));
+  if (int e = run_solve(c, prm, st)) return e;
+  if (!c->h_st->converged)
+    return fail(c, PD_ERR_NOT_CONVERGED, "CG reached the iteration limit before rtol");
+  return PD_OK;
+}
+
+int pd_update(pd_ctx* c, const pd_step_params* prm) {
+  if (!c) return fail(nullptr, PD_ERR_ARG, "null context");
+  if (int e = check_params(c, prm)) return e;
+  HIPCHK(c, hipSetDevice(c->device));
+  if (int e = enqueue_update(c, prm)) return e;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return PD_OK;
+}
+
+int pd_step(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st) {
+  if (!c) return fail(nullptr, PD_ERR_ARG, "null context");
+  if (int e = check_params(c, prm)) return e;
+  HIPCHK(c, hipSetDevice(c->device));
+  zero_timing(st);
+  const bool timing = prm->timing != 0;
+  if (timing)
+    if (int e = ensure_events(c, 8)) return e;
+  if (timing) HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+  if (prm->contact_radius > 0.0)
+    if (int e = enqueue_contact(c, prm)) return e;
+  if (timing) HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+  if (int e = enqueue_assemble(c, prm)) return e;
+  if (timing) HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+  std::swap(c->s, c->s_new);  // core/simulator.cuh:2612
+  if (int e = run_solve(c, prm, st)) return e;
+  read_assemble_stats(c, prm, st);  // the solve's waits covered the copies
+  if (timing) HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
+  if (int e = enqueue_update(c, prm)) return e;
+  if (timing) HIPCHK(c, hipEventRecord(c->ev[4], c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (st && timing) {
+    st->ms_contact = elapsed(c->ev[0], c->ev[1]);
+    st->ms_assemble = elapsed(c->ev[1], c->ev[2]);
+    st->ms_solve = elapsed(c->ev[2], c->ev[3]);
+    st->ms_update = elapsed(c->ev[3], c->ev[4]);
+  }
+  if (!c->h_st->converged)
+    return fail(c, PD_ERR_NOT_CONVERGED, "CG reached the iteration limit before rtol");
+  return PD_OK;
+}
+
+int pd_spmv(pd_ctx* c, const double* x4, double* y4) {
+  if (!c) return fail(nullptr, PD_ERR_ARG, "null context");
+  if (!c->bonds_ready) return fail(c, PD_ERR_STATE, "bonds not built");
+  if (c->n == 0) return PD_OK;
+  if (!x4 || !y4) return fail(c, PD_ERR_ARG, "null vector");
+  HIPCHK(c, hipSetDevice(c->device));
+  const size_t bytes = (size_t)c->n * 32;
+  HIPCHK(c, hipMemcpyAsync(c->z, x4, bytes, hipMemcpyHostToDevice, c->stream));
+  pd::launch_spmv_plain(spmv_view(c), c->z, c->y, c->stream);
+  CHECK_LAUNCH(c, "spmv");
+  HIPCHK(c, hipMemcpyAsync(y4, c->y, bytes, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return PD_OK;
+}
+
+int pd_backup(pd_ctx* c) {
+  if (!c) return fail(nullptr, PD_ERR_ARG, "null context");
+  HIPCHK(c, hipSetDevice(c->device));
+  const size_t n = (size_t)c->n;
+  if (n == 0) return PD_OK;
+  HIPCHK(c, hipMemcpyAsync(c->top_bak, c->top, n * 4, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->x_bak, c->x, n * 32, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->v_bak, c->v, n * 32, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return PD_OK;
+}
+
+int pd_restore(pd_ctx* c) {
+  if (!c) return fail(nullptr, PD_ERR_ARG, "null context");
+  HIPCHK(c, hipSetDevice(c->device));
+  const size_t n = (size_t)c->n;
+  if (n == 0) return PD_OK;
+  HIPCHK(c, hipMemcpyAsync(c->top, c->top_bak, n * 4, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->x, c->x_bak, n * 32, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->v, c->v_bak, n * 32, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return PD_OK;
+}
+
+int pd_synchronize(pd_ctx* c) {
+  if (!c) return fail(nullptr, PD_ERR_ARG, "null context");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return PD_OK;
+}
+
+int pd_spmv_bytes(pd_ctx* c, double* bytes) {
+  if (!c || !bytes) return fail(c, PD_ERR_ARG, "null argument");
+  *bytes = (double)c->live_bonds * (6.0 * 8.0 + 4.0) + (double)c->n * (6.0 * 8.0 + 4.0 + 2.0 * 24.0);
+  return PD_OK;
+}
+
+int pd_bench_spmv(pd_ctx* c, int32_t iters, double* ms_per_launch, double* bytes_per_launch) {
+  if (!c) return fail(nullptr, PD_ERR_ARG, "null context");
+  if (!c->bonds_ready) return fail(c, PD_ERR_STATE, "bonds not built");
+  if (iters < 1) return fail(c, PD_ERR_ARG, "iters must be >= 1");
+  HIPCHK(c, hipSetDevice(c->device));
+  if (int e = ensure_events(c, 2)) return e;
+  const pd::SpmvView A = spmv_view(c);
+  pd::launch_spmv_plain(A, c->sol, c->y, c->stream);  // warm
+  CHECK_LAUNCH(c, "spmv");
+  HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+  for (int i = 0; i < iters; ++i) pd::launch_spmv_plain(A, c->sol, c->y, c->stream);
+  HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (ms_per_launch) *ms_per_launch = elapsed(c->ev[0], c->ev[1]) / iters;
+  if (bytes_per_launch) pd_spmv_bytes(c, bytes_per_launch);
+  return PD_OK;
+}
+
+}  // extern "C"

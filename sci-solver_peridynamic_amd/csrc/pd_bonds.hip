@@ -1,0 +1,435 @@
+// pd_bonds.hip -- uniform-grid cell hashing, LDS radix sort, cell
+// start/end and the horizon bond-list build.
+//
+// Replaces the reference's host kd-tree (core/kdtree.h, core/simulator.cu:
+// 825-943) and reuses the grid / sort / cell-range scheme of its SPH path
+// (core/simulator.cuh:61-88, 862-887, 989-1062; core/simulator.cu:1019-1051)
+// for the GPU.  The neighbour predicate is the kd-tree's distance
+// sqrt(((0 + dx^2) + dy^2) + dz^2) < horizon with dx = x_p - x_q, so the
+// lists equal the reference's bit for bit once sorted.
+#include <math.h>
+
+#include "pd_internal.h"
+
+namespace pd {
+
+// ---------------------------------------------------------------- bbox
+__global__ __launch_bounds__(PD_BLOCK) void k_bbox_partial(const double4* x, int n,
+                                                          double* part) {
+  __shared__ double s[6][PD_BLOCK];
+  double lo0 = INFINITY, lo1 = INFINITY, lo2 = INFINITY;
+  double hi0 = -INFINITY, hi1 = -INFINITY, hi2 = -INFINITY;
+  for (int i = blockIdx.x * PD_BLOCK + threadIdx.x; i < n; i += gridDim.x * PD_BLOCK) {
+    const double4 p = x[i];
+    lo0 = fmin(lo0, p.x);
+    lo1 = fmin(lo1, p.y);
+    lo2 = fmin(lo2, p.z);
+    hi0 = fmax(hi0, p.x);
+    hi1 = fmax(hi1, p.y);
+    hi2 = fmax(hi2, p.z);
+  }
+  s[0][threadIdx.x] = lo0;
+  s[1][threadIdx.x] = lo1;
+  s[2][threadIdx.x] = lo2;
+  s[3][threadIdx.x] = hi0;
+  s[4][threadIdx.x] = hi1;
+  s[5][threadIdx.x] = hi2;
+  __syncthreads();
+  for (int w = PD_BLOCK / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      for (int k = 0; k < 3; ++k) s[k][threadIdx.x] = fmin(s[k][threadIdx.x], s[k][threadIdx.x + w]);
+      for (int k = 3; k < 6; ++k) s[k][threadIdx.x] = fmax(s[k][threadIdx.x], s[k][threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) part[blockIdx.x * 6 + threadIdx.x] = s[threadIdx.x][0];
+}
+
+__global__ void k_bbox_final(const double* part, int g, double* out6) {
+  if (threadIdx.x < 6) {
+    const int k = threadIdx.x;
+    double v = k < 3 ? INFINITY : -INFINITY;
+    for (int b = 0; b < g; ++b) v = k < 3 ? fmin(v, part[b * 6 + k]) : fmax(v, part[b * 6 + k]);
+    out6[k] = v;
+  }
+}
+
+static const int kBboxBlocks = 256;
+
+void launch_bbox(const double4* x, int32_t n, double* part, double* out6, hipStream_t s) {
+  hipLaunchKernelGGL(k_bbox_partial, dim3(kBboxBlocks), dim3(PD_BLOCK), 0, s, x, n, part);
+  hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(64), 0, s, part, kBboxBlocks, out6);
+}
+
+// ---------------------------------------------------------------- keys
+__device__ __forceinline__ void cell_of(const double4 p, const double* mn, double cell,
+                                        int& cx, int& cy, int& cz) {
+  cx = (int)floor((p.x - mn[0]) / cell);
+  cy = (int)floor((p.y - mn[1]) / cell);
+  cz = (int)floor((p.z - mn[2]) / cell);
+}
+
+struct KeyArgs {
+  const double4* x;
+  int n;
+  double mn[3];
+  double cell;
+  int dims[3];
+  uint32_t* keys;
+  int32_t* idx;
+};
+
+__global__ __launch_bounds__(PD_BLOCK) void k_cell_keys(KeyArgs a) {
+  const int i = blockIdx.x * PD_BLOCK + threadIdx.x;
+  if (i >= a.n) return;
+  int cx, cy, cz;
+  cell_of(a.x[i], a.mn, a.cell, cx, cy, cz);
+  a.keys[i] = (uint32_t)(((int64_t)cz * a.dims[1] + cy) * a.dims[0] + cx);
+  a.idx[i] = i;
+}
+
+void launch_cell_keys(const double4* x, int32_t n, const double* mn, double cell,
+                      const int32_t* dims, uint32_t* keys, int32_t* idx, hipStream_t s) {
+  KeyArgs a;
+  a.x = x;
+  a.n = n;
+  for (int k = 0; k < 3; ++k) {
+    a.mn[k] = mn[k];
+    a.dims[k] = dims[k];
+  }
+  a.cell = cell;
+  a.keys = keys;
+  a.idx = idx;
+  const int nb = (n + PD_BLOCK - 1) / PD_BLOCK;
+  if (nb) hipLaunchKernelGGL(k_cell_keys, dim3(nb), dim3(PD_BLOCK), 0, s, a);
+}
+
+// ------------------------------------------------------------- radix sort
+// Stable LSD radix sort, 8-bit digits.  A workgroup owns a tile of
+// 256 x 16 keys: (1) LDS histogram of the tile, (2) one-workgroup exclusive
+// scan of the digit-major histogram table, (3) stable scatter: the tile is
+// walked in 16 rounds of 256 keys; inside a wave the lanes holding the same
+// digit are found with eight ballots, counted per wave in LDS, and each key
+// lands at base[digit] + keys of that digit earlier in the tile.
+#define RS_ITEMS 16
+#define RS_TILE (PD_BLOCK * RS_ITEMS)
+
+size_t radix_hist_words(int32_t n) {
+  const size_t nb = (size_t)((n + RS_TILE - 1) / RS_TILE);
+  return 256 * (nb > 0 ? nb : 1);
+}
+
+__global__ __launch_bounds__(PD_BLOCK) void k_radix_hist(const uint32_t* keys, int n, int shift,
+                                                        uint32_t* hist, int nb) {
+  __shared__ uint32_t h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const int base = blockIdx.x * RS_TILE;
+  for (int j = 0; j < RS_ITEMS; ++j) {
+    const int i = base + j * PD_BLOCK + threadIdx.x;
+    if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+  }
+  __syncthreads();
+  hist[(int64_t)threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
+}
+
+// exclusive scan of hist[0..total) by one workgroup of 1024 threads
+__global__ __launch_bounds__(1024) void k_radix_scan(uint32_t* hist, int total) {
+  __shared__ uint32_t sums[1024];
+  const int t = threadIdx.x;
+  const int chunk = (total + 1023) / 1024;
+  const int lo = t * chunk;
+  const int hi = min(lo + chunk, total);
+  uint32_t acc = 0;
+  for (int i = lo; i < hi; ++i) acc += hist[i];
+  sums[t] = acc;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const uint32_t v = t >= off ? sums[t - off] : 0u;
+    __syncthreads();
+    sums[t] += v;
+    __syncthreads();
+  }
+  uint32_t run = t > 0 ? sums[t - 1] : 0u;
+  for (int i = lo; i < hi; ++i) {
+    const uint32_t c = hist[i];
+    hist[i] = run;
+    run += c;
+  }
+}
+
+__global__ __launch_bounds__(PD_BLOCK) void k_radix_scatter(const uint32_t* kin,
+                                                           const int32_t* vin,
+                                                           uint32_t* kout, int32_t* vout,
+                                                           int n, int shift,
+                                                           const uint32_t* hist, int nb) {
+  __shared__ uint32_t run[256];
+  __shared__ uint32_t base[256];
+  __shared__ uint32_t wcnt[4][256];
+  const int t = threadIdx.x;
+  const int w = t >> 6;
+  const int lane = t & 63;
+  run[t] = 0;
+  base[t] = hist[(int64_t)t * nb + blockIdx.x];
+  const unsigned long long lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+  for (int r = 0; r < RS_ITEMS; ++r) {
+    const int i = blockIdx.x * RS_TILE + r * PD_BLOCK + t;
+    const bool valid = i < n;
+    const uint32_t k = valid ? kin[i] : 0u;
+    const int32_t v = valid ? vin[i] : 0;
+    const uint32_t d = (k >> shift) & 255u;
+    wcnt[0][t] = 0;
+    wcnt[1][t] = 0;
+    wcnt[2][t] = 0;
+    wcnt[3][t] = 0;
+    __syncthreads();
+    unsigned long long peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const bool bit = (d >> b) & 1u;
+      const unsigned long long m = __ballot(bit);
+      peers &= bit ? m : ~m;
+    }
+    const uint32_t rank = (uint32_t)__popcll(peers & lt);
+    if (valid && rank == 0) wcnt[w][d] = (uint32_t)__popcll(peers);
+    __syncthreads();
+    if (valid) {
+      uint32_t pos = base[d] + run[d] + rank;
+      for (int ww = 0; ww < w; ++ww) pos += wcnt[ww][d];
+      kout[pos] = k;
+      vout[pos] = v;
+    }
+    __syncthreads();
+    run[t] += wcnt[0][t] + wcnt[1][t] + wcnt[2][t] + wcnt[3][t];
+    __syncthreads();
+  }
+}
+
+int radix_sort(uint32_t* keys_a, int32_t* idx_a, uint32_t* keys_b, int32_t* idx_b,
+               int32_t n, int32_t bits, uint32_t* hist, hipStream_t s) {
+  const int nb = (n + RS_TILE - 1) / RS_TILE;
+  if (n <= 0 || nb == 0) return 0;
+  const int passes = (bits + 7) / 8;
+  uint32_t* kin = keys_a;
+  int32_t* vin = idx_a;
+  uint32_t* kout = keys_b;
+  int32_t* vout = idx_b;
+  int where = 0;
+  for (int ps = 0; ps < passes; ++ps) {
+    const int shift = 8 * ps;
+    hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(PD_BLOCK), 0, s, kin, n, shift, hist, nb);
+    hipLaunchKernelGGL(k_radix_scan, dim3(1), dim3(1024), 0, s, hist, 256 * nb);
+    hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(PD_BLOCK), 0, s, kin, vin, kout, vout, n,
+                       shift, hist, nb);
+    uint32_t* tk = kin;
+    kin = kout;
+    kout = tk;
+    int32_t* tv = vin;
+    vin = vout;
+    vout = tv;
+    where ^= 1;
+  }
+  return where;
+}
+
+// ------------------------------------------------------------ cell ranges
+__global__ __launch_bounds__(PD_BLOCK) void k_fill2(int32_t* a, int32_t* b, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * PD_BLOCK + threadIdx.x;
+  if (i < n) {
+    a[i] = 0;
+    b[i] = 0;
+  }
+}
+
+__global__ __launch_bounds__(PD_BLOCK) void k_cell_ranges(const uint32_t* keys, int n,
+                                                         int32_t* start, int32_t* end) {
+  const int i = blockIdx.x * PD_BLOCK + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t k = keys[i];
+  if (i == 0 || keys[i - 1] != k) start[k] = i;
+  if (i == n - 1 || keys[i + 1] != k) end[k] = i + 1;
+}
+
+void launch_cell_ranges(const uint32_t* keys, int32_t n, int32_t* start, int32_t* end,
+                        int64_t ncell, hipStream_t s) {
+  const int64_t nbc = (ncell + PD_BLOCK - 1) / PD_BLOCK;
+  if (nbc) hipLaunchKernelGGL(k_fill2, dim3((unsigned)nbc), dim3(PD_BLOCK), 0, s, start, end, ncell);
+  const int nb = (n + PD_BLOCK - 1) / PD_BLOCK;
+  if (nb) hipLaunchKernelGGL(k_cell_ranges, dim3(nb), dim3(PD_BLOCK), 0, s, keys, n, start, end);
+}
+
+// ---------------------------------------------------------- neighbour test
+// core/kdtree.h:66-79 dist() with target p: dx = x_p - x_q
+__device__ __forceinline__ bool in_horizon(const double4 a, const double4 b, double h) {
+  const double dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+  const double r = sqrt(dx * dx + dy * dy + dz * dz);
+  return r < h;
+}
+
+struct NbArgs {
+  int n;
+  const double4* x;
+  const int32_t* body;
+  const double* horizon;  // per body
+  GridView g;
+  int32_t* counts;
+  const int64_t* slice_off;
+  int32_t* col;
+  int32_t* top;
+  int32_t* cap;
+};
+
+__global__ __launch_bounds__(PD_BLOCK) void k_bond_count(NbArgs a) {
+  const int p = blockIdx.x * PD_BLOCK + threadIdx.x;
+  if (p >= a.n) return;
+  const double4 xp = a.x[p];
+  const int bp = a.body[p];
+  const double h = a.horizon[bp];
+  int cx, cy, cz;
+  cell_of(xp, a.g.mn, a.g.cell, cx, cy, cz);
+  int count = 0;
+  for (int dz = -1; dz <= 1; ++dz)
+    for (int dy = -1; dy <= 1; ++dy)
+      for (int dx = -1; dx <= 1; ++dx) {
+        const int ix = cx + dx, iy = cy + dy, iz = cz + dz;
+        if (ix < 0 || iy < 0 || iz < 0 || ix >= a.g.dims[0] || iy >= a.g.dims[1] ||
+            iz >= a.g.dims[2])
+          continue;
+        const int64_t cell = ((int64_t)iz * a.g.dims[1] + iy) * a.g.dims[0] + ix;
+        const int e = a.g.cell_end[cell];
+        for (int m = a.g.cell_start[cell]; m < e; ++m) {
+          const int q = a.g.sorted_idx[m];
+          if (q == p || a.body[q] != bp) continue;
+          if (in_horizon(xp, a.x[q], h)) ++count;
+        }
+      }
+  a.counts[p] = count;
+}
+
+void launch_bond_count(int32_t n, const double4* x, const int32_t* body, const double* horizon,
+                       const GridView& g, int32_t* counts, hipStream_t s) {
+  NbArgs a = {};
+  a.n = n;
+  a.x = x;
+  a.body = body;
+  a.horizon = horizon;
+  a.g = g;
+  a.counts = counts;
+  const int nb = (n + PD_BLOCK - 1) / PD_BLOCK;
+  if (nb) hipLaunchKernelGGL(k_bond_count, dim3(nb), dim3(PD_BLOCK), 0, s, a);
+}
+
+// max bond count of each 64-row slice
+__global__ __launch_bounds__(PD_BLOCK) void k_slice_width(const int32_t* counts, int n,
+                                                         int32_t* widths) {
+  const int p = blockIdx.x * PD_BLOCK + threadIdx.x;
+  int c = p < n ? counts[p] : 0;
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) c = max(c, __shfl_xor(c, m, 64));
+  if ((threadIdx.x & 63) == 0 && p < n + 63) {
+    const int s = p >> 6;
+    if ((int64_t)s * 64 < n) widths[s] = c;
+  }
+}
+
+void launch_slice_width(const int32_t* counts, int32_t n, int32_t* widths, hipStream_t s) {
+  const int nb = (n + PD_BLOCK - 1) / PD_BLOCK;
+  if (nb) hipLaunchKernelGGL(k_slice_width, dim3(nb), dim3(PD_BLOCK), 0, s, counts, n, widths);
+}
+
+// One 64-thread workgroup per particle: the wave tests the 27 cells' members
+// 64 at a time, compacts the hits into LDS with a ballot prefix, sorts them
+// with a bitonic network and writes the row into its slice column.
+#define FILL_CAP PD_MAX_LIST
+#ifndef PD_MAX_LIST
+#define PD_MAX_LIST 1024
+#endif
+
+__global__ __launch_bounds__(64) void k_bond_fill(NbArgs a) {
+  __shared__ int list[PD_MAX_LIST];
+  const int p = blockIdx.x;
+  const int lane = threadIdx.x;
+  const double4 xp = a.x[p];
+  const int bp = a.body[p];
+  const double h = a.horizon[bp];
+  int cx, cy, cz;
+  cell_of(xp, a.g.mn, a.g.cell, cx, cy, cz);
+  const unsigned long long lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+  int cnt = 0;
+  for (int dz = -1; dz <= 1; ++dz)
+    for (int dy = -1; dy <= 1; ++dy)
+      for (int dx = -1; dx <= 1; ++dx) {
+        const int ix = cx + dx, iy = cy + dy, iz = cz + dz;
+        if (ix < 0 || iy < 0 || iz < 0 || ix >= a.g.dims[0] || iy >= a.g.dims[1] ||
+            iz >= a.g.dims[2])
+          continue;
+        const int64_t cell = ((int64_t)iz * a.g.dims[1] + iy) * a.g.dims[0] + ix;
+        const int st = a.g.cell_start[cell];
+        const int e = a.g.cell_end[cell];
+        for (int m0 = st; m0 < e; m0 += 64) {
+          const int m = m0 + lane;
+          bool hit = false;
+          int q = 0;
+          if (m < e) {
+            q = a.g.sorted_idx[m];
+            hit = (q != p) && (a.body[q] == bp) && in_horizon(xp, a.x[q], h);
+          }
+          const unsigned long long mask = __ballot(hit);
+          if (hit) {
+            const int pos = cnt + __popcll(mask & lt);
+            if (pos < PD_MAX_LIST) list[pos] = q;
+          }
+          cnt += __popcll(mask);
+        }
+      }
+  // the slice width was sized from k_bond_count's counts; never write past it
+  if (cnt > a.counts[p]) cnt = a.counts[p];
+  if (cnt > PD_MAX_LIST) cnt = PD_MAX_LIST;
+  int P = 1;
+  while (P < cnt) P <<= 1;
+  for (int i = cnt + lane; i < P; i += 64) list[i] = 0x7fffffff;
+  __syncthreads();
+  for (int k = 2; k <= P; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = lane; i < P; i += 64) {
+        const int ixj = i ^ j;
+        if (ixj > i) {
+          const int u = list[i], v = list[ixj];
+          const bool asc = (i & k) == 0;
+          if ((u > v) == asc) {
+            list[i] = v;
+            list[ixj] = u;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  const int s = p >> 6;
+  const int pl = p & 63;
+  const int64_t base = a.slice_off[s];
+  for (int k = lane; k < cnt; k += 64) a.col[(base + k) * 64 + pl] = list[k];
+  if (lane == 0) {
+    a.top[p] = cnt - 1;
+    a.cap[p] = cnt;
+  }
+}
+
+void launch_bond_fill(int32_t n, const double4* x, const int32_t* body, const double* horizon,
+                      const GridView& g, const int32_t* counts, const int64_t* slice_off,
+                      int32_t* col, int32_t* top, int32_t* cap, hipStream_t s) {
+  NbArgs a = {};
+  a.counts = const_cast<int32_t*>(counts);
+  a.n = n;
+  a.x = x;
+  a.body = body;
+  a.horizon = horizon;
+  a.g = g;
+  a.slice_off = slice_off;
+  a.col = col;
+  a.top = top;
+  a.cap = cap;
+  if (n > 0) hipLaunchKernelGGL(k_bond_fill, dim3(n), dim3(64), 0, s, a);
+}
+
+}  // namespace pd

@@ -1,0 +1,83 @@
+// pd_device.h -- device-side layout and helpers shared by the HIP kernels.
+//
+// HBM layout (one context, N particles, S = ceil(N/64) slices):
+//   * particle vectors: double4 per particle (32 B, w lane 0) so that the
+//     gathered x_q of a bond is one aligned 32-byte segment;
+//   * bonds: sliced ELL with 64 rows per slice (one wave64 lane per row).
+//     Slice s owns slot rows [slice_off[s], slice_off[s+1]); the slot k of
+//     row p = 64 s + lane lives at (slice_off[s] + k) * 64 + lane, so the
+//     64 lanes of a wave touch 64 consecutive words per slot;
+//   * matrix values: the 3x3 blocks are symmetric bit for bit, so six
+//     values (a00 a01 a02 a11 a12 a22) are stored in component planes:
+//     value c of slot k of row p at ((slice_off[s] + k) * 6 + c) * 64 + lane;
+//   * diagonal and block-Jacobi inverse blocks: (s * 6 + c) * 64 + lane.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define PD_WAVE 64
+#define PD_BLOCK 256  // 4 waves = 4 slices = one reduction tile
+#define PD_INVISIBLE 3
+
+struct pd_mat_dev {
+  double horizon, radius, c_bond, mass;
+};
+
+// Scalars of one CG solve, resident in device memory.
+struct pd_cg_state {
+  double rz;     // <r, z>
+  double yp;     // <A p, p>
+  double alpha;
+  double beta;
+  double rr;     // <r, r>
+  double bnorm;  // ||S b||
+  double tol;    // rtol * bnorm
+  double rnorm;  // sqrt(rr)
+  double rtol;
+  int32_t iter;
+  int32_t done;
+  int32_t converged;
+  int32_t max_iter;
+  uint32_t arrive;  // last-workgroup arrival counter, reset by the last one
+  uint32_t pad[3];
+};
+
+// Per-step counters written by the assembly kernel.
+struct pd_step_counters {
+  unsigned long long broken;
+  unsigned long long live;
+};
+
+__device__ __forceinline__ double4 ld4(const double4* p) { return *p; }
+
+// Blocks of the workgroup grid are handed to the 8 XCDs round-robin; this
+// maps the hardware order onto contiguous tile ranges per XCD so that the
+// tiles sharing an L2 are neighbours in the particle order.
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+  const int xcd = b & 7;
+  const int idx = b >> 3;
+  const int q = nb >> 3;
+  const int r = nb & 7;
+  return xcd < r ? xcd * (q + 1) + idx : r * (q + 1) + (xcd - r) * q + idx;
+}
+
+// Sum of a double over the 64 lanes by the xor butterfly m = 32..1; every
+// lane ends with the same value (the order fixed by the oracle's orc_dot).
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v = v + __shfl_xor(v, m, 64);
+  return v;
+}
+
+// Tile reduction of one value per thread of a 256-thread workgroup:
+// (w0 + w1) + (w2 + w3) of the four wave sums.  All threads must call it.
+__device__ __forceinline__ double tile_sum(double v, double* lds4) {
+  v = wave_sum(v);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) lds4[w] = v;
+  __syncthreads();
+  const double r = (lds4[0] + lds4[1]) + (lds4[2] + lds4[3]);
+  __syncthreads();
+  return r;
+}

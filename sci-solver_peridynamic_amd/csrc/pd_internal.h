@@ -1,0 +1,131 @@
+// pd_internal.h -- launchers shared between the kernel files and the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pd_device.h"
+
+namespace pd {
+
+// Bond topology in the sliced layout (see pd_device.h).
+struct BondView {
+  int32_t n;
+  int32_t nslices;
+  const int64_t* slice_off;  // [nslices + 1], in slot rows
+  int32_t* top;              // [n] last live slot (-1: none)
+  int32_t* col;              // [slots * 64]
+};
+
+struct AsmArgs {
+  int32_t n;
+  const int64_t* slice_off;
+  int32_t* top;
+  int32_t* col;
+  double* val;
+  double* diag;
+  double* minv;
+  const double4* x;
+  const double4* x0;
+  const double4* v;
+  const double4* fc;
+  const double4* vbc;
+  double4* f;
+  double4* b;
+  double4* sol;
+  const double* s;
+  double* s_new;
+  const double* s00;
+  const int32_t* act;
+  const int32_t* body;
+  const int32_t* bc;
+  const pd_mat_dev* mat;  // device copy of the body table
+  double dt;
+  double dt2;
+  double clock;
+  double gravity;
+  int32_t integrator;
+  int32_t contact;
+  int32_t track_gap;
+  double* gap_val;   // [n] when track_gap
+  int32_t* gap_q;    // [n] when track_gap
+  pd_step_counters* counters;
+};
+
+struct SpmvView {
+  int32_t n;
+  const int64_t* slice_off;
+  const int32_t* top;
+  const int32_t* col;
+  const double* val;
+  const double* diag;
+};
+
+struct CgArgs {
+  SpmvView A;
+  const double* minv;
+  const int32_t* bc;
+  const double4* b;
+  double4* x;
+  double4* r;
+  double4* z;
+  double4* p;
+  double4* y;
+  double* part;  // [3 * ntiles]
+  pd_cg_state* st;
+  int32_t precond;
+};
+
+struct GridView {
+  double mn[3];
+  double cell;
+  int32_t dims[3];
+  const uint32_t* sorted_key;
+  const int32_t* sorted_idx;
+  const int32_t* cell_start;
+  const int32_t* cell_end;
+};
+
+void launch_assemble(const AsmArgs& a, hipStream_t s);
+void launch_gap_reduce(const double* gap_val, const int32_t* gap_q, int32_t n,
+                       double* part_val, int32_t* part_p, int32_t* part_q,
+                       double* out_val, int32_t* out_pq, hipStream_t s);
+void launch_cg_setup(pd_cg_state* st, double rtol, int32_t max_iter, hipStream_t s);
+void launch_cg_init(const CgArgs& a, hipStream_t s);
+void launch_cg_spmv(const CgArgs& a, hipStream_t s);
+void launch_cg_update(const CgArgs& a, hipStream_t s);
+void launch_cg_dir(const CgArgs& a, hipStream_t s);
+void launch_spmv_plain(const SpmvView& A, const double4* x, double4* y, hipStream_t s);
+void launch_update(int32_t n, double4* x, double4* v, const double4* sol,
+                   const int32_t* act, int32_t integrator, double dt, int32_t walls,
+                   const double* lo, const double* hi, double restitution,
+                   hipStream_t s);
+void launch_contact(int32_t n, const double4* x, const int32_t* body,
+                    const int32_t* act, const GridView& g, double Rc, double kc,
+                    double vol, double4* fc, hipStream_t s);
+void launch_reset_counters(pd_step_counters* c, hipStream_t s);
+
+// pd_bonds.hip
+void launch_bbox(const double4* x, int32_t n, double* part, double* out6,
+                 hipStream_t s);
+void launch_cell_keys(const double4* x, int32_t n, const double* mn, double cell,
+                      const int32_t* dims, uint32_t* keys, int32_t* idx,
+                      hipStream_t s);
+// Stable LSD radix sort of (keys, idx) by the low `bits` bits; returns which
+// buffer pair (0: a, 1: b) holds the result.
+int radix_sort(uint32_t* keys_a, int32_t* idx_a, uint32_t* keys_b, int32_t* idx_b,
+               int32_t n, int32_t bits, uint32_t* hist, hipStream_t s);
+size_t radix_hist_words(int32_t n);
+void launch_cell_ranges(const uint32_t* keys, int32_t n, int32_t* start,
+                        int32_t* end, int64_t ncell, hipStream_t s);
+void launch_bond_count(int32_t n, const double4* x, const int32_t* body,
+                       const double* horizon, const GridView& g, int32_t* counts,
+                       hipStream_t s);
+void launch_slice_width(const int32_t* counts, int32_t n, int32_t* widths,
+                        hipStream_t s);
+void launch_bond_fill(int32_t n, const double4* x, const int32_t* body,
+                      const double* horizon, const GridView& g, const int32_t* counts,
+                      const int64_t* slice_off, int32_t* col, int32_t* top,
+                      int32_t* cap, hipStream_t s);
+
+}  // namespace pd

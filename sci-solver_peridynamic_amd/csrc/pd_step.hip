@@ -1,0 +1,782 @@
+// pd_step.hip -- per-step kernels of the implicit peridynamics path:
+// fused force / Jacobian / breaking / fill, block-sliced SpMV with the CG
+// recurrences and their deterministic reductions, position update, contact.
+//
+// Floating point follows the reference's host semantics bit for bit: the
+// file is compiled with -ffp-contract=off (no FMA), IEEE division and
+// square root, and the two float roundings of cg_solver.cuh:256,264.
+#include <math.h>
+
+#include "pd_internal.h"
+
+namespace pd {
+
+// ---------------------------------------------------------------------------
+// Fused assembly: core/cg_solver.cuh:200-339 (calculatePDForceDerivative)
+// + core/implicit_euler.cuh:19-86 / core/newmark_beta.cuh:19-86 (fill A, b)
+// + the block-Jacobi inverse of A_pp.  One lane per particle; a wave is one
+// 64-row slice, so every slot read / write is 64 consecutive words.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int p = tile * PD_BLOCK + threadIdx.x;
+  unsigned long long broken = 0, live = 0;
+  if (p < a.n && a.act[p] != PD_INVISIBLE) {
+    const pd_mat_dev mt = a.mat[a.body[p]];
+    const int s = p >> 6;
+    const int lane = p & 63;
+    const int64_t base = a.slice_off[s];
+    const double4 pos_p = a.x[p];
+    const double4 opos_p = a.x0[p];
+    double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+    int top = a.top[p];
+    const int otop = top;
+    const double stretch = a.s[p];
+    const float stretch_f = (float)stretch;
+    double smin = 0.0;
+    double k00 = 0.0, k01 = 0.0, k02 = 0.0, k11 = 0.0, k12 = 0.0, k22 = 0.0;
+    double gmin = INFINITY;
+    int gq = -1;
+    const double hr = mt.horizon - mt.radius;
+    int bnd = 0;
+    while (bnd <= top) {
+      const int64_t slot = base + bnd;
+      const int q = a.col[slot * 64 + lane];
+      const double4 xq = a.x[q];
+      double e0 = xq.x - pos_p.x, e1 = xq.y - pos_p.y, e2 = xq.z - pos_p.z;
+      const double dij = sqrt(e0 * e0 + e1 * e1 + e2 * e2);
+      // cg_solver.cuh:256: divide by (float)max(dij, SMALL)
+      const float dij_f = (float)dij;
+      const float small_f = (float)1.0e-10;
+      const double fd = (double)(dij_f > small_f ? dij_f : small_f);
+      e0 /= fd;
+      e1 /= fd;
+      e2 /= fd;
+      const double4 oq = a.x0[q];
+      const double o0 = opos_p.x - oq.x, o1 = opos_p.y - oq.y, o2 = opos_p.z - oq.z;
+      const double d0 = sqrt(o0 * o0 + o1 * o1 + o2 * o2);
+      const double drdivd = dij / d0 - 1.0;
+      // cg_solver.cuh:264: threshold (float)max(s_p, s_q)
+      const float sq_f = (float)a.s[q];
+      const double thr = (double)(stretch_f > sq_f ? stretch_f : sq_f);
+      if (a.track_gap) {
+        const double gap = fabs(drdivd - thr);
+        if (gap < gmin) {
+          gmin = gap;
+          gq = q;
+        }
+      }
+      if (drdivd > thr) {
+        // swap-remove: the last live bond takes this slot, walker stays
+        const int64_t last_slot = (base + top) * 64 + lane;
+        const int last = a.col[last_slot];
+        a.col[slot * 64 + lane] = last;
+        a.col[last_slot] = q;
+        --top;
+        continue;
+      }
+      double vscale = 1.0;
+      if (d0 > hr) vscale = 0.5 + (mt.horizon - d0) / 2.0 / mt.radius;
+      if (drdivd < smin) smin = drdivd;
+      const double kk = drdivd * vscale;
+      f0 += kk * e0;
+      f1 += kk * e1;
+      f2 += kk * e2;
+      const double ratio = d0 / dij;
+      const double dg = 1.0 - ratio;
+      const double sc = mt.c_bond / d0;
+      const double F00 = (dg + (e0 * e0) * ratio) * sc;
+      const double F01 = (0.0 + (e0 * e1) * ratio) * sc;
+      const double F02 = (0.0 + (e0 * e2) * ratio) * sc;
+      const double F11 = (dg + (e1 * e1) * ratio) * sc;
+      const double F12 = (0.0 + (e1 * e2) * ratio) * sc;
+      const double F22 = (dg + (e2 * e2) * ratio) * sc;
+      k00 += F00;
+      k01 += F01;
+      k02 += F02;
+      k11 += F11;
+      k12 += F12;
+      k22 += F22;
+      double* vo = a.val + slot * 6 * 64 + lane;
+      vo[0 * 64] = (F00 * -1.0) * a.dt2;
+      vo[1 * 64] = (F01 * -1.0) * a.dt2;
+      vo[2 * 64] = (F02 * -1.0) * a.dt2;
+      vo[3 * 64] = (F11 * -1.0) * a.dt2;
+      vo[4 * 64] = (F12 * -1.0) * a.dt2;
+      vo[5 * 64] = (F22 * -1.0) * a.dt2;
+      ++bnd;
+    }
+    // A_pp = K_pp dt^2 + m I
+    const double d00 = k00 * a.dt2 + mt.mass;
+    const double d01 = k01 * a.dt2 + 0.0;
+    const double d02 = k02 * a.dt2 + 0.0;
+    const double d11 = k11 * a.dt2 + mt.mass;
+    const double d12 = k12 * a.dt2 + 0.0;
+    const double d22 = k22 * a.dt2 + mt.mass;
+    double* dgo = a.diag + (int64_t)s * 6 * 64 + lane;
+    dgo[0 * 64] = d00;
+    dgo[1 * 64] = d01;
+    dgo[2 * 64] = d02;
+    dgo[3 * 64] = d11;
+    dgo[4 * 64] = d12;
+    dgo[5 * 64] = d22;
+    // block-Jacobi inverse by cofactors (six distinct entries)
+    {
+      const double c00 = d11 * d22 - d12 * d12;
+      const double c01 = d02 * d12 - d01 * d22;
+      const double c02 = d01 * d12 - d02 * d11;
+      const double c11 = d00 * d22 - d02 * d02;
+      const double c12 = d01 * d02 - d00 * d12;
+      const double c22 = d00 * d11 - d01 * d01;
+      const double det = (d00 * c00 + d01 * c01) + d02 * c02;
+      const double id = 1.0 / det;
+      double* mo = a.minv + (int64_t)s * 6 * 64 + lane;
+      mo[0 * 64] = c00 * id;
+      mo[1 * 64] = c01 * id;
+      mo[2 * 64] = c02 * id;
+      mo[3 * 64] = c11 * id;
+      mo[4 * 64] = c12 * id;
+      mo[5 * 64] = c22 * id;
+    }
+    f0 *= mt.c_bond;
+    f1 *= mt.c_bond;
+    f2 *= mt.c_bond;
+    const double ss0 = (10.0 - 9.0 * a.clock) * a.s00[p];
+    a.s_new[p] = ss0 - 0.25 * smin;
+    f1 -= (a.clock * a.gravity * mt.mass);
+    if (a.contact) {
+      const double4 c = a.fc[p];
+      const double len = sqrt(c.x * c.x + c.y * c.y + c.z * c.z);
+      if (len > 1e-10) {
+        f0 += c.x;
+        f1 += c.y;
+        f2 += c.z;
+      }
+    }
+    a.f[p] = make_double4(f0, f1, f2, 0.0);
+    const double4 vp = a.v[p];
+    double4 bo;
+    if (a.integrator == 1) {
+      const double hdt = 0.5 * a.dt;
+      bo.x = a.dt * (mt.mass * vp.x + hdt * f0);
+      bo.y = a.dt * (mt.mass * vp.y + hdt * f1);
+      bo.z = a.dt * (mt.mass * vp.z + hdt * f2);
+    } else {
+      bo.x = mt.mass * vp.x + a.dt * f0;
+      bo.y = mt.mass * vp.y + a.dt * f1;
+      bo.z = mt.mass * vp.z + a.dt * f2;
+    }
+    bo.w = 0.0;
+    a.b[p] = bo;
+    if (a.bc[p]) {
+      const double4 vb = a.vbc[p];
+      if (a.integrator == 1)
+        a.sol[p] = make_double4(a.dt * vb.x, a.dt * vb.y, a.dt * vb.z, 0.0);
+      else
+        a.sol[p] = make_double4(vb.x, vb.y, vb.z, 0.0);
+    }
+    if (top != otop) {
+      a.top[p] = top;
+      broken = (unsigned long long)(otop - top);
+    }
+    live = (unsigned long long)(top + 1);
+    if (a.track_gap) {
+      a.gap_val[p] = gmin;
+      a.gap_q[p] = gq;
+    }
+  } else if (p < a.n && a.track_gap) {
+    a.gap_val[p] = INFINITY;
+    a.gap_q[p] = -1;
+  }
+  // integer counters: order-free
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) {
+    broken += __shfl_xor(broken, m, 64);
+    live += __shfl_xor(live, m, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    if (broken) atomicAdd(&a.counters->broken, broken);
+    if (live) atomicAdd(&a.counters->live, live);
+  }
+}
+
+void launch_assemble(const AsmArgs& a, hipStream_t s) {
+  const int nb = (a.n + PD_BLOCK - 1) / PD_BLOCK;
+  if (nb) hipLaunchKernelGGL(k_assemble, dim3(nb), dim3(PD_BLOCK), 0, s, a);
+}
+
+__global__ void k_reset_counters(pd_step_counters* c) {
+  if (threadIdx.x == 0) {
+    c->broken = 0;
+    c->live = 0;
+  }
+}
+
+void launch_reset_counters(pd_step_counters* c, hipStream_t s) {
+  hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, s, c);
+}
+
+// Deterministic arg-min of the per-row gaps: smallest gap, then smallest p.
+__global__ __launch_bounds__(PD_BLOCK) void k_gap_partial(const double* gv,
+                                                         const int32_t* gq, int n,
+                                                         double* pv, int32_t* pp,
+                                                         int32_t* pq) {
+  __shared__ double sv[PD_BLOCK];
+  __shared__ int sp[PD_BLOCK];
+  const int i = blockIdx.x * PD_BLOCK + threadIdx.x;
+  sv[threadIdx.x] = i < n ? gv[i] : INFINITY;
+  sp[threadIdx.x] = i < n ? i : 0x7fffffff;
+  __syncthreads();
+  for (int w = PD_BLOCK / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      const double a = sv[threadIdx.x], b = sv[threadIdx.x + w];
+      const int ia = sp[threadIdx.x], ib = sp[threadIdx.x + w];
+      if (b < a || (b == a && ib < ia)) {
+        sv[threadIdx.x] = b;
+        sp[threadIdx.x] = ib;
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    pv[blockIdx.x] = sv[0];
+    pp[blockIdx.x] = sp[0];
+    pq[blockIdx.x] = (sp[0] < n) ? gq[sp[0]] : -1;
+  }
+}
+
+__global__ __launch_bounds__(PD_BLOCK) void k_gap_final(const double* pv,
+                                                       const int32_t* pp,
+                                                       const int32_t* pq, int g,
+                                                       double* out_val,
+                                                       int32_t* out_pq) {
+  __shared__ double sv[PD_BLOCK];
+  __shared__ int sp[PD_BLOCK];
+  __shared__ int sq[PD_BLOCK];
+  double bv = INFINITY;
+  int bp = 0x7fffffff, bq = -1;
+  for (int k = threadIdx.x; k < g; k += PD_BLOCK) {
+    if (pv[k] < bv || (pv[k] == bv && pp[k] < bp)) {
+      bv = pv[k];
+      bp = pp[k];
+      bq = pq[k];
+    }
+  }
+  sv[threadIdx.x] = bv;
+  sp[threadIdx.x] = bp;
+  sq[threadIdx.x] = bq;
+  __syncthreads();
+  for (int w = PD_BLOCK / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      const double a = sv[threadIdx.x], b = sv[threadIdx.x + w];
+      const int ia = sp[threadIdx.x], ib = sp[threadIdx.x + w];
+      if (b < a || (b == a && ib < ia)) {
+        sv[threadIdx.x] = b;
+        sp[threadIdx.x] = ib;
+        sq[threadIdx.x] = sq[threadIdx.x + w];
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    *out_val = sv[0];
+    out_pq[0] = sp[0] == 0x7fffffff ? -1 : sp[0];
+    out_pq[1] = sq[0];
+  }
+}
+
+void launch_gap_reduce(const double* gap_val, const int32_t* gap_q, int32_t n,
+                       double* part_val, int32_t* part_p, int32_t* part_q,
+                       double* out_val, int32_t* out_pq, hipStream_t s) {
+  const int g = (n + PD_BLOCK - 1) / PD_BLOCK;
+  if (g == 0) return;
+  hipLaunchKernelGGL(k_gap_partial, dim3(g), dim3(PD_BLOCK), 0, s, gap_val, gap_q, n,
+                     part_val, part_p, part_q);
+  hipLaunchKernelGGL(k_gap_final, dim3(1), dim3(PD_BLOCK), 0, s, part_val, part_p, part_q,
+                     g, out_val, out_pq);
+}
+
+// ---------------------------------------------------------------------------
+// SpMV row: core/cg_solver.cuh:343-383 computeAx.
+//   y_p = A_pp x_p, then y_p += dot(rows of A_{p,b}, x_{q_b}) for the live
+//   slots in list order; every block product is (a0 x0 + a1 x1) + a2 x2.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void blk6(double a00, double a01, double a02, double a11,
+                                     double a12, double a22, const double4 x,
+                                     double& o0, double& o1, double& o2) {
+  o0 = a00 * x.x + a01 * x.y + a02 * x.z;
+  o1 = a01 * x.x + a11 * x.y + a12 * x.z;
+  o2 = a02 * x.x + a12 * x.y + a22 * x.z;
+}
+
+__device__ __forceinline__ double ntl(const double* p) {
+  return __builtin_nontemporal_load(p);
+}
+__device__ __forceinline__ int ntl(const int* p) { return __builtin_nontemporal_load(p); }
+
+__device__ __forceinline__ void spmv_row(const SpmvView& A, const double4* __restrict__ x,
+                                         int p, double& y0, double& y1, double& y2) {
+  const int s = p >> 6;
+  const int lane = p & 63;
+  const double* dg = A.diag + (int64_t)s * 6 * 64 + lane;
+  blk6(dg[0], dg[64], dg[128], dg[192], dg[256], dg[320], x[p], y0, y1, y2);
+  const int64_t base = A.slice_off[s];
+  const int top = A.top[p];
+  const int* __restrict__ cp = A.col + base * 64 + lane;
+  const double* __restrict__ vp = A.val + base * 6 * 64 + lane;
+  int bnd = 0;
+  // two slots per trip: both slots' loads are issued before either product
+  for (; bnd + 1 <= top; bnd += 2) {
+    const int q0 = ntl(cp + (int64_t)bnd * 64);
+    const int q1 = ntl(cp + (int64_t)(bnd + 1) * 64);
+    const double* v0 = vp + (int64_t)bnd * 384;
+    const double* v1 = v0 + 384;
+    const double a0 = ntl(v0), a1 = ntl(v0 + 64), a2 = ntl(v0 + 128), a3 = ntl(v0 + 192),
+                 a4 = ntl(v0 + 256), a5 = ntl(v0 + 320);
+    const double b0 = ntl(v1), b1 = ntl(v1 + 64), b2 = ntl(v1 + 128), b3 = ntl(v1 + 192),
+                 b4 = ntl(v1 + 256), b5 = ntl(v1 + 320);
+    const double4 x0 = x[q0];
+    const double4 x1 = x[q1];
+    double t0, t1, t2;
+    blk6(a0, a1, a2, a3, a4, a5, x0, t0, t1, t2);
+    y0 += t0;
+    y1 += t1;
+    y2 += t2;
+    blk6(b0, b1, b2, b3, b4, b5, x1, t0, t1, t2);
+    y0 += t0;
+    y1 += t1;
+    y2 += t2;
+  }
+  if (bnd <= top) {
+    const int q0 = ntl(cp + (int64_t)bnd * 64);
+    const double* v0 = vp + (int64_t)bnd * 384;
+    double t0, t1, t2;
+    blk6(ntl(v0), ntl(v0 + 64), ntl(v0 + 128), ntl(v0 + 192), ntl(v0 + 256), ntl(v0 + 320),
+         x[q0], t0, t1, t2);
+    y0 += t0;
+    y1 += t1;
+    y2 += t2;
+  }
+}
+
+__global__ __launch_bounds__(PD_BLOCK) void k_spmv_plain(SpmvView A,
+                                                        const double4* __restrict__ x,
+                                                        double4* __restrict__ y) {
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int p = tile * PD_BLOCK + threadIdx.x;
+  if (p >= A.n) return;
+  double y0, y1, y2;
+  spmv_row(A, x, p, y0, y1, y2);
+  y[p] = make_double4(y0, y1, y2, 0.0);
+}
+
+void launch_spmv_plain(const SpmvView& A, const double4* x, double4* y, hipStream_t s) {
+  const int nb = (A.n + PD_BLOCK - 1) / PD_BLOCK;
+  if (nb) hipLaunchKernelGGL(k_spmv_plain, dim3(nb), dim3(PD_BLOCK), 0, s, A, x, y);
+}
+
+// ---------------------------------------------------------------------------
+// CG (CUSP krylov::cg structure, core/simulator.cu:763-770) with scalars
+// kept on the device.  Each kernel writes one partial per 256-row tile; the
+// last workgroup to arrive folds the partials in the fixed order of the
+// oracle's orc_dot and publishes the scalar for the next kernel.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void precond(const double* minv, int pc, int p, double r0,
+                                        double r1, double r2, double& z0, double& z1,
+                                        double& z2) {
+  if (pc == 1) {
+    const double* m = minv + (int64_t)(p >> 6) * 6 * 64 + (p & 63);
+    const double m00 = m[0], m01 = m[64], m02 = m[128], m11 = m[192], m12 = m[256],
+                 m22 = m[320];
+    z0 = m00 * r0 + m01 * r1 + m02 * r2;
+    z1 = m01 * r0 + m11 * r1 + m12 * r2;
+    z2 = m02 * r0 + m12 * r1 + m22 * r2;
+  } else {
+    z0 = r0;
+    z1 = r1;
+    z2 = r2;
+  }
+}
+
+__device__ __forceinline__ double term3(double a0, double a1, double a2, double b0,
+                                        double b1, double b2) {
+  return (a0 * b0 + a1 * b1) + a2 * b2;
+}
+
+// Publishes up to three tile partials and returns true in the last
+// workgroup, after which part[] holds every tile's values.
+__device__ bool publish_partials(double* part, int ntiles, int tile, int nq,
+                                 const double* v, pd_cg_state* st) {
+  __shared__ int s_last;
+  if (threadIdx.x == 0) {
+    for (int k = 0; k < nq; ++k)
+      __hip_atomic_store(part + (int64_t)k * ntiles + tile, v[k], __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    __threadfence();
+    const unsigned prev = atomicAdd(&st->arrive, 1u);
+    s_last = (prev == (unsigned)(ntiles - 1));
+  }
+  __syncthreads();
+  const bool last = s_last != 0;
+  if (last) __threadfence();
+  return last;
+}
+
+// Final fold of one quantity over the tiles (orc_dot's final_combine).
+__device__ double fold_partials(const double* part, int ntiles, double* lds4) {
+  double acc = 0.0;
+  for (int k = threadIdx.x; k < ntiles; k += PD_BLOCK)
+    acc = acc + __hip_atomic_load(part + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return tile_sum(acc, lds4);
+}
+
+__global__ void k_cg_setup(pd_cg_state* st, double rtol, int max_iter) {
+  if (threadIdx.x == 0) {
+    st->rz = 0.0;
+    st->yp = 0.0;
+    st->alpha = 0.0;
+    st->beta = 0.0;
+    st->rr = 0.0;
+    st->bnorm = 0.0;
+    st->tol = 0.0;
+    st->rnorm = 0.0;
+    st->rtol = rtol;
+    st->iter = 0;
+    st->done = 0;
+    st->converged = 0;
+    st->max_iter = max_iter;
+    st->arrive = 0u;
+  }
+}
+
+void launch_cg_setup(pd_cg_state* st, double rtol, int32_t max_iter, hipStream_t s) {
+  hipLaunchKernelGGL(k_cg_setup, dim3(1), dim3(64), 0, s, st, rtol, max_iter);
+}
+
+// y = A x0; r = S(b - y); z = M r; p = z; partials <r,z>, <r,r>, <Sb,Sb>
+__global__ __launch_bounds__(PD_BLOCK) void k_cg_init(CgArgs a) {
+  __shared__ double lds4[4];
+  const int ntiles = gridDim.x;
+  const int tile = xcd_tile(blockIdx.x, ntiles);
+  const int p = tile * PD_BLOCK + threadIdx.x;
+  double t_rz = 0.0, t_rr = 0.0, t_bb = 0.0;
+  if (p < a.A.n) {
+    double y0, y1, y2;
+    spmv_row(a.A, a.x, p, y0, y1, y2);
+    const double4 bp = a.b[p];
+    const bool fr = a.bc[p] == 0;
+    const double r0 = fr ? bp.x - y0 : 0.0;
+    const double r1 = fr ? bp.y - y1 : 0.0;
+    const double r2 = fr ? bp.z - y2 : 0.0;
+    const double sb0 = fr ? bp.x : 0.0, sb1 = fr ? bp.y : 0.0, sb2 = fr ? bp.z : 0.0;
+    double z0, z1, z2;
+    precond(a.minv, a.precond, p, r0, r1, r2, z0, z1, z2);
+    a.r[p] = make_double4(r0, r1, r2, 0.0);
+    a.p[p] = make_double4(z0, z1, z2, 0.0);
+    t_rz = term3(r0, r1, r2, z0, z1, z2);
+    t_rr = term3(r0, r1, r2, r0, r1, r2);
+    t_bb = term3(sb0, sb1, sb2, sb0, sb1, sb2);
+  }
+  double v[3];
+  v[0] = tile_sum(t_rz, lds4);
+  v[1] = tile_sum(t_rr, lds4);
+  v[2] = tile_sum(t_bb, lds4);
+  if (publish_partials(a.part, ntiles, tile, 3, v, a.st)) {
+    const double rz = fold_partials(a.part, ntiles, lds4);
+    const double rr = fold_partials(a.part + ntiles, ntiles, lds4);
+    const double bb = fold_partials(a.part + 2 * (int64_t)ntiles, ntiles, lds4);
+    if (threadIdx.x == 0) {
+      pd_cg_state* st = a.st;
+      const double bnorm = sqrt(bb);
+      const double tol = st->rtol * bnorm;
+      const double rnorm = sqrt(rr);
+      st->rz = rz;
+      st->rr = rr;
+      st->bnorm = bnorm;
+      st->tol = tol;
+      st->rnorm = rnorm;
+      st->iter = 0;
+      if (rnorm <= tol) {
+        st->done = 1;
+        st->converged = 1;
+      } else if (st->max_iter <= 0) {
+        st->done = 1;
+      }
+      st->arrive = 0u;
+    }
+  }
+}
+
+// y = A p; partial <y,p>; last: alpha = rz / <y,p>
+__global__ __launch_bounds__(PD_BLOCK) void k_cg_spmv(CgArgs a) {
+  if (a.st->done) return;
+  __shared__ double lds4[4];
+  const int ntiles = gridDim.x;
+  const int tile = xcd_tile(blockIdx.x, ntiles);
+  const int p = tile * PD_BLOCK + threadIdx.x;
+  double t = 0.0;
+  if (p < a.A.n) {
+    double y0, y1, y2;
+    spmv_row(a.A, a.p, p, y0, y1, y2);
+    a.y[p] = make_double4(y0, y1, y2, 0.0);
+    const double4 pp = a.p[p];
+    t = term3(y0, y1, y2, pp.x, pp.y, pp.z);
+  }
+  double v[1];
+  v[0] = tile_sum(t, lds4);
+  if (publish_partials(a.part, ntiles, tile, 1, v, a.st)) {
+    const double yp = fold_partials(a.part, ntiles, lds4);
+    if (threadIdx.x == 0) {
+      a.st->yp = yp;
+      a.st->alpha = a.st->rz / yp;
+      a.st->arrive = 0u;
+    }
+  }
+}
+
+// x += alpha p; r -= alpha y (free rows); z = M r; partials <r,z>, <r,r>;
+// last: beta, iteration count and the monitor test of the next iteration.
+__global__ __launch_bounds__(PD_BLOCK) void k_cg_update(CgArgs a) {
+  if (a.st->done) return;
+  __shared__ double lds4[4];
+  const int ntiles = gridDim.x;
+  const int tile = xcd_tile(blockIdx.x, ntiles);
+  const int p = tile * PD_BLOCK + threadIdx.x;
+  const double alpha = a.st->alpha;
+  double t_rz = 0.0, t_rr = 0.0;
+  if (p < a.A.n) {
+    const double4 pp = a.p[p];
+    double4 xx = a.x[p];
+    xx.x = alpha * pp.x + xx.x;
+    xx.y = alpha * pp.y + xx.y;
+    xx.z = alpha * pp.z + xx.z;
+    a.x[p] = xx;
+    double4 rr = a.r[p];
+    if (a.bc[p] == 0) {
+      const double4 yy = a.y[p];
+      const double na = -alpha;
+      rr.x = na * yy.x + rr.x;
+      rr.y = na * yy.y + rr.y;
+      rr.z = na * yy.z + rr.z;
+      a.r[p] = rr;
+    }
+    double z0, z1, z2;
+    precond(a.minv, a.precond, p, rr.x, rr.y, rr.z, z0, z1, z2);
+    a.z[p] = make_double4(z0, z1, z2, 0.0);
+    t_rz = term3(rr.x, rr.y, rr.z, z0, z1, z2);
+    t_rr = term3(rr.x, rr.y, rr.z, rr.x, rr.y, rr.z);
+  }
+  double v[2];
+  v[0] = tile_sum(t_rz, lds4);
+  v[1] = tile_sum(t_rr, lds4);
+  if (publish_partials(a.part, ntiles, tile, 2, v, a.st)) {
+    const double rz = fold_partials(a.part, ntiles, lds4);
+    const double rr = fold_partials(a.part + ntiles, ntiles, lds4);
+    if (threadIdx.x == 0) {
+      pd_cg_state* st = a.st;
+      const double rz_old = st->rz;
+      st->rz = rz;
+      st->beta = rz / rz_old;
+      st->iter = st->iter + 1;
+      st->rr = rr;
+      const double rnorm = sqrt(rr);
+      st->rnorm = rnorm;
+      if (rnorm <= st->tol) {
+        st->done = 1;
+        st->converged = 1;
+      } else if (st->iter >= st->max_iter) {
+        st->done = 1;
+      }
+      st->arrive = 0u;
+    }
+  }
+}
+
+// p = z + beta p
+__global__ __launch_bounds__(PD_BLOCK) void k_cg_dir(CgArgs a) {
+  if (a.st->done) return;
+  const int p = xcd_tile(blockIdx.x, gridDim.x) * PD_BLOCK + threadIdx.x;
+  if (p >= a.A.n) return;
+  const double beta = a.st->beta;
+  const double4 zz = a.z[p];
+  double4 pp = a.p[p];
+  pp.x = zz.x + beta * pp.x;
+  pp.y = zz.y + beta * pp.y;
+  pp.z = zz.z + beta * pp.z;
+  a.p[p] = pp;
+}
+
+static inline int tiles_of(int n) { return (n + PD_BLOCK - 1) / PD_BLOCK; }
+
+void launch_cg_init(const CgArgs& a, hipStream_t s) {
+  if (a.A.n) hipLaunchKernelGGL(k_cg_init, dim3(tiles_of(a.A.n)), dim3(PD_BLOCK), 0, s, a);
+}
+void launch_cg_spmv(const CgArgs& a, hipStream_t s) {
+  if (a.A.n) hipLaunchKernelGGL(k_cg_spmv, dim3(tiles_of(a.A.n)), dim3(PD_BLOCK), 0, s, a);
+}
+void launch_cg_update(const CgArgs& a, hipStream_t s) {
+  if (a.A.n) hipLaunchKernelGGL(k_cg_update, dim3(tiles_of(a.A.n)), dim3(PD_BLOCK), 0, s, a);
+}
+void launch_cg_dir(const CgArgs& a, hipStream_t s) {
+  if (a.A.n) hipLaunchKernelGGL(k_cg_dir, dim3(tiles_of(a.A.n)), dim3(PD_BLOCK), 0, s, a);
+}
+
+// ---------------------------------------------------------------------------
+// Position update: core/implicit_euler.cuh:90-176, core/newmark_beta.cuh:
+// 89-199 (velocity then position with the per-axis wall clamp).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void clamp_axis(double& pos, double& vel, double lo, double hi,
+                                           double rest) {
+  if (pos > hi) {
+    vel *= -rest;
+    pos = hi - rest * (pos - hi);
+  }
+  if (pos < lo) {
+    vel *= -rest;
+    pos = lo + rest * (lo - pos);
+  }
+}
+
+struct UpdArgs {
+  int n;
+  double4* x;
+  double4* v;
+  const double4* sol;
+  const int32_t* act;
+  int integrator;
+  double dt;
+  int walls;
+  double lo[3];
+  double hi[3];
+  double rest;
+};
+
+__global__ __launch_bounds__(PD_BLOCK) void k_update(UpdArgs a) {
+  const int p = blockIdx.x * PD_BLOCK + threadIdx.x;
+  if (p >= a.n || a.act[p] == PD_INVISIBLE) return;
+  const double4 s = a.sol[p];
+  const double4 xo = a.x[p];
+  double nv0, nv1, nv2, np0, np1, np2;
+  if (a.integrator == 1) {
+    const double4 vo = a.v[p];
+    const double c = 2.0 / a.dt;
+    nv0 = c * s.x - vo.x;
+    nv1 = c * s.y - vo.y;
+    nv2 = c * s.z - vo.z;
+    np0 = xo.x + s.x;
+    np1 = xo.y + s.y;
+    np2 = xo.z + s.z;
+  } else {
+    nv0 = s.x;
+    nv1 = s.y;
+    nv2 = s.z;
+    np0 = xo.x + nv0 * a.dt;
+    np1 = xo.y + nv1 * a.dt;
+    np2 = xo.z + nv2 * a.dt;
+  }
+  if (a.walls) {
+    clamp_axis(np0, nv0, a.lo[0], a.hi[0], a.rest);
+    clamp_axis(np1, nv1, a.lo[1], a.hi[1], a.rest);
+    clamp_axis(np2, nv2, a.lo[2], a.hi[2], a.rest);
+  }
+  a.x[p] = make_double4(np0, np1, np2, 0.0);
+  a.v[p] = make_double4(nv0, nv1, nv2, 0.0);
+}
+
+void launch_update(int32_t n, double4* x, double4* v, const double4* sol,
+                   const int32_t* act, int32_t integrator, double dt, int32_t walls,
+                   const double* lo, const double* hi, double restitution, hipStream_t s) {
+  UpdArgs a;
+  a.n = n;
+  a.x = x;
+  a.v = v;
+  a.sol = sol;
+  a.act = act;
+  a.integrator = integrator;
+  a.dt = dt;
+  a.walls = walls;
+  for (int k = 0; k < 3; ++k) {
+    a.lo[k] = lo[k];
+    a.hi[k] = hi[k];
+  }
+  a.rest = restitution;
+  const int nb = (n + PD_BLOCK - 1) / PD_BLOCK;
+  if (nb) hipLaunchKernelGGL(k_update, dim3(nb), dim3(PD_BLOCK), 0, s, a);
+}
+
+// ---------------------------------------------------------------------------
+// Contact: core/simulator.cuh:679-766 collidePDParticles on the uniform grid
+// (27 cells in z, y, x order; members in ascending index, as the stable
+// sort leaves them), between particles of different bodies.
+// ---------------------------------------------------------------------------
+struct ContactArgs {
+  int n;
+  const double4* x;
+  const int32_t* body;
+  const int32_t* act;
+  GridView g;
+  double Rc, kc, vol;
+  double4* fc;
+};
+
+__global__ __launch_bounds__(PD_BLOCK) void k_contact(ContactArgs a) {
+  const int p = blockIdx.x * PD_BLOCK + threadIdx.x;
+  if (p >= a.n) return;
+  if (a.act[p] == PD_INVISIBLE) {
+    a.fc[p] = make_double4(0.0, 0.0, 0.0, 0.0);
+    return;
+  }
+  const double4 pp = a.x[p];
+  const int bp = a.body[p];
+  const int cx = (int)floor((pp.x - a.g.mn[0]) / a.g.cell);
+  const int cy = (int)floor((pp.y - a.g.mn[1]) / a.g.cell);
+  const int cz = (int)floor((pp.z - a.g.mn[2]) / a.g.cell);
+  double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+  for (int dz = -1; dz <= 1; ++dz)
+    for (int dy = -1; dy <= 1; ++dy)
+      for (int dx = -1; dx <= 1; ++dx) {
+        const int ix = cx + dx, iy = cy + dy, iz = cz + dz;
+        if (ix < 0 || iy < 0 || iz < 0 || ix >= a.g.dims[0] || iy >= a.g.dims[1] ||
+            iz >= a.g.dims[2])
+          continue;
+        const int64_t cell = ((int64_t)iz * a.g.dims[1] + iy) * a.g.dims[0] + ix;
+        const int e = a.g.cell_end[cell];
+        for (int m = a.g.cell_start[cell]; m < e; ++m) {
+          const int q = a.g.sorted_idx[m];
+          if (q == p) continue;
+          if (a.body[q] == bp) continue;
+          if (a.act[q] == PD_INVISIBLE) continue;
+          const double4 xq = a.x[q];
+          const double q0 = pp.x - xq.x, q1 = pp.y - xq.y, q2 = pp.z - xq.z;
+          const double r = sqrt(q0 * q0 + q1 * q1 + q2 * q2);
+          if (r >= a.Rc) continue;
+          if (r == 0.0) continue;
+          const double howfar = (r - a.Rc);
+          const double mag = a.kc * (howfar * howfar);
+          const double coef = mag / a.vol;
+          f0 += coef * (q0 / r);
+          f1 += coef * (q1 / r);
+          f2 += coef * (q2 / r);
+        }
+      }
+  a.fc[p] = make_double4(f0, f1, f2, 0.0);
+}
+
+void launch_contact(int32_t n, const double4* x, const int32_t* body, const int32_t* act,
+                    const GridView& g, double Rc, double kc, double vol, double4* fc,
+                    hipStream_t s) {
+  ContactArgs a;
+  a.n = n;
+  a.x = x;
+  a.body = body;
+  a.act = act;
+  a.g = g;
+  a.Rc = Rc;
+  a.kc = kc;
+  a.vol = vol;
+  a.fc = fc;
+  const int nb = (n + PD_BLOCK - 1) / PD_BLOCK;
+  if (nb) hipLaunchKernelGGL(k_contact, dim3(nb), dim3(PD_BLOCK), 0, s, a);
+}
+
+}  // namespace pd

@@ -1,0 +1,302 @@
+"""ctypes binding of libpdsolver.so (the C ABI of include/pd_solver.h).
+
+This is plumbing for Python callers (tests, bench): every computation runs
+in the HIP library.  Loading fails loudly when the library is missing; there
+is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpdsolver.so")
+MAX_BODIES = 8
+
+PD_OK = 0
+PD_ERR_HIP = 1
+PD_ERR_ARG = 2
+PD_ERR_NOT_CONVERGED = 3
+PD_ERR_BOND_OVERFLOW = 4
+PD_ERR_STATE = 5
+PD_ERR_NO_DEVICE = 6
+
+FIELD = {
+    "position": 0, "velocity": 1, "original_position": 2, "stretch": 3,
+    "original_stretch": 4, "activity": 5, "body": 6, "bc_flag": 7, "bc_velocity": 8,
+    "solution": 9, "force": 10, "rhs": 11, "new_stretch": 12, "bond_top": 13,
+    "contact_force": 14,
+}
+_FIELD_KIND = {  # (dtype, components)
+    0: (np.float64, 4), 1: (np.float64, 4), 2: (np.float64, 4), 3: (np.float64, 1),
+    4: (np.float64, 1), 5: (np.int32, 1), 6: (np.int32, 1), 7: (np.int32, 1),
+    8: (np.float64, 4), 9: (np.float64, 4), 10: (np.float64, 4), 11: (np.float64, 4),
+    12: (np.float64, 1), 13: (np.int32, 1), 14: (np.float64, 4),
+}
+
+
+class Material(C.Structure):
+    _fields_ = [("horizon", C.c_double), ("radius", C.c_double),
+                ("c_bond", C.c_double), ("mass", C.c_double)]
+
+
+class Config(C.Structure):
+    _fields_ = [("num_particles", C.c_int32), ("device", C.c_int32),
+                ("max_bonds", C.c_int32), ("num_bodies", C.c_int32),
+                ("material", Material * MAX_BODIES)]
+
+
+class StepParams(C.Structure):
+    _fields_ = [
+        ("dt", C.c_double), ("clock_scale", C.c_double), ("gravity", C.c_double),
+        ("integrator", C.c_int32), ("precond", C.c_int32), ("rtol", C.c_double),
+        ("max_iter", C.c_int32), ("walls", C.c_int32),
+        ("bmin", C.c_double * 3), ("bmax", C.c_double * 3),
+        ("wall_radius", C.c_double), ("restitution", C.c_double),
+        ("contact_radius", C.c_double), ("contact_stiffness", C.c_double),
+        ("contact_volume", C.c_double), ("contact_cell", C.c_double),
+        ("track_gap", C.c_int32), ("timing", C.c_int32),
+    ]
+
+
+class StepStats(C.Structure):
+    _fields_ = [
+        ("cg_iterations", C.c_int32), ("converged", C.c_int32),
+        ("residual_norm", C.c_double), ("b_norm", C.c_double),
+        ("broken_bonds", C.c_int64), ("live_bonds", C.c_int64),
+        ("min_gap", C.c_double), ("gap_p", C.c_int32), ("gap_q", C.c_int32),
+        ("ms_contact", C.c_double), ("ms_assemble", C.c_double), ("ms_solve", C.c_double),
+        ("ms_update", C.c_double), ("ms_spmv_total", C.c_double),
+        ("spmv_launches", C.c_int32), ("host_syncs", C.c_int32),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class BondStats(C.Structure):
+    _fields_ = [("total_bonds", C.c_int64), ("min_bonds", C.c_int32),
+                ("max_bonds", C.c_int32), ("slots", C.c_int64), ("ms_build", C.c_double)]
+
+
+DEFAULT_PARAMS = dict(dt=1e-4, clock_scale=1.0, gravity=9.8, integrator=0, precond=1,
+                      rtol=1e-6, max_iter=100000, walls=0, bmin=(-1e30,) * 3,
+                      bmax=(1e30,) * 3, wall_radius=0.0, restitution=0.001,
+                      contact_radius=0.0, contact_stiffness=0.0, contact_volume=1.0,
+                      contact_cell=0.0, track_gap=0, timing=0)
+
+
+def make_params(**kw) -> StepParams:
+    p = StepParams()
+    d = dict(DEFAULT_PARAMS)
+    d.update(kw)
+    for k, v in d.items():
+        if k in ("bmin", "bmax"):
+            getattr(p, k)[:] = [float(t) for t in v]
+        else:
+            setattr(p, k, v)
+    return p
+
+
+class PDError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"pd error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    """Load libpdsolver.so; raises if it is not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"HIP library not built: {LIB_PATH} (run build.py)")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.pd_ctx_create.argtypes = [C.POINTER(Config), C.POINTER(vp)]
+        L.pd_ctx_destroy.argtypes = [vp]
+        L.pd_last_error.argtypes = [vp]
+        L.pd_last_error.restype = C.c_char_p
+        L.pd_upload.argtypes = [vp, C.c_int32, vp, C.c_size_t]
+        L.pd_download.argtypes = [vp, C.c_int32, vp, C.c_size_t]
+        L.pd_build_bonds.argtypes = [vp, C.POINTER(BondStats)]
+        L.pd_set_bonds.argtypes = [vp, vp, vp, C.c_int32]
+        L.pd_get_bonds.argtypes = [vp, vp, vp, C.c_int32]
+        L.pd_get_matrix.argtypes = [vp, vp, vp, C.c_int32]
+        for name in ("pd_step", "pd_assemble", "pd_solve"):
+            getattr(L, name).argtypes = [vp, C.POINTER(StepParams), C.POINTER(StepStats)]
+        L.pd_update.argtypes = [vp, C.POINTER(StepParams)]
+        L.pd_commit_stretch.argtypes = [vp]
+        L.pd_spmv.argtypes = [vp, vp, vp]
+        L.pd_backup.argtypes = [vp]
+        L.pd_restore.argtypes = [vp]
+        L.pd_synchronize.argtypes = [vp]
+        L.pd_bench_spmv.argtypes = [vp, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.pd_spmv_bytes.argtypes = [vp, C.POINTER(C.c_double)]
+        for name in ("pd_ctx_create", "pd_ctx_destroy", "pd_upload", "pd_download",
+                     "pd_build_bonds", "pd_set_bonds", "pd_get_bonds", "pd_get_matrix",
+                     "pd_step", "pd_assemble", "pd_solve", "pd_update", "pd_commit_stretch",
+                     "pd_spmv", "pd_backup", "pd_restore", "pd_synchronize", "pd_bench_spmv",
+                     "pd_spmv_bytes"):
+            getattr(L, name).restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Context:
+    """One particle system resident on a GPU (pd_ctx)."""
+
+    def __init__(self, n, materials, device=0, max_bonds=528):
+        cfg = Config()
+        cfg.num_particles = int(n)
+        cfg.device = int(device)
+        cfg.max_bonds = int(max_bonds)
+        cfg.num_bodies = len(materials)
+        for i, m in enumerate(materials):
+            cfg.material[i] = Material(*[float(t) for t in m])
+        self.n = int(n)
+        self.max_bonds = int(max_bonds)
+        self._h = C.c_void_p()
+        L = lib()
+        rc = L.pd_ctx_create(C.byref(cfg), C.byref(self._h))
+        if rc != PD_OK:
+            raise PDError(rc, L.pd_last_error(None).decode())
+
+    def _check(self, rc):
+        if rc != PD_OK:
+            raise PDError(rc, lib().pd_last_error(self._h).decode())
+
+    def close(self):
+        if self._h:
+            lib().pd_ctx_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- fields
+    def upload(self, field, arr):
+        f = FIELD[field]
+        dt, comp = _FIELD_KIND[f]
+        a = np.ascontiguousarray(arr, dtype=dt)
+        if a.size != self.n * comp:
+            raise ValueError(f"{field}: expected {self.n * comp} values, got {a.size}")
+        self._check(lib().pd_upload(self._h, f, _ptr(a), a.nbytes))
+
+    def download(self, field):
+        f = FIELD[field]
+        dt, comp = _FIELD_KIND[f]
+        shape = (self.n, comp) if comp > 1 else (self.n,)
+        a = np.zeros(shape, dtype=dt)
+        self._check(lib().pd_download(self._h, f, _ptr(a), a.nbytes))
+        return a
+
+    # -- bonds
+    def build_bonds(self):
+        st = BondStats()
+        self._check(lib().pd_build_bonds(self._h, C.byref(st)))
+        return st
+
+    def set_bonds(self, counts, lists):
+        counts = np.ascontiguousarray(counts, dtype=np.int32)
+        lists = np.ascontiguousarray(lists, dtype=np.int32)
+        self._check(lib().pd_set_bonds(self._h, _ptr(counts), _ptr(lists), lists.shape[1]))
+
+    def get_bonds(self, stride=None):
+        stride = stride or self.max_bonds
+        cap = np.zeros(self.n, np.int32)
+        lists = np.zeros((self.n, stride), np.int32)
+        self._check(lib().pd_get_bonds(self._h, _ptr(cap), _ptr(lists), stride))
+        return cap, lists
+
+    def get_matrix(self, stride=None):
+        stride = stride or self.max_bonds
+        diag = np.zeros((self.n, 9))
+        vals = np.zeros((self.n, stride, 9))
+        self._check(lib().pd_get_matrix(self._h, _ptr(diag), _ptr(vals), stride))
+        return diag, vals
+
+    # -- step pieces
+    def assemble(self, prm):
+        st = StepStats()
+        self._check(lib().pd_assemble(self._h, C.byref(prm), C.byref(st)))
+        return st
+
+    def commit_stretch(self):
+        self._check(lib().pd_commit_stretch(self._h))
+
+    def solve(self, prm, allow_fail=False):
+        st = StepStats()
+        rc = lib().pd_solve(self._h, C.byref(prm), C.byref(st))
+        if not (allow_fail and rc == PD_ERR_NOT_CONVERGED):
+            self._check(rc)
+        return st
+
+    def update(self, prm):
+        self._check(lib().pd_update(self._h, C.byref(prm)))
+
+    def step(self, prm, allow_fail=False):
+        st = StepStats()
+        rc = lib().pd_step(self._h, C.byref(prm), C.byref(st))
+        if not (allow_fail and rc == PD_ERR_NOT_CONVERGED):
+            self._check(rc)
+        return st
+
+    def spmv(self, x4):
+        x4 = np.ascontiguousarray(x4, dtype=np.float64)
+        y = np.zeros((self.n, 4))
+        self._check(lib().pd_spmv(self._h, _ptr(x4), _ptr(y)))
+        return y
+
+    def backup(self):
+        self._check(lib().pd_backup(self._h))
+
+    def restore(self):
+        self._check(lib().pd_restore(self._h))
+
+    def synchronize(self):
+        self._check(lib().pd_synchronize(self._h))
+
+    def bench_spmv(self, iters):
+        ms = C.c_double()
+        by = C.c_double()
+        self._check(lib().pd_bench_spmv(self._h, iters, C.byref(ms), C.byref(by)))
+        return ms.value, by.value
+
+    def spmv_bytes(self):
+        by = C.c_double()
+        self._check(lib().pd_spmv_bytes(self._h, C.byref(by)))
+        return by.value
+
+
+def load_state(ctx: Context, x4, v4, s00, s=None, body=None, act=None, bc=None, vbc=None,
+               x0=None):
+    """Upload the particle state the reference's makeReady() uploads."""
+    n = ctx.n
+    ctx.upload("position", x4)
+    ctx.upload("original_position", x4 if x0 is None else x0)
+    ctx.upload("velocity", v4)
+    ctx.upload("original_stretch", s00)
+    ctx.upload("stretch", s00 if s is None else s)
+    ctx.upload("body", np.zeros(n, np.int32) if body is None else body)
+    ctx.upload("activity", np.zeros(n, np.int32) if act is None else act)
+    ctx.upload("bc_flag", np.zeros(n, np.int32) if bc is None else bc)
+    ctx.upload("bc_velocity", np.zeros((n, 4)) if vbc is None else vbc)
+    ctx.upload("solution", np.zeros((n, 4)))

@@ -30,3 +30,30 @@ def make_system(x4, materials, body=None, s0=0.02, seed=1973, v4=None, **kw):
     s00 = orc.stretch_init(n, s0, seed)
     v4 = np.zeros((n, 4)) if v4 is None else v4
     return orc.OracleSystem(x4, v4, s00, rs, col, materials, body=body, **kw)
+
+
+def make_pair(x4, materials, body=None, s0=0.02, seed=1973, v4=None, bc=None, vbc=None,
+              act=None):
+    """The same initial state in the oracle and in a GPU context (bonds built
+    on the GPU); returns (oracle_system, gpu_context)."""
+    import pdsolver as pds
+    n = x4.shape[0]
+    sys_ = make_system(x4, materials, body=body, s0=s0, seed=seed, v4=v4, bc=bc, vbc=vbc,
+                       act=act)
+    ctx = pds.Context(n, materials)
+    pds.load_state(ctx, x4, sys_.v, sys_.s00, body=body, act=act, bc=bc, vbc=vbc)
+    ctx.build_bonds()
+    return sys_, ctx
+
+
+def config0_state(nx=20, ny=20, nz=20, h=0.01, jitter=0.05, seed=1973, top_speed=1.0):
+    """Tension specimen: x-fastest jittered lattice, bottom z-layer fixed,
+    top z-layer moved at +top_speed along z."""
+    x4 = lattice(nx, ny, nz, h, jitter, seed)
+    n = x4.shape[0]
+    k = np.arange(n) // (nx * ny)
+    bc = ((k == 0) | (k == nz - 1)).astype(np.int32)
+    vbc = np.zeros((n, 4))
+    vbc[k == nz - 1, 2] = top_speed
+    v4 = vbc.copy()
+    return x4, bc, vbc, v4
