@@ -70,7 +70,8 @@ typedef enum pd_field {
   PD_FIELD_FORCE = 10,            /* double4[N]  pd_force (download) */
   PD_FIELD_RHS = 11,              /* double4[N]  pd_system_vector */
   PD_FIELD_NEW_STRETCH = 12,      /* double[N]   pd_new_stretch */
-  PD_FIELD_BOND_TOP = 13,         /* int32[N]    pd_bond_list_top */
+  PD_FIELD_BOND_TOP = 13,         /* int32[N]    pd_bond_list_top (upload:
+                                   *              -1 <= top < capacity) */
   PD_FIELD_CONTACT_FORCE = 14,    /* double4[N]  contact force */
   PD_FIELD_COUNT = 15
 } pd_field;

@@ -36,6 +36,7 @@ struct pd_ctx {
   // bonds and matrix (sliced layout, see pd_device.h)
   int64_t* slice_off = nullptr;
   std::vector<int64_t> h_slice_off;
+  std::vector<int32_t> h_cap;  // row capacities (initial bond counts)
   int64_t slots = 0;      // slot rows (each 64 lanes)
   int64_t slots_cap = 0;  // allocated slot rows
   int32_t* col = nullptr;
@@ -152,7 +153,7 @@ FieldInfo field_info(pd_ctx* c, int32_t f) {
     case PD_FIELD_FORCE: return {(void**)&c->f, 32, true, false};
     case PD_FIELD_RHS: return {(void**)&c->b, 32, true, false};
     case PD_FIELD_NEW_STRETCH: return {(void**)&c->s_new, 8, false, false};
-    case PD_FIELD_BOND_TOP: return {(void**)&c->top, 4, false, false};
+    case PD_FIELD_BOND_TOP: return {(void**)&c->top, 4, false, true};
     case PD_FIELD_CONTACT_FORCE: return {(void**)&c->fc, 32, true, false};
     default: return {nullptr, 0, false, false};
   }
@@ -586,6 +587,13 @@ int pd_upload(pd_ctx* c, int32_t field, const void* src, size_t bytes) {
   if (bytes != fi.elem * (size_t)c->n) return fail(c, PD_ERR_ARG, "upload size mismatch");
   if (c->n == 0) return PD_OK;
   if (!src) return fail(c, PD_ERR_ARG, "null source");
+  if (field == PD_FIELD_BOND_TOP) {
+    if (!c->bonds_ready) return fail(c, PD_ERR_STATE, "bonds not built");
+    const int32_t* t = (const int32_t*)src;
+    for (int32_t i = 0; i < c->n; ++i)
+      if (t[i] < -1 || t[i] >= c->h_cap[i])
+        return fail(c, PD_ERR_ARG, "bond top outside [-1, capacity)");
+  }
   if (field == PD_FIELD_BODY) {
     const int32_t* bt = (const int32_t*)src;
     for (int32_t i = 0; i < c->n; ++i)
@@ -671,6 +679,7 @@ int pd_build_bonds(pd_ctx* c, pd_bond_stats* stats) {
   c->live_bonds = total;
   c->bonds_ready = true;
   c->last_iters = 0;
+  c->h_cap = counts;
   if (stats) {
     stats->total_bonds = total;
     stats->min_bonds = mn;
@@ -722,6 +731,7 @@ int pd_set_bonds(pd_ctx* c, const int32_t* counts, const int32_t* lists, int32_t
   c->live_bonds = total;
   c->bonds_ready = true;
   c->last_iters = 0;
+  c->h_cap = cap;
   return PD_OK;
 }
 

@@ -1,0 +1,87 @@
+// host_capi.cpp -- C entry points over the host classes for scripted
+// callers (Python tests and the benchmark): parse a params file, build its
+// scene, describe the derived run settings.  No device code.
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "parameters.h"
+#include "scene.h"
+
+extern "C" {
+
+// Parses params_path and builds its scene.  With null arrays only the
+// particle count is returned in *n_out; otherwise the arrays (sized for
+// *n_out particles) receive positions, velocities, body, Dirichlet flag and
+// prescribed velocity.  Returns 0 or -1 (message on stderr).
+int pdh_scene(const char* params_path, int32_t* n_out, double* x4, double* v4, int32_t* body,
+              int32_t* bc, double* vbc) {
+  SystemParameter sp(params_path);
+  if (!sp.ok()) return -1;
+  SimulationParameters& s = sp.getSimulationParameters();
+  RunningParameters& r = sp.getRunningParameters();
+  Scene scene(s, r, Scene::REGULAR_GRID);
+  if (!scene.ok()) {
+    fprintf(stderr, "%s\n", scene.error());
+    return -1;
+  }
+  const int n = s.num_pd_particle;
+  if (n_out) *n_out = n;
+  if (!x4) return 0;
+  std::vector<int> act(n);
+  scene.initPeridynamicsParticles(act.data(), (real4_t*)x4, (real4_t*)v4);
+  scene.initPeridynamicsBoundary(body, bc, (real4_t*)vbc);
+  return 0;
+}
+
+// Writes a JSON object with the derived run settings of params_path into
+// buf: materials per body, time step, step parameters, s0, step counts.
+// Returns the JSON length, or -1.
+int pdh_describe(const char* params_path, char* buf, int32_t len) {
+  SystemParameter sp(params_path);
+  if (!sp.ok()) return -1;
+  SimulationParameters& s = sp.getSimulationParameters();
+  RunningParameters& r = sp.getRunningParameters();
+  Scene scene(s, r, Scene::REGULAR_GRID);
+  const bool scene_ok = scene.ok();
+  const double dt = s.pd_dt > 0.0 ? s.pd_dt * ((double)s.pd_time_step / (double)TIMESTEP_PD)
+                                  : s.pd_time_step * SYSTEM_TIMESTEP_BASE;
+  char tmp[8192];
+  int w = snprintf(
+      tmp, sizeof(tmp),
+      "{\"num_pd_particle\": %d, \"scene_ok\": %d, \"num_bodies\": %d, "
+      "\"materials\": [[%.17g, %.17g, %.17g, %.17g], [%.17g, %.17g, %.17g, %.17g]], "
+      "\"dt\": %.17g, \"gravity\": %.17g, \"phase_in\": %d, \"integrator\": %d, "
+      "\"precond\": %d, \"rtol\": %.17g, \"max_iter\": %d, \"walls\": %d, "
+      "\"bmin\": [%.17g, %.17g, %.17g], \"bmax\": [%.17g, %.17g, %.17g], "
+      "\"wall_radius\": %.17g, \"restitution\": %.17g, \"contact_radius\": %.17g, "
+      "\"contact_stiffness\": %.17g, \"contact_volume\": %.17g, \"s0\": %.9g, "
+      "\"stretch_seed\": %u, \"final_step\": %d, \"adaptive_integration\": %d, "
+      "\"scale_factor\": %.17g, \"projectile_spacing\": %.17g}",
+      s.num_pd_particle, scene_ok ? 1 : 0, s.num_pd_bodies, s.pd_horizon, s.pd_particle_radius,
+      s.pd_c_bond, s.pd_particle_mass, s.projectile_horizon, s.projectile_particle_radius,
+      s.projectile_c_bond, s.pd_particle_mass, dt, s.pd_gravity, s.pd_phase_in,
+      r.integrator == IMPLICIT_EULER ? 0 : 1, r.cg_preconditioner, r.cg_relative_tolerance,
+      r.cg_max_iterations, r.walls, s.boundary_min_x, s.boundary_min_y, s.boundary_min_z,
+      s.boundary_max_x, s.boundary_max_y, s.boundary_max_z, s.pd_particle_radius, RESTITUTION,
+      r.contact_radius, r.contact_stiffness, r.contact_volume, (double)r.pd_stretch_limit_s0,
+      r.stretch_seed, r.final_step, r.adaptive_integration, s.scaleFactor, s.projectile_spacing);
+  if (w < 0 || w >= len || !buf) return -1;
+  memcpy(buf, tmp, (size_t)w + 1);
+  return w;
+}
+
+// getstretch() of the host path (C library rand(), core/simulator.cu:802-864)
+void pdh_stretch_init(int32_t n, float s0, uint32_t seed, double* out) {
+  srand(seed);
+  for (int32_t i = 0; i < n; ++i) {
+    double sum = 0.0;
+    for (int k = 0; k < 12; ++k) sum += rand() / (double)RAND_MAX;
+    sum -= 6.0;
+    out[i] = ((sum * 0.01 + 1.0) * (double)s0);
+  }
+}
+
+}  // extern "C"

@@ -286,6 +286,78 @@ class Context:
         return by.value
 
 
+HOST_LIB_PATH = os.path.join(HERE, "libpdhost.so")
+CONFIG_DIR = os.path.join(HERE, "configs")
+_hlib = None
+
+
+def host_lib():
+    """libpdhost.so: the C++ SystemParameter / Scene classes (no device code)."""
+    global _hlib
+    if _hlib is None:
+        if not os.path.exists(HOST_LIB_PATH):
+            raise RuntimeError(f"host library not built: {HOST_LIB_PATH} (run build.py)")
+        L = C.CDLL(HOST_LIB_PATH)
+        vp = C.c_void_p
+        L.pdh_scene.argtypes = [C.c_char_p, C.POINTER(C.c_int32), vp, vp, vp, vp, vp]
+        L.pdh_scene.restype = C.c_int
+        L.pdh_describe.argtypes = [C.c_char_p, C.c_char_p, C.c_int32]
+        L.pdh_describe.restype = C.c_int
+        L.pdh_stretch_init.argtypes = [C.c_int32, C.c_float, C.c_uint32, vp]
+        _hlib = L
+    return _hlib
+
+
+def describe(params_path):
+    """Derived run settings of a params file (materials, dt, solver keys)."""
+    import json
+    buf = C.create_string_buffer(16384)
+    n = host_lib().pdh_describe(params_path.encode(), buf, len(buf))
+    if n < 0:
+        raise RuntimeError(f"cannot parse {params_path}")
+    return json.loads(buf.value.decode())
+
+
+def scene(params_path):
+    """Particles of the params file's scene: dict of x4, v4, body, bc, vbc."""
+    L = host_lib()
+    n = C.c_int32()
+    if L.pdh_scene(params_path.encode(), C.byref(n), None, None, None, None, None) != 0:
+        raise RuntimeError(f"cannot build the scene of {params_path}")
+    n = n.value
+    out = dict(x4=np.zeros((n, 4)), v4=np.zeros((n, 4)), body=np.zeros(n, np.int32),
+               bc=np.zeros(n, np.int32), vbc=np.zeros((n, 4)))
+    rc = L.pdh_scene(params_path.encode(), C.byref(C.c_int32()), _ptr(out["x4"]),
+                     _ptr(out["v4"]), _ptr(out["body"]), _ptr(out["bc"]), _ptr(out["vbc"]))
+    if rc != 0:
+        raise RuntimeError(f"cannot build the scene of {params_path}")
+    return out
+
+
+def stretch_init(n, s0, seed=1973):
+    """getstretch() limits of the host path (C library rand())."""
+    out = np.zeros(n)
+    host_lib().pdh_stretch_init(n, s0, seed, _ptr(out))
+    return out
+
+
+def step_kwargs(desc, step):
+    """pd_step_params fields of step `step` (1-based) of a described run."""
+    ph = desc["phase_in"]
+    clock = 1.0 if ph <= 0 else min(step, ph) / float(ph)
+    return dict(dt=desc["dt"], clock_scale=clock, gravity=desc["gravity"],
+                integrator=desc["integrator"], precond=desc["precond"], rtol=desc["rtol"],
+                max_iter=desc["max_iter"], walls=desc["walls"], bmin=desc["bmin"],
+                bmax=desc["bmax"], wall_radius=desc["wall_radius"],
+                restitution=desc["restitution"], contact_radius=desc["contact_radius"],
+                contact_stiffness=desc["contact_stiffness"],
+                contact_volume=desc["contact_volume"], contact_cell=desc["contact_radius"])
+
+
+def materials_of(desc):
+    return [tuple(m) for m in desc["materials"][:desc["num_bodies"]]]
+
+
 def load_state(ctx: Context, x4, v4, s00, s=None, body=None, act=None, bc=None, vbc=None,
                x0=None):
     """Upload the particle state the reference's makeReady() uploads."""
