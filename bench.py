@@ -48,7 +48,7 @@ def read_traffic():
     return None
 
 
-def gpu_run(steps, warmup):
+def gpu_run(steps, warmup, copy_calibration=0):
     desc = pds.describe(CONFIG)
     sc = pds.scene(CONFIG)
     n = desc["num_pd_particle"]
@@ -85,10 +85,19 @@ def gpu_run(steps, warmup):
     elapsed = time.perf_counter() - t0
     spmv_bytes = ctx.spmv_bytes()
     live = st.live_bonds
+    # after the timed region: the plain SpMV kernel y = A x alone
+    plain_ms, plain_bytes = ctx.bench_spmv(20)
+    for _ in range(copy_calibration):
+        # device-to-device copies of known size (positions and velocities
+        # 32 B per particle, bond tops 4 B): the scale of the HBM counters
+        ctx.backup()
+    if copy_calibration:
+        log(f"copy calibration: {copy_calibration} x (2 copies of {32 * n} B + 1 of {4 * n} B)")
     ctx.close()
     return dict(n=n, bonds=int(bst.total_bonds), live=int(live), elapsed=elapsed, steps=steps,
                 spmv_ms=spmv_ms, spmv_n=spmv_n, iters=iters, broken=int(broken),
-                spmv_bytes=spmv_bytes, phase=phase, build_ms=bst.ms_build)
+                spmv_bytes=spmv_bytes, phase=phase, build_ms=bst.ms_build,
+                plain_ms=plain_ms, plain_bytes=plain_bytes)
 
 
 def cpu_baseline(sample_steps=2):
@@ -136,13 +145,15 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--copy-calibration", type=int, default=0,
+                    help="after the run, N rounds of device copies of known size")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU: the implicit peridynamics path is built and "
               "measured on a single MI355X; multi-GPU sharding is design-only.",
               file=sys.stderr)
         sys.exit(2)
-    r = gpu_run(a.steps, a.warmup)
+    r = gpu_run(a.steps, a.warmup, a.copy_calibration)
     ms_step = 1000.0 * r["elapsed"] / r["steps"]
     spmv_avg_ms = r["spmv_ms"] / max(r["spmv_n"], 1)
     achieved = r["spmv_bytes"] / (spmv_avg_ms * 1e-3) / 1e9
@@ -171,6 +182,7 @@ def main():
             "broken_bonds": r["broken"],
             "phase_ms_per_step": {k: v / r["steps"] for k, v in r["phase"].items()},
             "bond_build_ms": r["build_ms"],
+            "plain_spmv_gbs": r["plain_bytes"] / (r["plain_ms"] * 1e-3) / 1e9,
         },
         "roofline": {
             "kernel": "k_cg_spmv (CG SpMV y = A p + <y,p> partial)",

@@ -74,6 +74,8 @@ struct pd_ctx {
   int64_t cell_cap = 0;
   int32_t* counts = nullptr;
   int32_t* widths = nullptr;
+  uint32_t* zone_cnt = nullptr;  // contact candidate tile counts + total
+  int32_t* cand = nullptr;       // contact candidates
 
   // rollback copies
   double4 *x_bak = nullptr, *v_bak = nullptr;
@@ -85,6 +87,7 @@ struct pd_ctx {
   double* h_bbox = nullptr;
   double* h_gap = nullptr;
   int32_t* h_gap_pq = nullptr;
+  uint32_t* h_zone_count = nullptr;
 
   std::vector<hipEvent_t> ev;  // event pool for timing
 };
@@ -189,24 +192,29 @@ int check_params(pd_ctx* c, const pd_step_params* prm) {
   return PD_OK;
 }
 
-// Uniform grid over the current positions: bounds, keys, sort, ranges.
-int build_grid(pd_ctx* c, double cell, pd::GridView* g) {
-  const int32_t n = c->n;
-  pd::launch_bbox(c->x, n, c->bbox_part, c->bbox, c->stream);
+// Bounds of the particles of body `sel` (all when sel < 0), read back.
+int read_bbox(pd_ctx* c, int32_t sel, double* out6) {
+  pd::launch_bbox(c->x, c->body, c->n, sel, c->bbox_part, c->bbox, c->stream);
   CHECK_LAUNCH(c, "bbox");
   HIPCHK(c, hipMemcpyAsync(c->h_bbox, c->bbox, 6 * sizeof(double), hipMemcpyDeviceToHost,
                            c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  memcpy(out6, c->h_bbox, 6 * sizeof(double));
+  return PD_OK;
+}
+
+// Uniform grid with origin mn and edge `cell`: the sub-grid of cells
+// c0 .. c0 + dims - 1 over the m particles subset[0..m) (all n particles
+// when subset is null), sorted by cell with a stable radix sort so each
+// cell lists its particles in ascending index, plus cell start / end.
+int grid_over(pd_ctx* c, double cell, const double* mn, const int32_t* c0, const int32_t* dims,
+              const int32_t* subset, int32_t m, pd::GridView* g) {
   int64_t ncell = 1;
   for (int k = 0; k < 3; ++k) {
-    const double lo = c->h_bbox[k], hi = c->h_bbox[3 + k];
-    if (!(isfinite(lo) && isfinite(hi)))
-      return fail(c, PD_ERR_ARG, "non-finite particle position");
-    const double span = floor((hi - lo) / cell) + 1.0;
-    if (span > 2.0e9) return fail(c, PD_ERR_ARG, "grid too large for the cell size");
-    g->dims[k] = (int32_t)span;
-    g->mn[k] = lo;
-    ncell *= g->dims[k];
+    g->mn[k] = mn[k];
+    g->c0[k] = c0[k];
+    g->dims[k] = dims[k];
+    ncell *= dims[k];
   }
   if (ncell > (int64_t)1 << 31) return fail(c, PD_ERR_ARG, "grid has more than 2^31 cells");
   g->cell = cell;
@@ -218,16 +226,16 @@ int build_grid(pd_ctx* c, double cell, pd::GridView* g) {
     if (int e = dalloc(c, &c->cell_end, (size_t)cap)) return e;
     c->cell_cap = cap;
   }
-  pd::launch_cell_keys(c->x, n, g->mn, cell, g->dims, c->keys_a, c->idx_a, c->stream);
+  pd::launch_cell_keys(c->x, subset, m, mn, cell, c0, dims, c->keys_a, c->idx_a, c->stream);
   CHECK_LAUNCH(c, "cell keys");
   int bits = 1;
   while (bits < 32 && ((int64_t)1 << bits) < ncell) ++bits;
-  const int where = pd::radix_sort(c->keys_a, c->idx_a, c->keys_b, c->idx_b, n, bits, c->hist,
+  const int where = pd::radix_sort(c->keys_a, c->idx_a, c->keys_b, c->idx_b, m, bits, c->hist,
                                    c->stream);
   CHECK_LAUNCH(c, "radix sort");
   const uint32_t* keys = where ? c->keys_b : c->keys_a;
   const int32_t* idx = where ? c->idx_b : c->idx_a;
-  pd::launch_cell_ranges(keys, n, c->cell_start, c->cell_end, ncell, c->stream);
+  pd::launch_cell_ranges(keys, m, c->cell_start, c->cell_end, ncell, c->stream);
   CHECK_LAUNCH(c, "cell ranges");
   g->sorted_key = keys;
   g->sorted_idx = idx;
@@ -236,6 +244,22 @@ int build_grid(pd_ctx* c, double cell, pd::GridView* g) {
   return PD_OK;
 }
 
+int cell_index(double x, double mn, double cell) { return (int)floor((x - mn) / cell); }
+
+// Grid over all particles (bond build): origin at the componentwise minimum.
+int build_grid(pd_ctx* c, double cell, pd::GridView* g) {
+  double bb[6];
+  if (int e = read_bbox(c, -1, bb)) return e;
+  int32_t c0[3] = {0, 0, 0}, dims[3];
+  for (int k = 0; k < 3; ++k) {
+    if (!(isfinite(bb[k]) && isfinite(bb[3 + k])))
+      return fail(c, PD_ERR_ARG, "non-finite particle position");
+    const double span = floor((bb[3 + k] - bb[k]) / cell) + 1.0;
+    if (span > 2.0e9) return fail(c, PD_ERR_ARG, "grid too large for the cell size");
+    dims[k] = (int32_t)span;
+  }
+  return grid_over(c, cell, bb, c0, dims, nullptr, c->n, g);
+}
 int ensure_slots(pd_ctx* c, int64_t slots) {
   if (slots <= c->slots_cap) return PD_OK;
   dfree(c->col);
@@ -284,11 +308,62 @@ pd::CgArgs cg_args(pd_ctx* c, int precond) {
   return a;
 }
 
+// Contact: only particles within the contact radius of another body's
+// bounding box can touch it, so the per-step grid is built over those
+// candidates alone.  The grid keeps the origin (minimum over all particles)
+// and cell edges of the full grid, and candidates stay in ascending order,
+// so every particle sums its contacts in the same order as on the full grid.
 int enqueue_contact(pd_ctx* c, const pd_step_params* prm) {
-  pd::GridView g;
-  if (int e = build_grid(c, prm->contact_cell, &g)) return e;
-  pd::launch_contact(c->n, c->x, c->body, c->act, g, prm->contact_radius, prm->contact_stiffness,
-                     prm->contact_volume, c->fc, c->stream);
+  const int nb = c->nbody;
+  double bb[PD_MAX_BODIES][6];
+  double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int b = 0; b < nb; ++b) {
+    if (int e = read_bbox(c, b, bb[b])) return e;
+    for (int k = 0; k < 3; ++k) {
+      mn[k] = std::min(mn[k], bb[b][k]);
+      mx[k] = std::max(mx[k], bb[b][3 + k]);
+    }
+  }
+  for (int k = 0; k < 3; ++k)
+    if (!(isfinite(mn[k]) && isfinite(mx[k])))
+      return fail(c, PD_ERR_ARG, "non-finite particle position");
+  const double cell = prm->contact_cell;
+  // a small widening keeps pairs at a distance rounding to Rc inside
+  const double w = prm->contact_radius * (1.0 + 1e-6);
+  pd::ZoneBoxes z;
+  z.nbody = nb;
+  double ulo[3] = {INFINITY, INFINITY, INFINITY}, uhi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int b = 0; b < nb; ++b)
+    for (int k = 0; k < 3; ++k) {
+      z.lo[b][k] = bb[b][k] - w;  // empty bodies keep lo = +inf: no zone
+      z.hi[b][k] = bb[b][3 + k] + w;
+      ulo[k] = std::min(ulo[k], z.lo[b][k]);
+      uhi[k] = std::max(uhi[k], z.hi[b][k]);
+    }
+  const size_t tw = pd::zone_tile_words(c->n);
+  pd::launch_zone_select(c->x, c->body, c->n, z, c->zone_cnt, c->cand, c->stream);
+  CHECK_LAUNCH(c, "contact candidates");
+  uint32_t m = 0;
+  if (c->n > 0) {
+    HIPCHK(c, hipMemcpyAsync(c->h_zone_count, c->zone_cnt + tw - 1, sizeof(uint32_t),
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    m = *c->h_zone_count;
+  }
+  pd::GridView g = {};
+  int32_t c0[3] = {0, 0, 0}, dims[3] = {1, 1, 1};
+  if (m > 0) {
+    for (int k = 0; k < 3; ++k) {
+      const double lo = std::max(ulo[k], mn[k]);
+      const double hi = std::min(uhi[k], mx[k]);
+      c0[k] = cell_index(lo, mn[k], cell);
+      dims[k] = cell_index(hi, mn[k], cell) - c0[k] + 1;
+      if (dims[k] < 1) dims[k] = 1;
+    }
+    if (int e = grid_over(c, cell, mn, c0, dims, c->cand, (int32_t)m, &g)) return e;
+  }
+  pd::launch_contact(c->n, c->x, c->body, c->act, g, c->cand, (int32_t)m, prm->contact_radius,
+                     prm->contact_stiffness, prm->contact_volume, c->fc, c->stream);
   CHECK_LAUNCH(c, "contact");
   return PD_OK;
 }
@@ -370,8 +445,11 @@ int run_solve(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st) {
   CHECK_LAUNCH(c, "cg init");
   int launched = 0;
   int syncs = 0;
-  int chunk = std::max(1, c->last_iters);
+  // launch one iteration fewer than the previous solve needed, then single
+  // iterations, then doubling chunks: launches after convergence are rare
+  int chunk = std::max(1, c->last_iters - 1);
   int next = 1;
+  int singles = 2;
   const bool timing = prm->timing != 0;
   for (;;) {
     if (timing)
@@ -392,8 +470,13 @@ int run_solve(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st) {
     if (c->h_st->done) break;
     if (launched > prm->max_iter + 1)
       return fail(c, PD_ERR_HIP, "CG state did not terminate on the device");
-    chunk = next;
-    next = std::min(next * 2, 32);
+    if (singles > 0) {
+      --singles;
+      chunk = 1;
+    } else {
+      next = std::min(next * 2, 32);
+      chunk = next;
+    }
   }
   c->last_iters = c->h_st->iter;
   if (st) {
@@ -526,6 +609,8 @@ int pd_ctx_create(const pd_config* cfg, pd_ctx** out) {
   ALLOC(c->bbox_part, 256 * 6);
   ALLOC(c->bbox, 6);
   ALLOC(c->counts, n);
+  ALLOC(c->zone_cnt, pd::zone_tile_words(c->n));
+  ALLOC(c->cand, n);
   ALLOC(c->widths, (size_t)c->nslices);
   ALLOC(c->x_bak, n);
   ALLOC(c->v_bak, n);
@@ -535,7 +620,8 @@ int pd_ctx_create(const pd_config* cfg, pd_ctx** out) {
       hipHostMalloc((void**)&c->h_cnt, sizeof(pd_step_counters), hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc((void**)&c->h_bbox, 6 * sizeof(double), hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc((void**)&c->h_gap, sizeof(double), hipHostMallocDefault) != hipSuccess ||
-      hipHostMalloc((void**)&c->h_gap_pq, 2 * sizeof(int32_t), hipHostMallocDefault) != hipSuccess)
+      hipHostMalloc((void**)&c->h_gap_pq, 2 * sizeof(int32_t), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&c->h_zone_count, sizeof(uint32_t), hipHostMallocDefault) != hipSuccess)
     return bail(fail(c, PD_ERR_HIP, "hipHostMalloc failed"));
   pd_mat_dev hm[PD_MAX_BODIES];
   double hh[PD_MAX_BODIES];
@@ -568,12 +654,13 @@ int pd_ctx_destroy(pd_ctx* c) {
   dfree(c->gap_out); dfree(c->gap_pq);
   dfree(c->keys_a); dfree(c->keys_b); dfree(c->idx_a); dfree(c->idx_b); dfree(c->hist);
   dfree(c->bbox_part); dfree(c->bbox); dfree(c->cell_start); dfree(c->cell_end);
-  dfree(c->counts); dfree(c->widths); dfree(c->x_bak); dfree(c->v_bak); dfree(c->top_bak);
+  dfree(c->counts); dfree(c->widths); dfree(c->zone_cnt); dfree(c->cand); dfree(c->x_bak); dfree(c->v_bak); dfree(c->top_bak);
   if (c->h_st) (void)hipHostFree(c->h_st);
   if (c->h_cnt) (void)hipHostFree(c->h_cnt);
   if (c->h_bbox) (void)hipHostFree(c->h_bbox);
   if (c->h_gap) (void)hipHostFree(c->h_gap);
   if (c->h_gap_pq) (void)hipHostFree(c->h_gap_pq);
+  if (c->h_zone_count) (void)hipHostFree(c->h_zone_count);
   for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;

@@ -14,12 +14,14 @@
 namespace pd {
 
 // ---------------------------------------------------------------- bbox
-__global__ __launch_bounds__(PD_BLOCK) void k_bbox_partial(const double4* x, int n,
-                                                          double* part) {
+// Bounding box of the particles of one body (body_sel >= 0) or of all.
+__global__ __launch_bounds__(PD_BLOCK) void k_bbox_partial(const double4* x, const int32_t* body,
+                                                          int n, int body_sel, double* part) {
   __shared__ double s[6][PD_BLOCK];
   double lo0 = INFINITY, lo1 = INFINITY, lo2 = INFINITY;
   double hi0 = -INFINITY, hi1 = -INFINITY, hi2 = -INFINITY;
   for (int i = blockIdx.x * PD_BLOCK + threadIdx.x; i < n; i += gridDim.x * PD_BLOCK) {
+    if (body_sel >= 0 && body[i] != body_sel) continue;
     const double4 p = x[i];
     lo0 = fmin(lo0, p.x);
     lo1 = fmin(lo1, p.y);
@@ -45,21 +47,127 @@ __global__ __launch_bounds__(PD_BLOCK) void k_bbox_partial(const double4* x, int
   if (threadIdx.x < 6) part[blockIdx.x * 6 + threadIdx.x] = s[threadIdx.x][0];
 }
 
-__global__ void k_bbox_final(const double* part, int g, double* out6) {
-  if (threadIdx.x < 6) {
-    const int k = threadIdx.x;
+__global__ __launch_bounds__(PD_BLOCK) void k_bbox_final(const double* part, int g, double* out6) {
+  __shared__ double s[6][PD_BLOCK];
+  for (int k = 0; k < 6; ++k) {
     double v = k < 3 ? INFINITY : -INFINITY;
-    for (int b = 0; b < g; ++b) v = k < 3 ? fmin(v, part[b * 6 + k]) : fmax(v, part[b * 6 + k]);
-    out6[k] = v;
+    for (int b = threadIdx.x; b < g; b += PD_BLOCK)
+      v = k < 3 ? fmin(v, part[b * 6 + k]) : fmax(v, part[b * 6 + k]);
+    s[k][threadIdx.x] = v;
   }
+  __syncthreads();
+  for (int w = PD_BLOCK / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      for (int k = 0; k < 3; ++k) s[k][threadIdx.x] = fmin(s[k][threadIdx.x], s[k][threadIdx.x + w]);
+      for (int k = 3; k < 6; ++k) s[k][threadIdx.x] = fmax(s[k][threadIdx.x], s[k][threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) out6[threadIdx.x] = s[threadIdx.x][0];
 }
 
 static const int kBboxBlocks = 256;
 
-void launch_bbox(const double4* x, int32_t n, double* part, double* out6, hipStream_t s) {
-  hipLaunchKernelGGL(k_bbox_partial, dim3(kBboxBlocks), dim3(PD_BLOCK), 0, s, x, n, part);
-  hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(64), 0, s, part, kBboxBlocks, out6);
+void launch_bbox(const double4* x, const int32_t* body, int32_t n, int32_t body_sel,
+                 double* part, double* out6, hipStream_t s) {
+  hipLaunchKernelGGL(k_bbox_partial, dim3(kBboxBlocks), dim3(PD_BLOCK), 0, s, x, body, n,
+                     body_sel, part);
+  hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(PD_BLOCK), 0, s, part, kBboxBlocks, out6);
 }
+
+// ------------------------------------------------------------ compaction
+// Stable selection of the particles inside any zone box of another body:
+// per-tile counts, an exclusive scan of the counts, then each tile writes
+// its selected indices in ascending order.
+#define CP_ITEMS 16
+#define CP_TILE (PD_BLOCK * CP_ITEMS)
+
+__device__ __forceinline__ bool in_zone(const double4 p, int b, const ZoneBoxes& z) {
+  for (int k = 0; k < z.nbody; ++k) {
+    if (k == b) continue;
+    if (p.x >= z.lo[k][0] && p.x <= z.hi[k][0] && p.y >= z.lo[k][1] && p.y <= z.hi[k][1] &&
+        p.z >= z.lo[k][2] && p.z <= z.hi[k][2])
+      return true;
+  }
+  return false;
+}
+
+__global__ __launch_bounds__(PD_BLOCK) void k_zone_count(const double4* x, const int32_t* body,
+                                                        int n, ZoneBoxes z, uint32_t* tile_cnt) {
+  __shared__ uint32_t c;
+  if (threadIdx.x == 0) c = 0;
+  __syncthreads();
+  uint32_t mine = 0;
+  for (int j = 0; j < CP_ITEMS; ++j) {
+    const int i = blockIdx.x * CP_TILE + j * PD_BLOCK + threadIdx.x;
+    if (i < n && in_zone(x[i], body[i], z)) ++mine;
+  }
+  if (mine) atomicAdd(&c, mine);
+  __syncthreads();
+  if (threadIdx.x == 0) tile_cnt[blockIdx.x] = c;
+}
+
+// exclusive scan of g tile counts (one workgroup); total at tile_cnt[g]
+__global__ __launch_bounds__(1024) void k_scan_small(uint32_t* v, int g) {
+  __shared__ uint32_t sums[1024];
+  const int t = threadIdx.x;
+  const int chunk = (g + 1023) / 1024;
+  const int lo = t * chunk;
+  const int hi = min(lo + chunk, g);
+  uint32_t acc = 0;
+  for (int i = lo; i < hi; ++i) acc += v[i];
+  sums[t] = acc;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const uint32_t w = t >= off ? sums[t - off] : 0u;
+    __syncthreads();
+    sums[t] += w;
+    __syncthreads();
+  }
+  uint32_t run = t > 0 ? sums[t - 1] : 0u;
+  for (int i = lo; i < hi; ++i) {
+    const uint32_t c = v[i];
+    v[i] = run;
+    run += c;
+  }
+  if (t == 1023) v[g] = sums[1023];
+}
+
+__global__ __launch_bounds__(PD_BLOCK) void k_zone_scatter(const double4* x, const int32_t* body,
+                                                          int n, ZoneBoxes z,
+                                                          const uint32_t* tile_off,
+                                                          int32_t* out) {
+  __shared__ uint32_t wcnt[4];
+  __shared__ uint32_t run;
+  const int w = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const unsigned long long lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+  if (threadIdx.x == 0) run = tile_off[blockIdx.x];
+  for (int j = 0; j < CP_ITEMS; ++j) {
+    const int i = blockIdx.x * CP_TILE + j * PD_BLOCK + threadIdx.x;
+    const bool sel = i < n && in_zone(x[i], body[i], z);
+    const unsigned long long m = __ballot(sel);
+    __syncthreads();
+    if (lane == 0) wcnt[w] = (uint32_t)__popcll(m);
+    __syncthreads();
+    uint32_t pos = run + (uint32_t)__popcll(m & lt);
+    for (int ww = 0; ww < w; ++ww) pos += wcnt[ww];
+    if (sel) out[pos] = i;
+    __syncthreads();
+    if (threadIdx.x == 0) run += wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
+  }
+}
+
+void launch_zone_select(const double4* x, const int32_t* body, int32_t n, const ZoneBoxes& z,
+                        uint32_t* tile_cnt, int32_t* out, hipStream_t s) {
+  const int g = (n + CP_TILE - 1) / CP_TILE;
+  if (g == 0) return;
+  hipLaunchKernelGGL(k_zone_count, dim3(g), dim3(PD_BLOCK), 0, s, x, body, n, z, tile_cnt);
+  hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(1024), 0, s, tile_cnt, g);
+  hipLaunchKernelGGL(k_zone_scatter, dim3(g), dim3(PD_BLOCK), 0, s, x, body, n, z, tile_cnt, out);
+}
+
+size_t zone_tile_words(int32_t n) { return (size_t)(n + CP_TILE - 1) / CP_TILE + 1; }
 
 // ---------------------------------------------------------------- keys
 __device__ __forceinline__ void cell_of(const double4 p, const double* mn, double cell,
@@ -71,9 +179,11 @@ __device__ __forceinline__ void cell_of(const double4 p, const double* mn, doubl
 
 struct KeyArgs {
   const double4* x;
+  const int32_t* subset;  // particle of entry i (null: entry i is particle i)
   int n;
   double mn[3];
   double cell;
+  int c0[3];
   int dims[3];
   uint32_t* keys;
   int32_t* idx;
@@ -82,20 +192,27 @@ struct KeyArgs {
 __global__ __launch_bounds__(PD_BLOCK) void k_cell_keys(KeyArgs a) {
   const int i = blockIdx.x * PD_BLOCK + threadIdx.x;
   if (i >= a.n) return;
+  const int p = a.subset ? a.subset[i] : i;
   int cx, cy, cz;
-  cell_of(a.x[i], a.mn, a.cell, cx, cy, cz);
+  cell_of(a.x[p], a.mn, a.cell, cx, cy, cz);
+  cx -= a.c0[0];
+  cy -= a.c0[1];
+  cz -= a.c0[2];
   a.keys[i] = (uint32_t)(((int64_t)cz * a.dims[1] + cy) * a.dims[0] + cx);
-  a.idx[i] = i;
+  a.idx[i] = p;
 }
 
-void launch_cell_keys(const double4* x, int32_t n, const double* mn, double cell,
-                      const int32_t* dims, uint32_t* keys, int32_t* idx, hipStream_t s) {
+void launch_cell_keys(const double4* x, const int32_t* subset, int32_t n, const double* mn,
+                      double cell, const int32_t* c0, const int32_t* dims, uint32_t* keys,
+                      int32_t* idx, hipStream_t s) {
   KeyArgs a;
   a.x = x;
+  a.subset = subset;
   a.n = n;
   for (int k = 0; k < 3; ++k) {
     a.mn[k] = mn[k];
     a.dims[k] = dims[k];
+    a.c0[k] = c0 ? c0[k] : 0;
   }
   a.cell = cell;
   a.keys = keys;

@@ -39,7 +39,7 @@ struct pd_cg_state {
   int32_t done;
   int32_t converged;
   int32_t max_iter;
-  uint32_t arrive;  // last-workgroup arrival counter, reset by the last one
+  uint32_t arrive;  // reserved
   uint32_t pad[3];
 };
 

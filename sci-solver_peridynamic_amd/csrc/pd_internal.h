@@ -76,9 +76,12 @@ struct CgArgs {
   int32_t precond;
 };
 
+#define PD_MAX_ZONE_BODIES 8
+
 struct GridView {
   double mn[3];
   double cell;
+  int32_t c0[3];  // cell of the sub-grid's first cell in the full grid
   int32_t dims[3];
   const uint32_t* sorted_key;
   const int32_t* sorted_idx;
@@ -100,15 +103,36 @@ void launch_update(int32_t n, double4* x, double4* v, const double4* sol,
                    const int32_t* act, int32_t integrator, double dt, int32_t walls,
                    const double* lo, const double* hi, double restitution,
                    hipStream_t s);
+// contact forces of the m candidates cand[0..m); fc of every other particle
+// is set to zero
 void launch_contact(int32_t n, const double4* x, const int32_t* body,
-                    const int32_t* act, const GridView& g, double Rc, double kc,
-                    double vol, double4* fc, hipStream_t s);
+                    const int32_t* act, const GridView& g, const int32_t* cand,
+                    int32_t m, double Rc, double kc, double vol, double4* fc,
+                    hipStream_t s);
 void launch_reset_counters(pd_step_counters* c, hipStream_t s);
 
+// Per-body boxes, already widened by the contact radius: particle p is a
+// contact candidate if it lies in the box of a body other than its own.
+struct ZoneBoxes {
+  int32_t nbody;
+  double lo[PD_MAX_ZONE_BODIES][3];
+  double hi[PD_MAX_ZONE_BODIES][3];
+};
+
 // pd_bonds.hip
-void launch_bbox(const double4* x, int32_t n, double* part, double* out6,
-                 hipStream_t s);
-void launch_cell_keys(const double4* x, int32_t n, const double* mn, double cell,
+// bounds of the particles of body `body_sel` (all particles if < 0)
+void launch_bbox(const double4* x, const int32_t* body, int32_t n, int32_t body_sel,
+                 double* part, double* out6, hipStream_t s);
+// stable (ascending) list of contact candidates; the count lands in
+// tile_cnt[zone_tile_words(n) - 1]
+void launch_zone_select(const double4* x, const int32_t* body, int32_t n,
+                        const ZoneBoxes& z, uint32_t* tile_cnt, int32_t* out,
+                        hipStream_t s);
+size_t zone_tile_words(int32_t n);
+// keys of the cells (sub-grid origin cell c0, dims) of particles subset[i]
+// (or i when subset is null); idx[i] receives the particle index
+void launch_cell_keys(const double4* x, const int32_t* subset, int32_t n,
+                      const double* mn, double cell, const int32_t* c0,
                       const int32_t* dims, uint32_t* keys, int32_t* idx,
                       hipStream_t s);
 // Stable LSD radix sort of (keys, idx) by the low `bits` bits; returns which

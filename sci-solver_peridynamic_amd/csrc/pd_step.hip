@@ -15,7 +15,10 @@ namespace pd {
 // Fused assembly: core/cg_solver.cuh:200-339 (calculatePDForceDerivative)
 // + core/implicit_euler.cuh:19-86 / core/newmark_beta.cuh:19-86 (fill A, b)
 // + the block-Jacobi inverse of A_pp.  One lane per particle; a wave is one
-// 64-row slice, so every slot read / write is 64 consecutive words.
+// 64-row slice, so every slot read / write is 64 consecutive words.  The
+// next bond's column and neighbour data are fetched while the current bond
+// is computed; the matrix values are written with non-temporal stores (they
+// are read back once per CG iteration, never soon enough to gain from L2).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
@@ -39,10 +42,27 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
     int gq = -1;
     const double hr = mt.horizon - mt.radius;
     int bnd = 0;
+    int q_n = 0;
+    double4 xq_n = make_double4(0, 0, 0, 0), oq_n = make_double4(0, 0, 0, 0);
+    double sq_n = 0.0;
+    if (top >= 0) {
+      q_n = a.col[base * 64 + lane];
+      xq_n = a.x[q_n];
+      oq_n = a.x0[q_n];
+      sq_n = a.s[q_n];
+    }
     while (bnd <= top) {
       const int64_t slot = base + bnd;
-      const int q = a.col[slot * 64 + lane];
-      const double4 xq = a.x[q];
+      const int q = q_n;
+      const double4 xq = xq_n;
+      const double4 oq = oq_n;
+      const double sqv = sq_n;
+      if (bnd + 1 <= top) {
+        q_n = a.col[(slot + 1) * 64 + lane];
+        xq_n = a.x[q_n];
+        oq_n = a.x0[q_n];
+        sq_n = a.s[q_n];
+      }
       double e0 = xq.x - pos_p.x, e1 = xq.y - pos_p.y, e2 = xq.z - pos_p.z;
       const double dij = sqrt(e0 * e0 + e1 * e1 + e2 * e2);
       // cg_solver.cuh:256: divide by (float)max(dij, SMALL)
@@ -52,12 +72,11 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
       e0 /= fd;
       e1 /= fd;
       e2 /= fd;
-      const double4 oq = a.x0[q];
       const double o0 = opos_p.x - oq.x, o1 = opos_p.y - oq.y, o2 = opos_p.z - oq.z;
       const double d0 = sqrt(o0 * o0 + o1 * o1 + o2 * o2);
       const double drdivd = dij / d0 - 1.0;
       // cg_solver.cuh:264: threshold (float)max(s_p, s_q)
-      const float sq_f = (float)a.s[q];
+      const float sq_f = (float)sqv;
       const double thr = (double)(stretch_f > sq_f ? stretch_f : sq_f);
       if (a.track_gap) {
         const double gap = fabs(drdivd - thr);
@@ -73,6 +92,10 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
         a.col[slot * 64 + lane] = last;
         a.col[last_slot] = q;
         --top;
+        q_n = last;
+        xq_n = a.x[last];
+        oq_n = a.x0[last];
+        sq_n = a.s[last];
         continue;
       }
       double vscale = 1.0;
@@ -98,12 +121,12 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
       k12 += F12;
       k22 += F22;
       double* vo = a.val + slot * 6 * 64 + lane;
-      vo[0 * 64] = (F00 * -1.0) * a.dt2;
-      vo[1 * 64] = (F01 * -1.0) * a.dt2;
-      vo[2 * 64] = (F02 * -1.0) * a.dt2;
-      vo[3 * 64] = (F11 * -1.0) * a.dt2;
-      vo[4 * 64] = (F12 * -1.0) * a.dt2;
-      vo[5 * 64] = (F22 * -1.0) * a.dt2;
+      __builtin_nontemporal_store((F00 * -1.0) * a.dt2, vo + 0 * 64);
+      __builtin_nontemporal_store((F01 * -1.0) * a.dt2, vo + 1 * 64);
+      __builtin_nontemporal_store((F02 * -1.0) * a.dt2, vo + 2 * 64);
+      __builtin_nontemporal_store((F11 * -1.0) * a.dt2, vo + 3 * 64);
+      __builtin_nontemporal_store((F12 * -1.0) * a.dt2, vo + 4 * 64);
+      __builtin_nontemporal_store((F22 * -1.0) * a.dt2, vo + 5 * 64);
       ++bnd;
     }
     // A_pp = K_pp dt^2 + m I
@@ -325,29 +348,31 @@ __device__ __forceinline__ void spmv_row(const SpmvView& A, const double4* __res
   const int* __restrict__ cp = A.col + base * 64 + lane;
   const double* __restrict__ vp = A.val + base * 6 * 64 + lane;
   int bnd = 0;
-  // two slots per trip: both slots' loads are issued before either product
-  for (; bnd + 1 <= top; bnd += 2) {
-    const int q0 = ntl(cp + (int64_t)bnd * 64);
-    const int q1 = ntl(cp + (int64_t)(bnd + 1) * 64);
-    const double* v0 = vp + (int64_t)bnd * 384;
-    const double* v1 = v0 + 384;
-    const double a0 = ntl(v0), a1 = ntl(v0 + 64), a2 = ntl(v0 + 128), a3 = ntl(v0 + 192),
-                 a4 = ntl(v0 + 256), a5 = ntl(v0 + 320);
-    const double b0 = ntl(v1), b1 = ntl(v1 + 64), b2 = ntl(v1 + 128), b3 = ntl(v1 + 192),
-                 b4 = ntl(v1 + 256), b5 = ntl(v1 + 320);
-    const double4 x0 = x[q0];
-    const double4 x1 = x[q1];
-    double t0, t1, t2;
-    blk6(a0, a1, a2, a3, a4, a5, x0, t0, t1, t2);
-    y0 += t0;
-    y1 += t1;
-    y2 += t2;
-    blk6(b0, b1, b2, b3, b4, b5, x1, t0, t1, t2);
-    y0 += t0;
-    y1 += t1;
-    y2 += t2;
+  // four slots per trip: all their loads are issued before any product
+  for (; bnd + 3 <= top; bnd += 4) {
+    int q[4];
+    double v[4][6];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) q[u] = ntl(cp + (int64_t)(bnd + u) * 64);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const double* vv = vp + (int64_t)(bnd + u) * 384;
+#pragma unroll
+      for (int c = 0; c < 6; ++c) v[u][c] = ntl(vv + c * 64);
+    }
+    double4 xx[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) xx[u] = x[q[u]];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      double t0, t1, t2;
+      blk6(v[u][0], v[u][1], v[u][2], v[u][3], v[u][4], v[u][5], xx[u], t0, t1, t2);
+      y0 += t0;
+      y1 += t1;
+      y2 += t2;
+    }
   }
-  if (bnd <= top) {
+  for (; bnd <= top; ++bnd) {
     const int q0 = ntl(cp + (int64_t)bnd * 64);
     const double* v0 = vp + (int64_t)bnd * 384;
     double t0, t1, t2;
@@ -377,9 +402,11 @@ void launch_spmv_plain(const SpmvView& A, const double4* x, double4* y, hipStrea
 
 // ---------------------------------------------------------------------------
 // CG (CUSP krylov::cg structure, core/simulator.cu:763-770) with scalars
-// kept on the device.  Each kernel writes one partial per 256-row tile; the
-// last workgroup to arrive folds the partials in the fixed order of the
-// oracle's orc_dot and publishes the scalar for the next kernel.
+// kept on the device.  Every streaming kernel writes one partial sum per
+// 256-row tile; a one-workgroup kernel then folds the partials in the fixed
+// order of the oracle's orc_dot and publishes alpha, or beta plus the
+// monitor test.  Kernel boundaries order the writes, so no fences are
+// needed.  All kernels return at once when the solve is done.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void precond(const double* minv, int pc, int p, double r0,
                                         double r1, double r2, double& z0, double& z1,
@@ -403,30 +430,10 @@ __device__ __forceinline__ double term3(double a0, double a1, double a2, double 
   return (a0 * b0 + a1 * b1) + a2 * b2;
 }
 
-// Publishes up to three tile partials and returns true in the last
-// workgroup, after which part[] holds every tile's values.
-__device__ bool publish_partials(double* part, int ntiles, int tile, int nq,
-                                 const double* v, pd_cg_state* st) {
-  __shared__ int s_last;
-  if (threadIdx.x == 0) {
-    for (int k = 0; k < nq; ++k)
-      __hip_atomic_store(part + (int64_t)k * ntiles + tile, v[k], __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
-    __threadfence();
-    const unsigned prev = atomicAdd(&st->arrive, 1u);
-    s_last = (prev == (unsigned)(ntiles - 1));
-  }
-  __syncthreads();
-  const bool last = s_last != 0;
-  if (last) __threadfence();
-  return last;
-}
-
 // Final fold of one quantity over the tiles (orc_dot's final_combine).
 __device__ double fold_partials(const double* part, int ntiles, double* lds4) {
   double acc = 0.0;
-  for (int k = threadIdx.x; k < ntiles; k += PD_BLOCK)
-    acc = acc + __hip_atomic_load(part + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  for (int k = threadIdx.x; k < ntiles; k += PD_BLOCK) acc = acc + part[k];
   return tile_sum(acc, lds4);
 }
 
@@ -477,37 +484,42 @@ __global__ __launch_bounds__(PD_BLOCK) void k_cg_init(CgArgs a) {
     t_rr = term3(r0, r1, r2, r0, r1, r2);
     t_bb = term3(sb0, sb1, sb2, sb0, sb1, sb2);
   }
-  double v[3];
-  v[0] = tile_sum(t_rz, lds4);
-  v[1] = tile_sum(t_rr, lds4);
-  v[2] = tile_sum(t_bb, lds4);
-  if (publish_partials(a.part, ntiles, tile, 3, v, a.st)) {
-    const double rz = fold_partials(a.part, ntiles, lds4);
-    const double rr = fold_partials(a.part + ntiles, ntiles, lds4);
-    const double bb = fold_partials(a.part + 2 * (int64_t)ntiles, ntiles, lds4);
-    if (threadIdx.x == 0) {
-      pd_cg_state* st = a.st;
-      const double bnorm = sqrt(bb);
-      const double tol = st->rtol * bnorm;
-      const double rnorm = sqrt(rr);
-      st->rz = rz;
-      st->rr = rr;
-      st->bnorm = bnorm;
-      st->tol = tol;
-      st->rnorm = rnorm;
-      st->iter = 0;
-      if (rnorm <= tol) {
-        st->done = 1;
-        st->converged = 1;
-      } else if (st->max_iter <= 0) {
-        st->done = 1;
-      }
-      st->arrive = 0u;
+  const double v0 = tile_sum(t_rz, lds4);
+  const double v1 = tile_sum(t_rr, lds4);
+  const double v2 = tile_sum(t_bb, lds4);
+  if (threadIdx.x == 0) {
+    a.part[tile] = v0;
+    a.part[ntiles + tile] = v1;
+    a.part[2 * (int64_t)ntiles + tile] = v2;
+  }
+}
+
+__global__ __launch_bounds__(PD_BLOCK) void k_cg_init_fold(CgArgs a, int ntiles) {
+  __shared__ double lds4[4];
+  const double rz = fold_partials(a.part, ntiles, lds4);
+  const double rr = fold_partials(a.part + ntiles, ntiles, lds4);
+  const double bb = fold_partials(a.part + 2 * (int64_t)ntiles, ntiles, lds4);
+  if (threadIdx.x == 0) {
+    pd_cg_state* st = a.st;
+    const double bnorm = sqrt(bb);
+    const double tol = st->rtol * bnorm;
+    const double rnorm = sqrt(rr);
+    st->rz = rz;
+    st->rr = rr;
+    st->bnorm = bnorm;
+    st->tol = tol;
+    st->rnorm = rnorm;
+    st->iter = 0;
+    if (rnorm <= tol) {
+      st->done = 1;
+      st->converged = 1;
+    } else if (st->max_iter <= 0) {
+      st->done = 1;
     }
   }
 }
 
-// y = A p; partial <y,p>; last: alpha = rz / <y,p>
+// y = A p; partial <y,p>
 __global__ __launch_bounds__(PD_BLOCK) void k_cg_spmv(CgArgs a) {
   if (a.st->done) return;
   __shared__ double lds4[4];
@@ -522,20 +534,22 @@ __global__ __launch_bounds__(PD_BLOCK) void k_cg_spmv(CgArgs a) {
     const double4 pp = a.p[p];
     t = term3(y0, y1, y2, pp.x, pp.y, pp.z);
   }
-  double v[1];
-  v[0] = tile_sum(t, lds4);
-  if (publish_partials(a.part, ntiles, tile, 1, v, a.st)) {
-    const double yp = fold_partials(a.part, ntiles, lds4);
-    if (threadIdx.x == 0) {
-      a.st->yp = yp;
-      a.st->alpha = a.st->rz / yp;
-      a.st->arrive = 0u;
-    }
+  const double v = tile_sum(t, lds4);
+  if (threadIdx.x == 0) a.part[tile] = v;
+}
+
+// alpha = rz / <y,p>
+__global__ __launch_bounds__(PD_BLOCK) void k_cg_alpha(CgArgs a, int ntiles) {
+  if (a.st->done) return;
+  __shared__ double lds4[4];
+  const double yp = fold_partials(a.part, ntiles, lds4);
+  if (threadIdx.x == 0) {
+    a.st->yp = yp;
+    a.st->alpha = a.st->rz / yp;
   }
 }
 
-// x += alpha p; r -= alpha y (free rows); z = M r; partials <r,z>, <r,r>;
-// last: beta, iteration count and the monitor test of the next iteration.
+// x += alpha p; r -= alpha y (free rows); z = M r; partials <r,z>, <r,r>
 __global__ __launch_bounds__(PD_BLOCK) void k_cg_update(CgArgs a) {
   if (a.st->done) return;
   __shared__ double lds4[4];
@@ -566,28 +580,35 @@ __global__ __launch_bounds__(PD_BLOCK) void k_cg_update(CgArgs a) {
     t_rz = term3(rr.x, rr.y, rr.z, z0, z1, z2);
     t_rr = term3(rr.x, rr.y, rr.z, rr.x, rr.y, rr.z);
   }
-  double v[2];
-  v[0] = tile_sum(t_rz, lds4);
-  v[1] = tile_sum(t_rr, lds4);
-  if (publish_partials(a.part, ntiles, tile, 2, v, a.st)) {
-    const double rz = fold_partials(a.part, ntiles, lds4);
-    const double rr = fold_partials(a.part + ntiles, ntiles, lds4);
-    if (threadIdx.x == 0) {
-      pd_cg_state* st = a.st;
-      const double rz_old = st->rz;
-      st->rz = rz;
-      st->beta = rz / rz_old;
-      st->iter = st->iter + 1;
-      st->rr = rr;
-      const double rnorm = sqrt(rr);
-      st->rnorm = rnorm;
-      if (rnorm <= st->tol) {
-        st->done = 1;
-        st->converged = 1;
-      } else if (st->iter >= st->max_iter) {
-        st->done = 1;
-      }
-      st->arrive = 0u;
+  const double v0 = tile_sum(t_rz, lds4);
+  const double v1 = tile_sum(t_rr, lds4);
+  if (threadIdx.x == 0) {
+    a.part[tile] = v0;
+    a.part[ntiles + tile] = v1;
+  }
+}
+
+// beta = rz' / rz, iteration count and the monitor test ahead of the next
+// iteration (||r|| <= tol converged, iter >= max_iter failed)
+__global__ __launch_bounds__(PD_BLOCK) void k_cg_beta(CgArgs a, int ntiles) {
+  if (a.st->done) return;
+  __shared__ double lds4[4];
+  const double rz = fold_partials(a.part, ntiles, lds4);
+  const double rr = fold_partials(a.part + ntiles, ntiles, lds4);
+  if (threadIdx.x == 0) {
+    pd_cg_state* st = a.st;
+    const double rz_old = st->rz;
+    st->rz = rz;
+    st->beta = rz / rz_old;
+    st->iter = st->iter + 1;
+    st->rr = rr;
+    const double rnorm = sqrt(rr);
+    st->rnorm = rnorm;
+    if (rnorm <= st->tol) {
+      st->done = 1;
+      st->converged = 1;
+    } else if (st->iter >= st->max_iter) {
+      st->done = 1;
     }
   }
 }
@@ -609,13 +630,22 @@ __global__ __launch_bounds__(PD_BLOCK) void k_cg_dir(CgArgs a) {
 static inline int tiles_of(int n) { return (n + PD_BLOCK - 1) / PD_BLOCK; }
 
 void launch_cg_init(const CgArgs& a, hipStream_t s) {
-  if (a.A.n) hipLaunchKernelGGL(k_cg_init, dim3(tiles_of(a.A.n)), dim3(PD_BLOCK), 0, s, a);
+  if (!a.A.n) return;
+  const int g = tiles_of(a.A.n);
+  hipLaunchKernelGGL(k_cg_init, dim3(g), dim3(PD_BLOCK), 0, s, a);
+  hipLaunchKernelGGL(k_cg_init_fold, dim3(1), dim3(PD_BLOCK), 0, s, a, g);
 }
 void launch_cg_spmv(const CgArgs& a, hipStream_t s) {
-  if (a.A.n) hipLaunchKernelGGL(k_cg_spmv, dim3(tiles_of(a.A.n)), dim3(PD_BLOCK), 0, s, a);
+  if (!a.A.n) return;
+  const int g = tiles_of(a.A.n);
+  hipLaunchKernelGGL(k_cg_spmv, dim3(g), dim3(PD_BLOCK), 0, s, a);
 }
 void launch_cg_update(const CgArgs& a, hipStream_t s) {
-  if (a.A.n) hipLaunchKernelGGL(k_cg_update, dim3(tiles_of(a.A.n)), dim3(PD_BLOCK), 0, s, a);
+  if (!a.A.n) return;
+  const int g = tiles_of(a.A.n);
+  hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(PD_BLOCK), 0, s, a, g);
+  hipLaunchKernelGGL(k_cg_update, dim3(g), dim3(PD_BLOCK), 0, s, a);
+  hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(PD_BLOCK), 0, s, a, g);
 }
 void launch_cg_dir(const CgArgs& a, hipStream_t s) {
   if (a.A.n) hipLaunchKernelGGL(k_cg_dir, dim3(tiles_of(a.A.n)), dim3(PD_BLOCK), 0, s, a);
@@ -715,22 +745,27 @@ struct ContactArgs {
   const int32_t* body;
   const int32_t* act;
   GridView g;
+  const int32_t* cand;
+  int m;
   double Rc, kc, vol;
   double4* fc;
 };
 
+__global__ __launch_bounds__(PD_BLOCK) void k_zero4(double4* v, int n) {
+  const int i = blockIdx.x * PD_BLOCK + threadIdx.x;
+  if (i < n) v[i] = make_double4(0.0, 0.0, 0.0, 0.0);
+}
+
 __global__ __launch_bounds__(PD_BLOCK) void k_contact(ContactArgs a) {
-  const int p = blockIdx.x * PD_BLOCK + threadIdx.x;
-  if (p >= a.n) return;
-  if (a.act[p] == PD_INVISIBLE) {
-    a.fc[p] = make_double4(0.0, 0.0, 0.0, 0.0);
-    return;
-  }
+  const int i = blockIdx.x * PD_BLOCK + threadIdx.x;
+  if (i >= a.m) return;
+  const int p = a.cand[i];
+  if (a.act[p] == PD_INVISIBLE) return;  // fc[p] stays zero
   const double4 pp = a.x[p];
   const int bp = a.body[p];
-  const int cx = (int)floor((pp.x - a.g.mn[0]) / a.g.cell);
-  const int cy = (int)floor((pp.y - a.g.mn[1]) / a.g.cell);
-  const int cz = (int)floor((pp.z - a.g.mn[2]) / a.g.cell);
+  const int cx = (int)floor((pp.x - a.g.mn[0]) / a.g.cell) - a.g.c0[0];
+  const int cy = (int)floor((pp.y - a.g.mn[1]) / a.g.cell) - a.g.c0[1];
+  const int cz = (int)floor((pp.z - a.g.mn[2]) / a.g.cell) - a.g.c0[2];
   double f0 = 0.0, f1 = 0.0, f2 = 0.0;
   for (int dz = -1; dz <= 1; ++dz)
     for (int dy = -1; dy <= 1; ++dy)
@@ -763,20 +798,25 @@ __global__ __launch_bounds__(PD_BLOCK) void k_contact(ContactArgs a) {
 }
 
 void launch_contact(int32_t n, const double4* x, const int32_t* body, const int32_t* act,
-                    const GridView& g, double Rc, double kc, double vol, double4* fc,
-                    hipStream_t s) {
+                    const GridView& g, const int32_t* cand, int32_t m, double Rc, double kc,
+                    double vol, double4* fc, hipStream_t s) {
+  const int nb = (n + PD_BLOCK - 1) / PD_BLOCK;
+  if (nb) hipLaunchKernelGGL(k_zero4, dim3(nb), dim3(PD_BLOCK), 0, s, fc, n);
+  if (m <= 0) return;
   ContactArgs a;
   a.n = n;
   a.x = x;
   a.body = body;
   a.act = act;
   a.g = g;
+  a.cand = cand;
+  a.m = m;
   a.Rc = Rc;
   a.kc = kc;
   a.vol = vol;
   a.fc = fc;
-  const int nb = (n + PD_BLOCK - 1) / PD_BLOCK;
-  if (nb) hipLaunchKernelGGL(k_contact, dim3(nb), dim3(PD_BLOCK), 0, s, a);
+  const int mb = (m + PD_BLOCK - 1) / PD_BLOCK;
+  hipLaunchKernelGGL(k_contact, dim3(mb), dim3(PD_BLOCK), 0, s, a);
 }
 
 }  // namespace pd

@@ -1,0 +1,185 @@
+"""Workload-level parity: the shipped params.txt configurations run through the
+engine's own Scene / params surface on the GPU, against the CPU oracle.
+
+Bar (north-star parity contract): bond topology bit-exact; positions within
+1e-9 relative over 100 FP64 steps; broken-bond counts equal until the first
+step with a bond closer than 1e-6 to its threshold (reported)."""
+import json
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+import pytest
+
+import oracle as orc
+
+pytestmark = pytest.mark.gpu
+pds = pytest.importorskip("pdsolver")
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CFG = os.path.join(ROOT, "sci-solver_peridynamic_amd", "configs")
+REL_TOL = 1e-9
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def setup_pair(cfg_path):
+    desc = pds.describe(cfg_path)
+    sc = pds.scene(cfg_path)
+    n = desc["num_pd_particle"]
+    mats = pds.materials_of(desc)
+    counts, lists, total = orc.bonds_brute(sc["x4"], [m[0] for m in mats], body=sc["body"])
+    rs, col = orc.lists_to_csr(counts, lists)
+    s00 = orc.stretch_init(n, desc["s0"], desc["stretch_seed"])
+    S = orc.OracleSystem(sc["x4"], sc["v4"], s00, rs, col, mats, body=sc["body"], bc=sc["bc"],
+                         vbc=sc["vbc"])
+    ctx = pds.Context(n, mats)
+    pds.load_state(ctx, sc["x4"], sc["v4"], pds.stretch_init(n, desc["s0"], desc["stretch_seed"]),
+                   body=sc["body"], bc=sc["bc"], vbc=sc["vbc"])
+    bst = ctx.build_bonds()
+    assert bst.total_bonds == total
+    cap, glists = ctx.get_bonds()
+    assert np.array_equal(cap, counts)
+    for p in range(n):
+        assert np.array_equal(glists[p, :cap[p]], lists[p, :counts[p]])
+    return desc, sc, S, ctx
+
+
+@pytest.mark.parametrize("cfg,steps", [("tension_20.txt", 100), ("impact_30.txt", 60)])
+def test_config_run_parity(cfg, steps):
+    say("case", cfg, steps)
+    desc, sc, S, ctx = setup_pair(os.path.join(CFG, cfg))
+    first_near = None
+    worst_rel = 0.0
+    worst_rel_disp = 0.0
+    exact_steps = 0
+    for k in range(1, steps + 1):
+        kw = pds.step_kwargs(desc, k)
+        so = S.step(orc.make_params(**kw))
+        sg = ctx.step(pds.make_params(track_gap=1, **kw))
+        xg = ctx.download("position")
+        d = np.abs(xg[:, :3] - S.x[:, :3]).max()
+        rel = d / np.abs(S.x[:, :3]).max()
+        disp = np.abs(S.x[:, :3] - S.x0[:, :3]).max()
+        rel_disp = d / disp if disp > 0 else 0.0
+        worst_rel = max(worst_rel, rel)
+        worst_rel_disp = max(worst_rel_disp, rel_disp)
+        exact_steps += int(d == 0.0)
+        if first_near is None and so.min_gap < 1e-6:
+            first_near = (k, so.gap_p, so.gap_q, so.min_gap)
+            say("first near-threshold bond", first_near)
+        if first_near is None:
+            assert sg.broken_bonds == so.broken_bonds, (k, sg.broken_bonds, so.broken_bonds)
+            assert sg.cg_iterations == so.cg_iterations, k
+        assert so.converged and sg.converged
+        assert rel <= REL_TOL, (k, rel)
+        if k % 10 == 0:
+            say(dict(step=k, it=(so.cg_iterations, sg.cg_iterations),
+                     broken=(so.broken_bonds, sg.broken_bonds), live=sg.live_bonds, rel=rel,
+                     rel_disp=rel_disp, gap=so.min_gap))
+    say(dict(cfg=cfg, steps=steps, worst_rel=worst_rel, worst_rel_disp=worst_rel_disp,
+             bitwise_equal_steps=exact_steps, first_near=first_near))
+    ctx.close()
+
+
+def test_peri_run_driver_matches_oracle():
+    """The C++ driver (Scene + Simulator + DataIO over the C ABI) on the
+    tension config writes frames equal to the oracle's positions."""
+    say("case peri_run")
+    exe = os.path.join(ROOT, "sci-solver_peridynamic_amd", "peri_run")
+    cfg = os.path.join(CFG, "tension_20.txt")
+    desc = pds.describe(cfg)
+    with tempfile.TemporaryDirectory() as out:
+        params = os.path.join(out, "params.txt")
+        with open(cfg) as f:
+            text = f.read()
+        text = text.replace("final_step 100", "final_step 20").replace(
+            "steps_per_frame 10", "steps_per_frame 5") + "\nprint_steps 1\n"
+        with open(params, "w") as f:
+            f.write(text)
+        r = subprocess.run([exe, params, "none", out], capture_output=True, text=True,
+                           timeout=120)
+        say(r.stdout[-2000:], r.stderr[-2000:])
+        assert r.returncode == 0
+        recs = [json.loads(l) for l in r.stdout.splitlines() if l.startswith("{")]
+        # the reference's main loop (examples/main.cpp:86-116) advances until
+        # current_step > final_step (core/simulator.cu:141-144): final_step + 1 steps
+        assert [q["step"] for q in recs] == list(range(1, 22))
+        # oracle on the same scene for the same steps
+        sc = pds.scene(cfg)
+        n = desc["num_pd_particle"]
+        mats = pds.materials_of(desc)
+        counts, lists, _ = orc.bonds_brute(sc["x4"], [m[0] for m in mats], body=sc["body"])
+        rs, col = orc.lists_to_csr(counts, lists)
+        S = orc.OracleSystem(sc["x4"], sc["v4"], orc.stretch_init(n, desc["s0"]), rs, col, mats,
+                             body=sc["body"], bc=sc["bc"], vbc=sc["vbc"])
+        frames = {}
+        for k in range(1, 21):
+            st = S.step(orc.make_params(**pds.step_kwargs(desc, k)))
+            assert st.cg_iterations == recs[k - 1]["cg_iterations"]
+            frames[k] = S.x.copy()
+        # frame.1 is the initial state; frame j (j >= 2) follows step 5 (j - 1)
+        f1 = np.fromfile(os.path.join(out, "DATA", "PD_POSITION", "frame.1")).reshape(-1, 4)
+        assert np.array_equal(f1[:, :3], sc["x4"][:, :3])
+        for j, k in ((2, 5), (3, 10), (5, 20)):
+            fj = np.fromfile(os.path.join(out, "DATA", "PD_POSITION", f"frame.{j}")).reshape(-1, 4)
+            rel = np.abs(fj[:, :3] - frames[k][:, :3]).max() / np.abs(frames[k][:, :3]).max()
+            assert rel <= REL_TOL, (j, rel)
+        assert os.path.exists(os.path.join(out, "DATA", "viz_info.dat"))
+
+
+def test_full_size_bonds_and_operator():
+    """The 1,005,000-particle impact scene: GPU bond lists equal the CPU
+    kd-tree restatement's, row by row; then size-independent properties of
+    the assembled operator (symmetry, translation) and of one solve."""
+    say("case full-size impact_100")
+    cfg = os.path.join(CFG, "impact_100.txt")
+    desc = pds.describe(cfg)
+    sc = pds.scene(cfg)
+    n = desc["num_pd_particle"]
+    mats = pds.materials_of(desc)
+    ctx = pds.Context(n, mats)
+    pds.load_state(ctx, sc["x4"], sc["v4"], pds.stretch_init(n, desc["s0"]), body=sc["body"],
+                   bc=sc["bc"], vbc=sc["vbc"])
+    bst = ctx.build_bonds()
+    cap, glists = ctx.get_bonds(stride=128)
+    for b in range(len(mats)):
+        idx = np.nonzero(sc["body"] == b)[0]
+        c, l, dropped = orc.bonds_kdtree(sc["x4"][idx], mats[b][0], max_bonds=128)
+        assert dropped == 0
+        assert np.array_equal(cap[idx], c)
+        mask = np.arange(128)[None, :] < c[:, None]
+        assert np.array_equal(np.where(mask, glists[idx], -1), np.where(mask, idx[l], -1))
+        del l
+    assert bst.total_bonds == int(cap.sum())
+    say("bonds equal", bst.total_bonds)
+    kw = pds.step_kwargs(desc, 1)
+    ctx.assemble(pds.make_params(**kw))
+    rng = np.random.default_rng(0)
+    x = np.zeros((n, 4))
+    y = np.zeros((n, 4))
+    x[:, :3] = rng.normal(size=(n, 3))
+    y[:, :3] = rng.normal(size=(n, 3))
+    Ax = ctx.spmv(x)
+    Ay = ctx.spmv(y)
+    lhs = float(np.sum(Ax[:, :3] * y[:, :3]))
+    rhs = float(np.sum(x[:, :3] * Ay[:, :3]))
+    assert abs(lhs - rhs) <= 1e-10 * abs(lhs)
+    t = np.zeros((n, 4))
+    t[:, :3] = [0.3, -0.2, 0.7]
+    At = ctx.spmv(t)
+    m = np.where(sc["body"] == 0, mats[0][3], mats[-1][3])[:, None]
+    np.testing.assert_allclose(At[:, :3], m * t[:, :3], rtol=1e-6, atol=1e-9)
+    ctx.commit_stretch()
+    st = ctx.solve(pds.make_params(**kw))
+    sol = ctx.download("solution")
+    b = ctx.download("rhs")
+    r = b[:, :3] - ctx.spmv(sol)[:, :3]
+    free = sc["bc"] == 0
+    assert np.linalg.norm(r[free]) <= 1.0001 * desc["rtol"] * np.linalg.norm(b[free, :3])
+    say("solve", st.cg_iterations, st.residual_norm / st.b_norm)
+    ctx.close()
