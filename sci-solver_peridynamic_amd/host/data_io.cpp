@@ -171,10 +171,24 @@ bool DataIO::saveState(size_t _timestep) {
   return ok;
 }
 
+// core/dataIO.cu:152-198: the saved SimulationParameters (time step,
+// phase-in clock) come back from sim_info.dat after the particle counts are
+// checked; the step counter resumes at the saved step.
 bool DataIO::loadState(size_t _timestep) {
   pd_ctx* ctx = sim_.context();
   const int n = simParams_.num_pd_particle;
   if (!ctx || n <= 0) return false;
+  {
+    SimulationParameters saved = simParams_;
+    readSimulationParameter(saved);
+    if (saved.num_pd_particle != simParams_.num_pd_particle ||
+        saved.num_sph_particle != simParams_.num_sph_particle) {
+      fprintf(stderr, "Inconsistent number of particles with saved data.\n");
+      return false;
+    }
+    simParams_ = saved;
+  }
+  writtenSimulationParameter_ = true;
   char buff[1200];
   snprintf(buff, sizeof(buff), "%s/DATA/STATE/state.%zu", runningParams_.saving_path, _timestep);
   FILE* f = fopen(buff, "rb");
@@ -206,5 +220,7 @@ bool DataIO::loadState(size_t _timestep) {
   if (!ok) return false;
   // restore the full lists, then the live counts
   if (pd_set_bonds(ctx, cap.data(), lists.data(), MAX_PD_BOND_COUNT) != PD_OK) return false;
-  return pd_upload(ctx, PD_FIELD_BOND_TOP, top.data(), sizeof(int) * n) == PD_OK;
+  if (pd_upload(ctx, PD_FIELD_BOND_TOP, top.data(), sizeof(int) * n) != PD_OK) return false;
+  sim_.resumeAt((int)_timestep);
+  return true;
 }

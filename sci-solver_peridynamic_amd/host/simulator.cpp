@@ -176,6 +176,12 @@ bool Simulator::integratePeridynamics() {
         hostSimParams_.pd_time_step *= 2;
         printf("        Try to increase time step twice to %d\n", hostSimParams_.pd_time_step);
         current_substep_ = (hostSimParams_.pd_time_step >= TIMESTEP_PD) ? 0 : 1;
+        // the reference fills A and b after this point with the doubled step
+        // (core/simulator.cu:381-392, 430-446); the fill is fused into the
+        // assembly here, so assemble again: no bond breaks on this path and
+        // the force kernel does not depend on the step
+        prm = stepParams();
+        if (!check(pd_assemble(ctx_, &prm, &last_), "pd_assemble")) return true;
       }
     }
     if (!check(pd_backup(ctx_), "pd_backup")) return true;

@@ -26,6 +26,8 @@ class Simulator {
   bool CGConvergence();
 
   int get_current_step() const;
+  // added: continue a restored state from step `step` (DataIO::loadState)
+  void resumeAt(int step) { current_step_ = step; }
   int* get_pd_activity();
   real4_t* get_pd_position();
   real4_t* get_pd_velocity();
