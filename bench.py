@@ -10,6 +10,11 @@ Prints one JSON line with the metric, the roofline of the dominant kernel
 baseline (the CPU oracle on a bounded sample of the same workload).
 
     python bench.py [--gpus 1] [--steps 195] [--warmup 5] [--no-cpu-baseline]
+                    [--precision fp64|mixed] [--no-mixed]
+
+The FP64 line also carries a "mixed_precision" object: the same run with the
+float inner CG and FP64 refinement, its SpMV roofline, and its per-step
+position error against the FP64 path.
 """
 from __future__ import annotations
 
@@ -48,34 +53,48 @@ def read_traffic():
     return None
 
 
-def gpu_run(steps, warmup, copy_calibration=0):
+def new_context(desc, sc):
+    n = desc["num_pd_particle"]
+    s00 = pds.stretch_init(n, desc["s0"], desc["stretch_seed"])
+    ctx = pds.Context(n, pds.materials_of(desc))
+    pds.load_state(ctx, sc["x4"], sc["v4"], s00, body=sc["body"], bc=sc["bc"], vbc=sc["vbc"])
+    return ctx, ctx.build_bonds()
+
+
+def step_params(desc, step, precision, **kw):
+    return pds.make_params(**dict(pds.step_kwargs(desc, step), precision=precision, **kw))
+
+
+def gpu_run(steps, warmup, precision=0, copy_calibration=0):
     desc = pds.describe(CONFIG)
     sc = pds.scene(CONFIG)
     n = desc["num_pd_particle"]
-    mats = pds.materials_of(desc)
-    s00 = pds.stretch_init(n, desc["s0"], desc["stretch_seed"])
-    ctx = pds.Context(n, mats)
-    pds.load_state(ctx, sc["x4"], sc["v4"], s00, body=sc["body"], bc=sc["bc"], vbc=sc["vbc"])
-    bst = ctx.build_bonds()
-    log(f"particles {n}, directed bonds {bst.total_bonds} ({bst.min_bonds}..{bst.max_bonds} per "
-        f"particle), bond build {bst.ms_build:.1f} ms")
+    ctx, bst = new_context(desc, sc)
+    log(f"[{'mixed' if precision else 'fp64'}] particles {n}, directed bonds {bst.total_bonds} "
+        f"({bst.min_bonds}..{bst.max_bonds} per particle), bond build {bst.ms_build:.1f} ms")
     step = 0
     for _ in range(warmup):
         step += 1
-        ctx.step(pds.make_params(**pds.step_kwargs(desc, step)))
+        ctx.step(step_params(desc, step, precision))
     ctx.synchronize()
     spmv_ms = 0.0
     spmv_n = 0
+    resid_ms = 0.0
+    resid_n = 0
     iters = []
+    refinements = []
     broken = 0
     phase = dict(contact=0.0, assemble=0.0, solve=0.0, update=0.0)
     t0 = time.perf_counter()
     for _ in range(steps):
         step += 1
-        st = ctx.step(pds.make_params(timing=1, **pds.step_kwargs(desc, step)))
+        st = ctx.step(step_params(desc, step, precision, timing=1))
         spmv_ms += st.ms_spmv_total
         spmv_n += st.spmv_launches
+        resid_ms += st.ms_resid_total
+        resid_n += st.resid_launches
         iters.append(st.cg_iterations)
+        refinements.append(st.refinements)
         broken += st.broken_bonds
         phase["contact"] += st.ms_contact
         phase["assemble"] += st.ms_assemble
@@ -97,7 +116,57 @@ def gpu_run(steps, warmup, copy_calibration=0):
     return dict(n=n, bonds=int(bst.total_bonds), live=int(live), elapsed=elapsed, steps=steps,
                 spmv_ms=spmv_ms, spmv_n=spmv_n, iters=iters, broken=int(broken),
                 spmv_bytes=spmv_bytes, phase=phase, build_ms=bst.ms_build,
-                plain_ms=plain_ms, plain_bytes=plain_bytes)
+                plain_ms=plain_ms, plain_bytes=plain_bytes, resid_ms=resid_ms, resid_n=resid_n,
+                refinements=refinements)
+
+
+def mixed_error(steps):
+    """Steps 1..steps of the FP64 and the mixed-precision path side by side:
+    per-step max |x_mixed - x_fp64| / max |x_fp64| over all coordinates."""
+    desc = pds.describe(CONFIG)
+    sc = pds.scene(CONFIG)
+    c64, _ = new_context(desc, sc)
+    c32, _ = new_context(desc, sc)
+    errs = []
+    broken = [0, 0]
+    split = None  # first step whose broken-bond count differs
+    for k in range(1, steps + 1):
+        s64 = c64.step(step_params(desc, k, 0))
+        s32 = c32.step(step_params(desc, k, 1))
+        broken[0] += s64.broken_bonds
+        broken[1] += s32.broken_bonds
+        if split is None and s64.broken_bonds != s32.broken_bonds:
+            split = k
+        x64 = c64.download("position")[:, :3]
+        x32 = c32.download("position")[:, :3]
+        errs.append(float(np.abs(x32 - x64).max() / np.abs(x64).max()))
+        if k % 50 == 0:
+            log(f"mixed vs fp64 step {k}: rel position error {errs[-1]:.3e}")
+    c64.close()
+    c32.close()
+    before = errs[:split - 1] if split else errs
+    return dict(max_rel=max(errs), final_rel=errs[-1], steps=steps,
+                first_step_broken_counts_differ=split,
+                max_rel_before_that_step=max(before) if before else None,
+                per_step_rel_every_10=errs[9::10], broken_bonds_fp64=broken[0],
+                broken_bonds_mixed=broken[1])
+
+
+def spmv_roofline(r, kernel, traffic):
+    avg_ms = r["spmv_ms"] / max(r["spmv_n"], 1)
+    achieved = r["spmv_bytes"] / (avg_ms * 1e-3) / 1e9
+    return {
+        "kernel": kernel,
+        "bound": "hbm",
+        "achieved": achieved,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBS,
+        "traffic": traffic,
+        "bytes_per_launch": r["spmv_bytes"],
+        "avg_launch_ms": avg_ms,
+        "launches": r["spmv_n"],
+    }
 
 
 def cpu_baseline(sample_steps=2):
@@ -143,7 +212,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=195)
     ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64",
+                    help="solver of the timed line (mixed: float inner CG, FP64 refinement)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true",
+                    help="skip the mixed-precision report of the fp64 line")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="after the run, N rounds of device copies of known size")
@@ -153,11 +226,11 @@ def main():
               "measured on a single MI355X; multi-GPU sharding is design-only.",
               file=sys.stderr)
         sys.exit(2)
-    r = gpu_run(a.steps, a.warmup, a.copy_calibration)
+    prec = 1 if a.precision == "mixed" else 0
+    r = gpu_run(a.steps, a.warmup, prec, a.copy_calibration)
     ms_step = 1000.0 * r["elapsed"] / r["steps"]
-    spmv_avg_ms = r["spmv_ms"] / max(r["spmv_n"], 1)
-    achieved = r["spmv_bytes"] / (spmv_avg_ms * 1e-3) / 1e9
-    traffic = read_traffic()
+    kernel = ("k_mx_spmv (float inner CG SpMV y = A32 p + <y,p> partial)" if prec else
+              "k_cg_spmv (CG SpMV y = A p + <y,p> partial)")
     out = {
         "metric": "implicit steps/s @1M particles, CG rtol 1e-6; SpMV HBM GB/s vs ~8 TB/s peak",
         "value": r["steps"] / r["elapsed"],
@@ -169,11 +242,12 @@ def main():
         "higher_is_better": True,
         "scaling": "strong",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64" if prec else "f64",
         "data": "synthetic (seeded jittered lattice + projectile scene)",
         "config": {
             "workload": "impact_100: 100^3 jittered lattice + 5000-particle projectile at "
-                        "20 m/s, implicit Euler dt 5e-5 s, block-Jacobi CG rtol 1e-6, FP64",
+                        "20 m/s, implicit Euler dt 5e-5 s, block-Jacobi CG rtol 1e-6, " +
+                        ("float inner CG with FP64 residual / refinement" if prec else "FP64"),
             "particles": r["n"],
             "directed_bonds": r["bonds"],
             "live_bonds_end": r["live"],
@@ -184,19 +258,27 @@ def main():
             "bond_build_ms": r["build_ms"],
             "plain_spmv_gbs": r["plain_bytes"] / (r["plain_ms"] * 1e-3) / 1e9,
         },
-        "roofline": {
-            "kernel": "k_cg_spmv (CG SpMV y = A p + <y,p> partial)",
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic,
-            "bytes_per_launch": r["spmv_bytes"],
-            "avg_launch_ms": spmv_avg_ms,
-            "launches": r["spmv_n"],
-        },
+        "roofline": spmv_roofline(r, kernel, None if prec else read_traffic()),
     }
+    if prec:
+        out["config"]["refinements_mean"] = float(np.mean(r["refinements"]))
+        out["config"]["resid_spmv_ms_mean"] = r["resid_ms"] / max(r["resid_n"], 1)
+    if not prec and not a.no_mixed:
+        m = gpu_run(a.steps, a.warmup, 1)
+        m_rate = m["steps"] / m["elapsed"]
+        out["mixed_precision"] = {
+            "value": m_rate,
+            "unit": "steps/s",
+            "speedup_vs_fp64": m_rate / out["value"],
+            "ms_per_step": 1000.0 * m["elapsed"] / m["steps"],
+            "cg_iterations_mean": float(np.mean(m["iters"])),
+            "refinements_mean": float(np.mean(m["refinements"])),
+            "phase_ms_per_step": {k: v / m["steps"] for k, v in m["phase"].items()},
+            "roofline": spmv_roofline(m, "k_mx_spmv (float inner CG SpMV)", None),
+            "resid_spmv_ms_mean": m["resid_ms"] / max(m["resid_n"], 1),
+            "plain_spmv_gbs": m["plain_bytes"] / (m["plain_ms"] * 1e-3) / 1e9,
+            "position_error_vs_fp64": mixed_error(a.warmup + a.steps),
+        }
     if not a.no_cpu_baseline:
         log("cpu baseline: CPU oracle on the same workload ...")
         out["cpu_baseline"] = cpu_baseline(a.cpu_steps)

@@ -95,6 +95,17 @@ typedef struct pd_step_params {
   double contact_cell;      /* grid cell edge for contact, >= Rc */
   int32_t track_gap;        /* 1: report the bond closest to breaking */
   int32_t timing;           /* 1: time phases and SpMV launches */
+  /* Solver precision (not in the reference, which is FP64 only).
+   * 0: FP64 CG.  1: the assembly also rounds the bond blocks, the diagonal
+   * and its block-Jacobi inverse to float, and the solve is iterative
+   * refinement: r = S(b - A32 x) in FP64 (diagonal FP64, float bond blocks
+   * widened), stop when ||r|| <= rtol ||S b||, else a PCG on float vectors
+   * solves A32 d = r until ||r32|| <= max(inner_rtol ||r||, tol / 2) and
+   * x += d in FP64.  Dot products and norms are FP64 throughout. */
+  int32_t precision;
+  int32_t max_refine;     /* precision 1: refinement limit (0: 20) */
+  double inner_rtol;      /* precision 1: inner reduction (0: 1e-5) */
+  int32_t inner_max_iter; /* precision 1: per refinement (0: max_iter) */
 } pd_step_params;
 
 typedef struct pd_step_stats {
@@ -115,6 +126,9 @@ typedef struct pd_step_stats {
   double ms_spmv_total; /* sum over the CG SpMV launches that ran */
   int32_t spmv_launches;
   int32_t host_syncs;
+  int32_t refinements;    /* precision 1: FP64 corrections applied */
+  double ms_resid_total;  /* precision 1: FP64 residual launches that ran */
+  int32_t resid_launches;
 } pd_step_stats;
 
 typedef struct pd_bond_stats {
@@ -174,6 +188,8 @@ int pd_get_bonds(pd_ctx* ctx, int32_t* capacity, int32_t* lists,
  * diag9[N*9] = A_pp and vals9[(p*stride + k)*9] = A_{p,list[p][k]} for
  * the live slots k <= top[p] (the remaining entries are left untouched). */
 int pd_get_matrix(pd_ctx* ctx, double* diag9, double* vals9, int32_t stride);
+/* (After an assembly with precision 1, vals9 holds the float blocks
+ * widened to double; diag9 is the FP64 diagonal.) */
 
 /* Simulator::integratePeridynamicsImplicitEuler (core/simulator.cu:
  * 297-467) and integratePeridynamicsNewmarkBeta (core/simulator.cu:
@@ -195,7 +211,9 @@ int pd_assemble(pd_ctx* ctx, const pd_step_params* prm, pd_step_stats* stats);
 int pd_commit_stretch(pd_ctx* ctx);
 
 /* PeridynamicsMatrix::operator() / computeAx (core/cg_solver.cuh:343-428):
- * y = A x for host real4 vectors x, y of num_particles entries. */
+ * y = A x for host real4 vectors x, y of num_particles entries.  After an
+ * assembly with precision 1 the operator is A32 in FP64 arithmetic (the
+ * refinement's residual operator). */
 int pd_spmv(pd_ctx* ctx, const double* x4, double* y4);
 
 /* Simulator::solveLinearSystem (core/simulator.cu:735-799): CG on the
@@ -217,12 +235,15 @@ int pd_synchronize(pd_ctx* ctx);
 
 /* Times `iters` launches of the plain SpMV kernel y = A x on the resident
  * matrix with HIP events on the context stream; returns the mean kernel
- * time in ms and the algorithmic bytes one launch moves. */
+ * time in ms and the algorithmic bytes one launch moves.  After an
+ * assembly with precision 1 the kernel is the float inner SpMV. */
 int pd_bench_spmv(pd_ctx* ctx, int32_t iters, double* ms_per_launch,
                   double* bytes_per_launch);
 
-/* Algorithmic HBM bytes of one CG SpMV launch on the current bonds
- * (live blocks * (6*8 + 4) + N * (6*8 + 4 + 2*24)). */
+/* Algorithmic HBM bytes of one CG SpMV launch on the current bonds:
+ * FP64, live blocks * (6*8 + 4) + N * (6*8 + 4 + 2*24); after an assembly
+ * with precision 1 (float inner SpMV), live blocks * (6*4 + 4) +
+ * N * (6*4 + 4 + 2*12). */
 int pd_spmv_bytes(pd_ctx* ctx, double* bytes);
 
 #ifdef __cplusplus

@@ -22,6 +22,7 @@ MAX_BODIES = 8
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
 _lp = C.POINTER(C.c_int64)
+_fp = C.POINTER(C.c_float)
 
 
 class Material(C.Structure):
@@ -39,6 +40,7 @@ class System(C.Structure):
         ("act", _ip), ("body", _ip), ("bc", _ip),
         ("row_start", _lp), ("col", _ip), ("top", _ip),
         ("val", _dp), ("diag", _dp), ("minv", _dp),
+        ("val32", _fp), ("diag32", _fp), ("minv32", _fp),
     ]
 
 
@@ -51,6 +53,8 @@ class StepParams(C.Structure):
         ("wall_radius", C.c_double), ("restitution", C.c_double),
         ("contact_radius", C.c_double), ("contact_stiffness", C.c_double),
         ("contact_volume", C.c_double), ("contact_cell", C.c_double),
+        ("precision", C.c_int32), ("inner_rtol", C.c_double),
+        ("inner_max_iter", C.c_int32), ("max_refine", C.c_int32),
     ]
 
 
@@ -60,6 +64,7 @@ class StepStats(C.Structure):
         ("residual_norm", C.c_double), ("b_norm", C.c_double),
         ("broken_bonds", C.c_int64), ("live_bonds", C.c_int64),
         ("min_gap", C.c_double), ("gap_p", C.c_int32), ("gap_q", C.c_int32),
+        ("refinements", C.c_int32),
     ]
 
     def as_dict(self):
@@ -92,7 +97,9 @@ def lib():
         L.orc_bonds_brute.restype = C.c_int64
         L.orc_dot.argtypes = [C.c_int32, _dp, _dp]
         L.orc_dot.restype = C.c_double
-        for name in ("orc_assemble", "orc_cg"):
+        L.orc_dot_f32.argtypes = [C.c_int32, _fp, _fp]
+        L.orc_dot_f32.restype = C.c_double
+        for name in ("orc_assemble", "orc_cg", "orc_cg_mixed"):
             getattr(L, name).argtypes = [C.POINTER(System), C.POINTER(StepParams),
                                          C.POINTER(StepStats)]
         L.orc_step.argtypes = [C.POINTER(System), C.POINTER(StepParams),
@@ -100,6 +107,8 @@ def lib():
         L.orc_step.restype = C.c_int32
         L.orc_spmv.argtypes = [C.POINTER(System), _dp, _dp]
         L.orc_block_jacobi.argtypes = [C.POINTER(System)]
+        L.orc_mixed_matrix.argtypes = [C.POINTER(System)]
+        L.orc_spmv_wide.argtypes = [C.POINTER(System), _dp, _dp]
         for name in ("orc_update", "orc_contact"):
             getattr(L, name).argtypes = [C.POINTER(System), C.POINTER(StepParams)]
         _lib = L
@@ -166,6 +175,12 @@ def lists_to_csr(counts, lists):
     return row_start, col
 
 
+def dot_f32(a4, b4) -> float:
+    a4 = np.ascontiguousarray(a4, dtype=np.float32)
+    b4 = np.ascontiguousarray(b4, dtype=np.float32)
+    return lib().orc_dot_f32(a4.shape[0], _ptr(a4, _fp), _ptr(b4, _fp))
+
+
 def dot(a4, b4) -> float:
     a4 = np.ascontiguousarray(a4, dtype=np.float64)
     b4 = np.ascontiguousarray(b4, dtype=np.float64)
@@ -179,8 +194,9 @@ def make_params(**kw) -> StepParams:
                     rtol=1e-6, max_iter=100000, walls=0, bmin=(-1e30,) * 3,
                     bmax=(1e30,) * 3, wall_radius=0.0, restitution=0.001,
                     contact_radius=0.0, contact_stiffness=0.0, contact_volume=1.0,
-                    contact_cell=0.0)
+                    contact_cell=0.0, precision=0, inner_rtol=1e-5, max_refine=20)
     defaults.update(kw)
+    defaults.setdefault("inner_max_iter", defaults["max_iter"])
     for k, v in defaults.items():
         if k in ("bmin", "bmax"):
             getattr(p, k)[:] = [float(t) for t in v]
@@ -219,6 +235,9 @@ class OracleSystem:
         self.val = np.zeros((max(nnz, 1), 9))
         self.diag = np.zeros((n, 9))
         self.minv = np.zeros((n, 9))
+        self.val32 = np.zeros((max(nnz, 1), 9), np.float32)
+        self.diag32 = np.zeros((n, 9), np.float32)
+        self.minv32 = np.zeros((n, 9), np.float32)
         self.materials = list(materials)
         self._struct = System()
         self._refresh()
@@ -232,6 +251,8 @@ class OracleSystem:
         for name in ("x", "v", "x0", "sol", "f", "b", "fc", "vbc", "s", "s_new",
                      "s00", "val", "diag", "minv"):
             setattr(s, name, _ptr(getattr(self, name), _dp))
+        for name in ("val32", "diag32", "minv32"):
+            setattr(s, name, _ptr(getattr(self, name), _fp))
         for name in ("act", "body", "bc", "col", "top"):
             setattr(s, name, _ptr(getattr(self, name), _ip))
         s.row_start = _ptr(self.row_start, _lp)
@@ -254,6 +275,24 @@ class OracleSystem:
         st = StepStats()
         lib().orc_cg(C.byref(self._struct), C.byref(prm), C.byref(st))
         return st
+
+    def cg_mixed(self, prm):
+        st = StepStats()
+        lib().orc_cg_mixed(C.byref(self._struct), C.byref(prm), C.byref(st))
+        return st
+
+    def mixed_matrix(self):
+        lib().orc_mixed_matrix(C.byref(self._struct))
+
+    def spmv_wide(self, x4):
+        x4 = np.ascontiguousarray(x4, dtype=np.float64)
+        y = np.zeros((self.n, 4))
+        lib().orc_spmv_wide(C.byref(self._struct), _ptr(x4, _dp), _ptr(y, _dp))
+        return y
+
+    def solve(self, prm):
+        """orc_cg or orc_cg_mixed by prm.precision."""
+        return self.cg_mixed(prm) if prm.precision == 1 else self.cg(prm)
 
     def update(self, prm):
         lib().orc_update(C.byref(self._struct), C.byref(prm))

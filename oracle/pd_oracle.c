@@ -364,6 +364,40 @@ double orc_dot(int32_t n, const double* a4, const double* b4) {
   return r;
 }
 
+/* orc_dot's tree over precomputed per-row terms */
+static double orc_tree(int32_t n, const double* terms) {
+  int64_t g = (n + ORC_TILE - 1) / ORC_TILE;
+  if (g == 0) return final_combine(0, NULL);
+  double* part = (double*)malloc(sizeof(double) * (size_t)g);
+#pragma omp parallel for schedule(static)
+  for (int64_t t = 0; t < g; ++t) {
+    double v[256];
+    for (int l = 0; l < 256; ++l) {
+      int64_t i = t * ORC_TILE + l;
+      v[l] = (i < n) ? terms[i] : 0.0;
+    }
+    part[t] = tile_combine(v);
+  }
+  double r = final_combine(g, part);
+  free(part);
+  return r;
+}
+
+/* FP64 product of FP32 operands: every float is widened exactly */
+static inline double term3f(const float* a, const float* b) {
+  return ((double)a[0] * (double)b[0] + (double)a[1] * (double)b[1]) +
+         (double)a[2] * (double)b[2];
+}
+
+double orc_dot_f32(int32_t n, const float* a4, const float* b4) {
+  double* t = (double*)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+#pragma omp parallel for schedule(static)
+  for (int32_t i = 0; i < n; ++i) t[i] = term3f(a4 + 4 * (int64_t)i, b4 + 4 * (int64_t)i);
+  const double r = orc_tree(n, t);
+  free(t);
+  return r;
+}
+
 /* dot over three vector pairs in one sweep (same tree for each) */
 static void orc_dot3(int32_t n, const double* a0, const double* b0,
                      const double* a1, const double* b1, const double* a2,
@@ -711,6 +745,7 @@ void orc_cg(orc_system* sys, const orc_step_params* prm, orc_step_stats* st) {
     st->converged = converged;
     st->residual_norm = rnorm;
     st->b_norm = bnorm;
+    st->refinements = 0;
   }
   free(y);
   free(r);
@@ -718,6 +753,236 @@ void orc_cg(orc_system* sys, const orc_step_params* prm, orc_step_stats* st) {
   free(p);
   free(sb);
   (void)orc_dot3;
+}
+
+/* ======================================================================
+ * Mixed-precision solve (precision = 1).  Not in the reference, which
+ * solves in FP64 only; this is the textbook iterative refinement (Wilkinson;
+ * Higham, "Accuracy and Stability of Numerical Algorithms", ch. 12) with a
+ * PCG inner solver:
+ *   A32 = the assembled off-diagonal blocks rounded to float, FP64 diagonal
+ *   loop:  r = S(b - A32 x) in FP64 (float blocks widened exactly)
+ *          stop if ||r|| <= rtol ||S b|| (FP64 norms)
+ *          inner PCG on float vectors and float arithmetic for A32 d = r,
+ *            diagonal block and block-Jacobi inverse rounded to float,
+ *            dot products and norms in FP64 (orc_dot's tree over widened
+ *            floats), alpha and beta FP64 and rounded to float for the
+ *            vector updates, stop when ||r32|| <= max(inner_rtol ||r||,
+ *            tol / 2) or after inner_max_iter iterations
+ *          x += d (FP64)
+ * The FP32 blocks are float((F * -1) * dt2), the same rounding the GPU
+ * assembly applies; minv32 = float(minv), minv from orc_block_jacobi.
+ * ====================================================================== */
+static inline void blk_mul_f(const float* A, const float* x, float* o) {
+  o[0] = A[0] * x[0] + A[1] * x[1] + A[2] * x[2];
+  o[1] = A[3] * x[0] + A[4] * x[1] + A[5] * x[2];
+  o[2] = A[6] * x[0] + A[7] * x[1] + A[8] * x[2];
+}
+
+static inline void blk_mul_wide(const float* A, const double* x, double* o) {
+  o[0] = (double)A[0] * x[0] + (double)A[1] * x[1] + (double)A[2] * x[2];
+  o[1] = (double)A[3] * x[0] + (double)A[4] * x[1] + (double)A[5] * x[2];
+  o[2] = (double)A[6] * x[0] + (double)A[7] * x[1] + (double)A[8] * x[2];
+}
+
+/* y = A32 x in FP64: FP64 diagonal, float bonds widened, FP64 x */
+void orc_spmv_wide(const orc_system* sys, const double* x4, double* y4) {
+  const int32_t n = sys->n;
+#pragma omp parallel for schedule(static)
+  for (int32_t p = 0; p < n; ++p) {
+    double acc[3], t[3];
+    blk_mul(sys->diag + 9 * (int64_t)p, x4 + 4 * (int64_t)p, acc);
+    const int64_t rs = sys->row_start[p];
+    for (int32_t bnd = 0; bnd <= sys->top[p]; ++bnd) {
+      const int32_t q = sys->col[rs + bnd];
+      blk_mul_wide(sys->val32 + 9 * (rs + bnd), x4 + 4 * (int64_t)q, t);
+      acc[0] += t[0];
+      acc[1] += t[1];
+      acc[2] += t[2];
+    }
+    double* y = y4 + 4 * (int64_t)p;
+    y[0] = acc[0];
+    y[1] = acc[1];
+    y[2] = acc[2];
+    y[3] = 0.0;
+  }
+}
+
+/* y = A32 x in float arithmetic, same order as orc_spmv */
+static void orc_spmv_f32(const orc_system* sys, const float* x4, float* y4) {
+  const int32_t n = sys->n;
+#pragma omp parallel for schedule(static)
+  for (int32_t p = 0; p < n; ++p) {
+    float acc[3], t[3];
+    blk_mul_f(sys->diag32 + 9 * (int64_t)p, x4 + 4 * (int64_t)p, acc);
+    const int64_t rs = sys->row_start[p];
+    for (int32_t bnd = 0; bnd <= sys->top[p]; ++bnd) {
+      const int32_t q = sys->col[rs + bnd];
+      blk_mul_f(sys->val32 + 9 * (rs + bnd), x4 + 4 * (int64_t)q, t);
+      acc[0] += t[0];
+      acc[1] += t[1];
+      acc[2] += t[2];
+    }
+    float* y = y4 + 4 * (int64_t)p;
+    y[0] = acc[0];
+    y[1] = acc[1];
+    y[2] = acc[2];
+    y[3] = 0.0f;
+  }
+}
+
+static inline void precond_apply_f(const orc_system* sys, int precond, int64_t p,
+                                   const float* r, float* z) {
+  if (precond == 1) {
+    const float* m = sys->minv32 + 9 * p;
+    z[0] = m[0] * r[0] + m[1] * r[1] + m[2] * r[2];
+    z[1] = m[3] * r[0] + m[4] * r[1] + m[5] * r[2];
+    z[2] = m[6] * r[0] + m[7] * r[1] + m[8] * r[2];
+  } else {
+    z[0] = r[0];
+    z[1] = r[1];
+    z[2] = r[2];
+  }
+  z[3] = 0.0f;
+}
+
+void orc_mixed_matrix(orc_system* sys) {
+  const int32_t n = sys->n;
+  const int64_t nslot = sys->row_start[n];
+  orc_block_jacobi(sys);
+#pragma omp parallel for schedule(static)
+  for (int64_t k = 0; k < 9 * nslot; ++k) sys->val32[k] = (float)sys->val[k];
+#pragma omp parallel for schedule(static)
+  for (int64_t k = 0; k < 9 * (int64_t)n; ++k) {
+    sys->diag32[k] = (float)sys->diag[k];
+    sys->minv32[k] = (float)sys->minv[k];
+  }
+}
+
+void orc_cg_mixed(orc_system* sys, const orc_step_params* prm, orc_step_stats* st) {
+  const int32_t n = sys->n;
+  const size_t nn = (size_t)(n > 0 ? n : 1);
+  double* x = sys->sol;
+  double* y = (double*)malloc(sizeof(double) * 4 * nn);
+  float* r32 = (float*)malloc(sizeof(float) * 4 * nn);
+  float* z32 = (float*)malloc(sizeof(float) * 4 * nn);
+  float* p32 = (float*)malloc(sizeof(float) * 4 * nn);
+  float* y32 = (float*)malloc(sizeof(float) * 4 * nn);
+  float* d32 = (float*)malloc(sizeof(float) * 4 * nn);
+  double* t_rr = (double*)malloc(sizeof(double) * nn);
+  double* t_rz = (double*)malloc(sizeof(double) * nn);
+  double* t_r32 = (double*)malloc(sizeof(double) * nn);
+  double* t_bb = (double*)malloc(sizeof(double) * nn);
+  const double dt = prm->dt;
+  orc_mixed_matrix(sys);
+#pragma omp parallel for schedule(static)
+  for (int32_t i = 0; i < n; ++i) {
+    if (sys->bc[i]) {
+      for (int k = 0; k < 3; ++k) {
+        const double vb = sys->vbc[4 * (int64_t)i + k];
+        x[4 * (int64_t)i + k] = (prm->integrator == 1) ? dt * vb : vb;
+      }
+      x[4 * (int64_t)i + 3] = 0.0;
+    }
+  }
+  int32_t it = 0, ref = 0, converged = 0, first = 1;
+  double bnorm = 0.0, tol = 0.0, rnorm = 0.0;
+  for (;;) {
+    /* FP64 residual, then the inner solve's start: d = 0, z = M r, p = z */
+    orc_spmv_wide(sys, x, y);
+#pragma omp parallel for schedule(static)
+    for (int32_t i = 0; i < n; ++i) {
+      const int64_t o = 4 * (int64_t)i;
+      const int fr = !sys->bc[i];
+      double r[3], sb[3];
+      for (int k = 0; k < 3; ++k) {
+        r[k] = fr ? (sys->b[o + k] - y[o + k]) : 0.0;
+        sb[k] = fr ? sys->b[o + k] : 0.0;
+        r32[o + k] = (float)r[k];
+        d32[o + k] = 0.0f;
+      }
+      r32[o + 3] = 0.0f;
+      d32[o + 3] = 0.0f;
+      precond_apply_f(sys, prm->precond, i, r32 + o, z32 + o);
+      for (int k = 0; k < 4; ++k) p32[o + k] = z32[o + k];
+      t_rr[i] = term3(r, r);
+      t_bb[i] = term3(sb, sb);
+      t_rz[i] = term3f(r32 + o, z32 + o);
+      t_r32[i] = term3f(r32 + o, r32 + o);
+    }
+    const double rr = orc_tree(n, t_rr);
+    double rz = orc_tree(n, t_rz);
+    double rr32 = orc_tree(n, t_r32);
+    const double bb = orc_tree(n, t_bb);
+    if (first) {
+      bnorm = sqrt(bb);
+      tol = prm->rtol * bnorm;
+      first = 0;
+    }
+    rnorm = sqrt(rr);
+    if (rnorm <= tol) {
+      converged = 1;
+      break;
+    }
+    if (ref >= prm->max_refine || it >= prm->max_iter) break;
+    ++ref;
+    const double a_tol = prm->inner_rtol * rnorm, b_tol = 0.5 * tol;
+    const double inner_tol = a_tol > b_tol ? a_tol : b_tol;
+    int32_t k_in = 0;
+    for (;;) {
+      if (sqrt(rr32) <= inner_tol || k_in >= prm->inner_max_iter || it >= prm->max_iter) break;
+      orc_spmv_f32(sys, p32, y32);
+      const double alpha = rz / orc_dot_f32(n, y32, p32);
+      const float af = (float)alpha;
+#pragma omp parallel for schedule(static)
+      for (int32_t i = 0; i < n; ++i) {
+        const int64_t o = 4 * (int64_t)i;
+        for (int k = 0; k < 3; ++k) d32[o + k] = af * p32[o + k] + d32[o + k];
+        if (!sys->bc[i]) {
+          for (int k = 0; k < 3; ++k) r32[o + k] = (-af) * y32[o + k] + r32[o + k];
+        }
+        precond_apply_f(sys, prm->precond, i, r32 + o, z32 + o);
+        t_rz[i] = term3f(r32 + o, z32 + o);
+        t_r32[i] = term3f(r32 + o, r32 + o);
+      }
+      const double rz_new = orc_tree(n, t_rz);
+      rr32 = orc_tree(n, t_r32);
+      const double beta = rz_new / rz;
+      rz = rz_new;
+      const float bf = (float)beta;
+      ++k_in;
+      ++it;
+      if (sqrt(rr32) <= inner_tol || k_in >= prm->inner_max_iter || it >= prm->max_iter) break;
+#pragma omp parallel for schedule(static)
+      for (int32_t i = 0; i < n; ++i) {
+        const int64_t o = 4 * (int64_t)i;
+        for (int k = 0; k < 3; ++k) p32[o + k] = z32[o + k] + bf * p32[o + k];
+      }
+    }
+    /* x += d in FP64 */
+#pragma omp parallel for schedule(static)
+    for (int32_t i = 0; i < n; ++i) {
+      const int64_t o = 4 * (int64_t)i;
+      for (int k = 0; k < 3; ++k) x[o + k] = x[o + k] + (double)d32[o + k];
+    }
+  }
+  if (st) {
+    st->cg_iterations = it;
+    st->converged = converged;
+    st->residual_norm = rnorm;
+    st->b_norm = bnorm;
+    st->refinements = ref;
+  }
+  free(y);
+  free(r32);
+  free(z32);
+  free(p32);
+  free(y32);
+  free(d32);
+  free(t_rr);
+  free(t_rz);
+  free(t_r32);
+  free(t_bb);
 }
 
 /* ======================================================================
@@ -864,7 +1129,10 @@ int32_t orc_step(orc_system* sys, const orc_step_params* prm,
   orc_assemble(sys, prm, st);
   /* simulator.cuh:2612 s <- s_new */
   memcpy(sys->s, sys->s_new, sizeof(double) * (size_t)sys->n);
-  orc_cg(sys, prm, st);
+  if (prm->precision == 1)
+    orc_cg_mixed(sys, prm, st);
+  else
+    orc_cg(sys, prm, st);
   orc_update(sys, prm);
   return (st && !st->converged) ? 1 : 0;
 }

@@ -28,6 +28,9 @@
  *   orc_update          free-fall closed form, wall-clamp closed form
  *   orc_contact         two-particle closed form, symmetry
  *   orc_dot             exact rational sum on integer-valued data
+ *   orc_cg_mixed        numpy dense solve of the float-rounded system,
+ *                       FP64 stopping rule on the true residual, and the
+ *                       float inner iterations replayed in numpy float32
  */
 #ifndef PD_ORACLE_H
 #define PD_ORACLE_H
@@ -104,6 +107,10 @@ typedef struct orc_system {
   double* val;  /* 9 doubles per slot: A_{p,col} row-major 3x3 */
   double* diag; /* 9 doubles per row: A_{pp} */
   double* minv; /* 9 doubles per row: inverse of A_{pp} (block Jacobi) */
+  /* mixed precision (precision = 1): the same blocks rounded to float */
+  float* val32;  /* 9 per slot */
+  float* diag32; /* 9 per row */
+  float* minv32; /* 9 per row: float(inverse of the FP64 diagonal block) */
 } orc_system;
 
 typedef struct orc_step_params {
@@ -123,6 +130,10 @@ typedef struct orc_step_params {
   double contact_stiffness; /* Kc */
   double contact_volume;    /* PD_V divisor */
   double contact_cell;      /* uniform-grid cell edge, >= contact_radius */
+  int32_t precision;        /* 0 FP64 CG; 1 FP32 inner CG + FP64 refinement */
+  double inner_rtol;        /* inner stop: ||r32|| <= max(inner_rtol ||r||, tol/2) */
+  int32_t inner_max_iter;   /* per refinement */
+  int32_t max_refine;       /* outer FP64 refinements */
 } orc_step_params;
 
 typedef struct orc_step_stats {
@@ -135,6 +146,7 @@ typedef struct orc_step_stats {
   double min_gap;       /* min |sigma - threshold| over bonds tested */
   int32_t gap_p;        /* row of that bond */
   int32_t gap_q;        /* column of that bond */
+  int32_t refinements;  /* mixed precision: FP64 corrections applied */
 } orc_step_stats;
 
 /* Deterministic reduction shared with the HIP kernels (see pd_oracle.c). */
@@ -152,6 +164,15 @@ void orc_block_jacobi(orc_system* sys);
 /* core/simulator.cu:735-799 (CUSP krylov::cg structure), projected onto
  * the free rows when bc[] marks Dirichlet particles. */
 void orc_cg(orc_system* sys, const orc_step_params* prm, orc_step_stats* st);
+/* Mixed-precision solve (precision = 1): iterative refinement in FP64
+ * around a block-Jacobi PCG on FP32 vectors and FP32 blocks whose dot
+ * products and norms are FP64 (orc_dot's tree over float operands). */
+void orc_cg_mixed(orc_system* sys, const orc_step_params* prm, orc_step_stats* st);
+/* block-Jacobi inverse, then val32 / diag32 / minv32 = float(val / diag / minv) */
+void orc_mixed_matrix(orc_system* sys);
+/* y = A32 x in FP64 arithmetic (FP64 diagonal, float bonds widened) */
+void orc_spmv_wide(const orc_system* sys, const double* x4, double* y4);
+double orc_dot_f32(int32_t n, const float* a4, const float* b4);
 /* core/implicit_euler.cuh:90-176 / core/newmark_beta.cuh:89-199 */
 void orc_update(orc_system* sys, const orc_step_params* prm);
 /* core/simulator.cuh:602-768 collidePDParticles restated on a uniform

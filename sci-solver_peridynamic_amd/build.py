@@ -27,11 +27,13 @@ HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17",
     # bit-exact parity with the host oracle: no FMA contraction
     "-ffp-contract=off",
+    # IEEE float denormals in the float inner solve, as on the host
+    "-fno-gpu-flush-denormals-to-zero",
     # no packed FP32 instructions
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", f"-I{INCLUDE}", f"-I{CSRC}",
 ]
-HIP_SOURCES = ["pd_step.hip", "pd_bonds.hip", "pd_api.hip"]
+HIP_SOURCES = ["pd_step.hip", "pd_bonds.hip", "pd_mixed.hip", "pd_api.hip"]
 HOST_SOURCES = ["parameters.cpp", "scene.cpp", "simulator.cpp", "data_io.cpp", "host_capi.cpp"]
 CXX = os.environ.get("CXX", "g++")
 CXX_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-Wall", f"-I{INCLUDE}", f"-I{HOST}"]

@@ -10,7 +10,9 @@
 //   * matrix values: the 3x3 blocks are symmetric bit for bit, so six
 //     values (a00 a01 a02 a11 a12 a22) are stored in component planes:
 //     value c of slot k of row p at ((slice_off[s] + k) * 6 + c) * 64 + lane;
-//   * diagonal and block-Jacobi inverse blocks: (s * 6 + c) * 64 + lane.
+//   * diagonal and block-Jacobi inverse blocks: (s * 6 + c) * 64 + lane;
+//   * mixed precision keeps float copies in the same positions (val32,
+//     diag32, minv32) and float4 vectors (16 B) for the inner solve.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -41,6 +43,34 @@ struct pd_cg_state {
   int32_t max_iter;
   uint32_t arrive;  // reserved
   uint32_t pad[3];
+};
+
+// Scalars of one mixed-precision solve (iterative refinement around a float
+// PCG).  phase: 0 residual due, 1 inner iterations, 2 inner finished
+// (correction pending, then residual), 3 done.  Grid kernels only read
+// phase; the one-workgroup fold kernels change it.
+struct pd_mx_state {
+  double rz;         // <r32, z32> (FP64 sum of widened floats)
+  double rr32;       // <r32, r32>
+  double alpha;
+  double beta;
+  double rnorm;      // ||S(b - A x)||, FP64
+  double bnorm;      // ||S b||
+  double tol;        // rtol * bnorm
+  double inner_tol;  // max(inner_rtol * rnorm, tol / 2)
+  double rtol;
+  double inner_rtol;
+  int32_t iter;        // inner iterations over the whole solve
+  int32_t inner_iter;  // of the current refinement
+  int32_t refinements;
+  int32_t phase;
+  int32_t converged;
+  int32_t first;
+  int32_t max_iter;
+  int32_t inner_max;
+  int32_t max_refine;
+  int32_t residuals;  // FP64 residual evaluations so far
+  int32_t pad[2];
 };
 
 // Per-step counters written by the assembly kernel.
@@ -80,4 +110,13 @@ __device__ __forceinline__ double tile_sum(double v, double* lds4) {
   const double r = (lds4[0] + lds4[1]) + (lds4[2] + lds4[3]);
   __syncthreads();
   return r;
+}
+
+// Final fold of one quantity over the tiles by one 256-thread workgroup:
+// thread t sums part[t], part[t + 256], ... left to right from 0.0, then
+// tile_sum (the oracle's final_combine).
+__device__ __forceinline__ double fold_partials(const double* part, int ntiles, double* lds4) {
+  double acc = 0.0;
+  for (int k = threadIdx.x; k < ntiles; k += PD_BLOCK) acc = acc + part[k];
+  return tile_sum(acc, lds4);
 }

@@ -25,6 +25,10 @@ struct AsmArgs {
   double* val;
   double* diag;
   double* minv;
+  float* val32;   // mixed precision: float blocks instead of val / minv
+  float* diag32;
+  float* minv32;
+  int32_t mixed;
   const double4* x;
   const double4* x0;
   const double4* v;
@@ -76,6 +80,30 @@ struct CgArgs {
   int32_t precond;
 };
 
+// Mixed-precision solve: float matrix copy, float inner vectors, FP64
+// solution and right-hand side.
+struct MxArgs {
+  int32_t n;
+  const int64_t* slice_off;
+  const int32_t* top;
+  const int32_t* col;
+  const float* val32;
+  const double* diag;
+  const float* diag32;
+  const float* minv32;
+  const int32_t* bc;
+  const double4* b;
+  double4* x;
+  float4* r;
+  float4* z;
+  float4* p;
+  float4* y;
+  float4* d;
+  double* part;  // [4 * ntiles]
+  pd_mx_state* st;
+  int32_t precond;
+};
+
 #define PD_MAX_ZONE_BODIES 8
 
 struct GridView {
@@ -99,6 +127,17 @@ void launch_cg_spmv(const CgArgs& a, hipStream_t s);
 void launch_cg_update(const CgArgs& a, hipStream_t s);
 void launch_cg_dir(const CgArgs& a, hipStream_t s);
 void launch_spmv_plain(const SpmvView& A, const double4* x, double4* y, hipStream_t s);
+// pd_mixed.hip
+void launch_mx_setup(pd_mx_state* st, double rtol, int32_t max_iter, double inner_rtol,
+                     int32_t inner_max, int32_t max_refine, hipStream_t s);
+void launch_mx_resid(const MxArgs& a, hipStream_t s);  // residual + its fold
+void launch_mx_spmv(const MxArgs& a, hipStream_t s);
+void launch_mx_update(const MxArgs& a, hipStream_t s);  // alpha, update, beta
+void launch_mx_dir(const MxArgs& a, hipStream_t s);
+void launch_mx_apply(const MxArgs& a, hipStream_t s);
+// y = A32 x with FP64 arithmetic (the residual operator), and y32 = A32 x32
+void launch_spmv_wide(const MxArgs& a, const double4* x, double4* y, hipStream_t s);
+void launch_spmv_f32(const MxArgs& a, const float4* x, float4* y, hipStream_t s);
 void launch_update(int32_t n, double4* x, double4* v, const double4* sol,
                    const int32_t* act, int32_t integrator, double dt, int32_t walls,
                    const double* lo, const double* hi, double restitution,

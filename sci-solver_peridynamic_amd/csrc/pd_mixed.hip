@@ -1,0 +1,452 @@
+// pd_mixed.hip -- mixed-precision solve: iterative refinement in FP64
+// around a block-Jacobi PCG on float vectors and float matrix blocks.
+//
+// The reference solves in FP64 only (core/simulator.cu:735-799); this solver
+// keeps its system and stopping rule (||S(b - A x)|| <= rtol ||S b||, with
+// the residual, its norm and every dot product in FP64) and moves the
+// bandwidth-bound inner iterations to 4-byte values and vectors:
+//   residual  r = S(b - A32 x): FP64 diagonal, float bond blocks widened
+//             exactly, FP64 x; fused with the inner start d = 0, z = M32 r32,
+//             p = z (r32 = float(r))
+//   inner     y = A32 p in float arithmetic (diagonal first, bonds in list
+//             order, (a0 x0 + a1 x1) + a2 x2 per block row), alpha and beta
+//             in FP64 from FP64 sums of widened float products, rounded to
+//             float for the updates d += a p, r -= a y, p = z + b p
+//   stop      inner: ||r32|| <= max(inner_rtol ||r||, tol / 2) or its
+//             iteration limit; outer: ||r|| <= tol, refinement or total
+//             iteration limit
+//   correct   x += d in FP64
+// Sums use the tree of the FP64 solver (wave butterfly, (w0+w1)+(w2+w3) per
+// 256-row tile, strided fold of the tile partials), so the result is
+// reproducible run to run and equal bit for bit to the oracle's restatement.
+#include <math.h>
+
+#include "pd_internal.h"
+
+namespace pd {
+
+namespace {
+
+__device__ __forceinline__ double ntl(const double* p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ float ntl(const float* p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ int ntl(const int* p) { return __builtin_nontemporal_load(p); }
+
+__device__ __forceinline__ void blk6w(float a00, float a01, float a02, float a11, float a12,
+                                      float a22, const double4 x, double& o0, double& o1,
+                                      double& o2) {
+  const double b00 = a00, b01 = a01, b02 = a02, b11 = a11, b12 = a12, b22 = a22;
+  o0 = b00 * x.x + b01 * x.y + b02 * x.z;
+  o1 = b01 * x.x + b11 * x.y + b12 * x.z;
+  o2 = b02 * x.x + b12 * x.y + b22 * x.z;
+}
+
+__device__ __forceinline__ void blk6f(float a00, float a01, float a02, float a11, float a12,
+                                      float a22, const float4 x, float& o0, float& o1,
+                                      float& o2) {
+  o0 = a00 * x.x + a01 * x.y + a02 * x.z;
+  o1 = a01 * x.x + a11 * x.y + a12 * x.z;
+  o2 = a02 * x.x + a12 * x.y + a22 * x.z;
+}
+
+// Row p of A32 x in FP64 arithmetic (FP64 diagonal, float bonds).
+__device__ __forceinline__ void row_wide(const MxArgs& A, const double4* __restrict__ x, int p,
+                                         double& y0, double& y1, double& y2) {
+  const int s = p >> 6;
+  const int lane = p & 63;
+  const double* dg = A.diag + (int64_t)s * 6 * 64 + lane;
+  {
+    const double4 xp = x[p];
+    const double a00 = dg[0], a01 = dg[64], a02 = dg[128], a11 = dg[192], a12 = dg[256],
+                 a22 = dg[320];
+    y0 = a00 * xp.x + a01 * xp.y + a02 * xp.z;
+    y1 = a01 * xp.x + a11 * xp.y + a12 * xp.z;
+    y2 = a02 * xp.x + a12 * xp.y + a22 * xp.z;
+  }
+  const int64_t base = A.slice_off[s];
+  const int top = A.top[p];
+  const int* __restrict__ cp = A.col + base * 64 + lane;
+  const float* __restrict__ vp = A.val32 + base * 6 * 64 + lane;
+  int bnd = 0;
+  for (; bnd + 3 <= top; bnd += 4) {
+    int q[4];
+    float v[4][6];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) q[u] = ntl(cp + (int64_t)(bnd + u) * 64);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const float* vv = vp + (int64_t)(bnd + u) * 384;
+#pragma unroll
+      for (int c = 0; c < 6; ++c) v[u][c] = ntl(vv + c * 64);
+    }
+    double4 xx[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) xx[u] = x[q[u]];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      double t0, t1, t2;
+      blk6w(v[u][0], v[u][1], v[u][2], v[u][3], v[u][4], v[u][5], xx[u], t0, t1, t2);
+      y0 += t0;
+      y1 += t1;
+      y2 += t2;
+    }
+  }
+  for (; bnd <= top; ++bnd) {
+    const int q0 = ntl(cp + (int64_t)bnd * 64);
+    const float* v0 = vp + (int64_t)bnd * 384;
+    double t0, t1, t2;
+    blk6w(ntl(v0), ntl(v0 + 64), ntl(v0 + 128), ntl(v0 + 192), ntl(v0 + 256), ntl(v0 + 320),
+          x[q0], t0, t1, t2);
+    y0 += t0;
+    y1 += t1;
+    y2 += t2;
+  }
+}
+
+// Row p of A32 x in float arithmetic.
+__device__ __forceinline__ void row_f32(const MxArgs& A, const float4* __restrict__ x, int p,
+                                        float& y0, float& y1, float& y2) {
+  const int s = p >> 6;
+  const int lane = p & 63;
+  const float* dg = A.diag32 + (int64_t)s * 6 * 64 + lane;
+  blk6f(dg[0], dg[64], dg[128], dg[192], dg[256], dg[320], x[p], y0, y1, y2);
+  const int64_t base = A.slice_off[s];
+  const int top = A.top[p];
+  const int* __restrict__ cp = A.col + base * 64 + lane;
+  const float* __restrict__ vp = A.val32 + base * 6 * 64 + lane;
+  int bnd = 0;
+  for (; bnd + 3 <= top; bnd += 4) {
+    int q[4];
+    float v[4][6];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) q[u] = ntl(cp + (int64_t)(bnd + u) * 64);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const float* vv = vp + (int64_t)(bnd + u) * 384;
+#pragma unroll
+      for (int c = 0; c < 6; ++c) v[u][c] = ntl(vv + c * 64);
+    }
+    float4 xx[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) xx[u] = x[q[u]];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      float t0, t1, t2;
+      blk6f(v[u][0], v[u][1], v[u][2], v[u][3], v[u][4], v[u][5], xx[u], t0, t1, t2);
+      y0 += t0;
+      y1 += t1;
+      y2 += t2;
+    }
+  }
+  for (; bnd <= top; ++bnd) {
+    const int q0 = ntl(cp + (int64_t)bnd * 64);
+    const float* v0 = vp + (int64_t)bnd * 384;
+    float t0, t1, t2;
+    blk6f(ntl(v0), ntl(v0 + 64), ntl(v0 + 128), ntl(v0 + 192), ntl(v0 + 256), ntl(v0 + 320),
+          x[q0], t0, t1, t2);
+    y0 += t0;
+    y1 += t1;
+    y2 += t2;
+  }
+}
+
+__device__ __forceinline__ void precond_f(const float* minv, int pc, int p, float r0, float r1,
+                                          float r2, float& z0, float& z1, float& z2) {
+  if (pc == 1) {
+    const float* m = minv + (int64_t)(p >> 6) * 6 * 64 + (p & 63);
+    const float m00 = m[0], m01 = m[64], m02 = m[128], m11 = m[192], m12 = m[256],
+                m22 = m[320];
+    z0 = m00 * r0 + m01 * r1 + m02 * r2;
+    z1 = m01 * r0 + m11 * r1 + m12 * r2;
+    z2 = m02 * r0 + m12 * r1 + m22 * r2;
+  } else {
+    z0 = r0;
+    z1 = r1;
+    z2 = r2;
+  }
+}
+
+__device__ __forceinline__ double term3d(double a0, double a1, double a2, double b0, double b1,
+                                         double b2) {
+  return (a0 * b0 + a1 * b1) + a2 * b2;
+}
+
+// FP64 product of float operands (each widened exactly)
+__device__ __forceinline__ double term3f(float a0, float a1, float a2, float b0, float b1,
+                                         float b2) {
+  return ((double)a0 * (double)b0 + (double)a1 * (double)b1) + (double)a2 * (double)b2;
+}
+
+__device__ __forceinline__ bool resid_due(int phase) { return phase == 0 || phase == 2; }
+
+__global__ void k_mx_setup(pd_mx_state* st, double rtol, int max_iter, double inner_rtol,
+                           int inner_max, int max_refine) {
+  if (threadIdx.x == 0) {
+    st->rz = 0.0;
+    st->rr32 = 0.0;
+    st->alpha = 0.0;
+    st->beta = 0.0;
+    st->rnorm = 0.0;
+    st->bnorm = 0.0;
+    st->tol = 0.0;
+    st->inner_tol = 0.0;
+    st->rtol = rtol;
+    st->inner_rtol = inner_rtol;
+    st->iter = 0;
+    st->inner_iter = 0;
+    st->refinements = 0;
+    st->phase = 0;
+    st->converged = 0;
+    st->first = 1;
+    st->max_iter = max_iter;
+    st->inner_max = inner_max;
+    st->max_refine = max_refine;
+    st->residuals = 0;
+  }
+}
+
+// r = S(b - A32 x) in FP64; r32 = float(r); z = M32 r32; p = z; d = 0;
+// partials <r,r>, <r32,z>, <r32,r32>, <Sb,Sb>
+__global__ __launch_bounds__(PD_BLOCK) void k_mx_resid(MxArgs a) {
+  if (!resid_due(a.st->phase)) return;
+  __shared__ double lds4[4];
+  const int ntiles = gridDim.x;
+  const int tile = xcd_tile(blockIdx.x, ntiles);
+  const int p = tile * PD_BLOCK + threadIdx.x;
+  double t_rr = 0.0, t_rz = 0.0, t_r32 = 0.0, t_bb = 0.0;
+  if (p < a.n) {
+    double y0, y1, y2;
+    row_wide(a, a.x, p, y0, y1, y2);
+    const double4 bp = a.b[p];
+    const bool fr = a.bc[p] == 0;
+    const double r0 = fr ? bp.x - y0 : 0.0;
+    const double r1 = fr ? bp.y - y1 : 0.0;
+    const double r2 = fr ? bp.z - y2 : 0.0;
+    const double sb0 = fr ? bp.x : 0.0, sb1 = fr ? bp.y : 0.0, sb2 = fr ? bp.z : 0.0;
+    const float f0 = (float)r0, f1 = (float)r1, f2 = (float)r2;
+    float z0, z1, z2;
+    precond_f(a.minv32, a.precond, p, f0, f1, f2, z0, z1, z2);
+    a.r[p] = make_float4(f0, f1, f2, 0.0f);
+    a.p[p] = make_float4(z0, z1, z2, 0.0f);
+    a.d[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    t_rr = term3d(r0, r1, r2, r0, r1, r2);
+    t_rz = term3f(f0, f1, f2, z0, z1, z2);
+    t_r32 = term3f(f0, f1, f2, f0, f1, f2);
+    t_bb = term3d(sb0, sb1, sb2, sb0, sb1, sb2);
+  }
+  const double v0 = tile_sum(t_rr, lds4);
+  const double v1 = tile_sum(t_rz, lds4);
+  const double v2 = tile_sum(t_r32, lds4);
+  const double v3 = tile_sum(t_bb, lds4);
+  if (threadIdx.x == 0) {
+    a.part[tile] = v0;
+    a.part[ntiles + tile] = v1;
+    a.part[2 * (int64_t)ntiles + tile] = v2;
+    a.part[3 * (int64_t)ntiles + tile] = v3;
+  }
+}
+
+// outer test; on failure of the test start the next inner solve
+__global__ __launch_bounds__(PD_BLOCK) void k_mx_resid_fold(MxArgs a, int ntiles) {
+  if (!resid_due(a.st->phase)) return;
+  __shared__ double lds4[4];
+  const double rr = fold_partials(a.part, ntiles, lds4);
+  const double rz = fold_partials(a.part + ntiles, ntiles, lds4);
+  const double rr32 = fold_partials(a.part + 2 * (int64_t)ntiles, ntiles, lds4);
+  const double bb = fold_partials(a.part + 3 * (int64_t)ntiles, ntiles, lds4);
+  if (threadIdx.x == 0) {
+    pd_mx_state* st = a.st;
+    st->residuals = st->residuals + 1;
+    if (st->first) {
+      st->bnorm = sqrt(bb);
+      st->tol = st->rtol * st->bnorm;
+      st->first = 0;
+    }
+    const double rnorm = sqrt(rr);
+    st->rnorm = rnorm;
+    if (rnorm <= st->tol) {
+      st->converged = 1;
+      st->phase = 3;
+    } else if (st->refinements >= st->max_refine || st->iter >= st->max_iter) {
+      st->phase = 3;
+    } else {
+      st->refinements = st->refinements + 1;
+      const double at = st->inner_rtol * rnorm, bt = 0.5 * st->tol;
+      st->inner_tol = at > bt ? at : bt;
+      st->rz = rz;
+      st->rr32 = rr32;
+      st->inner_iter = 0;
+      const bool stop = sqrt(rr32) <= st->inner_tol || st->inner_max <= 0 ||
+                        st->iter >= st->max_iter;
+      st->phase = stop ? 2 : 1;
+    }
+  }
+}
+
+// y = A32 p (float); partial <y,p>
+__global__ __launch_bounds__(PD_BLOCK) void k_mx_spmv(MxArgs a) {
+  if (a.st->phase != 1) return;
+  __shared__ double lds4[4];
+  const int ntiles = gridDim.x;
+  const int tile = xcd_tile(blockIdx.x, ntiles);
+  const int p = tile * PD_BLOCK + threadIdx.x;
+  double t = 0.0;
+  if (p < a.n) {
+    float y0, y1, y2;
+    row_f32(a, a.p, p, y0, y1, y2);
+    a.y[p] = make_float4(y0, y1, y2, 0.0f);
+    const float4 pp = a.p[p];
+    t = term3f(y0, y1, y2, pp.x, pp.y, pp.z);
+  }
+  const double v = tile_sum(t, lds4);
+  if (threadIdx.x == 0) a.part[tile] = v;
+}
+
+__global__ __launch_bounds__(PD_BLOCK) void k_mx_alpha(MxArgs a, int ntiles) {
+  if (a.st->phase != 1) return;
+  __shared__ double lds4[4];
+  const double yp = fold_partials(a.part, ntiles, lds4);
+  if (threadIdx.x == 0) a.st->alpha = a.st->rz / yp;
+}
+
+// d += a p; r -= a y (free rows); z = M32 r; partials <r,z>, <r,r>
+__global__ __launch_bounds__(PD_BLOCK) void k_mx_update(MxArgs a) {
+  if (a.st->phase != 1) return;
+  __shared__ double lds4[4];
+  const int ntiles = gridDim.x;
+  const int tile = xcd_tile(blockIdx.x, ntiles);
+  const int p = tile * PD_BLOCK + threadIdx.x;
+  const float af = (float)a.st->alpha;
+  double t_rz = 0.0, t_rr = 0.0;
+  if (p < a.n) {
+    const float4 pp = a.p[p];
+    float4 dd = a.d[p];
+    dd.x = af * pp.x + dd.x;
+    dd.y = af * pp.y + dd.y;
+    dd.z = af * pp.z + dd.z;
+    a.d[p] = dd;
+    float4 rr = a.r[p];
+    if (a.bc[p] == 0) {
+      const float4 yy = a.y[p];
+      const float na = -af;
+      rr.x = na * yy.x + rr.x;
+      rr.y = na * yy.y + rr.y;
+      rr.z = na * yy.z + rr.z;
+      a.r[p] = rr;
+    }
+    float z0, z1, z2;
+    precond_f(a.minv32, a.precond, p, rr.x, rr.y, rr.z, z0, z1, z2);
+    a.z[p] = make_float4(z0, z1, z2, 0.0f);
+    t_rz = term3f(rr.x, rr.y, rr.z, z0, z1, z2);
+    t_rr = term3f(rr.x, rr.y, rr.z, rr.x, rr.y, rr.z);
+  }
+  const double v0 = tile_sum(t_rz, lds4);
+  const double v1 = tile_sum(t_rr, lds4);
+  if (threadIdx.x == 0) {
+    a.part[tile] = v0;
+    a.part[ntiles + tile] = v1;
+  }
+}
+
+// beta and the inner test ahead of the next inner iteration
+__global__ __launch_bounds__(PD_BLOCK) void k_mx_beta(MxArgs a, int ntiles) {
+  if (a.st->phase != 1) return;
+  __shared__ double lds4[4];
+  const double rz = fold_partials(a.part, ntiles, lds4);
+  const double rr32 = fold_partials(a.part + ntiles, ntiles, lds4);
+  if (threadIdx.x == 0) {
+    pd_mx_state* st = a.st;
+    st->beta = rz / st->rz;
+    st->rz = rz;
+    st->rr32 = rr32;
+    st->iter = st->iter + 1;
+    st->inner_iter = st->inner_iter + 1;
+    if (sqrt(rr32) <= st->inner_tol || st->inner_iter >= st->inner_max ||
+        st->iter >= st->max_iter)
+      st->phase = 2;
+  }
+}
+
+// p = z + b p
+__global__ __launch_bounds__(PD_BLOCK) void k_mx_dir(MxArgs a) {
+  if (a.st->phase != 1) return;
+  const int p = xcd_tile(blockIdx.x, gridDim.x) * PD_BLOCK + threadIdx.x;
+  if (p >= a.n) return;
+  const float bf = (float)a.st->beta;
+  const float4 zz = a.z[p];
+  float4 pp = a.p[p];
+  pp.x = zz.x + bf * pp.x;
+  pp.y = zz.y + bf * pp.y;
+  pp.z = zz.z + bf * pp.z;
+  a.p[p] = pp;
+}
+
+// x += d (FP64)
+__global__ __launch_bounds__(PD_BLOCK) void k_mx_apply(MxArgs a) {
+  if (a.st->phase != 2) return;
+  const int p = xcd_tile(blockIdx.x, gridDim.x) * PD_BLOCK + threadIdx.x;
+  if (p >= a.n) return;
+  const float4 dd = a.d[p];
+  double4 xx = a.x[p];
+  xx.x = xx.x + (double)dd.x;
+  xx.y = xx.y + (double)dd.y;
+  xx.z = xx.z + (double)dd.z;
+  a.x[p] = xx;
+}
+
+__global__ __launch_bounds__(PD_BLOCK) void k_spmv_wide(MxArgs a, const double4* __restrict__ x,
+                                                       double4* __restrict__ y) {
+  const int p = xcd_tile(blockIdx.x, gridDim.x) * PD_BLOCK + threadIdx.x;
+  if (p >= a.n) return;
+  double y0, y1, y2;
+  row_wide(a, x, p, y0, y1, y2);
+  y[p] = make_double4(y0, y1, y2, 0.0);
+}
+
+__global__ __launch_bounds__(PD_BLOCK) void k_spmv_f32(MxArgs a, const float4* __restrict__ x,
+                                                      float4* __restrict__ y) {
+  const int p = xcd_tile(blockIdx.x, gridDim.x) * PD_BLOCK + threadIdx.x;
+  if (p >= a.n) return;
+  float y0, y1, y2;
+  row_f32(a, x, p, y0, y1, y2);
+  y[p] = make_float4(y0, y1, y2, 0.0f);
+}
+
+inline int tiles_of(int n) { return (n + PD_BLOCK - 1) / PD_BLOCK; }
+
+}  // namespace
+
+void launch_mx_setup(pd_mx_state* st, double rtol, int32_t max_iter, double inner_rtol,
+                     int32_t inner_max, int32_t max_refine, hipStream_t s) {
+  hipLaunchKernelGGL(k_mx_setup, dim3(1), dim3(64), 0, s, st, rtol, max_iter, inner_rtol,
+                     inner_max, max_refine);
+}
+void launch_mx_resid(const MxArgs& a, hipStream_t s) {
+  if (!a.n) return;
+  const int g = tiles_of(a.n);
+  hipLaunchKernelGGL(k_mx_resid, dim3(g), dim3(PD_BLOCK), 0, s, a);
+  hipLaunchKernelGGL(k_mx_resid_fold, dim3(1), dim3(PD_BLOCK), 0, s, a, g);
+}
+void launch_mx_spmv(const MxArgs& a, hipStream_t s) {
+  if (a.n) hipLaunchKernelGGL(k_mx_spmv, dim3(tiles_of(a.n)), dim3(PD_BLOCK), 0, s, a);
+}
+void launch_mx_update(const MxArgs& a, hipStream_t s) {
+  if (!a.n) return;
+  const int g = tiles_of(a.n);
+  hipLaunchKernelGGL(k_mx_alpha, dim3(1), dim3(PD_BLOCK), 0, s, a, g);
+  hipLaunchKernelGGL(k_mx_update, dim3(g), dim3(PD_BLOCK), 0, s, a);
+  hipLaunchKernelGGL(k_mx_beta, dim3(1), dim3(PD_BLOCK), 0, s, a, g);
+}
+void launch_mx_dir(const MxArgs& a, hipStream_t s) {
+  if (a.n) hipLaunchKernelGGL(k_mx_dir, dim3(tiles_of(a.n)), dim3(PD_BLOCK), 0, s, a);
+}
+void launch_mx_apply(const MxArgs& a, hipStream_t s) {
+  if (a.n) hipLaunchKernelGGL(k_mx_apply, dim3(tiles_of(a.n)), dim3(PD_BLOCK), 0, s, a);
+}
+void launch_spmv_wide(const MxArgs& a, const double4* x, double4* y, hipStream_t s) {
+  if (a.n) hipLaunchKernelGGL(k_spmv_wide, dim3(tiles_of(a.n)), dim3(PD_BLOCK), 0, s, a, x, y);
+}
+void launch_spmv_f32(const MxArgs& a, const float4* x, float4* y, hipStream_t s) {
+  if (a.n) hipLaunchKernelGGL(k_spmv_f32, dim3(tiles_of(a.n)), dim3(PD_BLOCK), 0, s, a, x, y);
+}
+
+}  // namespace pd

@@ -20,6 +20,9 @@ namespace pd {
 // is computed; the matrix values are written with non-temporal stores (they
 // are read back once per CG iteration, never soon enough to gain from L2).
 // ---------------------------------------------------------------------------
+// MIXED: the bond blocks, the diagonal and its inverse are also (or, for
+// the bonds and the inverse, only) written rounded to float.
+template <bool MIXED>
 __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int p = tile * PD_BLOCK + threadIdx.x;
@@ -120,13 +123,23 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
       k11 += F11;
       k12 += F12;
       k22 += F22;
-      double* vo = a.val + slot * 6 * 64 + lane;
-      __builtin_nontemporal_store((F00 * -1.0) * a.dt2, vo + 0 * 64);
-      __builtin_nontemporal_store((F01 * -1.0) * a.dt2, vo + 1 * 64);
-      __builtin_nontemporal_store((F02 * -1.0) * a.dt2, vo + 2 * 64);
-      __builtin_nontemporal_store((F11 * -1.0) * a.dt2, vo + 3 * 64);
-      __builtin_nontemporal_store((F12 * -1.0) * a.dt2, vo + 4 * 64);
-      __builtin_nontemporal_store((F22 * -1.0) * a.dt2, vo + 5 * 64);
+      if constexpr (MIXED) {
+        float* vo = a.val32 + slot * 6 * 64 + lane;
+        __builtin_nontemporal_store((float)((F00 * -1.0) * a.dt2), vo + 0 * 64);
+        __builtin_nontemporal_store((float)((F01 * -1.0) * a.dt2), vo + 1 * 64);
+        __builtin_nontemporal_store((float)((F02 * -1.0) * a.dt2), vo + 2 * 64);
+        __builtin_nontemporal_store((float)((F11 * -1.0) * a.dt2), vo + 3 * 64);
+        __builtin_nontemporal_store((float)((F12 * -1.0) * a.dt2), vo + 4 * 64);
+        __builtin_nontemporal_store((float)((F22 * -1.0) * a.dt2), vo + 5 * 64);
+      } else {
+        double* vo = a.val + slot * 6 * 64 + lane;
+        __builtin_nontemporal_store((F00 * -1.0) * a.dt2, vo + 0 * 64);
+        __builtin_nontemporal_store((F01 * -1.0) * a.dt2, vo + 1 * 64);
+        __builtin_nontemporal_store((F02 * -1.0) * a.dt2, vo + 2 * 64);
+        __builtin_nontemporal_store((F11 * -1.0) * a.dt2, vo + 3 * 64);
+        __builtin_nontemporal_store((F12 * -1.0) * a.dt2, vo + 4 * 64);
+        __builtin_nontemporal_store((F22 * -1.0) * a.dt2, vo + 5 * 64);
+      }
       ++bnd;
     }
     // A_pp = K_pp dt^2 + m I
@@ -143,6 +156,15 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
     dgo[3 * 64] = d11;
     dgo[4 * 64] = d12;
     dgo[5 * 64] = d22;
+    if constexpr (MIXED) {
+      float* dgf = a.diag32 + (int64_t)s * 6 * 64 + lane;
+      dgf[0 * 64] = (float)d00;
+      dgf[1 * 64] = (float)d01;
+      dgf[2 * 64] = (float)d02;
+      dgf[3 * 64] = (float)d11;
+      dgf[4 * 64] = (float)d12;
+      dgf[5 * 64] = (float)d22;
+    }
     // block-Jacobi inverse by cofactors (six distinct entries)
     {
       const double c00 = d11 * d22 - d12 * d12;
@@ -153,13 +175,23 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
       const double c22 = d00 * d11 - d01 * d01;
       const double det = (d00 * c00 + d01 * c01) + d02 * c02;
       const double id = 1.0 / det;
-      double* mo = a.minv + (int64_t)s * 6 * 64 + lane;
-      mo[0 * 64] = c00 * id;
-      mo[1 * 64] = c01 * id;
-      mo[2 * 64] = c02 * id;
-      mo[3 * 64] = c11 * id;
-      mo[4 * 64] = c12 * id;
-      mo[5 * 64] = c22 * id;
+      if constexpr (MIXED) {
+        float* mo = a.minv32 + (int64_t)s * 6 * 64 + lane;
+        mo[0 * 64] = (float)(c00 * id);
+        mo[1 * 64] = (float)(c01 * id);
+        mo[2 * 64] = (float)(c02 * id);
+        mo[3 * 64] = (float)(c11 * id);
+        mo[4 * 64] = (float)(c12 * id);
+        mo[5 * 64] = (float)(c22 * id);
+      } else {
+        double* mo = a.minv + (int64_t)s * 6 * 64 + lane;
+        mo[0 * 64] = c00 * id;
+        mo[1 * 64] = c01 * id;
+        mo[2 * 64] = c02 * id;
+        mo[3 * 64] = c11 * id;
+        mo[4 * 64] = c12 * id;
+        mo[5 * 64] = c22 * id;
+      }
     }
     f0 *= mt.c_bond;
     f1 *= mt.c_bond;
@@ -225,7 +257,11 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
 
 void launch_assemble(const AsmArgs& a, hipStream_t s) {
   const int nb = (a.n + PD_BLOCK - 1) / PD_BLOCK;
-  if (nb) hipLaunchKernelGGL(k_assemble, dim3(nb), dim3(PD_BLOCK), 0, s, a);
+  if (!nb) return;
+  if (a.mixed)
+    hipLaunchKernelGGL(k_assemble<true>, dim3(nb), dim3(PD_BLOCK), 0, s, a);
+  else
+    hipLaunchKernelGGL(k_assemble<false>, dim3(nb), dim3(PD_BLOCK), 0, s, a);
 }
 
 __global__ void k_reset_counters(pd_step_counters* c) {
@@ -428,13 +464,6 @@ __device__ __forceinline__ void precond(const double* minv, int pc, int p, doubl
 __device__ __forceinline__ double term3(double a0, double a1, double a2, double b0,
                                         double b1, double b2) {
   return (a0 * b0 + a1 * b1) + a2 * b2;
-}
-
-// Final fold of one quantity over the tiles (orc_dot's final_combine).
-__device__ double fold_partials(const double* part, int ntiles, double* lds4) {
-  double acc = 0.0;
-  for (int k = threadIdx.x; k < ntiles; k += PD_BLOCK) acc = acc + part[k];
-  return tile_sum(acc, lds4);
 }
 
 __global__ void k_cg_setup(pd_cg_state* st, double rtol, int max_iter) {

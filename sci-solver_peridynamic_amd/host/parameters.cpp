@@ -44,6 +44,9 @@ RunningParameters::RunningParameters()
       cg_preconditioner(0),
       cg_relative_tolerance(CG_RELATIVE_TOLERANCE),
       cg_max_iterations(MAX_CG_ITERATION),
+      cg_precision(0),
+      cg_inner_rtol(0.0),
+      cg_max_refine(0),
       scene_lattice(0),
       lattice_nx(0),
       lattice_ny(0),
@@ -178,6 +181,9 @@ void SystemParameter::loadParametersFromFile(const char* _paramFile) {
       r.cg_preconditioner = (value == "block_jacobi" || value == "1") ? 1 : 0;
     if (name == "cg_relative_tolerance") r.cg_relative_tolerance = atof(v);
     if (name == "cg_max_iterations") r.cg_max_iterations = atoi(v);
+    if (name == "cg_precision") r.cg_precision = (value == "mixed" || value == "1") ? 1 : 0;
+    if (name == "cg_inner_rtol") r.cg_inner_rtol = atof(v);
+    if (name == "cg_max_refine") r.cg_max_refine = atoi(v);
     if (name == "scene") r.scene_lattice = (value == "lattice") ? 1 : 0;
     if (name == "lattice_nx") r.lattice_nx = atoi(v);
     if (name == "lattice_ny") r.lattice_ny = atoi(v);

@@ -127,6 +127,9 @@ struct RunningParameters {
   int cg_preconditioner;        // 0 identity (reference), 1 block Jacobi
   double cg_relative_tolerance; // CG_RELATIVE_TOLERANCE
   int cg_max_iterations;        // MAX_CG_ITERATION
+  int cg_precision;             // 0 FP64, 1 float inner solve + FP64 refinement
+  double cg_inner_rtol;         // precision 1: inner reduction (0: 1e-5)
+  int cg_max_refine;            // precision 1: refinement limit (0: 20)
   int scene_lattice;            // 1: box lattice scene instead of a mesh fill
   int lattice_nx, lattice_ny, lattice_nz;
   double lattice_spacing;       // [m]

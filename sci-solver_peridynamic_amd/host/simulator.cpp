@@ -76,6 +76,10 @@ pd_step_params Simulator::stepParams() const {
   p.contact_cell = r.contact_radius;
   p.track_gap = 0;
   p.timing = 1;
+  p.precision = r.cg_precision;
+  p.inner_rtol = r.cg_inner_rtol;
+  p.max_refine = r.cg_max_refine;
+  p.inner_max_iter = 0;
   return p;
 }
 

@@ -58,6 +58,8 @@ class StepParams(C.Structure):
         ("contact_radius", C.c_double), ("contact_stiffness", C.c_double),
         ("contact_volume", C.c_double), ("contact_cell", C.c_double),
         ("track_gap", C.c_int32), ("timing", C.c_int32),
+        ("precision", C.c_int32), ("max_refine", C.c_int32),
+        ("inner_rtol", C.c_double), ("inner_max_iter", C.c_int32),
     ]
 
 
@@ -70,6 +72,8 @@ class StepStats(C.Structure):
         ("ms_contact", C.c_double), ("ms_assemble", C.c_double), ("ms_solve", C.c_double),
         ("ms_update", C.c_double), ("ms_spmv_total", C.c_double),
         ("spmv_launches", C.c_int32), ("host_syncs", C.c_int32),
+        ("refinements", C.c_int32), ("ms_resid_total", C.c_double),
+        ("resid_launches", C.c_int32),
     ]
 
     def as_dict(self):
@@ -85,13 +89,15 @@ DEFAULT_PARAMS = dict(dt=1e-4, clock_scale=1.0, gravity=9.8, integrator=0, preco
                       rtol=1e-6, max_iter=100000, walls=0, bmin=(-1e30,) * 3,
                       bmax=(1e30,) * 3, wall_radius=0.0, restitution=0.001,
                       contact_radius=0.0, contact_stiffness=0.0, contact_volume=1.0,
-                      contact_cell=0.0, track_gap=0, timing=0)
+                      contact_cell=0.0, track_gap=0, timing=0, precision=0, max_refine=20,
+                      inner_rtol=1e-5)
 
 
 def make_params(**kw) -> StepParams:
     p = StepParams()
     d = dict(DEFAULT_PARAMS)
     d.update(kw)
+    d.setdefault("inner_max_iter", d["max_iter"])
     for k, v in d.items():
         if k in ("bmin", "bmax"):
             getattr(p, k)[:] = [float(t) for t in v]
@@ -351,7 +357,9 @@ def step_kwargs(desc, step):
                 bmax=desc["bmax"], wall_radius=desc["wall_radius"],
                 restitution=desc["restitution"], contact_radius=desc["contact_radius"],
                 contact_stiffness=desc["contact_stiffness"],
-                contact_volume=desc["contact_volume"], contact_cell=desc["contact_radius"])
+                contact_volume=desc["contact_volume"], contact_cell=desc["contact_radius"],
+                precision=desc.get("precision", 0), inner_rtol=desc.get("inner_rtol", 1e-5),
+                max_refine=desc.get("max_refine", 20), inner_max_iter=desc["max_iter"])
 
 
 def materials_of(desc):

@@ -57,3 +57,21 @@ def config0_state(nx=20, ny=20, nz=20, h=0.01, jitter=0.05, seed=1973, top_speed
     vbc[k == nz - 1, 2] = top_speed
     v4 = vbc.copy()
     return x4, bc, vbc, v4
+
+
+def deformed_pair(nx=10, ny=9, nz=8, amp=0.03, seed=0, s_fixed=0.02):
+    """Oracle / GPU pair on a jittered lattice with random velocities and a
+    random displacement of amp * spacing; stretch limits set to s_fixed."""
+    h = 0.01
+    x4 = lattice(nx, ny, nz, h, 0.05, seed)
+    rng = np.random.default_rng(seed + 1)
+    v4 = np.zeros_like(x4)
+    v4[:, :3] = rng.uniform(-1, 1, (x4.shape[0], 3))
+    sys_, ctx = make_pair(x4, [material(h)], v4=v4)
+    disp = rng.uniform(-1, 1, (x4.shape[0], 3)) * amp * h
+    sys_.x[:, :3] += disp
+    ctx.upload("position", sys_.x)
+    if s_fixed is not None:
+        sys_.s[:] = s_fixed
+        ctx.upload("stretch", sys_.s)
+    return sys_, ctx

@@ -49,16 +49,21 @@ def setup_pair(cfg_path):
     return desc, sc, S, ctx
 
 
-@pytest.mark.parametrize("cfg,steps", [("tension_20.txt", 100), ("impact_30.txt", 60)])
-def test_config_run_parity(cfg, steps):
-    say("case", cfg, steps)
+@pytest.mark.parametrize("cfg,steps,precision", [("tension_20.txt", 100, 0),
+                                                 ("impact_30.txt", 60, 0),
+                                                 ("tension_20.txt", 100, 1),
+                                                 ("impact_30.txt", 60, 1)])
+def test_config_run_parity(cfg, steps, precision):
+    """precision 1 runs the mixed-precision solver in both the oracle and
+    the engine: the same bar applies."""
+    say("case", cfg, steps, "precision", precision)
     desc, sc, S, ctx = setup_pair(os.path.join(CFG, cfg))
     first_near = None
     worst_rel = 0.0
     worst_rel_disp = 0.0
     exact_steps = 0
     for k in range(1, steps + 1):
-        kw = pds.step_kwargs(desc, k)
+        kw = dict(pds.step_kwargs(desc, k), precision=precision)
         so = S.step(orc.make_params(**kw))
         sg = ctx.step(pds.make_params(track_gap=1, **kw))
         xg = ctx.download("position")

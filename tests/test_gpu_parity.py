@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 import oracle as orc
-from pdhelpers import config0_state, lattice, make_pair, make_system, material
+from pdhelpers import config0_state, deformed_pair, lattice, make_pair, make_system, material
 
 pytestmark = pytest.mark.gpu
 
@@ -79,22 +79,6 @@ def test_bond_overflow_is_an_error():
         ctx.build_bonds()
     assert e.value.code == pds.PD_ERR_BOND_OVERFLOW
     ctx.close()
-
-
-def deformed_pair(nx=10, ny=9, nz=8, amp=0.03, seed=0, s_fixed=0.02):
-    h = 0.01
-    x4 = lattice(nx, ny, nz, h, 0.05, seed)
-    rng = np.random.default_rng(seed + 1)
-    v4 = np.zeros_like(x4)
-    v4[:, :3] = rng.uniform(-1, 1, (x4.shape[0], 3))
-    sys_, ctx = make_pair(x4, [material(h)], v4=v4)
-    disp = rng.uniform(-1, 1, (x4.shape[0], 3)) * amp * h
-    sys_.x[:, :3] += disp
-    ctx.upload("position", sys_.x)
-    if s_fixed is not None:
-        sys_.s[:] = s_fixed
-        ctx.upload("stretch", sys_.s)
-    return sys_, ctx
 
 
 @pytest.mark.parametrize("integrator", [0, 1])
