@@ -106,6 +106,10 @@ typedef struct pd_step_params {
   int32_t max_refine;     /* precision 1: refinement limit (0: 20) */
   double inner_rtol;      /* precision 1: inner reduction (0: 1e-5) */
   int32_t inner_max_iter; /* precision 1: per refinement (0: max_iter) */
+  /* contact pairs: 0 particles of different bodies; 1 every pair closer
+   * than contact_radius, same body included (the pair set of the
+   * reference's collidePDParticles, core/simulator.cuh:672-765) */
+  int32_t contact_mode;
 } pd_step_params;
 
 typedef struct pd_step_stats {

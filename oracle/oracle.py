@@ -55,6 +55,7 @@ class StepParams(C.Structure):
         ("contact_volume", C.c_double), ("contact_cell", C.c_double),
         ("precision", C.c_int32), ("inner_rtol", C.c_double),
         ("inner_max_iter", C.c_int32), ("max_refine", C.c_int32),
+        ("contact_mode", C.c_int32),
     ]
 
 
@@ -194,7 +195,8 @@ def make_params(**kw) -> StepParams:
                     rtol=1e-6, max_iter=100000, walls=0, bmin=(-1e30,) * 3,
                     bmax=(1e30,) * 3, wall_radius=0.0, restitution=0.001,
                     contact_radius=0.0, contact_stiffness=0.0, contact_volume=1.0,
-                    contact_cell=0.0, precision=0, inner_rtol=1e-5, max_refine=20)
+                    contact_cell=0.0, precision=0, inner_rtol=1e-5, max_refine=20,
+                    contact_mode=0)
     defaults.update(kw)
     defaults.setdefault("inner_max_iter", defaults["max_iter"])
     for k, v in defaults.items():

@@ -1098,7 +1098,7 @@ void orc_contact(orc_system* sys, const orc_step_params* prm) {
           for (int64_t m = start[cell]; m < start[cell + 1]; ++m) {
             const int32_t q = sorted[m];
             if (q == p) continue;
-            if (sys->body[q] == sys->body[p]) continue;
+            if (prm->contact_mode == 0 && sys->body[q] == sys->body[p]) continue;
             if (sys->act[q] == ORC_INVISIBLE) continue;
             const double* xq = sys->x + 4 * (int64_t)q;
             const double pq[4] = {pp[0] - xq[0], pp[1] - xq[1], pp[2] - xq[2],

@@ -134,6 +134,7 @@ typedef struct orc_step_params {
   double inner_rtol;        /* inner stop: ||r32|| <= max(inner_rtol ||r||, tol/2) */
   int32_t inner_max_iter;   /* per refinement */
   int32_t max_refine;       /* outer FP64 refinements */
+  int32_t contact_mode;     /* 0 pairs of different bodies, 1 all pairs */
 } orc_step_params;
 
 typedef struct orc_step_stats {
@@ -176,7 +177,9 @@ double orc_dot_f32(int32_t n, const float* a4, const float* b4);
 /* core/implicit_euler.cuh:90-176 / core/newmark_beta.cuh:89-199 */
 void orc_update(orc_system* sys, const orc_step_params* prm);
 /* core/simulator.cuh:602-768 collidePDParticles restated on a uniform
- * grid: fc[p] = sum over other-body q with r < Rc of Kc (r-Rc)^2/V (p-q)/r */
+ * grid: fc[p] = sum over q != p with r < Rc of Kc (r-Rc)^2/V (p-q)/r, q of
+ * another body (contact_mode 0) or any (contact_mode 1, the reference's
+ * pair set) */
 void orc_contact(orc_system* sys, const orc_step_params* prm);
 /* core/simulator.cu:297-467: contact, assemble, s <- s_new, solve,
  * v <- sol, update.  Returns 0, or 1 if CG did not converge. */

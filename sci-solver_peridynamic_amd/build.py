@@ -34,7 +34,8 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-function", f"-I{INCLUDE}", f"-I{CSRC}",
 ]
 HIP_SOURCES = ["pd_step.hip", "pd_bonds.hip", "pd_mixed.hip", "pd_api.hip"]
-HOST_SOURCES = ["parameters.cpp", "scene.cpp", "simulator.cpp", "data_io.cpp", "host_capi.cpp"]
+HOST_SOURCES = ["parameters.cpp", "scene.cpp", "mesh_fill.cpp", "simulator.cpp", "data_io.cpp",
+                "host_capi.cpp"]
 CXX = os.environ.get("CXX", "g++")
 CXX_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-Wall", f"-I{INCLUDE}", f"-I{HOST}"]
 

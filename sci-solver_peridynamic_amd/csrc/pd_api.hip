@@ -199,6 +199,8 @@ int check_params(pd_ctx* c, const pd_step_params* prm) {
     return fail(c, PD_ERR_ARG, "contact_cell must be >= contact_radius");
   if (prm->contact_radius > 0.0 && !(prm->contact_volume != 0.0))
     return fail(c, PD_ERR_ARG, "contact_volume must be non-zero");
+  if (prm->contact_mode != 0 && prm->contact_mode != 1)
+    return fail(c, PD_ERR_ARG, "contact_mode must be 0 (between bodies) or 1 (all pairs)");
   if (prm->precision != 0 && prm->precision != 1)
     return fail(c, PD_ERR_ARG, "precision must be 0 (FP64) or 1 (float inner solve + FP64 refinement)");
   if (prm->precision == 1 && (prm->inner_rtol < 0.0 || prm->max_refine < 0 || prm->inner_max_iter < 0))
@@ -404,6 +406,21 @@ int enqueue_contact(pd_ctx* c, const pd_step_params* prm) {
     if (!(isfinite(mn[k]) && isfinite(mx[k])))
       return fail(c, PD_ERR_ARG, "non-finite particle position");
   const double cell = prm->contact_cell;
+  if (prm->contact_mode == 1) {
+    // every pair: the grid over all particles
+    pd::GridView g = {};
+    int32_t c0[3] = {0, 0, 0}, dims[3];
+    for (int k = 0; k < 3; ++k) {
+      const double span = floor((mx[k] - mn[k]) / cell) + 1.0;
+      if (span > 2.0e9) return fail(c, PD_ERR_ARG, "contact grid too large for contact_cell");
+      dims[k] = (int32_t)span;
+    }
+    if (int e = grid_over(c, cell, mn, c0, dims, nullptr, c->n, &g)) return e;
+    pd::launch_contact(c->n, c->x, c->body, c->act, g, nullptr, c->n, prm->contact_radius,
+                       prm->contact_stiffness, prm->contact_volume, 1, c->fc, c->stream);
+    CHECK_LAUNCH(c, "contact");
+    return PD_OK;
+  }
   // a small widening keeps pairs at a distance rounding to Rc inside
   const double w = prm->contact_radius * (1.0 + 1e-6);
   pd::ZoneBoxes z;
@@ -439,7 +456,7 @@ int enqueue_contact(pd_ctx* c, const pd_step_params* prm) {
     if (int e = grid_over(c, cell, mn, c0, dims, c->cand, (int32_t)m, &g)) return e;
   }
   pd::launch_contact(c->n, c->x, c->body, c->act, g, c->cand, (int32_t)m, prm->contact_radius,
-                     prm->contact_stiffness, prm->contact_volume, c->fc, c->stream);
+                     prm->contact_stiffness, prm->contact_volume, 0, c->fc, c->stream);
   CHECK_LAUNCH(c, "contact");
   return PD_OK;
 }

@@ -142,12 +142,12 @@ void launch_update(int32_t n, double4* x, double4* v, const double4* sol,
                    const int32_t* act, int32_t integrator, double dt, int32_t walls,
                    const double* lo, const double* hi, double restitution,
                    hipStream_t s);
-// contact forces of the m candidates cand[0..m); fc of every other particle
-// is set to zero
+// contact forces of the m candidates cand[0..m) (particles 0..m when cand
+// is null); fc of every other particle is set to zero
 void launch_contact(int32_t n, const double4* x, const int32_t* body,
                     const int32_t* act, const GridView& g, const int32_t* cand,
-                    int32_t m, double Rc, double kc, double vol, double4* fc,
-                    hipStream_t s);
+                    int32_t m, double Rc, double kc, double vol, int all_pairs,
+                    double4* fc, hipStream_t s);
 void launch_reset_counters(pd_step_counters* c, hipStream_t s);
 
 // Per-body boxes, already widened by the contact radius: particle p is a

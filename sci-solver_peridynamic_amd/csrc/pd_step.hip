@@ -766,7 +766,8 @@ void launch_update(int32_t n, double4* x, double4* v, const double4* sol,
 // ---------------------------------------------------------------------------
 // Contact: core/simulator.cuh:679-766 collidePDParticles on the uniform grid
 // (27 cells in z, y, x order; members in ascending index, as the stable
-// sort leaves them), between particles of different bodies.
+// sort leaves them), between particles of different bodies or, with
+// all_pairs, between any two particles.
 // ---------------------------------------------------------------------------
 struct ContactArgs {
   int n;
@@ -777,6 +778,7 @@ struct ContactArgs {
   const int32_t* cand;
   int m;
   double Rc, kc, vol;
+  int all_pairs;
   double4* fc;
 };
 
@@ -788,7 +790,7 @@ __global__ __launch_bounds__(PD_BLOCK) void k_zero4(double4* v, int n) {
 __global__ __launch_bounds__(PD_BLOCK) void k_contact(ContactArgs a) {
   const int i = blockIdx.x * PD_BLOCK + threadIdx.x;
   if (i >= a.m) return;
-  const int p = a.cand[i];
+  const int p = a.cand ? a.cand[i] : i;
   if (a.act[p] == PD_INVISIBLE) return;  // fc[p] stays zero
   const double4 pp = a.x[p];
   const int bp = a.body[p];
@@ -808,7 +810,7 @@ __global__ __launch_bounds__(PD_BLOCK) void k_contact(ContactArgs a) {
         for (int m = a.g.cell_start[cell]; m < e; ++m) {
           const int q = a.g.sorted_idx[m];
           if (q == p) continue;
-          if (a.body[q] == bp) continue;
+          if (!a.all_pairs && a.body[q] == bp) continue;
           if (a.act[q] == PD_INVISIBLE) continue;
           const double4 xq = a.x[q];
           const double q0 = pp.x - xq.x, q1 = pp.y - xq.y, q2 = pp.z - xq.z;
@@ -828,7 +830,7 @@ __global__ __launch_bounds__(PD_BLOCK) void k_contact(ContactArgs a) {
 
 void launch_contact(int32_t n, const double4* x, const int32_t* body, const int32_t* act,
                     const GridView& g, const int32_t* cand, int32_t m, double Rc, double kc,
-                    double vol, double4* fc, hipStream_t s) {
+                    double vol, int all_pairs, double4* fc, hipStream_t s) {
   const int nb = (n + PD_BLOCK - 1) / PD_BLOCK;
   if (nb) hipLaunchKernelGGL(k_zero4, dim3(nb), dim3(PD_BLOCK), 0, s, fc, n);
   if (m <= 0) return;
@@ -843,6 +845,7 @@ void launch_contact(int32_t n, const double4* x, const int32_t* body, const int3
   a.Rc = Rc;
   a.kc = kc;
   a.vol = vol;
+  a.all_pairs = all_pairs;
   a.fc = fc;
   const int mb = (m + PD_BLOCK - 1) / PD_BLOCK;
   hipLaunchKernelGGL(k_contact, dim3(mb), dim3(PD_BLOCK), 0, s, a);

@@ -7,6 +7,8 @@
 #include <string>
 #include <vector>
 
+#include "exact_predicates.h"
+#include "mesh_fill.h"
 #include "parameters.h"
 #include "scene.h"
 
@@ -60,7 +62,7 @@ int pdh_describe(const char* params_path, char* buf, int32_t len) {
       "\"contact_stiffness\": %.17g, \"contact_volume\": %.17g, \"s0\": %.9g, "
       "\"stretch_seed\": %u, \"final_step\": %d, \"adaptive_integration\": %d, "
       "\"scale_factor\": %.17g, \"projectile_spacing\": %.17g, \"precision\": %d, "
-      "\"inner_rtol\": %.17g, \"max_refine\": %d}",
+      "\"inner_rtol\": %.17g, \"max_refine\": %d, \"contact_mode\": %d}",
       s.num_pd_particle, scene_ok ? 1 : 0, s.num_pd_bodies, s.pd_horizon, s.pd_particle_radius,
       s.pd_c_bond, s.pd_particle_mass, s.projectile_horizon, s.projectile_particle_radius,
       s.projectile_c_bond, s.pd_particle_mass, dt, s.pd_gravity, s.pd_phase_in,
@@ -70,7 +72,7 @@ int pdh_describe(const char* params_path, char* buf, int32_t len) {
       r.contact_radius, r.contact_stiffness, r.contact_volume, (double)r.pd_stretch_limit_s0,
       r.stretch_seed, r.final_step, r.adaptive_integration, s.scaleFactor, s.projectile_spacing,
       r.cg_precision, r.cg_inner_rtol > 0.0 ? r.cg_inner_rtol : 1e-5,
-      r.cg_max_refine > 0 ? r.cg_max_refine : 20);
+      r.cg_max_refine > 0 ? r.cg_max_refine : 20, r.contact_mode);
   if (w < 0 || w >= len || !buf) return -1;
   memcpy(buf, tmp, (size_t)w + 1);
   return w;
@@ -84,6 +86,40 @@ void pdh_stretch_init(int32_t n, float s0, uint32_t seed, double* out) {
     for (int k = 0; k < 12; ++k) sum += rand() / (double)RAND_MAX;
     sum -= 6.0;
     out[i] = ((sum * 0.01 + 1.0) * (double)s0);
+  }
+}
+
+// The OBJ reader of mesh-filled scenes.  With null arrays only the counts
+// are returned.  Returns 0 or -1 (message on stderr).
+int pdh_load_obj(const char* path, int32_t* nv, int32_t* nf, float* v, int32_t* f) {
+  ObjMesh m;
+  std::string err;
+  if (!load_obj(path, m, err)) {
+    fprintf(stderr, "%s\n", err.c_str());
+    return -1;
+  }
+  *nv = (int32_t)(m.v.size() / 3);
+  *nf = (int32_t)(m.f.size() / 3);
+  if (v) memcpy(v, m.v.data(), m.v.size() * sizeof(float));
+  if (f) memcpy(f, m.f.data(), m.f.size() * sizeof(int32_t));
+  return 0;
+}
+
+// Exact point-in-mesh test of n points against a triangle mesh.
+void pdh_mesh_inside(int32_t nv, const double* xyz, int32_t nt, const int32_t* tri, int32_t n,
+                     const double* pts, int32_t* out) {
+  const std::vector<double> x(xyz, xyz + 3 * (size_t)nv);
+  const std::vector<int> t(tri, tri + 3 * (size_t)nt);
+  MeshInside m(x, t);
+  for (int32_t i = 0; i < n; ++i) out[i] = m.inside(pts + 3 * (size_t)i) ? 1 : 0;
+}
+
+// Signs of orient2d(a, b, c) and orient3d(a, b, c, d) for n point tuples.
+void pdh_orient(int32_t n, const double* a, const double* b, const double* c, const double* d,
+                int32_t* o2, int32_t* o3) {
+  for (int32_t i = 0; i < n; ++i) {
+    o2[i] = exact::orient2d(a + 3 * i, b + 3 * i, c + 3 * i);
+    o3[i] = exact::orient3d(a + 3 * i, b + 3 * i, c + 3 * i, d + 3 * i);
   }
 }
 

@@ -68,6 +68,7 @@ RunningParameters::RunningParameters()
       contact_radius(0.0),
       contact_stiffness(0.0),
       contact_volume(1.0),
+      contact_mode(0),
       walls(-1),
       stretch_seed(1973u),
       print_steps(0) {
@@ -205,6 +206,7 @@ void SystemParameter::loadParametersFromFile(const char* _paramFile) {
     if (name == "contact_radius") r.contact_radius = atof(v);
     if (name == "contact_stiffness") r.contact_stiffness = atof(v);
     if (name == "contact_volume") r.contact_volume = atof(v);
+    if (name == "contact_pairs") r.contact_mode = (value == "all" || value == "1") ? 1 : 0;
     if (name == "walls") r.walls = parse_bool_int(value) ? 1 : 0;
     if (name == "stretch_seed") r.stretch_seed = (unsigned)strtoul(v, nullptr, 10);
     if (name == "print_steps") r.print_steps = atoi(v);

@@ -147,6 +147,7 @@ struct RunningParameters {
   double contact_radius;        // Rc [m]; 0 disables contact
   double contact_stiffness;     // Kc
   double contact_volume;        // divisor of Kc (PD_V in the reference)
+  int contact_mode;             // 0 between bodies, 1 all pairs (contact_pairs all)
   int walls;                    // 1: clamp to the boundary box (-1: mesh 1, lattice 0)
   unsigned stretch_seed;        // srand seed of getstretch (1973)
   int print_steps;              // 1: one JSON line per step on stdout

@@ -1,4 +1,13 @@
-// scene.cpp -- box-lattice specimens and a seeded spherical projectile.
+// scene.cpp -- mesh-filled scenes, box-lattice specimens and a seeded
+// spherical projectile.
+//
+// Mesh (examples/scene.cpp:18-136, 360-395, 530-598): the OBJ mesh is
+// scaled to half of 0.9 x the smallest extent of the boundary box over its
+// largest extent, its bounding box moved to the boundary minimum, and the
+// lattice of spacing 2r over that box (x fastest) keeps the points inside
+// the mesh, clamped to the box; then the particles are rotated and moved
+// by mesh_translation / scaleFactor.  The float roundings of the reference
+// (float mesh coordinates and box, fminf / fmaxf) are kept.
 //
 // Specimen: nx x ny x nz simple-cubic lattice, x fastest (the reference's
 // fill order, examples/scene.cpp:536-540), spacing h, each coordinate
@@ -19,6 +28,8 @@
 
 #include <algorithm>
 
+#include "mesh_fill.h"
+
 double scene_uniform(unsigned seed, unsigned long long index) {
   uint64_t z = (uint64_t)seed * 0x9E3779B97F4A7C15ull + index;
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
@@ -31,7 +42,16 @@ Scene::Scene(SimulationParameters& simParams, RunningParameters& runningParams,
              ParticleArrangement arrangement)
     : simParams_(simParams), runningParams_(runningParams), arrangement_(arrangement) {
   if (!runningParams_.scene_lattice) {
-    err_ = "mesh-filled scenes are not available in this engine; use 'scene lattice'";
+    if (simParams_.num_pd_particle == 0) {  // examples/scene.cpp:25-28
+      ok_ = true;
+      return;
+    }
+    if (!buildMesh()) return;
+    simParams_.num_pd_particle = (int)pos_.size();
+    simParams_.num_total_particle = simParams_.num_sph_particle + simParams_.num_pd_particle;
+    simParams_.num_clists = simParams_.num_pd_particle;
+    while (simParams_.num_clists % 8 != 0) simParams_.num_clists++;
+    ok_ = true;
     return;
   }
   const RunningParameters& r = runningParams_;
@@ -181,4 +201,98 @@ void Scene::initPeridynamicsBoundary(int* pd_body, int* pd_bc, real4_t* pd_bc_ve
     if (pd_bc) pd_bc[p] = bc_[p];
     if (pd_bc_velocity) pd_bc_velocity[p] = vbc_[p];
   }
+}
+
+// examples/scene.cpp:18-136 (scaling, grid) and 530-598 (fill).
+bool Scene::buildMesh() {
+  const SimulationParameters& sp = simParams_;
+  ObjMesh mesh;
+  std::string e;
+  if (!load_obj(runningParams_.obj_file, mesh, e)) {
+    err_msg_ = e;
+    err_ = err_msg_.c_str();
+    return false;
+  }
+  const size_t nv = mesh.v.size() / 3;
+  // cyTriMesh::ComputeBoundingBox over all vertices, in float
+  float bmin[3] = {mesh.v[0], mesh.v[1], mesh.v[2]};
+  float bmax[3] = {mesh.v[0], mesh.v[1], mesh.v[2]};
+  for (size_t i = 1; i < nv; ++i)
+    for (int a = 0; a < 3; ++a) {
+      const float c = mesh.v[3 * i + a];
+      if (bmin[a] > c) bmin[a] = c;
+      if (bmax[a] < c) bmax[a] = c;
+    }
+  float dif[3];
+  for (int a = 0; a < 3; ++a) dif[a] = bmax[a] - bmin[a];
+  const double maxDiff = (double)fmaxf(fmaxf(fabsf(dif[0]), fabsf(dif[1])), fabsf(dif[2]));
+  const float ext = fminf(fminf((float)(sp.boundary_max_x - sp.boundary_min_x),
+                                (float)(sp.boundary_max_y - sp.boundary_min_y)),
+                          (float)(sp.boundary_max_z - sp.boundary_min_z));
+  double scale = (double)ext * 0.9 / maxDiff;
+  scale *= 0.5f;
+  const double bnd_min[3] = {sp.boundary_min_x, sp.boundary_min_y, sp.boundary_min_z};
+  double shift[3];
+  for (int a = 0; a < 3; ++a) {
+    bmin[a] = (float)((double)bmin[a] * scale);
+    bmax[a] = (float)((double)bmax[a] * scale);
+    shift[a] = bnd_min[a] - (double)bmin[a];
+    bmin[a] = (float)((double)bmin[a] + shift[a]);
+    bmax[a] = (float)((double)bmax[a] + shift[a]);
+  }
+  printf("    Bounding box for mesh object: [%g, %g, %g] -> [%g, %g, %g]\n", bmin[0], bmin[1],
+         bmin[2], bmax[0], bmax[1], bmax[2]);
+  const double r = sp.pd_particle_radius;
+  int grid[3];
+  for (int a = 0; a < 3; ++a) {
+    const float span = bmax[a] - bmin[a];
+    grid[a] = (int)ceil((double)span / (2.0f * r)) + 1;
+  }
+  printf("    Maximum Peridynamics grid: %dx%dx%d\n", grid[0], grid[1], grid[2]);
+  mesh_xyz_.resize(3 * nv);
+  for (size_t i = 0; i < nv; ++i)
+    for (int a = 0; a < 3; ++a) mesh_xyz_[3 * i + a] = (double)mesh.v[3 * i + a] * scale + shift[a];
+  mesh_tri_ = mesh.f;
+  MeshInside inside(mesh_xyz_, mesh_tri_);
+  const double jitter = arrangement_ == REGULAR_GRID ? 0.0 : kMeshJitter * r;
+  const double spacing = 2 * r;
+  pos_.clear();
+  for (int z = 0; z < grid[2]; ++z)
+    for (int y = 0; y < grid[1]; ++y)
+      for (int x = 0; x < grid[0]; ++x) {
+        // three rand() draws per lattice point, as the reference's frand()
+        double p[3];
+        const int idx[3] = {x, y, z};
+        for (int a = 0; a < 3; ++a) {
+          const double u = rand() / (double)RAND_MAX;
+          p[a] = (double)bmin[a] + idx[a] * spacing + (u * 2.0f - 1.0f) * jitter;
+        }
+        if (!inside.inside(p)) continue;
+        for (int a = 0; a < 3; ++a) {
+          p[a] = (p[a] < (double)bmin[a]) ? (double)bmin[a] : p[a];
+          p[a] = (p[a] > (double)bmax[a]) ? (double)bmax[a] : p[a];
+        }
+        pos_.push_back(make_real4(p[0], p[1], p[2], 0.0));
+      }
+  printf("    Total filled Peridynamics particles: %zu\n", pos_.size());
+  // examples/scene.cpp:360-395: rotation (zero angles) then translation
+  const double tr[3] = {(double)runningParams_.mesh_translation_x / sp.scaleFactor,
+                        (double)runningParams_.mesh_translation_y / sp.scaleFactor,
+                        (double)runningParams_.mesh_translation_z / sp.scaleFactor};
+  const double az = 0.0, el = 0.0, ro = 0.0;
+  const double sA = sin(az), cA = cos(az), sE = sin(el), cE = cos(el), sR = sin(ro), cR = cos(ro);
+  const double R[3][3] = {{cR * cA - sR * sA * sE, sR * cA + cR * sA * sE, -sA * cE},
+                          {-sR * cE, cR * cE, sE},
+                          {cR * sA + sR * cA * sE, sR * sA - cR * cA * sE, cA * cE}};
+  const double v0 = (double)runningParams_.pd_initial_velocity;
+  vel_.assign(pos_.size(), make_real4(v0, v0, v0, v0));
+  for (real4_t& q : pos_) {
+    double t[3];
+    for (int j = 0; j < 3; ++j) t[j] = R[j][0] * q.x + R[j][1] * q.y + R[j][2] * q.z;
+    q = make_real4(t[0] + tr[0], t[1] + tr[1], t[2] + tr[2], 0.0);
+  }
+  body_.assign(pos_.size(), 0);
+  bc_.assign(pos_.size(), 0);
+  vbc_.assign(pos_.size(), make_real4(0, 0, 0, 0));
+  return true;
 }

@@ -1,9 +1,11 @@
 // scene.h -- particle generation behind the reference's Scene interface
-// (examples/scene.h:28-68).  The reference fills an OBJ mesh; this engine
-// builds box-lattice specimens and an optional spherical projectile.
+// (examples/scene.h:28-68): an OBJ mesh filled with a lattice (the
+// reference's scene), or box-lattice specimens with an optional spherical
+// projectile (scene lattice).
 #ifndef PD_HOST_SCENE_H
 #define PD_HOST_SCENE_H
 
+#include <string>
 #include <vector>
 
 #include "definitions.h"
@@ -32,6 +34,9 @@ class Scene {
  private:
   void buildLattice();
   void buildProjectile();
+  bool buildMesh();
+
+  static constexpr double kMeshJitter = 0.01;  // JITTER, examples/scene.h:20
 
   SimulationParameters& simParams_;
   RunningParameters& runningParams_;
@@ -41,8 +46,11 @@ class Scene {
   std::vector<int> body_;
   std::vector<int> bc_;
   std::vector<real4_t> vbc_;
+  std::vector<double> mesh_xyz_;  // scaled mesh vertices (double)
+  std::vector<int> mesh_tri_;
   bool ok_ = false;
   const char* err_ = "";
+  std::string err_msg_;
 };
 
 // Counter-based uniform deviate in [0, 1): the splitmix64 finaliser of

@@ -74,6 +74,7 @@ pd_step_params Simulator::stepParams() const {
   p.contact_stiffness = r.contact_stiffness;
   p.contact_volume = r.contact_volume;
   p.contact_cell = r.contact_radius;
+  p.contact_mode = r.contact_mode;
   p.track_gap = 0;
   p.timing = 1;
   p.precision = r.cg_precision;

@@ -60,6 +60,7 @@ class StepParams(C.Structure):
         ("track_gap", C.c_int32), ("timing", C.c_int32),
         ("precision", C.c_int32), ("max_refine", C.c_int32),
         ("inner_rtol", C.c_double), ("inner_max_iter", C.c_int32),
+        ("contact_mode", C.c_int32),
     ]
 
 
@@ -90,7 +91,7 @@ DEFAULT_PARAMS = dict(dt=1e-4, clock_scale=1.0, gravity=9.8, integrator=0, preco
                       bmax=(1e30,) * 3, wall_radius=0.0, restitution=0.001,
                       contact_radius=0.0, contact_stiffness=0.0, contact_volume=1.0,
                       contact_cell=0.0, track_gap=0, timing=0, precision=0, max_refine=20,
-                      inner_rtol=1e-5)
+                      inner_rtol=1e-5, contact_mode=0)
 
 
 def make_params(**kw) -> StepParams:
@@ -310,13 +311,66 @@ def host_lib():
         L.pdh_describe.argtypes = [C.c_char_p, C.c_char_p, C.c_int32]
         L.pdh_describe.restype = C.c_int
         L.pdh_stretch_init.argtypes = [C.c_int32, C.c_float, C.c_uint32, vp]
+        L.pdh_load_obj.argtypes = [C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), vp, vp]
+        L.pdh_load_obj.restype = C.c_int
+        L.pdh_mesh_inside.argtypes = [C.c_int32, vp, C.c_int32, vp, C.c_int32, vp, vp]
+        L.pdh_orient.argtypes = [C.c_int32, vp, vp, vp, vp, vp, vp]
         _hlib = L
     return _hlib
 
 
-def describe(params_path):
+def load_obj(path):
+    """OBJ mesh as read by mesh-filled scenes: (float32 vertices [nv,3],
+    int32 triangles [nt,3])."""
+    L = host_lib()
+    nv, nf = C.c_int32(), C.c_int32()
+    if L.pdh_load_obj(path.encode(), C.byref(nv), C.byref(nf), None, None) != 0:
+        raise RuntimeError(f"cannot read {path}")
+    v = np.zeros((nv.value, 3), np.float32)
+    f = np.zeros((nf.value, 3), np.int32)
+    L.pdh_load_obj(path.encode(), C.byref(nv), C.byref(nf), _ptr(v), _ptr(f))
+    return v, f
+
+
+def mesh_inside(xyz, tri, pts):
+    """Exact point-in-mesh flags of pts [n,3] for the mesh (xyz [nv,3], tri [nt,3])."""
+    xyz = np.ascontiguousarray(xyz, np.float64)
+    tri = np.ascontiguousarray(tri, np.int32)
+    pts = np.ascontiguousarray(pts, np.float64)
+    out = np.zeros(len(pts), np.int32)
+    host_lib().pdh_mesh_inside(len(xyz), _ptr(xyz), len(tri), _ptr(tri), len(pts), _ptr(pts),
+                               _ptr(out))
+    return out
+
+
+def orient(a, b, c, d):
+    """Exact signs of orient2d(a, b, c) and orient3d(a, b, c, d) per row."""
+    a, b, c, d = (np.ascontiguousarray(t, np.float64) for t in (a, b, c, d))
+    o2 = np.zeros(len(a), np.int32)
+    o3 = np.zeros(len(a), np.int32)
+    host_lib().pdh_orient(len(a), _ptr(a), _ptr(b), _ptr(c), _ptr(d), _ptr(o2), _ptr(o3))
+    return o2, o3
+
+
+def with_obj(params_path, obj_file):
+    """A copy of the params file naming obj_file (the reference's driver
+    takes the OBJ path on its command line, examples/main.cpp:37)."""
+    import tempfile
+    fd, path = tempfile.mkstemp(suffix=".txt", prefix="params_")
+    with os.fdopen(fd, "w") as f, open(params_path) as g:
+        f.write(g.read().rstrip("\n") + f"\nobj_file {obj_file}\n")
+    return path
+
+
+def describe(params_path, obj_file=None):
     """Derived run settings of a params file (materials, dt, solver keys)."""
     import json
+    if obj_file is not None:
+        tmp = with_obj(params_path, obj_file)
+        try:
+            return describe(tmp)
+        finally:
+            os.unlink(tmp)
     buf = C.create_string_buffer(16384)
     n = host_lib().pdh_describe(params_path.encode(), buf, len(buf))
     if n < 0:
@@ -324,8 +378,14 @@ def describe(params_path):
     return json.loads(buf.value.decode())
 
 
-def scene(params_path):
+def scene(params_path, obj_file=None):
     """Particles of the params file's scene: dict of x4, v4, body, bc, vbc."""
+    if obj_file is not None:
+        tmp = with_obj(params_path, obj_file)
+        try:
+            return scene(tmp)
+        finally:
+            os.unlink(tmp)
     L = host_lib()
     n = C.c_int32()
     if L.pdh_scene(params_path.encode(), C.byref(n), None, None, None, None, None) != 0:
@@ -359,7 +419,8 @@ def step_kwargs(desc, step):
                 contact_stiffness=desc["contact_stiffness"],
                 contact_volume=desc["contact_volume"], contact_cell=desc["contact_radius"],
                 precision=desc.get("precision", 0), inner_rtol=desc.get("inner_rtol", 1e-5),
-                max_refine=desc.get("max_refine", 20), inner_max_iter=desc["max_iter"])
+                max_refine=desc.get("max_refine", 20), inner_max_iter=desc["max_iter"],
+                contact_mode=desc.get("contact_mode", 0))
 
 
 def materials_of(desc):

@@ -225,9 +225,40 @@ def test_contact_two_particles():
     np.testing.assert_allclose(sys.fc[0, :3], f, rtol=1e-14)
     np.testing.assert_allclose(sys.fc[1, :3], -f, rtol=1e-14)
     assert np.all(sys.fc[2] == 0)
-    sys.body[:] = 0  # same body: no contact
+    sys.body[:] = 0  # same body: no contact between bodies ...
     sys.contact(prm)
     assert np.all(sys.fc == 0)
+    # ... but the reference's pair set (all pairs) has it
+    sys.contact(orc.make_params(contact_radius=Rc, contact_stiffness=kc, contact_volume=vol,
+                                contact_cell=0.01, contact_mode=1))
+    np.testing.assert_allclose(sys.fc[0, :3], f, rtol=1e-14)
+    np.testing.assert_allclose(sys.fc[1, :3], -f, rtol=1e-14)
+    assert np.all(sys.fc[2] == 0)
+
+
+def test_contact_all_pairs_brute_force():
+    # every pair closer than Rc, by brute force in numpy, on a compressed cloud
+    rng = np.random.default_rng(12)
+    n = 400
+    x4 = np.zeros((n, 4))
+    x4[:, :3] = rng.uniform(0, 0.05, (n, 3))
+    body = rng.integers(0, 2, n).astype(np.int32)
+    sys = orc.OracleSystem(x4, np.zeros((n, 4)), np.ones(n), np.zeros(n + 1, np.int64),
+                           np.zeros(0, np.int32), [(0.1, 0.05, 1.0, 1.0)] * 2, body=body)
+    Rc, kc = 0.006, 3.0e6
+    for mode in (0, 1):
+        sys.contact(orc.make_params(contact_radius=Rc, contact_stiffness=kc, contact_volume=2.0,
+                                    contact_cell=0.007, contact_mode=mode))
+        d = x4[:, None, :3] - x4[None, :, :3]
+        r = np.linalg.norm(d, axis=2)
+        mask = (r < Rc) & (r > 0)
+        if mode == 0:
+            mask &= body[:, None] != body[None, :]
+        with np.errstate(invalid="ignore", divide="ignore"):
+            mag = np.where(mask, kc * (r - Rc) ** 2 / 2.0 / np.where(r > 0, r, 1), 0.0)
+        f = (mag[:, :, None] * d).sum(1)
+        assert mask.sum() > 20
+        np.testing.assert_allclose(sys.fc[:, :3], f, rtol=1e-12, atol=1e-12 * np.abs(f).max())
 
 
 def test_reference_timestep_units():
