@@ -135,6 +135,15 @@ typedef struct pd_step_stats {
   int32_t resid_launches;
 } pd_step_stats;
 
+/* Solver result of one specimen (pd_set_specimens). */
+typedef struct pd_specimen_stats {
+  int32_t cg_iterations;
+  int32_t converged;
+  double residual_norm;
+  double b_norm;
+  int32_t refinements;
+} pd_specimen_stats;
+
 typedef struct pd_bond_stats {
   int64_t total_bonds; /* directed bonds */
   int32_t min_bonds;
@@ -249,6 +258,22 @@ int pd_bench_spmv(pd_ctx* ctx, int32_t iters, double* ms_per_launch,
  * with precision 1 (float inner SpMV), live blocks * (6*4 + 4) +
  * N * (6*4 + 4 + 2*12). */
 int pd_spmv_bytes(pd_ctx* ctx, double* bytes);
+
+/* Independent specimens in one context (no counterpart in the reference,
+ * which runs one body per process).  Specimen k owns rows [first[k],
+ * first[k] + num[k]); first[k] must be a multiple of 256 and the ranges
+ * ascending and disjoint; rows outside every specimen must be INVISIBLE
+ * and far from the others' horizons (they carry no bonds and contribute
+ * zeros).  Afterwards every solve keeps its scalars, sums, stopping test
+ * and iteration count per specimen, in the same launches for all of them,
+ * so each specimen's result equals a context that holds it alone.
+ * count = 0 returns to one specimen over all rows. */
+int pd_set_specimens(pd_ctx* ctx, int32_t count, const int32_t* first, const int32_t* num);
+
+/* Per-specimen results of the last solve (count = the specimen count).
+ * pd_step_stats then reports the most iterations, converged = all, and the
+ * residual of the specimen farthest from its tolerance. */
+int pd_get_specimen_stats(pd_ctx* ctx, int32_t count, pd_specimen_stats* out);
 
 #ifdef __cplusplus
 }

@@ -52,15 +52,16 @@ struct pd_ctx {
   float* minv32 = nullptr;
   int64_t slots32_cap = 0;
   float4 *r32 = nullptr, *z32 = nullptr, *p32 = nullptr, *y32 = nullptr, *d32 = nullptr;
-  pd_mx_state* mx = nullptr;
-  pd_mx_state* h_mx = nullptr;
+  pd_mx_state* mx = nullptr;    // [nspec]
+  pd_mx_state* h_mx = nullptr;  // pinned [nspec]
   int last_mx_iters = 0;
+  int last_solve_prec = -1;
   int mat_prec = -1;  // precision of the last assembly (-1: none)
 
   // CG
   double4 *r = nullptr, *z = nullptr, *p = nullptr, *y = nullptr;
   double* part = nullptr;
-  pd_cg_state* st = nullptr;
+  pd_cg_state* st = nullptr;  // [nspec]
   pd_step_counters* counters = nullptr;
   pd_mat_dev* dmat = nullptr;
   double* dhorizon = nullptr;
@@ -92,8 +93,15 @@ struct pd_ctx {
   double4 *x_bak = nullptr, *v_bak = nullptr;
   int32_t* top_bak = nullptr;
 
+  // independent specimens (one by default): row ranges and tile tables
+  int32_t nspec = 1;
+  int32_t state_cap = 0;  // entries allocated in st / h_st (and mx / h_mx)
+  std::vector<int32_t> h_spec_first, h_spec_count;
+  int32_t *tile_spec = nullptr, *spec_tile0 = nullptr, *spec_ntiles = nullptr;
+  bool last_converged = false;  // every specimen of the last solve
+
   // pinned host mirrors
-  pd_cg_state* h_st = nullptr;
+  pd_cg_state* h_st = nullptr;  // [nspec]
   pd_step_counters* h_cnt = nullptr;
   double* h_bbox = nullptr;
   double* h_gap = nullptr;
@@ -218,11 +226,12 @@ int ensure_mixed(pd_ctx* c) {
     if ((e = dalloc(c, &c->diag32, npad * 6)) || (e = dalloc(c, &c->minv32, npad * 6)) ||
         (e = dalloc(c, &c->r32, n)) || (e = dalloc(c, &c->z32, n)) ||
         (e = dalloc(c, &c->p32, n)) || (e = dalloc(c, &c->y32, n)) ||
-        (e = dalloc(c, &c->d32, n)) || (e = dalloc(c, &c->mx, 1)))
+        (e = dalloc(c, &c->d32, n)) || (e = dalloc(c, &c->mx, (size_t)c->state_cap)))
       return e;
-    if (hipHostMalloc((void**)&c->h_mx, sizeof(pd_mx_state), hipHostMallocDefault) != hipSuccess)
+    if (hipHostMalloc((void**)&c->h_mx, sizeof(pd_mx_state) * c->state_cap,
+                      hipHostMallocDefault) != hipSuccess)
       return fail(c, PD_ERR_HIP, "hipHostMalloc failed");
-    memset(c->h_mx, 0, sizeof(pd_mx_state));
+    memset(c->h_mx, 0, sizeof(pd_mx_state) * c->state_cap);
   }
   if (c->slots32_cap < c->slots_cap) {
     dfree(c->val32);
@@ -345,9 +354,20 @@ pd::SpmvView spmv_view(pd_ctx* c) {
   return A;
 }
 
+pd::Batch batch_view(pd_ctx* c) {
+  pd::Batch b;
+  b.nspec = c->nspec;
+  const bool many = c->tile_spec != nullptr;
+  b.tile_spec = many ? c->tile_spec : nullptr;
+  b.spec_tile0 = many ? c->spec_tile0 : nullptr;
+  b.spec_ntiles = many ? c->spec_ntiles : nullptr;
+  return b;
+}
+
 pd::MxArgs mx_args(pd_ctx* c, int precond) {
   pd::MxArgs a;
   a.n = c->n;
+  a.batch = batch_view(c);
   a.slice_off = c->slice_off;
   a.top = c->top;
   a.col = c->col;
@@ -372,6 +392,7 @@ pd::MxArgs mx_args(pd_ctx* c, int precond) {
 pd::CgArgs cg_args(pd_ctx* c, int precond) {
   pd::CgArgs a;
   a.A = spmv_view(c);
+  a.batch = batch_view(c);
   a.minv = c->minv;
   a.bc = c->bc;
   a.b = c->b;
@@ -545,7 +566,8 @@ int run_solve(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st) {
     return fail(c, PD_ERR_STATE, "the matrix was not assembled with this precision");
   if (prm->precision == 1) return run_solve_mixed(c, prm, st);
   const pd::CgArgs a = cg_args(c, prm->precond);
-  pd::launch_cg_setup(c->st, prm->rtol, prm->max_iter, c->stream);
+  const int ns = c->nspec;
+  pd::launch_cg_setup(c->st, ns, prm->rtol, prm->max_iter, c->stream);
   pd::launch_cg_init(a, c->stream);
   CHECK_LAUNCH(c, "cg init");
   int launched = 0;
@@ -568,11 +590,13 @@ int run_solve(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st) {
     }
     CHECK_LAUNCH(c, "cg iteration");
     launched += chunk;
-    HIPCHK(c, hipMemcpyAsync(c->h_st, c->st, sizeof(pd_cg_state), hipMemcpyDeviceToHost,
+    HIPCHK(c, hipMemcpyAsync(c->h_st, c->st, sizeof(pd_cg_state) * ns, hipMemcpyDeviceToHost,
                              c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     ++syncs;
-    if (c->h_st->done) break;
+    bool all_done = true;
+    for (int k = 0; k < ns; ++k) all_done = all_done && c->h_st[k].done;
+    if (all_done) break;
     if (launched > prm->max_iter + 1)
       return fail(c, PD_ERR_HIP, "CG state did not terminate on the device");
     if (singles > 0) {
@@ -583,16 +607,29 @@ int run_solve(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st) {
       chunk = next;
     }
   }
-  c->last_iters = c->h_st->iter;
+  // the batch summary: most iterations, all converged, worst relative residual
+  int it = 0, worst = 0;
+  bool conv = true;
+  for (int k = 0; k < ns; ++k) {
+    const pd_cg_state& h = c->h_st[k];
+    it = std::max(it, h.iter);
+    conv = conv && h.converged;
+    const pd_cg_state& w = c->h_st[worst];
+    if (h.rnorm * w.bnorm > w.rnorm * h.bnorm) worst = k;
+  }
+  c->last_iters = it;
+  c->last_converged = conv;
+  c->last_solve_prec = 0;
   if (st) {
-    st->cg_iterations = c->h_st->iter;
-    st->converged = c->h_st->converged;
-    st->residual_norm = c->h_st->rnorm;
-    st->b_norm = c->h_st->bnorm;
+    st->cg_iterations = it;
+    st->converged = conv ? 1 : 0;
+    st->residual_norm = c->h_st[worst].rnorm;
+    st->b_norm = c->h_st[worst].bnorm;
+    st->refinements = 0;
     st->host_syncs += syncs;
     if (timing) {
       double tot = 0.0;
-      const int ran = std::min(c->h_st->iter, launched);
+      const int ran = std::min(it, launched);
       for (int k = 0; k < ran; ++k) tot += elapsed(c->ev[8 + 2 * k], c->ev[8 + 2 * k + 1]);
       st->ms_spmv_total = tot;
       st->spmv_launches = ran;
@@ -609,6 +646,7 @@ int run_solve_mixed(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st) {
   const pd::MxArgs a = mx_args(c, prm->precond);
   const MixedLimits lim = mixed_limits(prm);
   const bool timing = prm->timing != 0;
+  const int ns = c->nspec;
   // event pairs: [0] first residual, then per chunk `chunk` inner SpMVs and
   // one residual
   std::vector<int> spmv_ev, resid_ev;
@@ -619,7 +657,7 @@ int run_solve_mixed(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st) {
     used += 2;
     return PD_OK;
   };
-  pd::launch_mx_setup(c->mx, prm->rtol, prm->max_iter, lim.inner_rtol, lim.inner_max,
+  pd::launch_mx_setup(c->mx, ns, prm->rtol, prm->max_iter, lim.inner_rtol, lim.inner_max,
                       lim.max_refine, c->stream);
   if (timing) {
     if (int e = pair(resid_ev)) return e;
@@ -632,7 +670,7 @@ int run_solve_mixed(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st) {
   int next = 1;
   int singles = 2;
   int syncs = 0;
-  int iter_seen = 0, resid_seen = 1;
+  std::vector<int> iter_seen(ns, 0), resid_seen(ns, 1);
   double spmv_ms = 0.0, resid_ms = 0.0;
   int spmv_ran = 0, resid_ran = 0;
   std::vector<int> chunk_spmv;
@@ -659,14 +697,24 @@ int run_solve_mixed(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st) {
     pd::launch_mx_resid(a, c->stream);
     if (timing) HIPCHK(c, hipEventRecord(c->ev[rpair + 1], c->stream));
     CHECK_LAUNCH(c, "mixed iteration");
-    HIPCHK(c, hipMemcpyAsync(c->h_mx, c->mx, sizeof(pd_mx_state), hipMemcpyDeviceToHost,
+    HIPCHK(c, hipMemcpyAsync(c->h_mx, c->mx, sizeof(pd_mx_state) * ns, hipMemcpyDeviceToHost,
                              c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     ++syncs;
-    const pd_mx_state& h = *c->h_mx;
+    // a launch did work if any specimen needed it: the inner SpMVs that ran
+    // are the first max_k (iterations of k in this chunk) of the chunk
+    int ran = 0;
+    bool resid_ran_now = false, all_done = true;
+    for (int k = 0; k < ns; ++k) {
+      const pd_mx_state& h = c->h_mx[k];
+      ran = std::max(ran, h.iter - iter_seen[k]);
+      resid_ran_now = resid_ran_now || h.residuals > resid_seen[k];
+      iter_seen[k] = h.iter;
+      resid_seen[k] = h.residuals;
+      all_done = all_done && h.phase == 3;
+    }
     if (timing) {
-      // the inner SpMVs that ran are the first ones of the chunk
-      const int ran = std::min((int)chunk_spmv.size(), h.iter - iter_seen);
+      ran = std::min((int)chunk_spmv.size(), ran);
       for (int k = 0; k < ran; ++k)
         spmv_ms += elapsed(c->ev[chunk_spmv[k]], c->ev[chunk_spmv[k] + 1]);
       spmv_ran += ran;
@@ -674,16 +722,14 @@ int run_solve_mixed(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st) {
         resid_ms += elapsed(c->ev[resid_ev[0]], c->ev[resid_ev[0] + 1]);
         ++resid_ran;
       }
-      if (h.residuals > resid_seen) {
+      if (resid_ran_now) {
         resid_ms += elapsed(c->ev[rpair], c->ev[rpair + 1]);
         ++resid_ran;
       }
     }
-    iter_seen = h.iter;
-    resid_seen = h.residuals;
-    if (h.phase == 3) break;
-    // every chunk advances the inner iterations or the refinements
-    if (syncs > prm->max_iter + lim.max_refine + 2)
+    if (all_done) break;
+    // every chunk advances some specimen's inner iterations or refinements
+    if (syncs > ns * (prm->max_iter + lim.max_refine + 2))
       return fail(c, PD_ERR_HIP, "mixed solve state did not terminate on the device");
     if (singles > 0) {
       --singles;
@@ -693,19 +739,25 @@ int run_solve_mixed(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st) {
       chunk = next;
     }
   }
-  const pd_mx_state& h = *c->h_mx;
-  c->last_mx_iters = h.iter;
-  // the FP64 solver's view of the state, read by pd_solve / pd_step
-  c->h_st->converged = h.converged;
-  c->h_st->iter = h.iter;
-  c->h_st->rnorm = h.rnorm;
-  c->h_st->bnorm = h.bnorm;
+  int it = 0, refs = 0, worst = 0;
+  bool conv = true;
+  for (int k = 0; k < ns; ++k) {
+    const pd_mx_state& h = c->h_mx[k];
+    it = std::max(it, h.iter);
+    refs = std::max(refs, h.refinements);
+    conv = conv && h.converged;
+    const pd_mx_state& w = c->h_mx[worst];
+    if (h.rnorm * w.bnorm > w.rnorm * h.bnorm) worst = k;
+  }
+  c->last_mx_iters = it;
+  c->last_converged = conv;
+  c->last_solve_prec = 1;
   if (st) {
-    st->cg_iterations = h.iter;
-    st->converged = h.converged;
-    st->residual_norm = h.rnorm;
-    st->b_norm = h.bnorm;
-    st->refinements = h.refinements;
+    st->cg_iterations = it;
+    st->converged = conv ? 1 : 0;
+    st->residual_norm = c->h_mx[worst].rnorm;
+    st->b_norm = c->h_mx[worst].bnorm;
+    st->refinements = refs;
     st->host_syncs += syncs;
     if (timing) {
       st->ms_spmv_total = spmv_ms;
@@ -815,6 +867,7 @@ int pd_ctx_create(const pd_config* cfg, pd_ctx** out) {
   ALLOC(c->y, n);
   ALLOC(c->part, 4 * ntiles);
   ALLOC(c->st, 1);
+  c->state_cap = 1;
   ALLOC(c->counters, 1);
   ALLOC(c->dmat, PD_MAX_BODIES);
   ALLOC(c->dhorizon, PD_MAX_BODIES);
@@ -881,6 +934,7 @@ int pd_ctx_destroy(pd_ctx* c) {
   dfree(c->counts); dfree(c->widths); dfree(c->zone_cnt); dfree(c->cand); dfree(c->x_bak); dfree(c->v_bak); dfree(c->top_bak);
   dfree(c->val32); dfree(c->diag32); dfree(c->minv32); dfree(c->r32); dfree(c->z32);
   dfree(c->p32); dfree(c->y32); dfree(c->d32); dfree(c->mx);
+  dfree(c->tile_spec); dfree(c->spec_tile0); dfree(c->spec_ntiles);
   if (c->h_mx) (void)hipHostFree(c->h_mx);
   if (c->h_st) (void)hipHostFree(c->h_st);
   if (c->h_cnt) (void)hipHostFree(c->h_cnt);
@@ -958,7 +1012,7 @@ int pd_build_bonds(pd_ctx* c, pd_bond_stats* stats) {
     // one cell of each other despite rounding in the cell index
     pd::GridView g;
     if (int e = build_grid(c, hmax * (1.0 + 1e-7), &g)) return e;
-    pd::launch_bond_count(n, c->x, c->body, c->dhorizon, g, c->counts, c->stream);
+    pd::launch_bond_count(n, c->x, c->body, c->act, c->dhorizon, g, c->counts, c->stream);
     CHECK_LAUNCH(c, "bond count");
     pd::launch_slice_width(c->counts, n, c->widths, c->stream);
     CHECK_LAUNCH(c, "slice width");
@@ -975,7 +1029,7 @@ int pd_build_bonds(pd_ctx* c, pd_bond_stats* stats) {
                       std::to_string(c->max_bonds));
     }
     if (int e = set_slice_offsets(c, widths)) return e;
-    pd::launch_bond_fill(n, c->x, c->body, c->dhorizon, g, c->counts, c->slice_off, c->col,
+    pd::launch_bond_fill(n, c->x, c->body, c->act, c->dhorizon, g, c->counts, c->slice_off, c->col,
                          c->top, c->cap, c->stream);
     CHECK_LAUNCH(c, "bond fill");
   } else {
@@ -1142,7 +1196,7 @@ int pd_solve(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st) {
   if (int e = check_params(c, prm)) return e;
   HIPCHK(c, hipSetDevice(c->device));
   if (int e = run_solve(c, prm, st)) return e;
-  if (!c->h_st->converged)
+  if (!c->last_converged)
     return fail(c, PD_ERR_NOT_CONVERGED, "CG reached the iteration limit before rtol");
   return PD_OK;
 }
@@ -1183,7 +1237,7 @@ int pd_step(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st) {
     st->ms_solve = elapsed(c->ev[2], c->ev[3]);
     st->ms_update = elapsed(c->ev[3], c->ev[4]);
   }
-  if (!c->h_st->converged)
+  if (!c->last_converged)
     return fail(c, PD_ERR_NOT_CONVERGED, "CG reached the iteration limit before rtol");
   return PD_OK;
 }
@@ -1267,6 +1321,114 @@ int pd_bench_spmv(pd_ctx* c, int32_t iters, double* ms_per_launch, double* bytes
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (ms_per_launch) *ms_per_launch = elapsed(c->ev[0], c->ev[1]) / iters;
   if (bytes_per_launch) pd_spmv_bytes(c, bytes_per_launch);
+  return PD_OK;
+}
+
+int pd_set_specimens(pd_ctx* c, int32_t count, const int32_t* first, const int32_t* num) {
+  if (!c) return fail(nullptr, PD_ERR_ARG, "null context");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const int32_t n = c->n;
+  if (count == 0) {  // back to one specimen over all rows
+    dfree(c->tile_spec);
+    dfree(c->spec_tile0);
+    dfree(c->spec_ntiles);
+    c->nspec = 1;
+    c->h_spec_first.clear();
+    c->h_spec_count.clear();
+    c->last_iters = c->last_mx_iters = 0;
+    return PD_OK;
+  }
+  if (count < 0 || count > std::max(n, 1)) return fail(c, PD_ERR_ARG, "specimen count out of range");
+  if (!first || !num) return fail(c, PD_ERR_ARG, "null specimen table");
+  for (int32_t k = 0; k < count; ++k) {
+    const int64_t end = (k + 1 < count) ? first[k + 1] : n;
+    if (first[k] < 0 || first[k] % PD_BLOCK != 0)
+      return fail(c, PD_ERR_ARG, "specimen first rows must be multiples of 256");
+    if (num[k] < 1 || (int64_t)first[k] + num[k] > end)
+      return fail(c, PD_ERR_ARG, "specimen rows must be non-empty, ascending and disjoint");
+  }
+  // rows outside every specimen must be INVISIBLE (they contribute zeros)
+  std::vector<int32_t> act(n);
+  if (n > 0) {
+    HIPCHK(c, hipMemcpy(act.data(), c->act, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+  }
+  std::vector<char> inside(n, 0);
+  for (int32_t k = 0; k < count; ++k)
+    for (int32_t r = first[k]; r < first[k] + num[k]; ++r) inside[r] = 1;
+  for (int32_t r = 0; r < n; ++r)
+    if (!inside[r] && act[r] != PD_INVISIBLE)
+      return fail(c, PD_ERR_ARG, "rows outside the specimens must be INVISIBLE");
+  const int32_t ntiles = (n + PD_BLOCK - 1) / PD_BLOCK;
+  std::vector<int32_t> tile_spec(std::max(ntiles, 1), 0), t0(count), nt(count);
+  for (int32_t k = 0; k < count; ++k) {
+    t0[k] = first[k] / PD_BLOCK;
+    nt[k] = (first[k] + num[k] + PD_BLOCK - 1) / PD_BLOCK - t0[k];
+    const int32_t hi = (k + 1 < count) ? first[k + 1] / PD_BLOCK : ntiles;
+    for (int32_t t = t0[k]; t < hi; ++t) tile_spec[t] = k;
+  }
+  if (count > c->state_cap) {
+    dfree(c->st);
+    if (int e = dalloc(c, &c->st, (size_t)count)) return e;
+    if (c->h_st) (void)hipHostFree(c->h_st);
+    c->h_st = nullptr;
+    if (hipHostMalloc((void**)&c->h_st, sizeof(pd_cg_state) * count, hipHostMallocDefault) !=
+        hipSuccess)
+      return fail(c, PD_ERR_HIP, "hipHostMalloc failed");
+    if (c->mx) {
+      dfree(c->mx);
+      if (int e = dalloc(c, &c->mx, (size_t)count)) return e;
+      if (c->h_mx) (void)hipHostFree(c->h_mx);
+      c->h_mx = nullptr;
+      if (hipHostMalloc((void**)&c->h_mx, sizeof(pd_mx_state) * count, hipHostMallocDefault) !=
+          hipSuccess)
+        return fail(c, PD_ERR_HIP, "hipHostMalloc failed");
+    }
+    c->state_cap = count;
+  }
+  dfree(c->tile_spec);
+  dfree(c->spec_tile0);
+  dfree(c->spec_ntiles);
+  if (int e = dalloc(c, &c->tile_spec, tile_spec.size())) return e;
+  if (int e = dalloc(c, &c->spec_tile0, (size_t)count)) return e;
+  if (int e = dalloc(c, &c->spec_ntiles, (size_t)count)) return e;
+  HIPCHK(c, hipMemcpyAsync(c->tile_spec, tile_spec.data(), sizeof(int32_t) * tile_spec.size(),
+                           hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->spec_tile0, t0.data(), sizeof(int32_t) * count,
+                           hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->spec_ntiles, nt.data(), sizeof(int32_t) * count,
+                           hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->nspec = count;
+  c->h_spec_first.assign(first, first + count);
+  c->h_spec_count.assign(num, num + count);
+  c->last_iters = c->last_mx_iters = 0;
+  c->last_solve_prec = -1;
+  return PD_OK;
+}
+
+int pd_get_specimen_stats(pd_ctx* c, int32_t count, pd_specimen_stats* out) {
+  if (!c) return fail(nullptr, PD_ERR_ARG, "null context");
+  if (!out || count != c->nspec) return fail(c, PD_ERR_ARG, "count must equal the specimen count");
+  if (c->last_solve_prec < 0) return fail(c, PD_ERR_STATE, "no solve since the specimens were set");
+  for (int32_t k = 0; k < count; ++k) {
+    pd_specimen_stats& o = out[k];
+    if (c->last_solve_prec == 1) {
+      const pd_mx_state& h = c->h_mx[k];
+      o.cg_iterations = h.iter;
+      o.converged = h.converged;
+      o.residual_norm = h.rnorm;
+      o.b_norm = h.bnorm;
+      o.refinements = h.refinements;
+    } else {
+      const pd_cg_state& h = c->h_st[k];
+      o.cg_iterations = h.iter;
+      o.converged = h.converged;
+      o.residual_norm = h.rnorm;
+      o.b_norm = h.bnorm;
+      o.refinements = 0;
+    }
+  }
   return PD_OK;
 }
 

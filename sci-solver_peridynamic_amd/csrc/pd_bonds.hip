@@ -387,6 +387,7 @@ struct NbArgs {
   int n;
   const double4* x;
   const int32_t* body;
+  const int32_t* act;  // INVISIBLE particles take no bonds
   const double* horizon;  // per body
   GridView g;
   int32_t* counts;
@@ -402,6 +403,10 @@ __global__ __launch_bounds__(PD_BLOCK) void k_bond_count(NbArgs a) {
   const double4 xp = a.x[p];
   const int bp = a.body[p];
   const double h = a.horizon[bp];
+  if (a.act[p] == PD_INVISIBLE) {
+    a.counts[p] = 0;
+    return;
+  }
   int cx, cy, cz;
   cell_of(xp, a.g.mn, a.g.cell, cx, cy, cz);
   int count = 0;
@@ -416,19 +421,20 @@ __global__ __launch_bounds__(PD_BLOCK) void k_bond_count(NbArgs a) {
         const int e = a.g.cell_end[cell];
         for (int m = a.g.cell_start[cell]; m < e; ++m) {
           const int q = a.g.sorted_idx[m];
-          if (q == p || a.body[q] != bp) continue;
+          if (q == p || a.body[q] != bp || a.act[q] == PD_INVISIBLE) continue;
           if (in_horizon(xp, a.x[q], h)) ++count;
         }
       }
   a.counts[p] = count;
 }
 
-void launch_bond_count(int32_t n, const double4* x, const int32_t* body, const double* horizon,
-                       const GridView& g, int32_t* counts, hipStream_t s) {
+void launch_bond_count(int32_t n, const double4* x, const int32_t* body, const int32_t* act,
+                       const double* horizon, const GridView& g, int32_t* counts, hipStream_t s) {
   NbArgs a = {};
   a.n = n;
   a.x = x;
   a.body = body;
+  a.act = act;
   a.horizon = horizon;
   a.g = g;
   a.counts = counts;
@@ -472,6 +478,7 @@ __global__ __launch_bounds__(64) void k_bond_fill(NbArgs a) {
   int cx, cy, cz;
   cell_of(xp, a.g.mn, a.g.cell, cx, cy, cz);
   const unsigned long long lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+  const bool visible = a.act[p] != PD_INVISIBLE;  // uniform over the workgroup
   int cnt = 0;
   for (int dz = -1; dz <= 1; ++dz)
     for (int dy = -1; dy <= 1; ++dy)
@@ -487,9 +494,10 @@ __global__ __launch_bounds__(64) void k_bond_fill(NbArgs a) {
           const int m = m0 + lane;
           bool hit = false;
           int q = 0;
-          if (m < e) {
+          if (visible && m < e) {
             q = a.g.sorted_idx[m];
-            hit = (q != p) && (a.body[q] == bp) && in_horizon(xp, a.x[q], h);
+            hit = (q != p) && (a.body[q] == bp) && (a.act[q] != PD_INVISIBLE) &&
+                  in_horizon(xp, a.x[q], h);
           }
           const unsigned long long mask = __ballot(hit);
           if (hit) {
@@ -532,14 +540,16 @@ __global__ __launch_bounds__(64) void k_bond_fill(NbArgs a) {
   }
 }
 
-void launch_bond_fill(int32_t n, const double4* x, const int32_t* body, const double* horizon,
-                      const GridView& g, const int32_t* counts, const int64_t* slice_off,
-                      int32_t* col, int32_t* top, int32_t* cap, hipStream_t s) {
+void launch_bond_fill(int32_t n, const double4* x, const int32_t* body, const int32_t* act,
+                      const double* horizon, const GridView& g, const int32_t* counts,
+                      const int64_t* slice_off, int32_t* col, int32_t* top, int32_t* cap,
+                      hipStream_t s) {
   NbArgs a = {};
   a.counts = const_cast<int32_t*>(counts);
   a.n = n;
   a.x = x;
   a.body = body;
+  a.act = act;
   a.horizon = horizon;
   a.g = g;
   a.slice_off = slice_off;

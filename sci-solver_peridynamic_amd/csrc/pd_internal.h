@@ -65,8 +65,34 @@ struct SpmvView {
   const double* diag;
 };
 
+// Independent specimens solved together: specimen k owns the 256-row
+// tiles [spec_tile0[k], spec_tile0[k] + spec_ntiles[k]); tile_spec maps a
+// tile to its specimen.  Every solver scalar is per specimen, every sum is
+// folded per specimen.  Null tables: one specimen over all tiles.
+struct Batch {
+  const int32_t* tile_spec;
+  const int32_t* spec_tile0;
+  const int32_t* spec_ntiles;
+  int32_t nspec;
+};
+
+__device__ __forceinline__ int spec_of(const Batch& b, int tile) {
+  return b.tile_spec ? b.tile_spec[tile] : 0;
+}
+// tile range of the specimen folded by this workgroup (blockIdx.x)
+__device__ __forceinline__ void spec_range(const Batch& b, int ntiles, int& t0, int& nt) {
+  if (b.spec_tile0) {
+    t0 = b.spec_tile0[blockIdx.x];
+    nt = b.spec_ntiles[blockIdx.x];
+  } else {
+    t0 = 0;
+    nt = ntiles;
+  }
+}
+
 struct CgArgs {
   SpmvView A;
+  Batch batch;
   const double* minv;
   const int32_t* bc;
   const double4* b;
@@ -76,7 +102,7 @@ struct CgArgs {
   double4* p;
   double4* y;
   double* part;  // [3 * ntiles]
-  pd_cg_state* st;
+  pd_cg_state* st;  // [batch.nspec]
   int32_t precond;
 };
 
@@ -84,6 +110,7 @@ struct CgArgs {
 // solution and right-hand side.
 struct MxArgs {
   int32_t n;
+  Batch batch;
   const int64_t* slice_off;
   const int32_t* top;
   const int32_t* col;
@@ -100,7 +127,7 @@ struct MxArgs {
   float4* y;
   float4* d;
   double* part;  // [4 * ntiles]
-  pd_mx_state* st;
+  pd_mx_state* st;  // [batch.nspec]
   int32_t precond;
 };
 
@@ -121,15 +148,16 @@ void launch_assemble(const AsmArgs& a, hipStream_t s);
 void launch_gap_reduce(const double* gap_val, const int32_t* gap_q, int32_t n,
                        double* part_val, int32_t* part_p, int32_t* part_q,
                        double* out_val, int32_t* out_pq, hipStream_t s);
-void launch_cg_setup(pd_cg_state* st, double rtol, int32_t max_iter, hipStream_t s);
+void launch_cg_setup(pd_cg_state* st, int32_t nspec, double rtol, int32_t max_iter,
+                     hipStream_t s);
 void launch_cg_init(const CgArgs& a, hipStream_t s);
 void launch_cg_spmv(const CgArgs& a, hipStream_t s);
 void launch_cg_update(const CgArgs& a, hipStream_t s);
 void launch_cg_dir(const CgArgs& a, hipStream_t s);
 void launch_spmv_plain(const SpmvView& A, const double4* x, double4* y, hipStream_t s);
 // pd_mixed.hip
-void launch_mx_setup(pd_mx_state* st, double rtol, int32_t max_iter, double inner_rtol,
-                     int32_t inner_max, int32_t max_refine, hipStream_t s);
+void launch_mx_setup(pd_mx_state* st, int32_t nspec, double rtol, int32_t max_iter,
+                     double inner_rtol, int32_t inner_max, int32_t max_refine, hipStream_t s);
 void launch_mx_resid(const MxArgs& a, hipStream_t s);  // residual + its fold
 void launch_mx_spmv(const MxArgs& a, hipStream_t s);
 void launch_mx_update(const MxArgs& a, hipStream_t s);  // alpha, update, beta
@@ -181,12 +209,13 @@ int radix_sort(uint32_t* keys_a, int32_t* idx_a, uint32_t* keys_b, int32_t* idx_
 size_t radix_hist_words(int32_t n);
 void launch_cell_ranges(const uint32_t* keys, int32_t n, int32_t* start,
                         int32_t* end, int64_t ncell, hipStream_t s);
-void launch_bond_count(int32_t n, const double4* x, const int32_t* body,
+// INVISIBLE particles take no bonds and are no one's neighbour
+void launch_bond_count(int32_t n, const double4* x, const int32_t* body, const int32_t* act,
                        const double* horizon, const GridView& g, int32_t* counts,
                        hipStream_t s);
 void launch_slice_width(const int32_t* counts, int32_t n, int32_t* widths,
                         hipStream_t s);
-void launch_bond_fill(int32_t n, const double4* x, const int32_t* body,
+void launch_bond_fill(int32_t n, const double4* x, const int32_t* body, const int32_t* act,
                       const double* horizon, const GridView& g, const int32_t* counts,
                       const int64_t* slice_off, int32_t* col, int32_t* top,
                       int32_t* cap, hipStream_t s);

@@ -178,9 +178,10 @@ __device__ __forceinline__ double term3f(float a0, float a1, float a2, float b0,
 
 __device__ __forceinline__ bool resid_due(int phase) { return phase == 0 || phase == 2; }
 
-__global__ void k_mx_setup(pd_mx_state* st, double rtol, int max_iter, double inner_rtol,
-                           int inner_max, int max_refine) {
-  if (threadIdx.x == 0) {
+__global__ void k_mx_setup(pd_mx_state* states, int nspec, double rtol, int max_iter,
+                           double inner_rtol, int inner_max, int max_refine) {
+  for (int k = threadIdx.x; k < nspec; k += blockDim.x) {
+    pd_mx_state* st = states + k;
     st->rz = 0.0;
     st->rr32 = 0.0;
     st->alpha = 0.0;
@@ -207,10 +208,10 @@ __global__ void k_mx_setup(pd_mx_state* st, double rtol, int max_iter, double in
 // r = S(b - A32 x) in FP64; r32 = float(r); z = M32 r32; p = z; d = 0;
 // partials <r,r>, <r32,z>, <r32,r32>, <Sb,Sb>
 __global__ __launch_bounds__(PD_BLOCK) void k_mx_resid(MxArgs a) {
-  if (!resid_due(a.st->phase)) return;
   __shared__ double lds4[4];
   const int ntiles = gridDim.x;
   const int tile = xcd_tile(blockIdx.x, ntiles);
+  if (!resid_due(a.st[spec_of(a.batch, tile)].phase)) return;
   const int p = tile * PD_BLOCK + threadIdx.x;
   double t_rr = 0.0, t_rz = 0.0, t_r32 = 0.0, t_bb = 0.0;
   if (p < a.n) {
@@ -246,15 +247,18 @@ __global__ __launch_bounds__(PD_BLOCK) void k_mx_resid(MxArgs a) {
 }
 
 // outer test; on failure of the test start the next inner solve
+// one workgroup per specimen
 __global__ __launch_bounds__(PD_BLOCK) void k_mx_resid_fold(MxArgs a, int ntiles) {
-  if (!resid_due(a.st->phase)) return;
+  if (!resid_due(a.st[blockIdx.x].phase)) return;
   __shared__ double lds4[4];
-  const double rr = fold_partials(a.part, ntiles, lds4);
-  const double rz = fold_partials(a.part + ntiles, ntiles, lds4);
-  const double rr32 = fold_partials(a.part + 2 * (int64_t)ntiles, ntiles, lds4);
-  const double bb = fold_partials(a.part + 3 * (int64_t)ntiles, ntiles, lds4);
+  int t0, nt;
+  spec_range(a.batch, ntiles, t0, nt);
+  const double rr = fold_partials(a.part + t0, nt, lds4);
+  const double rz = fold_partials(a.part + ntiles + t0, nt, lds4);
+  const double rr32 = fold_partials(a.part + 2 * (int64_t)ntiles + t0, nt, lds4);
+  const double bb = fold_partials(a.part + 3 * (int64_t)ntiles + t0, nt, lds4);
   if (threadIdx.x == 0) {
-    pd_mx_state* st = a.st;
+    pd_mx_state* st = a.st + blockIdx.x;
     st->residuals = st->residuals + 1;
     if (st->first) {
       st->bnorm = sqrt(bb);
@@ -284,10 +288,10 @@ __global__ __launch_bounds__(PD_BLOCK) void k_mx_resid_fold(MxArgs a, int ntiles
 
 // y = A32 p (float); partial <y,p>
 __global__ __launch_bounds__(PD_BLOCK) void k_mx_spmv(MxArgs a) {
-  if (a.st->phase != 1) return;
   __shared__ double lds4[4];
   const int ntiles = gridDim.x;
   const int tile = xcd_tile(blockIdx.x, ntiles);
+  if (a.st[spec_of(a.batch, tile)].phase != 1) return;
   const int p = tile * PD_BLOCK + threadIdx.x;
   double t = 0.0;
   if (p < a.n) {
@@ -302,20 +306,24 @@ __global__ __launch_bounds__(PD_BLOCK) void k_mx_spmv(MxArgs a) {
 }
 
 __global__ __launch_bounds__(PD_BLOCK) void k_mx_alpha(MxArgs a, int ntiles) {
-  if (a.st->phase != 1) return;
+  pd_mx_state* st = a.st + blockIdx.x;
+  if (st->phase != 1) return;
   __shared__ double lds4[4];
-  const double yp = fold_partials(a.part, ntiles, lds4);
-  if (threadIdx.x == 0) a.st->alpha = a.st->rz / yp;
+  int t0, nt;
+  spec_range(a.batch, ntiles, t0, nt);
+  const double yp = fold_partials(a.part + t0, nt, lds4);
+  if (threadIdx.x == 0) st->alpha = st->rz / yp;
 }
 
 // d += a p; r -= a y (free rows); z = M32 r; partials <r,z>, <r,r>
 __global__ __launch_bounds__(PD_BLOCK) void k_mx_update(MxArgs a) {
-  if (a.st->phase != 1) return;
   __shared__ double lds4[4];
   const int ntiles = gridDim.x;
   const int tile = xcd_tile(blockIdx.x, ntiles);
+  const pd_mx_state* st = a.st + spec_of(a.batch, tile);
+  if (st->phase != 1) return;
   const int p = tile * PD_BLOCK + threadIdx.x;
-  const float af = (float)a.st->alpha;
+  const float af = (float)st->alpha;
   double t_rz = 0.0, t_rr = 0.0;
   if (p < a.n) {
     const float4 pp = a.p[p];
@@ -349,12 +357,14 @@ __global__ __launch_bounds__(PD_BLOCK) void k_mx_update(MxArgs a) {
 
 // beta and the inner test ahead of the next inner iteration
 __global__ __launch_bounds__(PD_BLOCK) void k_mx_beta(MxArgs a, int ntiles) {
-  if (a.st->phase != 1) return;
+  if (a.st[blockIdx.x].phase != 1) return;
   __shared__ double lds4[4];
-  const double rz = fold_partials(a.part, ntiles, lds4);
-  const double rr32 = fold_partials(a.part + ntiles, ntiles, lds4);
+  int t0, nt;
+  spec_range(a.batch, ntiles, t0, nt);
+  const double rz = fold_partials(a.part + t0, nt, lds4);
+  const double rr32 = fold_partials(a.part + ntiles + t0, nt, lds4);
   if (threadIdx.x == 0) {
-    pd_mx_state* st = a.st;
+    pd_mx_state* st = a.st + blockIdx.x;
     st->beta = rz / st->rz;
     st->rz = rz;
     st->rr32 = rr32;
@@ -368,10 +378,12 @@ __global__ __launch_bounds__(PD_BLOCK) void k_mx_beta(MxArgs a, int ntiles) {
 
 // p = z + b p
 __global__ __launch_bounds__(PD_BLOCK) void k_mx_dir(MxArgs a) {
-  if (a.st->phase != 1) return;
-  const int p = xcd_tile(blockIdx.x, gridDim.x) * PD_BLOCK + threadIdx.x;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const pd_mx_state* st = a.st + spec_of(a.batch, tile);
+  if (st->phase != 1) return;
+  const int p = tile * PD_BLOCK + threadIdx.x;
   if (p >= a.n) return;
-  const float bf = (float)a.st->beta;
+  const float bf = (float)st->beta;
   const float4 zz = a.z[p];
   float4 pp = a.p[p];
   pp.x = zz.x + bf * pp.x;
@@ -382,8 +394,9 @@ __global__ __launch_bounds__(PD_BLOCK) void k_mx_dir(MxArgs a) {
 
 // x += d (FP64)
 __global__ __launch_bounds__(PD_BLOCK) void k_mx_apply(MxArgs a) {
-  if (a.st->phase != 2) return;
-  const int p = xcd_tile(blockIdx.x, gridDim.x) * PD_BLOCK + threadIdx.x;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  if (a.st[spec_of(a.batch, tile)].phase != 2) return;
+  const int p = tile * PD_BLOCK + threadIdx.x;
   if (p >= a.n) return;
   const float4 dd = a.d[p];
   double4 xx = a.x[p];
@@ -415,16 +428,16 @@ inline int tiles_of(int n) { return (n + PD_BLOCK - 1) / PD_BLOCK; }
 
 }  // namespace
 
-void launch_mx_setup(pd_mx_state* st, double rtol, int32_t max_iter, double inner_rtol,
-                     int32_t inner_max, int32_t max_refine, hipStream_t s) {
-  hipLaunchKernelGGL(k_mx_setup, dim3(1), dim3(64), 0, s, st, rtol, max_iter, inner_rtol,
+void launch_mx_setup(pd_mx_state* st, int32_t nspec, double rtol, int32_t max_iter,
+                     double inner_rtol, int32_t inner_max, int32_t max_refine, hipStream_t s) {
+  hipLaunchKernelGGL(k_mx_setup, dim3(1), dim3(256), 0, s, st, nspec, rtol, max_iter, inner_rtol,
                      inner_max, max_refine);
 }
 void launch_mx_resid(const MxArgs& a, hipStream_t s) {
   if (!a.n) return;
   const int g = tiles_of(a.n);
   hipLaunchKernelGGL(k_mx_resid, dim3(g), dim3(PD_BLOCK), 0, s, a);
-  hipLaunchKernelGGL(k_mx_resid_fold, dim3(1), dim3(PD_BLOCK), 0, s, a, g);
+  hipLaunchKernelGGL(k_mx_resid_fold, dim3(a.batch.nspec), dim3(PD_BLOCK), 0, s, a, g);
 }
 void launch_mx_spmv(const MxArgs& a, hipStream_t s) {
   if (a.n) hipLaunchKernelGGL(k_mx_spmv, dim3(tiles_of(a.n)), dim3(PD_BLOCK), 0, s, a);
@@ -432,9 +445,9 @@ void launch_mx_spmv(const MxArgs& a, hipStream_t s) {
 void launch_mx_update(const MxArgs& a, hipStream_t s) {
   if (!a.n) return;
   const int g = tiles_of(a.n);
-  hipLaunchKernelGGL(k_mx_alpha, dim3(1), dim3(PD_BLOCK), 0, s, a, g);
+  hipLaunchKernelGGL(k_mx_alpha, dim3(a.batch.nspec), dim3(PD_BLOCK), 0, s, a, g);
   hipLaunchKernelGGL(k_mx_update, dim3(g), dim3(PD_BLOCK), 0, s, a);
-  hipLaunchKernelGGL(k_mx_beta, dim3(1), dim3(PD_BLOCK), 0, s, a, g);
+  hipLaunchKernelGGL(k_mx_beta, dim3(a.batch.nspec), dim3(PD_BLOCK), 0, s, a, g);
 }
 void launch_mx_dir(const MxArgs& a, hipStream_t s) {
   if (a.n) hipLaunchKernelGGL(k_mx_dir, dim3(tiles_of(a.n)), dim3(PD_BLOCK), 0, s, a);

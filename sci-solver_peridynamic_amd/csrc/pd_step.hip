@@ -439,10 +439,11 @@ void launch_spmv_plain(const SpmvView& A, const double4* x, double4* y, hipStrea
 // ---------------------------------------------------------------------------
 // CG (CUSP krylov::cg structure, core/simulator.cu:763-770) with scalars
 // kept on the device.  Every streaming kernel writes one partial sum per
-// 256-row tile; a one-workgroup kernel then folds the partials in the fixed
-// order of the oracle's orc_dot and publishes alpha, or beta plus the
-// monitor test.  Kernel boundaries order the writes, so no fences are
-// needed.  All kernels return at once when the solve is done.
+// 256-row tile; a fold kernel with one workgroup per specimen then folds
+// that specimen's partials in the fixed order of the oracle's orc_dot and
+// publishes alpha, or beta plus the monitor test.  Kernel boundaries order
+// the writes, so no fences are needed.  The kernels of a tile or specimen
+// return at once when its solve is done.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void precond(const double* minv, int pc, int p, double r0,
                                         double r1, double r2, double& z0, double& z1,
@@ -466,8 +467,9 @@ __device__ __forceinline__ double term3(double a0, double a1, double a2, double 
   return (a0 * b0 + a1 * b1) + a2 * b2;
 }
 
-__global__ void k_cg_setup(pd_cg_state* st, double rtol, int max_iter) {
-  if (threadIdx.x == 0) {
+__global__ void k_cg_setup(pd_cg_state* states, int nspec, double rtol, int max_iter) {
+  for (int k = threadIdx.x; k < nspec; k += blockDim.x) {
+    pd_cg_state* st = states + k;
     st->rz = 0.0;
     st->yp = 0.0;
     st->alpha = 0.0;
@@ -485,8 +487,9 @@ __global__ void k_cg_setup(pd_cg_state* st, double rtol, int max_iter) {
   }
 }
 
-void launch_cg_setup(pd_cg_state* st, double rtol, int32_t max_iter, hipStream_t s) {
-  hipLaunchKernelGGL(k_cg_setup, dim3(1), dim3(64), 0, s, st, rtol, max_iter);
+void launch_cg_setup(pd_cg_state* st, int32_t nspec, double rtol, int32_t max_iter,
+                     hipStream_t s) {
+  hipLaunchKernelGGL(k_cg_setup, dim3(1), dim3(256), 0, s, st, nspec, rtol, max_iter);
 }
 
 // y = A x0; r = S(b - y); z = M r; p = z; partials <r,z>, <r,r>, <Sb,Sb>
@@ -523,13 +526,16 @@ __global__ __launch_bounds__(PD_BLOCK) void k_cg_init(CgArgs a) {
   }
 }
 
+// one workgroup per specimen
 __global__ __launch_bounds__(PD_BLOCK) void k_cg_init_fold(CgArgs a, int ntiles) {
   __shared__ double lds4[4];
-  const double rz = fold_partials(a.part, ntiles, lds4);
-  const double rr = fold_partials(a.part + ntiles, ntiles, lds4);
-  const double bb = fold_partials(a.part + 2 * (int64_t)ntiles, ntiles, lds4);
+  int t0, nt;
+  spec_range(a.batch, ntiles, t0, nt);
+  const double rz = fold_partials(a.part + t0, nt, lds4);
+  const double rr = fold_partials(a.part + ntiles + t0, nt, lds4);
+  const double bb = fold_partials(a.part + 2 * (int64_t)ntiles + t0, nt, lds4);
   if (threadIdx.x == 0) {
-    pd_cg_state* st = a.st;
+    pd_cg_state* st = a.st + blockIdx.x;
     const double bnorm = sqrt(bb);
     const double tol = st->rtol * bnorm;
     const double rnorm = sqrt(rr);
@@ -550,10 +556,10 @@ __global__ __launch_bounds__(PD_BLOCK) void k_cg_init_fold(CgArgs a, int ntiles)
 
 // y = A p; partial <y,p>
 __global__ __launch_bounds__(PD_BLOCK) void k_cg_spmv(CgArgs a) {
-  if (a.st->done) return;
   __shared__ double lds4[4];
   const int ntiles = gridDim.x;
   const int tile = xcd_tile(blockIdx.x, ntiles);
+  if (a.st[spec_of(a.batch, tile)].done) return;
   const int p = tile * PD_BLOCK + threadIdx.x;
   double t = 0.0;
   if (p < a.A.n) {
@@ -569,23 +575,27 @@ __global__ __launch_bounds__(PD_BLOCK) void k_cg_spmv(CgArgs a) {
 
 // alpha = rz / <y,p>
 __global__ __launch_bounds__(PD_BLOCK) void k_cg_alpha(CgArgs a, int ntiles) {
-  if (a.st->done) return;
+  pd_cg_state* st = a.st + blockIdx.x;
+  if (st->done) return;
   __shared__ double lds4[4];
-  const double yp = fold_partials(a.part, ntiles, lds4);
+  int t0, nt;
+  spec_range(a.batch, ntiles, t0, nt);
+  const double yp = fold_partials(a.part + t0, nt, lds4);
   if (threadIdx.x == 0) {
-    a.st->yp = yp;
-    a.st->alpha = a.st->rz / yp;
+    st->yp = yp;
+    st->alpha = st->rz / yp;
   }
 }
 
 // x += alpha p; r -= alpha y (free rows); z = M r; partials <r,z>, <r,r>
 __global__ __launch_bounds__(PD_BLOCK) void k_cg_update(CgArgs a) {
-  if (a.st->done) return;
   __shared__ double lds4[4];
   const int ntiles = gridDim.x;
   const int tile = xcd_tile(blockIdx.x, ntiles);
+  const pd_cg_state* st = a.st + spec_of(a.batch, tile);
+  if (st->done) return;
   const int p = tile * PD_BLOCK + threadIdx.x;
-  const double alpha = a.st->alpha;
+  const double alpha = st->alpha;
   double t_rz = 0.0, t_rr = 0.0;
   if (p < a.A.n) {
     const double4 pp = a.p[p];
@@ -620,12 +630,14 @@ __global__ __launch_bounds__(PD_BLOCK) void k_cg_update(CgArgs a) {
 // beta = rz' / rz, iteration count and the monitor test ahead of the next
 // iteration (||r|| <= tol converged, iter >= max_iter failed)
 __global__ __launch_bounds__(PD_BLOCK) void k_cg_beta(CgArgs a, int ntiles) {
-  if (a.st->done) return;
+  if (a.st[blockIdx.x].done) return;
   __shared__ double lds4[4];
-  const double rz = fold_partials(a.part, ntiles, lds4);
-  const double rr = fold_partials(a.part + ntiles, ntiles, lds4);
+  int t0, nt;
+  spec_range(a.batch, ntiles, t0, nt);
+  const double rz = fold_partials(a.part + t0, nt, lds4);
+  const double rr = fold_partials(a.part + ntiles + t0, nt, lds4);
   if (threadIdx.x == 0) {
-    pd_cg_state* st = a.st;
+    pd_cg_state* st = a.st + blockIdx.x;
     const double rz_old = st->rz;
     st->rz = rz;
     st->beta = rz / rz_old;
@@ -644,10 +656,12 @@ __global__ __launch_bounds__(PD_BLOCK) void k_cg_beta(CgArgs a, int ntiles) {
 
 // p = z + beta p
 __global__ __launch_bounds__(PD_BLOCK) void k_cg_dir(CgArgs a) {
-  if (a.st->done) return;
-  const int p = xcd_tile(blockIdx.x, gridDim.x) * PD_BLOCK + threadIdx.x;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const pd_cg_state* st = a.st + spec_of(a.batch, tile);
+  if (st->done) return;
+  const int p = tile * PD_BLOCK + threadIdx.x;
   if (p >= a.A.n) return;
-  const double beta = a.st->beta;
+  const double beta = st->beta;
   const double4 zz = a.z[p];
   double4 pp = a.p[p];
   pp.x = zz.x + beta * pp.x;
@@ -662,7 +676,7 @@ void launch_cg_init(const CgArgs& a, hipStream_t s) {
   if (!a.A.n) return;
   const int g = tiles_of(a.A.n);
   hipLaunchKernelGGL(k_cg_init, dim3(g), dim3(PD_BLOCK), 0, s, a);
-  hipLaunchKernelGGL(k_cg_init_fold, dim3(1), dim3(PD_BLOCK), 0, s, a, g);
+  hipLaunchKernelGGL(k_cg_init_fold, dim3(a.batch.nspec), dim3(PD_BLOCK), 0, s, a, g);
 }
 void launch_cg_spmv(const CgArgs& a, hipStream_t s) {
   if (!a.A.n) return;
@@ -672,9 +686,9 @@ void launch_cg_spmv(const CgArgs& a, hipStream_t s) {
 void launch_cg_update(const CgArgs& a, hipStream_t s) {
   if (!a.A.n) return;
   const int g = tiles_of(a.A.n);
-  hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(PD_BLOCK), 0, s, a, g);
+  hipLaunchKernelGGL(k_cg_alpha, dim3(a.batch.nspec), dim3(PD_BLOCK), 0, s, a, g);
   hipLaunchKernelGGL(k_cg_update, dim3(g), dim3(PD_BLOCK), 0, s, a);
-  hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(PD_BLOCK), 0, s, a, g);
+  hipLaunchKernelGGL(k_cg_beta, dim3(a.batch.nspec), dim3(PD_BLOCK), 0, s, a, g);
 }
 void launch_cg_dir(const CgArgs& a, hipStream_t s) {
   if (a.A.n) hipLaunchKernelGGL(k_cg_dir, dim3(tiles_of(a.A.n)), dim3(PD_BLOCK), 0, s, a);

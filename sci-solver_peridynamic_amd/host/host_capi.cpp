@@ -123,4 +123,43 @@ void pdh_orient(int32_t n, const double* a, const double* b, const double* c, co
   }
 }
 
+// Activity, per-particle s0 and the specimen table of the params file's
+// scene.  With null arrays only *n and *nspec are returned.  A scene without
+// specimens reports nspec = 0 and s0 = pd_stretch_limit_s0 for every particle.
+int pdh_scene_extra(const char* params_path, int32_t* n, int32_t* nspec, int32_t* act, float* s0,
+                    int32_t* first, int32_t* count) {
+  SystemParameter sp(params_path);
+  if (!sp.ok()) return -1;
+  SimulationParameters& s = sp.getSimulationParameters();
+  RunningParameters& r = sp.getRunningParameters();
+  Scene scene(s, r, Scene::REGULAR_GRID);
+  if (!scene.ok()) {
+    fprintf(stderr, "%s\n", scene.error());
+    return -1;
+  }
+  *n = s.num_pd_particle;
+  *nspec = scene.numSpecimens();
+  if (!act) return 0;
+  std::vector<real4_t> x(*n), v(*n);
+  scene.initPeridynamicsParticles(act, x.data(), v.data());
+  const std::vector<float>& s0v = scene.particleS0();
+  for (int32_t i = 0; i < *n; ++i) s0[i] = s0v.empty() ? r.pd_stretch_limit_s0 : s0v[i];
+  for (int32_t k = 0; k < *nspec; ++k) {
+    first[k] = scene.specimenFirst()[k];
+    count[k] = scene.specimenCount()[k];
+  }
+  return 0;
+}
+
+// getstretch() with a stretch limit per particle, one srand(seed) first.
+void pdh_stretch_init_each(int32_t n, const float* s0, uint32_t seed, double* out) {
+  srand(seed);
+  for (int32_t i = 0; i < n; ++i) {
+    double sum = 0.0;
+    for (int k = 0; k < 12; ++k) sum += rand() / (double)RAND_MAX;
+    sum -= 6.0;
+    out[i] = ((sum * 0.01 + 1.0) * (double)s0[i]);
+  }
+}
+
 }  // extern "C"

@@ -30,6 +30,10 @@ int main(int argc, char* argv[]) {
                                   simulator.get_pd_velocity());
   scene.initPeridynamicsBoundary(simulator.get_pd_body(), simulator.get_pd_bc(),
                                  simulator.get_pd_bc_velocity());
+  if (scene.numSpecimens() > 0) {
+    simulator.setSpecimens(scene.specimenFirst(), scene.specimenCount());
+    simulator.setStretchLimits(scene.particleS0());
+  }
   simulator.makeReady();
   if (runningParams.start_step > 0 && !simulator.dataIO().loadState(runningParams.start_step)) {
     fprintf(stderr, "could not load state %d\n", runningParams.start_step);

@@ -71,7 +71,14 @@ RunningParameters::RunningParameters()
       contact_mode(0),
       walls(-1),
       stretch_seed(1973u),
-      print_steps(0) {
+      print_steps(0),
+      batch_count(0),
+      batch_seed(2025u),
+      batch_min_particles(5000),
+      batch_max_particles(60000),
+      batch_max_aspect(4.0),
+      batch_s0_min(0.01),
+      batch_s0_max(0.03) {
   obj_file[0] = 0;
   saving_path[0] = 0;
 }
@@ -185,7 +192,14 @@ void SystemParameter::loadParametersFromFile(const char* _paramFile) {
     if (name == "cg_precision") r.cg_precision = (value == "mixed" || value == "1") ? 1 : 0;
     if (name == "cg_inner_rtol") r.cg_inner_rtol = atof(v);
     if (name == "cg_max_refine") r.cg_max_refine = atoi(v);
-    if (name == "scene") r.scene_lattice = (value == "lattice") ? 1 : 0;
+    if (name == "scene") r.scene_lattice = (value == "lattice") ? 1 : (value == "batch" ? 2 : 0);
+    if (name == "batch_count") r.batch_count = atoi(v);
+    if (name == "batch_seed") r.batch_seed = (unsigned)strtoul(v, nullptr, 10);
+    if (name == "batch_min_particles") r.batch_min_particles = atoi(v);
+    if (name == "batch_max_particles") r.batch_max_particles = atoi(v);
+    if (name == "batch_max_aspect") r.batch_max_aspect = atof(v);
+    if (name == "batch_s0_min") r.batch_s0_min = atof(v);
+    if (name == "batch_s0_max") r.batch_s0_max = atof(v);
     if (name == "lattice_nx") r.lattice_nx = atoi(v);
     if (name == "lattice_ny") r.lattice_ny = atoi(v);
     if (name == "lattice_nz") r.lattice_nz = atoi(v);

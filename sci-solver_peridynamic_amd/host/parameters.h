@@ -130,7 +130,8 @@ struct RunningParameters {
   int cg_precision;             // 0 FP64, 1 float inner solve + FP64 refinement
   double cg_inner_rtol;         // precision 1: inner reduction (0: 1e-5)
   int cg_max_refine;            // precision 1: refinement limit (0: 20)
-  int scene_lattice;            // 1: box lattice scene instead of a mesh fill
+  int scene_lattice;            // 1: box lattice scene, 2: batch of lattice specimens
+                                //    (0: the reference's mesh fill)
   int lattice_nx, lattice_ny, lattice_nz;
   double lattice_spacing;       // [m]
   double lattice_jitter;        // uniform jitter, fraction of the spacing
@@ -151,6 +152,12 @@ struct RunningParameters {
   int walls;                    // 1: clamp to the boundary box (-1: mesh 1, lattice 0)
   unsigned stretch_seed;        // srand seed of getstretch (1973)
   int print_steps;              // 1: one JSON line per step on stdout
+  // scene batch: independent tension specimens
+  int batch_count;
+  unsigned batch_seed;
+  int batch_min_particles, batch_max_particles;  // log-uniform
+  double batch_max_aspect;                       // long axis z : x = y, uniform in [1, max]
+  double batch_s0_min, batch_s0_max;             // s0 per specimen, uniform
 };
 
 class SystemParameter {

@@ -15,6 +15,17 @@
 // Dirichlet layers: z-layers k < bc_fix_zmin_layers are held at v = 0,
 // k >= nz - bc_move_zmax_layers move at (0, 0, bc_move_zmax_velocity).
 //
+// Batch (scene batch): batch_count independent tension specimens.  Specimen
+// k draws, from scene_uniform(batch_seed, 3k + {0, 1, 2}), its particle count
+// log-uniformly in [batch_min, batch_max], its aspect a uniformly in
+// [1, batch_max_aspect] (nx = ny = round((N/a)^(1/3)), nz = round(N/nx^2),
+// the long axis along the pull) and its s0 uniformly in [s0_min, s0_max].
+// Specimens stand on an xy grid, one horizon plus the jitter apart at least;
+// each is jittered like the lattice (index (k << 40) + 3p + axis) and loaded
+// like it (bc_* layers).  Rows of specimen k start on a multiple of 256;
+// the rows between specimens are INVISIBLE copies of the specimen's first
+// particle.
+//
 // Projectile (body 1): the n points of a cubic lattice of spacing s closest
 // to the sphere centre (ties by lattice index), s the largest spacing whose
 // lattice has at least n points strictly inside radius R (bisection), then
@@ -55,6 +66,15 @@ Scene::Scene(SimulationParameters& simParams, RunningParameters& runningParams,
     return;
   }
   const RunningParameters& r = runningParams_;
+  if (r.scene_lattice == 2) {
+    if (!buildBatch()) return;
+    simParams_.num_pd_particle = (int)pos_.size();
+    simParams_.num_total_particle = simParams_.num_sph_particle + simParams_.num_pd_particle;
+    simParams_.num_clists = simParams_.num_pd_particle;
+    while (simParams_.num_clists % 8 != 0) simParams_.num_clists++;
+    ok_ = true;
+    return;
+  }
   if (r.lattice_nx <= 0 || r.lattice_ny <= 0 || r.lattice_nz <= 0 || !(r.lattice_spacing > 0.0)) {
     err_ = "lattice_nx/ny/nz and lattice_spacing must be positive";
     return;
@@ -189,7 +209,7 @@ void Scene::buildProjectile() {
 void Scene::initPeridynamicsParticles(int* pd_activity, real4_t* pd_position,
                                       real4_t* pd_velocity) {
   for (size_t p = 0; p < pos_.size(); ++p) {
-    if (pd_activity) pd_activity[p] = ACTIVE;
+    if (pd_activity) pd_activity[p] = act_.empty() ? ACTIVE : act_[p];
     if (pd_position) pd_position[p] = pos_[p];
     if (pd_velocity) pd_velocity[p] = vel_[p];
   }
@@ -294,5 +314,94 @@ bool Scene::buildMesh() {
   body_.assign(pos_.size(), 0);
   bc_.assign(pos_.size(), 0);
   vbc_.assign(pos_.size(), make_real4(0, 0, 0, 0));
+  return true;
+}
+
+bool Scene::buildBatch() {
+  const RunningParameters& r = runningParams_;
+  const int nspec = r.batch_count;
+  const double h = r.lattice_spacing;
+  if (nspec < 1 || !(h > 0.0) || r.batch_min_particles < 8 ||
+      r.batch_max_particles < r.batch_min_particles || !(r.batch_max_aspect >= 1.0)) {
+    err_ = "batch_count, lattice_spacing, batch_min/max_particles and batch_max_aspect are invalid";
+    return false;
+  }
+  const double jit = r.lattice_jitter;
+  struct Spec {
+    int nx, ny, nz;
+    float s0;
+  };
+  std::vector<Spec> spec(nspec);
+  int max_nx = 0;
+  for (int k = 0; k < nspec; ++k) {
+    const double u0 = scene_uniform(r.batch_seed, 3ull * k);
+    const double u1 = scene_uniform(r.batch_seed, 3ull * k + 1);
+    const double u2 = scene_uniform(r.batch_seed, 3ull * k + 2);
+    const double lo = log((double)r.batch_min_particles), hi = log((double)r.batch_max_particles);
+    const double N = exp(lo + u0 * (hi - lo));
+    const double a = 1.0 + u1 * (r.batch_max_aspect - 1.0);
+    Spec& s = spec[k];
+    s.nx = s.ny = std::max(2, (int)llround(cbrt(N / a)));
+    s.nz = std::max(2, (int)llround(N / ((double)s.nx * s.ny)));
+    s.s0 = (float)(r.batch_s0_min + u2 * (r.batch_s0_max - r.batch_s0_min));
+    max_nx = std::max(max_nx, s.nx);
+  }
+  const int cols = (int)ceil(sqrt((double)nspec));
+  const double gap = simParams_.pd_horizon + 2.0 * jit * h + h;
+  const double pitch = (max_nx - 1) * h + gap;
+  pos_.clear();
+  vel_.clear();
+  body_.clear();
+  bc_.clear();
+  vbc_.clear();
+  act_.clear();
+  s0_.clear();
+  spec_first_.clear();
+  spec_count_.clear();
+  for (int k = 0; k < nspec; ++k) {
+    const Spec& s = spec[k];
+    // pad to the next 256-row boundary with INVISIBLE copies of the previous
+    // specimen's first particle
+    while (pos_.size() % 256 != 0) {
+      const size_t src = spec_first_.back();
+      pos_.push_back(pos_[src]);
+      vel_.push_back(make_real4(0, 0, 0, 0));
+      body_.push_back(0);
+      bc_.push_back(0);
+      vbc_.push_back(make_real4(0, 0, 0, 0));
+      act_.push_back(INVISIBLE);
+      s0_.push_back(s0_.back());
+    }
+    spec_first_.push_back((int)pos_.size());
+    spec_count_.push_back(s.nx * s.ny * s.nz);
+    const double org[3] = {(k % cols) * pitch, (k / cols) * pitch, 0.0};
+    for (int kz = 0; kz < s.nz; ++kz)
+      for (int j = 0; j < s.ny; ++j)
+        for (int i = 0; i < s.nx; ++i) {
+          const uint64_t p = ((uint64_t)kz * s.ny + j) * s.nx + i;
+          const int idx[3] = {i, j, kz};
+          double c[3];
+          for (int a = 0; a < 3; ++a) {
+            const double u = scene_uniform(r.lattice_seed, ((uint64_t)k << 40) + 3 * p + a);
+            c[a] = org[a] + idx[a] * h + (2.0 * u - 1.0) * jit * h;
+          }
+          pos_.push_back(make_real4(c[0], c[1], c[2], 0.0));
+          real4_t v = make_real4(0, 0, 0, 0), vb = v;
+          int bc = 0;
+          if (kz < r.bc_fix_zmin_layers) {
+            bc = 1;
+          } else if (kz >= s.nz - r.bc_move_zmax_layers) {
+            bc = 1;
+            vb = make_real4(0.0, 0.0, r.bc_move_zmax_velocity, 0.0);
+            v = vb;
+          }
+          vel_.push_back(v);
+          body_.push_back(0);
+          bc_.push_back(bc);
+          vbc_.push_back(vb);
+          act_.push_back(ACTIVE);
+          s0_.push_back(s.s0);
+        }
+  }
   return true;
 }

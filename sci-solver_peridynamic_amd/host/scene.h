@@ -28,6 +28,15 @@ class Scene {
   // Added: body index, Dirichlet flag and prescribed velocity per particle.
   void initPeridynamicsBoundary(int* pd_body, int* pd_bc, real4_t* pd_bc_velocity);
 
+  // Added (scene batch): independent specimens.  Specimen k owns rows
+  // [first[k], first[k] + count[k]); the rows between specimens are
+  // INVISIBLE padding (each specimen starts on a 256-row boundary).
+  int numSpecimens() const { return (int)spec_first_.size(); }
+  const std::vector<int>& specimenFirst() const { return spec_first_; }
+  const std::vector<int>& specimenCount() const { return spec_count_; }
+  // stretch limit s0 per particle (empty: pd_stretch_limit_s0 for all)
+  const std::vector<float>& particleS0() const { return s0_; }
+
   bool ok() const { return ok_; }
   const char* error() const { return err_; }
 
@@ -35,6 +44,7 @@ class Scene {
   void buildLattice();
   void buildProjectile();
   bool buildMesh();
+  bool buildBatch();
 
   static constexpr double kMeshJitter = 0.01;  // JITTER, examples/scene.h:20
 
@@ -46,6 +56,9 @@ class Scene {
   std::vector<int> body_;
   std::vector<int> bc_;
   std::vector<real4_t> vbc_;
+  std::vector<int> act_;          // empty: all ACTIVE
+  std::vector<int> spec_first_, spec_count_;
+  std::vector<float> s0_;
   std::vector<double> mesh_xyz_;  // scaled mesh vertices (double)
   std::vector<int> mesh_tri_;
   bool ok_ = false;

@@ -96,8 +96,9 @@ void Simulator::makeReady() {
 void Simulator::initPeridynamicsBonds() {
   const int n = hostSimParams_.num_pd_particle;
   srand(runningParams_.stretch_seed);
-  const double s0 = (double)runningParams_.pd_stretch_limit_s0;
+  const bool each = s0_.size() == (size_t)n;
   for (int i = 0; i < n; ++i) {
+    const double s0 = (double)(each ? s0_[i] : runningParams_.pd_stretch_limit_s0);
     double sum = 0.0;
     for (int k = 0; k < 12; ++k) sum += rand() / (double)RAND_MAX;
     sum -= 6.0;
@@ -118,6 +119,11 @@ void Simulator::initPeridynamicsBonds() {
   if (!check(pd_upload(ctx_, PD_FIELD_SOLUTION, zero.data(), v4), "upload solution")) return;
   if (!check(pd_build_bonds(ctx_, &bond_stats_), "pd_build_bonds")) return;
   hostSimParams_.pd_max_num_bonds = bond_stats_.max_bonds;
+  if (!spec_first_.empty() &&
+      !check(pd_set_specimens(ctx_, (int32_t)spec_first_.size(), spec_first_.data(),
+                              spec_count_.data()),
+             "pd_set_specimens"))
+    return;
   printf("    Number of Peridynamics particle bonds: %d -> %d\n", bond_stats_.min_bonds,
          bond_stats_.max_bonds);
   printf("    Sum of bonds: %lld\n", (long long)bond_stats_.total_bonds);

@@ -32,6 +32,14 @@ class Simulator {
   real4_t* get_pd_position();
   real4_t* get_pd_velocity();
 
+  // added: independent specimens and per-particle stretch limits (scene
+  // batch); call before makeReady()
+  void setSpecimens(const std::vector<int>& first, const std::vector<int>& count) {
+    spec_first_ = first;
+    spec_count_ = count;
+  }
+  void setStretchLimits(const std::vector<float>& s0) { s0_ = s0; }
+
   // added: body index, Dirichlet flag / velocity (filled by the Scene)
   int* get_pd_body();
   int* get_pd_bc();
@@ -63,6 +71,8 @@ class Simulator {
   std::vector<int> activity_, body_, bc_;
   std::vector<real4_t> position_, velocity_, bc_velocity_;
   std::vector<double> stretch_;
+  std::vector<int> spec_first_, spec_count_;
+  std::vector<float> s0_;
   pd_step_stats last_ = {};
   pd_bond_stats bond_stats_ = {};
   double time_total_ms_ = 0.0, time_assemble_ms_ = 0.0, time_solve_ms_ = 0.0,

@@ -81,6 +81,12 @@ class StepStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class SpecimenStats(C.Structure):
+    _fields_ = [("cg_iterations", C.c_int32), ("converged", C.c_int32),
+                ("residual_norm", C.c_double), ("b_norm", C.c_double),
+                ("refinements", C.c_int32)]
+
+
 class BondStats(C.Structure):
     _fields_ = [("total_bonds", C.c_int64), ("min_bonds", C.c_int32),
                 ("max_bonds", C.c_int32), ("slots", C.c_int64), ("ms_build", C.c_double)]
@@ -144,11 +150,13 @@ def lib():
         L.pd_synchronize.argtypes = [vp]
         L.pd_bench_spmv.argtypes = [vp, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.pd_spmv_bytes.argtypes = [vp, C.POINTER(C.c_double)]
+        L.pd_set_specimens.argtypes = [vp, C.c_int32, vp, vp]
+        L.pd_get_specimen_stats.argtypes = [vp, C.c_int32, C.POINTER(SpecimenStats)]
         for name in ("pd_ctx_create", "pd_ctx_destroy", "pd_upload", "pd_download",
                      "pd_build_bonds", "pd_set_bonds", "pd_get_bonds", "pd_get_matrix",
                      "pd_step", "pd_assemble", "pd_solve", "pd_update", "pd_commit_stretch",
                      "pd_spmv", "pd_backup", "pd_restore", "pd_synchronize", "pd_bench_spmv",
-                     "pd_spmv_bytes"):
+                     "pd_spmv_bytes", "pd_set_specimens", "pd_get_specimen_stats"):
             getattr(L, name).restype = C.c_int
         _lib = L
     return _lib
@@ -266,6 +274,18 @@ class Context:
             self._check(rc)
         return st
 
+    def set_specimens(self, first, count):
+        first = np.ascontiguousarray(first, np.int32)
+        count = np.ascontiguousarray(count, np.int32)
+        self._check(lib().pd_set_specimens(self._h, len(first), _ptr(first), _ptr(count)))
+
+    def specimen_stats(self, nspec):
+        out = (SpecimenStats * nspec)()
+        self._check(lib().pd_get_specimen_stats(self._h, nspec, out))
+        return [dict(cg_iterations=o.cg_iterations, converged=o.converged,
+                     residual_norm=o.residual_norm, b_norm=o.b_norm, refinements=o.refinements)
+                for o in out]
+
     def spmv(self, x4):
         x4 = np.ascontiguousarray(x4, dtype=np.float64)
         y = np.zeros((self.n, 4))
@@ -312,6 +332,10 @@ def host_lib():
         L.pdh_describe.restype = C.c_int
         L.pdh_stretch_init.argtypes = [C.c_int32, C.c_float, C.c_uint32, vp]
         L.pdh_load_obj.argtypes = [C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), vp, vp]
+        L.pdh_scene_extra.argtypes = [C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), vp,
+                                      vp, vp, vp]
+        L.pdh_scene_extra.restype = C.c_int
+        L.pdh_stretch_init_each.argtypes = [C.c_int32, vp, C.c_uint32, vp]
         L.pdh_load_obj.restype = C.c_int
         L.pdh_mesh_inside.argtypes = [C.c_int32, vp, C.c_int32, vp, C.c_int32, vp, vp]
         L.pdh_orient.argtypes = [C.c_int32, vp, vp, vp, vp, vp, vp]
@@ -397,6 +421,31 @@ def scene(params_path, obj_file=None):
                      _ptr(out["v4"]), _ptr(out["body"]), _ptr(out["bc"]), _ptr(out["vbc"]))
     if rc != 0:
         raise RuntimeError(f"cannot build the scene of {params_path}")
+    return out
+
+
+def scene_extra(params_path):
+    """Activity, per-particle s0 and the specimen table (first, count) of the
+    params file's scene."""
+    L = host_lib()
+    n, ns = C.c_int32(), C.c_int32()
+    if L.pdh_scene_extra(params_path.encode(), C.byref(n), C.byref(ns), None, None, None,
+                         None) != 0:
+        raise RuntimeError(f"cannot build the scene of {params_path}")
+    act = np.zeros(n.value, np.int32)
+    s0 = np.zeros(n.value, np.float32)
+    first = np.zeros(ns.value, np.int32)
+    count = np.zeros(ns.value, np.int32)
+    L.pdh_scene_extra(params_path.encode(), C.byref(n), C.byref(ns), _ptr(act), _ptr(s0),
+                      _ptr(first), _ptr(count))
+    return dict(act=act, s0=s0, first=first, count=count)
+
+
+def stretch_init_each(s0, seed=1973):
+    """getstretch() limits with one s0 per particle (float32 array)."""
+    s0 = np.ascontiguousarray(s0, np.float32)
+    out = np.zeros(len(s0))
+    host_lib().pdh_stretch_init_each(len(s0), _ptr(s0), seed, _ptr(out))
     return out
 
 
