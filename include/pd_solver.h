@@ -110,6 +110,17 @@ typedef struct pd_step_params {
    * than contact_radius, same body included (the pair set of the
    * reference's collidePDParticles, core/simulator.cuh:672-765) */
   int32_t contact_mode;
+  /* SpMV slots per trip (2, 4 or 8; the summation order, and so every
+   * result, is the same for all); 0: the value pd_tune_spmv picked for this
+   * precision, else 4 */
+  int32_t spmv_unroll;
+  /* pd_step only: 1 replays the step from hipGraphs captured on first use
+   * (assembly, stretch commit, the solve's start and as many iterations as
+   * the last step needed, then the position update), one launch and one
+   * wait per step; the host-driven loop finishes any specimen still
+   * iterating.  Needs contact off and track_gap 0.  Results are identical
+   * to graph = 0. */
+  int32_t graph;
 } pd_step_params;
 
 typedef struct pd_step_stats {
@@ -258,6 +269,13 @@ int pd_bench_spmv(pd_ctx* ctx, int32_t iters, double* ms_per_launch,
  * with precision 1 (float inner SpMV), live blocks * (6*4 + 4) +
  * N * (6*4 + 4 + 2*12). */
 int pd_spmv_bytes(pd_ctx* ctx, double* bytes);
+
+/* Autotuner (no counterpart in the reference): times `iters` launches of
+ * each SpMV variant (2, 4 and 8 slots per trip) on the resident matrix of
+ * the last assembly's precision, remembers the fastest for that precision
+ * (used when spmv_unroll = 0) and returns the mean times in ms3[3] and the
+ * choice in *best. */
+int pd_tune_spmv(pd_ctx* ctx, int32_t iters, double* ms3, int32_t* best);
 
 /* Independent specimens in one context (no counterpart in the reference,
  * which runs one body per process).  Specimen k owns rows [first[k],

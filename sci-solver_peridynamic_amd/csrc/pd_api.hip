@@ -9,6 +9,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -57,6 +58,8 @@ struct pd_ctx {
   int last_mx_iters = 0;
   int last_solve_prec = -1;
   int mat_prec = -1;  // precision of the last assembly (-1: none)
+  int unroll[2] = {4, 4};  // SpMV slots per trip in use (FP64, mixed)
+  int tuned[2] = {0, 0};   // pd_tune_spmv's choice (0: not tuned)
 
   // CG
   double4 *r = nullptr, *z = nullptr, *p = nullptr, *y = nullptr;
@@ -109,6 +112,14 @@ struct pd_ctx {
   uint32_t* h_zone_count = nullptr;
 
   std::vector<hipEvent_t> ev;  // event pool for timing
+
+  // captured steps (graph = 1): assembly + solve start + K iterations, and
+  // the position update; step scalars come from consts
+  pd_step_consts* consts = nullptr;
+  pd_step_consts* h_consts = nullptr;
+  hipGraphExec_t g_solve = nullptr, g_update = nullptr;
+  std::string graph_key;
+  int graph_k = 0;
 };
 
 namespace {
@@ -209,6 +220,11 @@ int check_params(pd_ctx* c, const pd_step_params* prm) {
     return fail(c, PD_ERR_ARG, "contact_volume must be non-zero");
   if (prm->contact_mode != 0 && prm->contact_mode != 1)
     return fail(c, PD_ERR_ARG, "contact_mode must be 0 (between bodies) or 1 (all pairs)");
+  if (prm->spmv_unroll != 0 && prm->spmv_unroll != 2 && prm->spmv_unroll != 4 &&
+      prm->spmv_unroll != 8)
+    return fail(c, PD_ERR_ARG, "spmv_unroll must be 0 (tuned / default), 2, 4 or 8");
+  for (int k = 0; k < 2; ++k)
+    c->unroll[k] = prm->spmv_unroll ? prm->spmv_unroll : (c->tuned[k] ? c->tuned[k] : 4);
   if (prm->precision != 0 && prm->precision != 1)
     return fail(c, PD_ERR_ARG, "precision must be 0 (FP64) or 1 (float inner solve + FP64 refinement)");
   if (prm->precision == 1 && (prm->inner_rtol < 0.0 || prm->max_refine < 0 || prm->inner_max_iter < 0))
@@ -351,6 +367,7 @@ pd::SpmvView spmv_view(pd_ctx* c) {
   A.col = c->col;
   A.val = c->val;
   A.diag = c->diag;
+  A.unroll = c->unroll[0];
   return A;
 }
 
@@ -386,6 +403,7 @@ pd::MxArgs mx_args(pd_ctx* c, int precond) {
   a.part = c->part;
   a.st = c->mx;
   a.precond = precond;
+  a.unroll = c->unroll[1];
   return a;
 }
 
@@ -482,7 +500,8 @@ int enqueue_contact(pd_ctx* c, const pd_step_params* prm) {
   return PD_OK;
 }
 
-int enqueue_assemble(pd_ctx* c, const pd_step_params* prm) {
+int enqueue_assemble(pd_ctx* c, const pd_step_params* prm,
+                     const pd_step_consts* consts = nullptr) {
   if (prm->precision == 1)
     if (int e = ensure_mixed(c)) return e;
   pd::launch_reset_counters(c->counters, c->stream);
@@ -523,6 +542,7 @@ int enqueue_assemble(pd_ctx* c, const pd_step_params* prm) {
   a.gap_val = c->gap_val;
   a.gap_q = c->gap_q;
   a.counters = c->counters;
+  a.consts = consts;
   pd::launch_assemble(a, c->stream);
   CHECK_LAUNCH(c, "assemble");
   c->mat_prec = prm->precision;
@@ -559,26 +579,35 @@ void read_assemble_stats(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st
 }
 
 // CG on the device; the host waits once per chunk of iterations.
-int run_solve_mixed(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st);
+int run_solve_mixed(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st, int resumed);
 
-int run_solve(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st) {
+// resumed > 0: a captured step already ran the start and `resumed`
+// iterations; the device state is in h_st / h_mx.  No per-launch timing then.
+int run_solve(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st, int resumed = 0) {
   if (c->mat_prec != prm->precision)
     return fail(c, PD_ERR_STATE, "the matrix was not assembled with this precision");
-  if (prm->precision == 1) return run_solve_mixed(c, prm, st);
+  if (prm->precision == 1) return run_solve_mixed(c, prm, st, resumed);
   const pd::CgArgs a = cg_args(c, prm->precond);
   const int ns = c->nspec;
-  pd::launch_cg_setup(c->st, ns, prm->rtol, prm->max_iter, c->stream);
-  pd::launch_cg_init(a, c->stream);
-  CHECK_LAUNCH(c, "cg init");
-  int launched = 0;
+  if (!resumed) {
+    pd::launch_cg_setup(c->st, ns, prm->rtol, prm->max_iter, c->stream);
+    pd::launch_cg_init(a, c->stream);
+    CHECK_LAUNCH(c, "cg init");
+  }
+  int launched = resumed;
   int syncs = 0;
   // launch one iteration fewer than the previous solve needed, then single
   // iterations, then doubling chunks: launches after convergence are rare
-  int chunk = std::max(1, c->last_iters - 1);
+  int chunk = resumed ? 1 : std::max(1, c->last_iters - 1);
   int next = 1;
   int singles = 2;
-  const bool timing = prm->timing != 0;
-  for (;;) {
+  const bool timing = prm->timing != 0 && !resumed;
+  bool all_done = false;
+  if (resumed) {
+    all_done = true;
+    for (int k = 0; k < ns; ++k) all_done = all_done && c->h_st[k].done;
+  }
+  while (!all_done) {
     if (timing)
       if (int e = ensure_events(c, 2 * (size_t)(launched + chunk) + 8)) return e;
     for (int i = 0; i < chunk; ++i) {
@@ -594,7 +623,7 @@ int run_solve(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st) {
                              c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     ++syncs;
-    bool all_done = true;
+    all_done = true;
     for (int k = 0; k < ns; ++k) all_done = all_done && c->h_st[k].done;
     if (all_done) break;
     if (launched > prm->max_iter + 1)
@@ -642,10 +671,10 @@ int run_solve(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st) {
 // iterations, then the correction, the FP64 residual and its test; every
 // kernel returns at once unless the device-side phase asks for it, so the
 // host only predicts how many inner iterations to enqueue before it waits.
-int run_solve_mixed(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st) {
+int run_solve_mixed(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st, int resumed) {
   const pd::MxArgs a = mx_args(c, prm->precond);
   const MixedLimits lim = mixed_limits(prm);
-  const bool timing = prm->timing != 0;
+  const bool timing = prm->timing != 0 && !resumed;
   const int ns = c->nspec;
   // event pairs: [0] first residual, then per chunk `chunk` inner SpMVs and
   // one residual
@@ -657,24 +686,35 @@ int run_solve_mixed(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st) {
     used += 2;
     return PD_OK;
   };
-  pd::launch_mx_setup(c->mx, ns, prm->rtol, prm->max_iter, lim.inner_rtol, lim.inner_max,
-                      lim.max_refine, c->stream);
-  if (timing) {
-    if (int e = pair(resid_ev)) return e;
-    HIPCHK(c, hipEventRecord(c->ev[resid_ev.back()], c->stream));
+  if (!resumed) {
+    pd::launch_mx_setup(c->mx, ns, prm->rtol, prm->max_iter, lim.inner_rtol, lim.inner_max,
+                        lim.max_refine, c->stream);
+    if (timing) {
+      if (int e = pair(resid_ev)) return e;
+      HIPCHK(c, hipEventRecord(c->ev[resid_ev.back()], c->stream));
+    }
+    pd::launch_mx_resid(a, c->stream);
+    if (timing) HIPCHK(c, hipEventRecord(c->ev[resid_ev.back() + 1], c->stream));
+    CHECK_LAUNCH(c, "mixed residual");
   }
-  pd::launch_mx_resid(a, c->stream);
-  if (timing) HIPCHK(c, hipEventRecord(c->ev[resid_ev.back() + 1], c->stream));
-  CHECK_LAUNCH(c, "mixed residual");
-  int chunk = std::max(1, c->last_mx_iters);
+  int chunk = resumed ? 1 : std::max(1, c->last_mx_iters);
   int next = 1;
   int singles = 2;
   int syncs = 0;
   std::vector<int> iter_seen(ns, 0), resid_seen(ns, 1);
+  bool all_done = false;
+  if (resumed) {
+    all_done = true;
+    for (int k = 0; k < ns; ++k) {
+      iter_seen[k] = c->h_mx[k].iter;
+      resid_seen[k] = c->h_mx[k].residuals;
+      all_done = all_done && c->h_mx[k].phase == 3;
+    }
+  }
   double spmv_ms = 0.0, resid_ms = 0.0;
   int spmv_ran = 0, resid_ran = 0;
   std::vector<int> chunk_spmv;
-  for (;;) {
+  while (!all_done) {
     chunk_spmv.clear();
     for (int i = 0; i < chunk; ++i) {
       if (timing) {
@@ -704,7 +744,8 @@ int run_solve_mixed(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st) {
     // a launch did work if any specimen needed it: the inner SpMVs that ran
     // are the first max_k (iterations of k in this chunk) of the chunk
     int ran = 0;
-    bool resid_ran_now = false, all_done = true;
+    bool resid_ran_now = false;
+    all_done = true;
     for (int k = 0; k < ns; ++k) {
       const pd_mx_state& h = c->h_mx[k];
       ran = std::max(ran, h.iter - iter_seen[k]);
@@ -718,7 +759,7 @@ int run_solve_mixed(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st) {
       for (int k = 0; k < ran; ++k)
         spmv_ms += elapsed(c->ev[chunk_spmv[k]], c->ev[chunk_spmv[k] + 1]);
       spmv_ran += ran;
-      if (syncs == 1) {
+      if (syncs == 1 && !resumed) {
         resid_ms += elapsed(c->ev[resid_ev[0]], c->ev[resid_ev[0] + 1]);
         ++resid_ran;
       }
@@ -769,15 +810,158 @@ int run_solve_mixed(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st) {
   return PD_OK;
 }
 
-int enqueue_update(pd_ctx* c, const pd_step_params* prm) {
+int enqueue_update(pd_ctx* c, const pd_step_params* prm,
+                   const pd_step_consts* consts = nullptr) {
   double lo[3], hi[3];
   for (int k = 0; k < 3; ++k) {
     lo[k] = prm->bmin[k] + prm->wall_radius;
     hi[k] = prm->bmax[k] - prm->wall_radius;
   }
   pd::launch_update(c->n, c->x, c->v, c->sol, c->act, prm->integrator, prm->dt, prm->walls, lo, hi,
-                    prm->restitution, c->stream);
+                    prm->restitution, consts, c->stream);
   CHECK_LAUNCH(c, "update");
+  return PD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Captured steps (graph = 1).  One hipGraph holds the assembly, the stretch
+// commit (a copy, so the captured pointers keep their roles), the solve's
+// start and K iterations (mixed: K inner iterations, the correction and the
+// residual test), and the pinned copies of the solver states and counters;
+// a second graph holds the position update.  Step scalars (dt, clock) are
+// read from device memory, so the graphs are replayed every step.  If a
+// specimen is still iterating after the graph, the host-driven loop goes on
+// from the device state before the update graph runs.
+// ---------------------------------------------------------------------------
+void destroy_graphs(pd_ctx* c) {
+  if (c->g_solve) (void)hipGraphExecDestroy(c->g_solve);
+  if (c->g_update) (void)hipGraphExecDestroy(c->g_update);
+  c->g_solve = c->g_update = nullptr;
+  c->graph_key.clear();
+}
+
+std::string graph_key(pd_ctx* c, const pd_step_params* prm, int K) {
+  const MixedLimits lim = mixed_limits(prm);
+  char buf[1024];
+  snprintf(buf, sizeof(buf),
+           "%d %d %d %a %d %a %d %d %d %a %a %a %a %a %a %a %a %d %d %d %p %p %p %p %p %p %p",
+           prm->precision, prm->precond, prm->integrator, prm->rtol, prm->max_iter,
+           lim.inner_rtol, lim.inner_max, lim.max_refine, prm->walls, prm->bmin[0],
+           prm->bmin[1], prm->bmin[2], prm->bmax[0], prm->bmax[1], prm->bmax[2],
+           prm->wall_radius, prm->restitution, c->unroll[prm->precision], c->nspec, K,
+           (void*)c->s, (void*)c->s_new, (void*)c->val, (void*)c->col, (void*)c->val32,
+           (void*)c->tile_spec, (void*)c->st);
+  return std::string(buf);
+}
+
+int capture(pd_ctx* c, hipGraphExec_t* out, const std::function<int()>& body) {
+  hipGraph_t g = nullptr;
+  HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  const int e = body();
+  const hipError_t ee = hipStreamEndCapture(c->stream, &g);
+  if (e != PD_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return e;
+  }
+  if (ee != hipSuccess || !g)
+    return fail(c, PD_ERR_HIP, std::string("stream capture: ") + hipGetErrorString(ee));
+  const hipError_t ei = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (ei != hipSuccess)
+    return fail(c, PD_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+  return PD_OK;
+}
+
+int capture_step(pd_ctx* c, const pd_step_params* prm, int K) {
+  destroy_graphs(c);
+  const int ns = c->nspec;
+  int e = capture(c, &c->g_solve, [&]() -> int {
+    if (int e2 = enqueue_assemble(c, prm, c->consts)) return e2;
+    pd::launch_copy(c->s, c->s_new, c->n, c->stream);  // core/simulator.cuh:2612
+    if (prm->precision == 1) {
+      const pd::MxArgs a = mx_args(c, prm->precond);
+      const MixedLimits lim = mixed_limits(prm);
+      pd::launch_mx_setup(c->mx, ns, prm->rtol, prm->max_iter, lim.inner_rtol, lim.inner_max,
+                          lim.max_refine, c->stream);
+      pd::launch_mx_resid(a, c->stream);
+      for (int i = 0; i < K; ++i) {
+        pd::launch_mx_spmv(a, c->stream);
+        pd::launch_mx_update(a, c->stream);
+        pd::launch_mx_dir(a, c->stream);
+      }
+      pd::launch_mx_apply(a, c->stream);
+      pd::launch_mx_resid(a, c->stream);
+      HIPCHK(c, hipMemcpyAsync(c->h_mx, c->mx, sizeof(pd_mx_state) * ns, hipMemcpyDeviceToHost,
+                               c->stream));
+    } else {
+      const pd::CgArgs a = cg_args(c, prm->precond);
+      pd::launch_cg_setup(c->st, ns, prm->rtol, prm->max_iter, c->stream);
+      pd::launch_cg_init(a, c->stream);
+      for (int i = 0; i < K; ++i) {
+        pd::launch_cg_spmv(a, c->stream);
+        pd::launch_cg_update(a, c->stream);
+        pd::launch_cg_dir(a, c->stream);
+      }
+      HIPCHK(c, hipMemcpyAsync(c->h_st, c->st, sizeof(pd_cg_state) * ns, hipMemcpyDeviceToHost,
+                               c->stream));
+    }
+    return PD_OK;
+  });
+  if (e == PD_OK) e = capture(c, &c->g_update, [&]() { return enqueue_update(c, prm, c->consts); });
+  if (e != PD_OK) {
+    destroy_graphs(c);
+    return e;
+  }
+  c->graph_key = graph_key(c, prm, K);
+  c->graph_k = K;
+  return PD_OK;
+}
+
+int run_step_graph(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st) {
+  if (prm->contact_radius > 0.0)
+    return fail(c, PD_ERR_ARG, "captured steps (graph = 1) need contact off");
+  if (prm->track_gap) return fail(c, PD_ERR_ARG, "captured steps (graph = 1) do not track gaps");
+  if (prm->precision == 1)
+    if (int e = ensure_mixed(c)) return e;
+  if (!c->consts) {
+    if (int e = dalloc(c, &c->consts, 1)) return e;
+    if (hipHostMalloc((void**)&c->h_consts, sizeof(pd_step_consts), hipHostMallocDefault) !=
+        hipSuccess)
+      return fail(c, PD_ERR_HIP, "hipHostMalloc failed");
+  }
+  const int last = prm->precision == 1 ? c->last_mx_iters : c->last_iters;
+  int K = c->graph_k;
+  if (!c->g_solve || last + 1 > K || last + 4 < K) K = std::max(2, last + 1);
+  if (graph_key(c, prm, K) != c->graph_key)
+    if (int e = capture_step(c, prm, K)) return e;
+  const bool timing = prm->timing != 0;
+  if (timing)
+    if (int e = ensure_events(c, 8)) return e;
+  c->h_consts->dt = prm->dt;
+  c->h_consts->dt2 = prm->integrator == 1 ? (prm->dt * prm->dt) / 4.0 : prm->dt * prm->dt;
+  c->h_consts->clock = prm->clock_scale;
+  c->h_consts->gravity = prm->gravity;
+  HIPCHK(c, hipMemcpyAsync(c->consts, c->h_consts, sizeof(pd_step_consts), hipMemcpyHostToDevice,
+                           c->stream));
+  if (timing) HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+  HIPCHK(c, hipGraphLaunch(c->g_solve, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->mat_prec = prm->precision;
+  if (int e = run_solve(c, prm, st, K)) return e;
+  read_assemble_stats(c, prm, st);
+  if (timing) HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+  HIPCHK(c, hipGraphLaunch(c->g_update, c->stream));
+  if (timing) HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (st) {
+    st->host_syncs += 1;
+    if (timing) {
+      st->ms_solve = elapsed(c->ev[0], c->ev[1]);  // assembly + solve
+      st->ms_update = elapsed(c->ev[1], c->ev[2]);
+    }
+  }
+  if (!c->last_converged)
+    return fail(c, PD_ERR_NOT_CONVERGED, "CG reached the iteration limit before rtol");
   return PD_OK;
 }
 
@@ -935,6 +1119,9 @@ int pd_ctx_destroy(pd_ctx* c) {
   dfree(c->val32); dfree(c->diag32); dfree(c->minv32); dfree(c->r32); dfree(c->z32);
   dfree(c->p32); dfree(c->y32); dfree(c->d32); dfree(c->mx);
   dfree(c->tile_spec); dfree(c->spec_tile0); dfree(c->spec_ntiles);
+  destroy_graphs(c);
+  dfree(c->consts);
+  if (c->h_consts) (void)hipHostFree(c->h_consts);
   if (c->h_mx) (void)hipHostFree(c->h_mx);
   if (c->h_st) (void)hipHostFree(c->h_st);
   if (c->h_cnt) (void)hipHostFree(c->h_cnt);
@@ -1215,6 +1402,7 @@ int pd_step(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st) {
   if (int e = check_params(c, prm)) return e;
   HIPCHK(c, hipSetDevice(c->device));
   zero_timing(st);
+  if (prm->graph) return run_step_graph(c, prm, st);
   const bool timing = prm->timing != 0;
   if (timing)
     if (int e = ensure_events(c, 8)) return e;
@@ -1429,6 +1617,43 @@ int pd_get_specimen_stats(pd_ctx* c, int32_t count, pd_specimen_stats* out) {
       o.refinements = 0;
     }
   }
+  return PD_OK;
+}
+
+int pd_tune_spmv(pd_ctx* c, int32_t iters, double* ms3, int32_t* best) {
+  if (!c) return fail(nullptr, PD_ERR_ARG, "null context");
+  if (c->mat_prec < 0) return fail(c, PD_ERR_STATE, "no matrix assembled");
+  if (iters < 1) return fail(c, PD_ERR_ARG, "iters must be >= 1");
+  HIPCHK(c, hipSetDevice(c->device));
+  if (int e = ensure_events(c, 2)) return e;
+  const int prec = c->mat_prec;
+  const int variants[3] = {2, 4, 8};
+  double ms[3];
+  const int keep = c->unroll[prec];
+  for (int v = 0; v < 3; ++v) {
+    c->unroll[prec] = variants[v];
+    auto launch = [&]() {
+      if (prec == 1)
+        pd::launch_spmv_f32(mx_args(c, 1), c->p32, c->y32, c->stream);
+      else
+        pd::launch_spmv_plain(spmv_view(c), c->sol, c->y, c->stream);
+    };
+    launch();  // warm
+    HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+    for (int i = 0; i < iters; ++i) launch();
+    HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    CHECK_LAUNCH(c, "spmv variant");
+    ms[v] = elapsed(c->ev[0], c->ev[1]) / iters;
+  }
+  int b = 0;
+  for (int v = 1; v < 3; ++v)
+    if (ms[v] < ms[b]) b = v;
+  c->tuned[prec] = variants[b];
+  c->unroll[prec] = keep;
+  if (ms3)
+    for (int v = 0; v < 3; ++v) ms3[v] = ms[v];
+  if (best) *best = variants[b];
   return PD_OK;
 }
 

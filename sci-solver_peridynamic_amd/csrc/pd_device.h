@@ -73,6 +73,15 @@ struct pd_mx_state {
   int32_t pad[2];
 };
 
+// Step scalars read from device memory by captured (graph) steps, so one
+// graph serves every step.
+struct pd_step_consts {
+  double dt;
+  double dt2;
+  double clock;
+  double gravity;
+};
+
 // Per-step counters written by the assembly kernel.
 struct pd_step_counters {
   unsigned long long broken;

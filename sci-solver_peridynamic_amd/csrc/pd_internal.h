@@ -51,6 +51,7 @@ struct AsmArgs {
   int32_t integrator;
   int32_t contact;
   int32_t track_gap;
+  const pd_step_consts* consts;  // non-null: dt, dt2, clock, gravity from here
   double* gap_val;   // [n] when track_gap
   int32_t* gap_q;    // [n] when track_gap
   pd_step_counters* counters;
@@ -63,6 +64,7 @@ struct SpmvView {
   const int32_t* col;
   const double* val;
   const double* diag;
+  int32_t unroll;  // slots per trip of the SpMV row loop: 2, 4 or 8
 };
 
 // Independent specimens solved together: specimen k owns the 256-row
@@ -129,6 +131,7 @@ struct MxArgs {
   double* part;  // [4 * ntiles]
   pd_mx_state* st;  // [batch.nspec]
   int32_t precond;
+  int32_t unroll;  // slots per trip of the SpMV row loops: 2, 4 or 8
 };
 
 #define PD_MAX_ZONE_BODIES 8
@@ -166,10 +169,12 @@ void launch_mx_apply(const MxArgs& a, hipStream_t s);
 // y = A32 x with FP64 arithmetic (the residual operator), and y32 = A32 x32
 void launch_spmv_wide(const MxArgs& a, const double4* x, double4* y, hipStream_t s);
 void launch_spmv_f32(const MxArgs& a, const float4* x, float4* y, hipStream_t s);
+// consts non-null: dt from there
 void launch_update(int32_t n, double4* x, double4* v, const double4* sol,
                    const int32_t* act, int32_t integrator, double dt, int32_t walls,
                    const double* lo, const double* hi, double restitution,
-                   hipStream_t s);
+                   const pd_step_consts* consts, hipStream_t s);
+void launch_copy(double* dst, const double* src, int32_t n, hipStream_t s);
 // contact forces of the m candidates cand[0..m) (particles 0..m when cand
 // is null); fc of every other particle is set to zero
 void launch_contact(int32_t n, const double4* x, const int32_t* body,

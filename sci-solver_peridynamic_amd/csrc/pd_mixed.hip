@@ -48,7 +48,9 @@ __device__ __forceinline__ void blk6f(float a00, float a01, float a02, float a11
   o2 = a02 * x.x + a12 * x.y + a22 * x.z;
 }
 
-// Row p of A32 x in FP64 arithmetic (FP64 diagonal, float bonds).
+// Row p of A32 x in FP64 arithmetic (FP64 diagonal, float bonds); U slots
+// per trip.
+template <int U>
 __device__ __forceinline__ void row_wide(const MxArgs& A, const double4* __restrict__ x, int p,
                                          double& y0, double& y1, double& y2) {
   const int s = p >> 6;
@@ -67,22 +69,22 @@ __device__ __forceinline__ void row_wide(const MxArgs& A, const double4* __restr
   const int* __restrict__ cp = A.col + base * 64 + lane;
   const float* __restrict__ vp = A.val32 + base * 6 * 64 + lane;
   int bnd = 0;
-  for (; bnd + 3 <= top; bnd += 4) {
-    int q[4];
-    float v[4][6];
+  for (; bnd + U - 1 <= top; bnd += U) {
+    int q[U];
+    float v[U][6];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) q[u] = ntl(cp + (int64_t)(bnd + u) * 64);
+    for (int u = 0; u < U; ++u) q[u] = ntl(cp + (int64_t)(bnd + u) * 64);
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < U; ++u) {
       const float* vv = vp + (int64_t)(bnd + u) * 384;
 #pragma unroll
       for (int c = 0; c < 6; ++c) v[u][c] = ntl(vv + c * 64);
     }
-    double4 xx[4];
+    double4 xx[U];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) xx[u] = x[q[u]];
+    for (int u = 0; u < U; ++u) xx[u] = x[q[u]];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < U; ++u) {
       double t0, t1, t2;
       blk6w(v[u][0], v[u][1], v[u][2], v[u][3], v[u][4], v[u][5], xx[u], t0, t1, t2);
       y0 += t0;
@@ -102,7 +104,8 @@ __device__ __forceinline__ void row_wide(const MxArgs& A, const double4* __restr
   }
 }
 
-// Row p of A32 x in float arithmetic.
+// Row p of A32 x in float arithmetic; U slots per trip.
+template <int U>
 __device__ __forceinline__ void row_f32(const MxArgs& A, const float4* __restrict__ x, int p,
                                         float& y0, float& y1, float& y2) {
   const int s = p >> 6;
@@ -114,22 +117,22 @@ __device__ __forceinline__ void row_f32(const MxArgs& A, const float4* __restric
   const int* __restrict__ cp = A.col + base * 64 + lane;
   const float* __restrict__ vp = A.val32 + base * 6 * 64 + lane;
   int bnd = 0;
-  for (; bnd + 3 <= top; bnd += 4) {
-    int q[4];
-    float v[4][6];
+  for (; bnd + U - 1 <= top; bnd += U) {
+    int q[U];
+    float v[U][6];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) q[u] = ntl(cp + (int64_t)(bnd + u) * 64);
+    for (int u = 0; u < U; ++u) q[u] = ntl(cp + (int64_t)(bnd + u) * 64);
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < U; ++u) {
       const float* vv = vp + (int64_t)(bnd + u) * 384;
 #pragma unroll
       for (int c = 0; c < 6; ++c) v[u][c] = ntl(vv + c * 64);
     }
-    float4 xx[4];
+    float4 xx[U];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) xx[u] = x[q[u]];
+    for (int u = 0; u < U; ++u) xx[u] = x[q[u]];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < U; ++u) {
       float t0, t1, t2;
       blk6f(v[u][0], v[u][1], v[u][2], v[u][3], v[u][4], v[u][5], xx[u], t0, t1, t2);
       y0 += t0;
@@ -207,6 +210,7 @@ __global__ void k_mx_setup(pd_mx_state* states, int nspec, double rtol, int max_
 
 // r = S(b - A32 x) in FP64; r32 = float(r); z = M32 r32; p = z; d = 0;
 // partials <r,r>, <r32,z>, <r32,r32>, <Sb,Sb>
+template <int U>
 __global__ __launch_bounds__(PD_BLOCK) void k_mx_resid(MxArgs a) {
   __shared__ double lds4[4];
   const int ntiles = gridDim.x;
@@ -216,7 +220,7 @@ __global__ __launch_bounds__(PD_BLOCK) void k_mx_resid(MxArgs a) {
   double t_rr = 0.0, t_rz = 0.0, t_r32 = 0.0, t_bb = 0.0;
   if (p < a.n) {
     double y0, y1, y2;
-    row_wide(a, a.x, p, y0, y1, y2);
+    row_wide<U>(a, a.x, p, y0, y1, y2);
     const double4 bp = a.b[p];
     const bool fr = a.bc[p] == 0;
     const double r0 = fr ? bp.x - y0 : 0.0;
@@ -287,6 +291,7 @@ __global__ __launch_bounds__(PD_BLOCK) void k_mx_resid_fold(MxArgs a, int ntiles
 }
 
 // y = A32 p (float); partial <y,p>
+template <int U>
 __global__ __launch_bounds__(PD_BLOCK) void k_mx_spmv(MxArgs a) {
   __shared__ double lds4[4];
   const int ntiles = gridDim.x;
@@ -296,7 +301,7 @@ __global__ __launch_bounds__(PD_BLOCK) void k_mx_spmv(MxArgs a) {
   double t = 0.0;
   if (p < a.n) {
     float y0, y1, y2;
-    row_f32(a, a.p, p, y0, y1, y2);
+    row_f32<U>(a, a.p, p, y0, y1, y2);
     a.y[p] = make_float4(y0, y1, y2, 0.0f);
     const float4 pp = a.p[p];
     t = term3f(y0, y1, y2, pp.x, pp.y, pp.z);
@@ -406,21 +411,23 @@ __global__ __launch_bounds__(PD_BLOCK) void k_mx_apply(MxArgs a) {
   a.x[p] = xx;
 }
 
+template <int U>
 __global__ __launch_bounds__(PD_BLOCK) void k_spmv_wide(MxArgs a, const double4* __restrict__ x,
                                                        double4* __restrict__ y) {
   const int p = xcd_tile(blockIdx.x, gridDim.x) * PD_BLOCK + threadIdx.x;
   if (p >= a.n) return;
   double y0, y1, y2;
-  row_wide(a, x, p, y0, y1, y2);
+  row_wide<U>(a, x, p, y0, y1, y2);
   y[p] = make_double4(y0, y1, y2, 0.0);
 }
 
+template <int U>
 __global__ __launch_bounds__(PD_BLOCK) void k_spmv_f32(MxArgs a, const float4* __restrict__ x,
                                                       float4* __restrict__ y) {
   const int p = xcd_tile(blockIdx.x, gridDim.x) * PD_BLOCK + threadIdx.x;
   if (p >= a.n) return;
   float y0, y1, y2;
-  row_f32(a, x, p, y0, y1, y2);
+  row_f32<U>(a, x, p, y0, y1, y2);
   y[p] = make_float4(y0, y1, y2, 0.0f);
 }
 
@@ -436,11 +443,21 @@ void launch_mx_setup(pd_mx_state* st, int32_t nspec, double rtol, int32_t max_it
 void launch_mx_resid(const MxArgs& a, hipStream_t s) {
   if (!a.n) return;
   const int g = tiles_of(a.n);
-  hipLaunchKernelGGL(k_mx_resid, dim3(g), dim3(PD_BLOCK), 0, s, a);
+  switch (a.unroll) {
+    case 2: hipLaunchKernelGGL(k_mx_resid<2>, dim3(g), dim3(PD_BLOCK), 0, s, a); break;
+    case 8: hipLaunchKernelGGL(k_mx_resid<8>, dim3(g), dim3(PD_BLOCK), 0, s, a); break;
+    default: hipLaunchKernelGGL(k_mx_resid<4>, dim3(g), dim3(PD_BLOCK), 0, s, a);
+  }
   hipLaunchKernelGGL(k_mx_resid_fold, dim3(a.batch.nspec), dim3(PD_BLOCK), 0, s, a, g);
 }
 void launch_mx_spmv(const MxArgs& a, hipStream_t s) {
-  if (a.n) hipLaunchKernelGGL(k_mx_spmv, dim3(tiles_of(a.n)), dim3(PD_BLOCK), 0, s, a);
+  if (!a.n) return;
+  const dim3 g(tiles_of(a.n));
+  switch (a.unroll) {
+    case 2: hipLaunchKernelGGL(k_mx_spmv<2>, g, dim3(PD_BLOCK), 0, s, a); break;
+    case 8: hipLaunchKernelGGL(k_mx_spmv<8>, g, dim3(PD_BLOCK), 0, s, a); break;
+    default: hipLaunchKernelGGL(k_mx_spmv<4>, g, dim3(PD_BLOCK), 0, s, a);
+  }
 }
 void launch_mx_update(const MxArgs& a, hipStream_t s) {
   if (!a.n) return;
@@ -456,10 +473,22 @@ void launch_mx_apply(const MxArgs& a, hipStream_t s) {
   if (a.n) hipLaunchKernelGGL(k_mx_apply, dim3(tiles_of(a.n)), dim3(PD_BLOCK), 0, s, a);
 }
 void launch_spmv_wide(const MxArgs& a, const double4* x, double4* y, hipStream_t s) {
-  if (a.n) hipLaunchKernelGGL(k_spmv_wide, dim3(tiles_of(a.n)), dim3(PD_BLOCK), 0, s, a, x, y);
+  if (!a.n) return;
+  const dim3 g(tiles_of(a.n));
+  switch (a.unroll) {
+    case 2: hipLaunchKernelGGL(k_spmv_wide<2>, g, dim3(PD_BLOCK), 0, s, a, x, y); break;
+    case 8: hipLaunchKernelGGL(k_spmv_wide<8>, g, dim3(PD_BLOCK), 0, s, a, x, y); break;
+    default: hipLaunchKernelGGL(k_spmv_wide<4>, g, dim3(PD_BLOCK), 0, s, a, x, y);
+  }
 }
 void launch_spmv_f32(const MxArgs& a, const float4* x, float4* y, hipStream_t s) {
-  if (a.n) hipLaunchKernelGGL(k_spmv_f32, dim3(tiles_of(a.n)), dim3(PD_BLOCK), 0, s, a, x, y);
+  if (!a.n) return;
+  const dim3 g(tiles_of(a.n));
+  switch (a.unroll) {
+    case 2: hipLaunchKernelGGL(k_spmv_f32<2>, g, dim3(PD_BLOCK), 0, s, a, x, y); break;
+    case 8: hipLaunchKernelGGL(k_spmv_f32<8>, g, dim3(PD_BLOCK), 0, s, a, x, y); break;
+    default: hipLaunchKernelGGL(k_spmv_f32<4>, g, dim3(PD_BLOCK), 0, s, a, x, y);
+  }
 }
 
 }  // namespace pd

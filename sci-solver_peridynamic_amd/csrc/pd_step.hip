@@ -24,6 +24,12 @@ namespace pd {
 // the bonds and the inverse, only) written rounded to float.
 template <bool MIXED>
 __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
+  if (a.consts) {
+    a.dt = a.consts->dt;
+    a.dt2 = a.consts->dt2;
+    a.clock = a.consts->clock;
+    a.gravity = a.consts->gravity;
+  }
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int p = tile * PD_BLOCK + threadIdx.x;
   unsigned long long broken = 0, live = 0;
@@ -373,6 +379,9 @@ __device__ __forceinline__ double ntl(const double* p) {
 }
 __device__ __forceinline__ int ntl(const int* p) { return __builtin_nontemporal_load(p); }
 
+// U slots per trip: all their loads are issued before any product (the
+// summation order does not depend on U)
+template <int U>
 __device__ __forceinline__ void spmv_row(const SpmvView& A, const double4* __restrict__ x,
                                          int p, double& y0, double& y1, double& y2) {
   const int s = p >> 6;
@@ -384,23 +393,22 @@ __device__ __forceinline__ void spmv_row(const SpmvView& A, const double4* __res
   const int* __restrict__ cp = A.col + base * 64 + lane;
   const double* __restrict__ vp = A.val + base * 6 * 64 + lane;
   int bnd = 0;
-  // four slots per trip: all their loads are issued before any product
-  for (; bnd + 3 <= top; bnd += 4) {
-    int q[4];
-    double v[4][6];
+  for (; bnd + U - 1 <= top; bnd += U) {
+    int q[U];
+    double v[U][6];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) q[u] = ntl(cp + (int64_t)(bnd + u) * 64);
+    for (int u = 0; u < U; ++u) q[u] = ntl(cp + (int64_t)(bnd + u) * 64);
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < U; ++u) {
       const double* vv = vp + (int64_t)(bnd + u) * 384;
 #pragma unroll
       for (int c = 0; c < 6; ++c) v[u][c] = ntl(vv + c * 64);
     }
-    double4 xx[4];
+    double4 xx[U];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) xx[u] = x[q[u]];
+    for (int u = 0; u < U; ++u) xx[u] = x[q[u]];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < U; ++u) {
       double t0, t1, t2;
       blk6(v[u][0], v[u][1], v[u][2], v[u][3], v[u][4], v[u][5], xx[u], t0, t1, t2);
       y0 += t0;
@@ -420,6 +428,7 @@ __device__ __forceinline__ void spmv_row(const SpmvView& A, const double4* __res
   }
 }
 
+template <int U>
 __global__ __launch_bounds__(PD_BLOCK) void k_spmv_plain(SpmvView A,
                                                         const double4* __restrict__ x,
                                                         double4* __restrict__ y) {
@@ -427,13 +436,18 @@ __global__ __launch_bounds__(PD_BLOCK) void k_spmv_plain(SpmvView A,
   const int p = tile * PD_BLOCK + threadIdx.x;
   if (p >= A.n) return;
   double y0, y1, y2;
-  spmv_row(A, x, p, y0, y1, y2);
+  spmv_row<U>(A, x, p, y0, y1, y2);
   y[p] = make_double4(y0, y1, y2, 0.0);
 }
 
 void launch_spmv_plain(const SpmvView& A, const double4* x, double4* y, hipStream_t s) {
   const int nb = (A.n + PD_BLOCK - 1) / PD_BLOCK;
-  if (nb) hipLaunchKernelGGL(k_spmv_plain, dim3(nb), dim3(PD_BLOCK), 0, s, A, x, y);
+  if (!nb) return;
+  switch (A.unroll) {
+    case 2: hipLaunchKernelGGL(k_spmv_plain<2>, dim3(nb), dim3(PD_BLOCK), 0, s, A, x, y); break;
+    case 8: hipLaunchKernelGGL(k_spmv_plain<8>, dim3(nb), dim3(PD_BLOCK), 0, s, A, x, y); break;
+    default: hipLaunchKernelGGL(k_spmv_plain<4>, dim3(nb), dim3(PD_BLOCK), 0, s, A, x, y);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -493,6 +507,7 @@ void launch_cg_setup(pd_cg_state* st, int32_t nspec, double rtol, int32_t max_it
 }
 
 // y = A x0; r = S(b - y); z = M r; p = z; partials <r,z>, <r,r>, <Sb,Sb>
+template <int U>
 __global__ __launch_bounds__(PD_BLOCK) void k_cg_init(CgArgs a) {
   __shared__ double lds4[4];
   const int ntiles = gridDim.x;
@@ -501,7 +516,7 @@ __global__ __launch_bounds__(PD_BLOCK) void k_cg_init(CgArgs a) {
   double t_rz = 0.0, t_rr = 0.0, t_bb = 0.0;
   if (p < a.A.n) {
     double y0, y1, y2;
-    spmv_row(a.A, a.x, p, y0, y1, y2);
+    spmv_row<U>(a.A, a.x, p, y0, y1, y2);
     const double4 bp = a.b[p];
     const bool fr = a.bc[p] == 0;
     const double r0 = fr ? bp.x - y0 : 0.0;
@@ -555,6 +570,7 @@ __global__ __launch_bounds__(PD_BLOCK) void k_cg_init_fold(CgArgs a, int ntiles)
 }
 
 // y = A p; partial <y,p>
+template <int U>
 __global__ __launch_bounds__(PD_BLOCK) void k_cg_spmv(CgArgs a) {
   __shared__ double lds4[4];
   const int ntiles = gridDim.x;
@@ -564,7 +580,7 @@ __global__ __launch_bounds__(PD_BLOCK) void k_cg_spmv(CgArgs a) {
   double t = 0.0;
   if (p < a.A.n) {
     double y0, y1, y2;
-    spmv_row(a.A, a.p, p, y0, y1, y2);
+    spmv_row<U>(a.A, a.p, p, y0, y1, y2);
     a.y[p] = make_double4(y0, y1, y2, 0.0);
     const double4 pp = a.p[p];
     t = term3(y0, y1, y2, pp.x, pp.y, pp.z);
@@ -675,13 +691,21 @@ static inline int tiles_of(int n) { return (n + PD_BLOCK - 1) / PD_BLOCK; }
 void launch_cg_init(const CgArgs& a, hipStream_t s) {
   if (!a.A.n) return;
   const int g = tiles_of(a.A.n);
-  hipLaunchKernelGGL(k_cg_init, dim3(g), dim3(PD_BLOCK), 0, s, a);
+  switch (a.A.unroll) {
+    case 2: hipLaunchKernelGGL(k_cg_init<2>, dim3(g), dim3(PD_BLOCK), 0, s, a); break;
+    case 8: hipLaunchKernelGGL(k_cg_init<8>, dim3(g), dim3(PD_BLOCK), 0, s, a); break;
+    default: hipLaunchKernelGGL(k_cg_init<4>, dim3(g), dim3(PD_BLOCK), 0, s, a);
+  }
   hipLaunchKernelGGL(k_cg_init_fold, dim3(a.batch.nspec), dim3(PD_BLOCK), 0, s, a, g);
 }
 void launch_cg_spmv(const CgArgs& a, hipStream_t s) {
   if (!a.A.n) return;
   const int g = tiles_of(a.A.n);
-  hipLaunchKernelGGL(k_cg_spmv, dim3(g), dim3(PD_BLOCK), 0, s, a);
+  switch (a.A.unroll) {
+    case 2: hipLaunchKernelGGL(k_cg_spmv<2>, dim3(g), dim3(PD_BLOCK), 0, s, a); break;
+    case 8: hipLaunchKernelGGL(k_cg_spmv<8>, dim3(g), dim3(PD_BLOCK), 0, s, a); break;
+    default: hipLaunchKernelGGL(k_cg_spmv<4>, dim3(g), dim3(PD_BLOCK), 0, s, a);
+  }
 }
 void launch_cg_update(const CgArgs& a, hipStream_t s) {
   if (!a.A.n) return;
@@ -722,11 +746,13 @@ struct UpdArgs {
   double lo[3];
   double hi[3];
   double rest;
+  const pd_step_consts* consts;
 };
 
 __global__ __launch_bounds__(PD_BLOCK) void k_update(UpdArgs a) {
   const int p = blockIdx.x * PD_BLOCK + threadIdx.x;
   if (p >= a.n || a.act[p] == PD_INVISIBLE) return;
+  if (a.consts) a.dt = a.consts->dt;
   const double4 s = a.sol[p];
   const double4 xo = a.x[p];
   double nv0, nv1, nv2, np0, np1, np2;
@@ -758,8 +784,10 @@ __global__ __launch_bounds__(PD_BLOCK) void k_update(UpdArgs a) {
 
 void launch_update(int32_t n, double4* x, double4* v, const double4* sol,
                    const int32_t* act, int32_t integrator, double dt, int32_t walls,
-                   const double* lo, const double* hi, double restitution, hipStream_t s) {
+                   const double* lo, const double* hi, double restitution,
+                   const pd_step_consts* consts, hipStream_t s) {
   UpdArgs a;
+  a.consts = consts;
   a.n = n;
   a.x = x;
   a.v = v;
@@ -775,6 +803,16 @@ void launch_update(int32_t n, double4* x, double4* v, const double4* sol,
   a.rest = restitution;
   const int nb = (n + PD_BLOCK - 1) / PD_BLOCK;
   if (nb) hipLaunchKernelGGL(k_update, dim3(nb), dim3(PD_BLOCK), 0, s, a);
+}
+
+__global__ __launch_bounds__(PD_BLOCK) void k_copy(double* dst, const double* src, int n) {
+  const int i = blockIdx.x * PD_BLOCK + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
+void launch_copy(double* dst, const double* src, int32_t n, hipStream_t s) {
+  const int nb = (n + PD_BLOCK - 1) / PD_BLOCK;
+  if (nb) hipLaunchKernelGGL(k_copy, dim3(nb), dim3(PD_BLOCK), 0, s, dst, src, n);
 }
 
 // ---------------------------------------------------------------------------

@@ -60,7 +60,7 @@ class StepParams(C.Structure):
         ("track_gap", C.c_int32), ("timing", C.c_int32),
         ("precision", C.c_int32), ("max_refine", C.c_int32),
         ("inner_rtol", C.c_double), ("inner_max_iter", C.c_int32),
-        ("contact_mode", C.c_int32),
+        ("contact_mode", C.c_int32), ("spmv_unroll", C.c_int32), ("graph", C.c_int32),
     ]
 
 
@@ -97,7 +97,7 @@ DEFAULT_PARAMS = dict(dt=1e-4, clock_scale=1.0, gravity=9.8, integrator=0, preco
                       bmax=(1e30,) * 3, wall_radius=0.0, restitution=0.001,
                       contact_radius=0.0, contact_stiffness=0.0, contact_volume=1.0,
                       contact_cell=0.0, track_gap=0, timing=0, precision=0, max_refine=20,
-                      inner_rtol=1e-5, contact_mode=0)
+                      inner_rtol=1e-5, contact_mode=0, spmv_unroll=0, graph=0)
 
 
 def make_params(**kw) -> StepParams:
@@ -151,12 +151,14 @@ def lib():
         L.pd_bench_spmv.argtypes = [vp, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.pd_spmv_bytes.argtypes = [vp, C.POINTER(C.c_double)]
         L.pd_set_specimens.argtypes = [vp, C.c_int32, vp, vp]
+        L.pd_tune_spmv.argtypes = [vp, C.c_int32, vp, C.POINTER(C.c_int32)]
         L.pd_get_specimen_stats.argtypes = [vp, C.c_int32, C.POINTER(SpecimenStats)]
         for name in ("pd_ctx_create", "pd_ctx_destroy", "pd_upload", "pd_download",
                      "pd_build_bonds", "pd_set_bonds", "pd_get_bonds", "pd_get_matrix",
                      "pd_step", "pd_assemble", "pd_solve", "pd_update", "pd_commit_stretch",
                      "pd_spmv", "pd_backup", "pd_restore", "pd_synchronize", "pd_bench_spmv",
-                     "pd_spmv_bytes", "pd_set_specimens", "pd_get_specimen_stats"):
+                     "pd_spmv_bytes", "pd_set_specimens", "pd_get_specimen_stats",
+                     "pd_tune_spmv"):
             getattr(L, name).restype = C.c_int
         _lib = L
     return _lib
@@ -273,6 +275,14 @@ class Context:
         if not (allow_fail and rc == PD_ERR_NOT_CONVERGED):
             self._check(rc)
         return st
+
+    def tune_spmv(self, iters=5):
+        """Times the SpMV variants on the resident matrix; returns (ms per
+        variant for 2/4/8 slots per trip, chosen slots per trip)."""
+        ms = np.zeros(3)
+        best = C.c_int32()
+        self._check(lib().pd_tune_spmv(self._h, iters, _ptr(ms), C.byref(best)))
+        return ms, best.value
 
     def set_specimens(self, first, count):
         first = np.ascontiguousarray(first, np.int32)
