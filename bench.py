@@ -12,9 +12,11 @@ baseline (the CPU oracle on a bounded sample of the same workload).
     python bench.py [--gpus 1] [--steps 195] [--warmup 5] [--no-cpu-baseline]
                     [--precision fp64|mixed] [--no-mixed]
 
-The FP64 line also carries a "mixed_precision" object: the same run with the
+The FP64 line also carries a "mixed_precision" object (the same run with the
 float inner CG and FP64 refinement, its SpMV roofline, and its per-step
-position error against the FP64 path.
+position error against the FP64 path) and a "batched_specimens" object
+(64 independent tension specimens in one context, aggregate particle-steps/s,
+hipGraph-captured and eager).
 """
 from __future__ import annotations
 
@@ -77,6 +79,9 @@ def gpu_run(steps, warmup, precision=0, copy_calibration=0):
         step += 1
         ctx.step(step_params(desc, step, precision))
     ctx.synchronize()
+    # the autotuner picks the SpMV variant on the resident matrix (outside
+    # the timed region); spmv_unroll = 0 then uses it
+    tune_ms, unroll = ctx.tune_spmv(10) if warmup > 0 else (np.zeros(3), 4)
     spmv_ms = 0.0
     spmv_n = 0
     resid_ms = 0.0
@@ -117,7 +122,55 @@ def gpu_run(steps, warmup, precision=0, copy_calibration=0):
                 spmv_ms=spmv_ms, spmv_n=spmv_n, iters=iters, broken=int(broken),
                 spmv_bytes=spmv_bytes, phase=phase, build_ms=bst.ms_build,
                 plain_ms=plain_ms, plain_bytes=plain_bytes, resid_ms=resid_ms, resid_n=resid_n,
-                refinements=refinements)
+                refinements=refinements, unroll=unroll, tune_ms=[float(t) for t in tune_ms])
+
+
+BATCH_CONFIG = os.path.join(PKG, "configs", "batch_64.txt")
+
+
+def batched_run(steps, warmup, graph):
+    """batch_64: 64 independent tension specimens in one context, float
+    inner CG with FP64 refinement; aggregate particle-steps/s over the
+    specimens' particles (padding rows excluded)."""
+    desc = pds.describe(BATCH_CONFIG)
+    sc = pds.scene(BATCH_CONFIG)
+    ex = pds.scene_extra(BATCH_CONFIG)
+    n = desc["num_pd_particle"]
+    ctx = pds.Context(n, pds.materials_of(desc))
+    pds.load_state(ctx, sc["x4"], sc["v4"], pds.stretch_init_each(ex["s0"], desc["stretch_seed"]),
+                   body=sc["body"], act=ex["act"], bc=sc["bc"], vbc=sc["vbc"])
+    bst = ctx.build_bonds()
+    ctx.set_specimens(ex["first"], ex["count"])
+    nspec = len(ex["first"])
+    real = int(ex["count"].sum())
+    step = 0
+    for _ in range(warmup):
+        step += 1
+        ctx.step(pds.make_params(**dict(pds.step_kwargs(desc, step), graph=graph)))
+    ctx.synchronize()
+    tune_ms, unroll = ctx.tune_spmv(10)
+    its, spec_its, broken, syncs = [], [], 0, 0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step += 1
+        st = ctx.step(pds.make_params(**dict(pds.step_kwargs(desc, step), graph=graph)))
+        its.append(st.cg_iterations)
+        broken += st.broken_bonds
+        syncs += st.host_syncs
+    ctx.synchronize()
+    elapsed = time.perf_counter() - t0
+    per = ctx.specimen_stats(nspec)
+    spec_its = [p["cg_iterations"] for p in per]
+    ctx.close()
+    log(f"[batch graph={graph}] {nspec} specimens, {real} particles ({n} rows), "
+        f"{steps / elapsed:.1f} steps/s")
+    return dict(specimens=nspec, particles=real, rows=n, bonds=int(bst.total_bonds),
+                steps=steps, elapsed=elapsed, value=real * steps / elapsed,
+                ms_per_step=1000.0 * elapsed / steps, cg_iterations_max_mean=float(np.mean(its)),
+                last_step_iterations_min_max=[min(spec_its), max(spec_its)], broken_bonds=broken,
+                host_syncs_per_step=syncs / steps, spmv_unroll=unroll,
+                tune_ms=[float(t) for t in tune_ms],
+                count_min_max=[int(ex["count"].min()), int(ex["count"].max())])
 
 
 def mixed_error(steps):
@@ -217,6 +270,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the mixed-precision report of the fp64 line")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the batched-specimen report")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="after the run, N rounds of device copies of known size")
@@ -257,6 +312,8 @@ def main():
             "phase_ms_per_step": {k: v / r["steps"] for k, v in r["phase"].items()},
             "bond_build_ms": r["build_ms"],
             "plain_spmv_gbs": r["plain_bytes"] / (r["plain_ms"] * 1e-3) / 1e9,
+            "spmv_unroll": r["unroll"],
+            "spmv_tune_ms_2_4_8": r["tune_ms"],
         },
         "roofline": spmv_roofline(r, kernel, None if prec else read_traffic()),
     }
@@ -277,7 +334,25 @@ def main():
             "roofline": spmv_roofline(m, "k_mx_spmv (float inner CG SpMV)", None),
             "resid_spmv_ms_mean": m["resid_ms"] / max(m["resid_n"], 1),
             "plain_spmv_gbs": m["plain_bytes"] / (m["plain_ms"] * 1e-3) / 1e9,
+            "spmv_unroll": m["unroll"],
+            "spmv_tune_ms_2_4_8": m["tune_ms"],
             "position_error_vs_fp64": mixed_error(a.warmup + a.steps),
+        }
+    if not a.no_batched:
+        bw = max(1, min(a.warmup, 5))
+        eager = batched_run(95, bw, 0)
+        graph = batched_run(95, bw, 1)
+        out["batched_specimens"] = {
+            "metric": "aggregate particle-steps/s",
+            "value": graph["value"],
+            "unit": "particle-steps/s",
+            "workload": "batch_64: 64 independent tension specimens in one context, counts "
+                        "log-uniform in [5000, 60000], aspect 1..4, s0 U[0.01, 0.03], dt 1e-4 s, "
+                        "block-Jacobi CG rtol 1e-6 with float inner solve and FP64 refinement, "
+                        "one launch per phase for all specimens, hipGraph-captured steps, "
+                        "autotuned SpMV slots per trip",
+            "graph": graph,
+            "eager": eager,
         }
     if not a.no_cpu_baseline:
         log("cpu baseline: CPU oracle on the same workload ...")
