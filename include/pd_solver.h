@@ -81,7 +81,8 @@ typedef struct pd_step_params {
   double clock_scale; /* phase-in factor (simulator.cu:1163-1167) */
   double gravity;     /* 9.8 in the reference, applied along -y */
   int32_t integrator; /* 0 implicit Euler, 1 Newmark-beta */
-  int32_t precond;    /* 0 identity (reference CUSP), 1 block Jacobi */
+  int32_t precond;    /* 0 identity (reference CUSP), 1 block Jacobi,
+                         2 learned blocks (pd_learned_set) */
   double rtol;        /* stop when ||r|| <= rtol * ||S b|| */
   int32_t max_iter;   /* CG iteration limit (reference 100000) */
   int32_t walls;      /* 1: clamp positions to [bmin+r, bmax-r] */
@@ -121,6 +122,10 @@ typedef struct pd_step_params {
    * iterating.  Needs contact off and track_gap 0.  Results are identical
    * to graph = 0. */
   int32_t graph;
+  /* precond 2: the learned solve gets at most learned_fallback x the
+   * iterations of this context's last block-Jacobi solve (0: 1.5), then
+   * continues with block Jacobi from its iterate */
+  double learned_fallback;
 } pd_step_params;
 
 typedef struct pd_step_stats {
@@ -144,6 +149,7 @@ typedef struct pd_step_stats {
   int32_t refinements;    /* precision 1: FP64 corrections applied */
   double ms_resid_total;  /* precision 1: FP64 residual launches that ran */
   int32_t resid_launches;
+  int32_t precond_fallbacks; /* precond 2: 1 if block Jacobi took over */
 } pd_step_stats;
 
 /* Solver result of one specimen (pd_set_specimens). */
@@ -276,6 +282,38 @@ int pd_spmv_bytes(pd_ctx* ctx, double* bytes);
  * (used when spmv_unroll = 0) and returns the mean times in ms3[3] and the
  * choice in *best. */
 int pd_tune_spmv(pd_ctx* ctx, int32_t iters, double* ms3, int32_t* best);
+
+/* Learned block preconditioner (no counterpart in the reference).  The
+ * network maps per-particle features of the assembled system (live bond
+ * count, mean and max bond stretch, the diagonal block relative to m I,
+ * distance to the nearest broken bond) through 16-64-64-8 ReLU layers (BF16
+ * weights, FP32 accumulation) to L_p, and precond 2 uses
+ * M_p = C_p^{-T} (L_p L_p^T + eps I) C_p^{-1} with D_p = C_p C_p^T in place
+ * of the block-Jacobi inverse D_p^{-1} (which is the case L_p = I).
+ * params: PD_LEARNED_PARAMS floats, W1[64][16] b1[64] W2[64][64] b2[64]
+ * W3[8][64] b3[8] feature_mean[16] feature_inv_std[16] eps. */
+#define PD_LEARNED_PARAMS 5801
+int pd_learned_set(pd_ctx* ctx, const float* params, int32_t count);
+/* The preconditioner blocks of the last solve, 9 doubles per particle. */
+int pd_get_precond(pd_ctx* ctx, double* minv9);
+
+/* Training of the network on systems the simulator assembled: samples are
+ * (features, D_p, (A e)_p, e_p) for random sign probes e on every
+ * stride-th row (offset from the seed); fitting minimises the mean of
+ * |M_p (A e)_p - e_p|^2 with Adam, `steps` steps of `batch` samples drawn
+ * with replacement (batch a multiple of 128).  losses[k] (if non-null)
+ * receives each step's mean loss. */
+typedef struct pd_trainer pd_trainer;
+int pd_trainer_create(int32_t device, int64_t capacity, uint64_t seed, pd_trainer** out);
+int pd_trainer_destroy(pd_trainer* t);
+const char* pd_trainer_last_error(const pd_trainer* t);
+int pd_trainer_add(pd_trainer* t, pd_ctx* ctx, uint64_t seed, int32_t stride, int64_t* added);
+int pd_trainer_fit(pd_trainer* t, int32_t steps, int32_t batch, float lr, uint64_t seed,
+                   float* losses);
+int pd_trainer_params(pd_trainer* t, float* params, int32_t count);
+/* samples [first, first + count), 28 floats each: features[16], D (6),
+ * (A e)_p (3), e_p (3) */
+int pd_trainer_samples(pd_trainer* t, int64_t first, int64_t count, float* out);
 
 /* Independent specimens in one context (no counterpart in the reference,
  * which runs one body per process).  Specimen k owns rows [first[k],

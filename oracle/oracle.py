@@ -109,6 +109,7 @@ def lib():
         L.orc_spmv.argtypes = [C.POINTER(System), _dp, _dp]
         L.orc_block_jacobi.argtypes = [C.POINTER(System)]
         L.orc_mixed_matrix.argtypes = [C.POINTER(System)]
+        L.orc_round_blocks.argtypes = [C.POINTER(System)]
         L.orc_spmv_wide.argtypes = [C.POINTER(System), _dp, _dp]
         for name in ("orc_update", "orc_contact"):
             getattr(L, name).argtypes = [C.POINTER(System), C.POINTER(StepParams)]

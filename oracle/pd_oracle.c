@@ -655,7 +655,7 @@ void orc_block_jacobi(orc_system* sys) {
  * ====================================================================== */
 static inline void precond_apply(const orc_system* sys, int precond,
                                  int64_t p, const double* r, double* z) {
-  if (precond == 1) {
+  if (precond != 0) { /* 1 block Jacobi, 2 given blocks */
     const double* m = sys->minv + 9 * p;
     z[0] = m[0] * r[0] + m[1] * r[1] + m[2] * r[2];
     z[1] = m[3] * r[0] + m[4] * r[1] + m[5] * r[2];
@@ -678,7 +678,7 @@ void orc_cg(orc_system* sys, const orc_step_params* prm, orc_step_stats* st) {
   double* p = (double*)malloc(bytes);
   double* sb = (double*)malloc(bytes);
   const double dt = prm->dt;
-  if (prm->precond == 1) orc_block_jacobi(sys);
+  if (prm->precond == 1) orc_block_jacobi(sys); /* precond 2: minv as given */
 
   /* Dirichlet rows start at their prescribed value (implicit Euler solves
    * for the velocity, Newmark-beta for the displacement dt * v). */
@@ -833,7 +833,7 @@ static void orc_spmv_f32(const orc_system* sys, const float* x4, float* y4) {
 
 static inline void precond_apply_f(const orc_system* sys, int precond, int64_t p,
                                    const float* r, float* z) {
-  if (precond == 1) {
+  if (precond != 0) {
     const float* m = sys->minv32 + 9 * p;
     z[0] = m[0] * r[0] + m[1] * r[1] + m[2] * r[2];
     z[1] = m[3] * r[0] + m[4] * r[1] + m[5] * r[2];
@@ -847,9 +847,13 @@ static inline void precond_apply_f(const orc_system* sys, int precond, int64_t p
 }
 
 void orc_mixed_matrix(orc_system* sys) {
+  orc_block_jacobi(sys);
+  orc_round_blocks(sys);
+}
+
+void orc_round_blocks(orc_system* sys) {
   const int32_t n = sys->n;
   const int64_t nslot = sys->row_start[n];
-  orc_block_jacobi(sys);
 #pragma omp parallel for schedule(static)
   for (int64_t k = 0; k < 9 * nslot; ++k) sys->val32[k] = (float)sys->val[k];
 #pragma omp parallel for schedule(static)
@@ -874,7 +878,10 @@ void orc_cg_mixed(orc_system* sys, const orc_step_params* prm, orc_step_stats* s
   double* t_r32 = (double*)malloc(sizeof(double) * nn);
   double* t_bb = (double*)malloc(sizeof(double) * nn);
   const double dt = prm->dt;
-  orc_mixed_matrix(sys);
+  if (prm->precond == 2)
+    orc_round_blocks(sys); /* minv as given */
+  else
+    orc_mixed_matrix(sys);
 #pragma omp parallel for schedule(static)
   for (int32_t i = 0; i < n; ++i) {
     if (sys->bc[i]) {

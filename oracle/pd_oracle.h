@@ -118,7 +118,7 @@ typedef struct orc_step_params {
   double clock_scale; /* phase-in factor c (core/simulator.cu:1163-1167) */
   double gravity;     /* 9.8 in the reference, along -y */
   int32_t integrator; /* 0 implicit Euler, 1 Newmark-beta */
-  int32_t precond;    /* 0 identity (reference), 1 block Jacobi */
+  int32_t precond;    /* 0 identity (reference), 1 block Jacobi, 2 minv as given */
   double rtol;        /* relative tolerance on ||r|| / ||S b|| */
   int32_t max_iter;
   int32_t walls;      /* 1: clamp to [bmin+r, bmax-r] with restitution */
@@ -171,6 +171,8 @@ void orc_cg(orc_system* sys, const orc_step_params* prm, orc_step_stats* st);
 void orc_cg_mixed(orc_system* sys, const orc_step_params* prm, orc_step_stats* st);
 /* block-Jacobi inverse, then val32 / diag32 / minv32 = float(val / diag / minv) */
 void orc_mixed_matrix(orc_system* sys);
+/* val32 / diag32 / minv32 = float(val / diag / minv) */
+void orc_round_blocks(orc_system* sys);
 /* y = A32 x in FP64 arithmetic (FP64 diagonal, float bonds widened) */
 void orc_spmv_wide(const orc_system* sys, const double* x4, double* y4);
 double orc_dot_f32(int32_t n, const float* a4, const float* b4);

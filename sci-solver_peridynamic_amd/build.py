@@ -33,7 +33,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", f"-I{INCLUDE}", f"-I{CSRC}",
 ]
-HIP_SOURCES = ["pd_step.hip", "pd_bonds.hip", "pd_mixed.hip", "pd_api.hip"]
+HIP_SOURCES = ["pd_step.hip", "pd_bonds.hip", "pd_mixed.hip", "pd_learned.hip", "pd_api.hip"]
 HOST_SOURCES = ["parameters.cpp", "scene.cpp", "mesh_fill.cpp", "simulator.cpp", "data_io.cpp",
                 "host_capi.cpp"]
 CXX = os.environ.get("CXX", "g++")

@@ -16,6 +16,8 @@
 #include "pd_internal.h"
 #include "../../include/pd_solver.h"
 
+static_assert(PD_LEARNED_PARAMS == PC_NPARAM, "learned parameter layout");
+
 static thread_local std::string g_last_error;
 
 struct pd_ctx {
@@ -117,9 +119,34 @@ struct pd_ctx {
   // the position update; step scalars come from consts
   pd_step_consts* consts = nullptr;
   pd_step_consts* h_consts = nullptr;
+
+  // learned preconditioner
+  float2* bond_stats = nullptr;  // (mean, max) live-bond stretch per row
+  float* pc_feat = nullptr;      // [n][PC_NIN] features (lazy)
+  float* pc_p = nullptr;         // [PC_NPARAM]
+  __bf16* pc_w = nullptr;        // [PC_NWEIGHT]
+  bool pc_loaded = false;
+  int bj_iters = 0;  // iterations of the last block-Jacobi solve
   hipGraphExec_t g_solve = nullptr, g_update = nullptr;
   std::string graph_key;
   int graph_k = 0;
+};
+
+struct pd_trainer {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  float* samples = nullptr;  // [cap][PC_SAMPLE]
+  int64_t cap = 0;
+  int64_t count = 0;
+  float *p = nullptr, *m = nullptr, *v = nullptr;  // parameters, Adam moments
+  __bf16* w = nullptr;
+  float* partial = nullptr;
+  int32_t partial_waves = 0;
+  float* loss = nullptr;
+  int32_t loss_cap = 0;
+  int32_t steps_done = 0;
+  bool normed = false;
+  std::string err;
 };
 
 namespace {
@@ -212,8 +239,12 @@ int check_params(pd_ctx* c, const pd_step_params* prm) {
   if (!(prm->dt > 0.0)) return fail(c, PD_ERR_ARG, "dt must be > 0");
   if (prm->integrator != 0 && prm->integrator != 1)
     return fail(c, PD_ERR_ARG, "integrator must be 0 (implicit Euler) or 1 (Newmark-beta)");
-  if (prm->precond != 0 && prm->precond != 1)
-    return fail(c, PD_ERR_ARG, "precond must be 0 (identity) or 1 (block Jacobi)");
+  if (prm->precond < 0 || prm->precond > 2)
+    return fail(c, PD_ERR_ARG, "precond must be 0 (identity), 1 (block Jacobi) or 2 (learned)");
+  if (prm->precond == 2 && !c->pc_loaded)
+    return fail(c, PD_ERR_STATE, "precond 2 needs a network (pd_learned_set)");
+  if (prm->precond == 2 && prm->graph)
+    return fail(c, PD_ERR_ARG, "captured steps (graph = 1) support precond 0 and 1");
   if (prm->contact_radius > 0.0 && !(prm->contact_cell >= prm->contact_radius))
     return fail(c, PD_ERR_ARG, "contact_cell must be >= contact_radius");
   if (prm->contact_radius > 0.0 && !(prm->contact_volume != 0.0))
@@ -543,6 +574,7 @@ int enqueue_assemble(pd_ctx* c, const pd_step_params* prm,
   a.gap_q = c->gap_q;
   a.counters = c->counters;
   a.consts = consts;
+  a.bond_stats = c->bond_stats;
   pd::launch_assemble(a, c->stream);
   CHECK_LAUNCH(c, "assemble");
   c->mat_prec = prm->precision;
@@ -823,6 +855,80 @@ int enqueue_update(pd_ctx* c, const pd_step_params* prm,
   return PD_OK;
 }
 
+// The learned blocks M_p replace the block-Jacobi inverse written by the
+// assembly (minv, and minv32 for the mixed solve).
+int apply_learned(pd_ctx* c, const pd_step_params* prm) {
+  if (!c->pc_feat)
+    if (int e = dalloc(c, &c->pc_feat, (size_t)std::max(c->n, 1) * PC_NIN)) return e;
+  pd::PcFeatArgs fa;
+  fa.n = c->n;
+  fa.slice_off = c->slice_off;
+  fa.top = c->top;
+  fa.cap = c->cap;
+  fa.col = c->col;
+  fa.x = c->x;
+  fa.diag = c->diag;
+  fa.bond_stats = c->bond_stats;
+  fa.act = c->act;
+  fa.body = c->body;
+  fa.mat = c->dmat;
+  fa.feat = c->pc_feat;
+  pd::launch_pc_features(fa, c->stream);
+  pd::PcNet net;
+  net.w = c->pc_w;
+  net.p = c->pc_p;
+  pd::launch_pc_infer(c->n, c->pc_feat, net, c->diag, c->act, c->minv,
+                      prm->precision == 1 ? c->minv32 : nullptr, c->stream);
+  CHECK_LAUNCH(c, "learned preconditioner");
+  return PD_OK;
+}
+
+// The solve with the preconditioner choice: block Jacobi records its
+// iteration count; the learned preconditioner gets at most
+// learned_fallback x that many iterations (when known), after which the
+// block-Jacobi inverse is restored and the solve goes on from its iterate.
+int run_solve_pc(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st) {
+  if (prm->precond != 2) {
+    if (int e = run_solve(c, prm, st)) return e;
+    if (prm->precond == 1) c->bj_iters = prm->precision == 1 ? c->last_mx_iters : c->last_iters;
+    return PD_OK;
+  }
+  if (c->mat_prec != prm->precision)
+    return fail(c, PD_ERR_STATE, "the matrix was not assembled with this precision");
+  if (int e = apply_learned(c, prm)) return e;
+  pd_step_params p2 = *prm;
+  const double f = prm->learned_fallback > 0.0 ? prm->learned_fallback : 1.5;
+  if (c->bj_iters > 0)
+    p2.max_iter = std::min(prm->max_iter, (int32_t)ceil(f * (double)c->bj_iters));
+  if (int e = run_solve(c, &p2, st)) return e;
+  if (c->last_converged || p2.max_iter >= prm->max_iter) return PD_OK;
+  // fall back to block Jacobi from the current iterate
+  const int used = prm->precision == 1 ? c->last_mx_iters : c->last_iters;
+  pd::launch_block_jacobi(c->n, c->diag, c->act, c->minv,
+                          prm->precision == 1 ? c->minv32 : nullptr, c->stream);
+  CHECK_LAUNCH(c, "block Jacobi");
+  pd_step_params p3 = *prm;
+  p3.precond = 1;
+  p3.max_iter = std::max(0, prm->max_iter - used);
+  p3.inner_max_iter = prm->inner_max_iter > 0 ? prm->inner_max_iter : prm->max_iter;
+  pd_step_stats s3 = {};
+  if (int e = run_solve(c, &p3, st ? &s3 : nullptr)) return e;
+  if (st) {
+    st->cg_iterations = used + s3.cg_iterations;
+    st->converged = s3.converged;
+    st->residual_norm = s3.residual_norm;
+    st->b_norm = s3.b_norm;
+    st->refinements += s3.refinements;
+    st->host_syncs += s3.host_syncs;
+    st->ms_spmv_total += s3.ms_spmv_total;
+    st->spmv_launches += s3.spmv_launches;
+    st->ms_resid_total += s3.ms_resid_total;
+    st->resid_launches += s3.resid_launches;
+    st->precond_fallbacks = 1;
+  }
+  return PD_OK;
+}
+
 // ---------------------------------------------------------------------------
 // Captured steps (graph = 1).  One hipGraph holds the assembly, the stretch
 // commit (a copy, so the captured pointers keep their roles), the solve's
@@ -974,6 +1080,7 @@ void zero_timing(pd_step_stats* st) {
   st->refinements = 0;
   st->ms_resid_total = 0.0;
   st->resid_launches = 0;
+  st->precond_fallbacks = 0;
 }
 
 }  // namespace
@@ -1073,6 +1180,7 @@ int pd_ctx_create(const pd_config* cfg, pd_ctx** out) {
   ALLOC(c->zone_cnt, pd::zone_tile_words(c->n));
   ALLOC(c->cand, n);
   ALLOC(c->widths, (size_t)c->nslices);
+  ALLOC(c->bond_stats, n);
   ALLOC(c->x_bak, n);
   ALLOC(c->v_bak, n);
   ALLOC(c->top_bak, n);
@@ -1122,6 +1230,7 @@ int pd_ctx_destroy(pd_ctx* c) {
   destroy_graphs(c);
   dfree(c->consts);
   if (c->h_consts) (void)hipHostFree(c->h_consts);
+  dfree(c->bond_stats); dfree(c->pc_feat); dfree(c->pc_p); dfree(c->pc_w);
   if (c->h_mx) (void)hipHostFree(c->h_mx);
   if (c->h_st) (void)hipHostFree(c->h_st);
   if (c->h_cnt) (void)hipHostFree(c->h_cnt);
@@ -1382,7 +1491,7 @@ int pd_solve(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st) {
   if (!c) return fail(nullptr, PD_ERR_ARG, "null context");
   if (int e = check_params(c, prm)) return e;
   HIPCHK(c, hipSetDevice(c->device));
-  if (int e = run_solve(c, prm, st)) return e;
+  if (int e = run_solve_pc(c, prm, st)) return e;
   if (!c->last_converged)
     return fail(c, PD_ERR_NOT_CONVERGED, "CG reached the iteration limit before rtol");
   return PD_OK;
@@ -1413,7 +1522,7 @@ int pd_step(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st) {
   if (int e = enqueue_assemble(c, prm)) return e;
   if (timing) HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
   std::swap(c->s, c->s_new);  // core/simulator.cuh:2612
-  if (int e = run_solve(c, prm, st)) return e;
+  if (int e = run_solve_pc(c, prm, st)) return e;
   read_assemble_stats(c, prm, st);  // the solve's waits covered the copies
   if (timing) HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
   if (int e = enqueue_update(c, prm)) return e;
@@ -1654,6 +1763,248 @@ int pd_tune_spmv(pd_ctx* c, int32_t iters, double* ms3, int32_t* best) {
   if (ms3)
     for (int v = 0; v < 3; ++v) ms3[v] = ms[v];
   if (best) *best = variants[b];
+  return PD_OK;
+}
+
+int pd_learned_set(pd_ctx* c, const float* params, int32_t count) {
+  if (!c) return fail(nullptr, PD_ERR_ARG, "null context");
+  if (!params || count != PC_NPARAM)
+    return fail(c, PD_ERR_ARG, "the network takes PD_LEARNED_PARAMS floats");
+  HIPCHK(c, hipSetDevice(c->device));
+  if (!c->pc_p) {
+    if (int e = dalloc(c, &c->pc_p, PC_NPARAM)) return e;
+    if (int e = dalloc(c, &c->pc_w, PC_NWEIGHT)) return e;
+  }
+  HIPCHK(c, hipMemcpyAsync(c->pc_p, params, sizeof(float) * PC_NPARAM, hipMemcpyHostToDevice,
+                           c->stream));
+  pd::launch_pc_to_bf16(c->pc_p, c->pc_w, c->stream);
+  CHECK_LAUNCH(c, "bf16 weights");
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->pc_loaded = true;
+  return PD_OK;
+}
+
+int pd_get_precond(pd_ctx* c, double* minv9) {
+  if (!c) return fail(nullptr, PD_ERR_ARG, "null context");
+  if (!minv9) return fail(c, PD_ERR_ARG, "null output");
+  HIPCHK(c, hipSetDevice(c->device));
+  std::vector<double> mv((size_t)c->nslices * 64 * 6);
+  if (!mv.empty())
+    HIPCHK(c, hipMemcpyAsync(mv.data(), c->minv, mv.size() * sizeof(double),
+                             hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int32_t p = 0; p < c->n; ++p) {
+    const double* six = mv.data() + (size_t)(p >> 6) * 384 + (p & 63);
+    const double m[9] = {six[0],   six[64],  six[128], six[64], six[192],
+                         six[256], six[128], six[256], six[320]};
+    memcpy(minv9 + 9 * (size_t)p, m, sizeof(m));
+  }
+  return PD_OK;
+}
+
+static int tfail(pd_trainer* t, int code, const std::string& msg) {
+  if (t) t->err = msg;
+  g_last_error = msg;
+  return code;
+}
+#define THIPCHK(t, expr)                                                              \
+  do {                                                                                \
+    hipError_t e_ = (expr);                                                           \
+    if (e_ != hipSuccess)                                                             \
+      return tfail((t), PD_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+int pd_trainer_create(int32_t device, int64_t capacity, uint64_t seed, pd_trainer** out) {
+  if (!out) return tfail(nullptr, PD_ERR_ARG, "null output pointer");
+  *out = nullptr;
+  if (capacity < 1) return tfail(nullptr, PD_ERR_ARG, "capacity must be >= 1");
+  pd_trainer* t = new pd_trainer();
+  t->device = device;
+  t->cap = capacity;
+  auto bail = [&](int code) {
+    std::string msg = t->err;
+    pd_trainer_destroy(t);
+    g_last_error = msg;
+    return code;
+  };
+  if (hipSetDevice(device) != hipSuccess) return bail(tfail(t, PD_ERR_ARG, "bad device"));
+  if (hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking) != hipSuccess)
+    return bail(tfail(t, PD_ERR_HIP, "hipStreamCreate failed"));
+  if (hipMalloc((void**)&t->samples, sizeof(float) * PC_SAMPLE * capacity) != hipSuccess ||
+      hipMalloc((void**)&t->p, sizeof(float) * PC_NPARAM) != hipSuccess ||
+      hipMalloc((void**)&t->m, sizeof(float) * PC_NWEIGHT) != hipSuccess ||
+      hipMalloc((void**)&t->v, sizeof(float) * PC_NWEIGHT) != hipSuccess ||
+      hipMalloc((void**)&t->w, sizeof(__bf16) * PC_NWEIGHT) != hipSuccess)
+    return bail(tfail(t, PD_ERR_HIP, "hipMalloc failed"));
+  // He-uniform weights from a seeded splitmix64, zero biases, a small last
+  // layer (S starts near I: block Jacobi), eps 1e-3
+  std::vector<float> p(PC_NPARAM, 0.0f);
+  uint64_t z = seed;
+  auto uni = [&]() {
+    z += 0x9E3779B97F4A7C15ull;
+    uint64_t x = z;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    x ^= x >> 31;
+    return (double)(x >> 11) * (1.0 / 9007199254740992.0);
+  };
+  auto fill = [&](int off, int rows, int cols, double scale) {
+    const double lim = scale * sqrt(6.0 / cols);
+    for (int i = 0; i < rows * cols; ++i) p[off + i] = (float)((2.0 * uni() - 1.0) * lim);
+  };
+  fill(PC_OFF_W1, PC_NH, PC_NIN, 1.0);
+  fill(PC_OFF_W2, PC_NH, PC_NH, 1.0);
+  fill(PC_OFF_W3, PC_NOUT, PC_NH, 0.01);
+  for (int k = 0; k < PC_NIN; ++k) p[PC_OFF_ISTD + k] = 1.0f;
+  p[PC_OFF_EPS] = 1e-3f;
+  if (hipMemcpy(t->p, p.data(), sizeof(float) * PC_NPARAM, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemset(t->m, 0, sizeof(float) * PC_NWEIGHT) != hipSuccess ||
+      hipMemset(t->v, 0, sizeof(float) * PC_NWEIGHT) != hipSuccess)
+    return bail(tfail(t, PD_ERR_HIP, "trainer initialisation failed"));
+  pd::launch_pc_to_bf16(t->p, t->w, t->stream);
+  if (hipStreamSynchronize(t->stream) != hipSuccess)
+    return bail(tfail(t, PD_ERR_HIP, "trainer initialisation failed"));
+  *out = t;
+  return PD_OK;
+}
+
+int pd_trainer_destroy(pd_trainer* t) {
+  if (!t) return PD_OK;
+  (void)hipSetDevice(t->device);
+  if (t->stream) (void)hipStreamSynchronize(t->stream);
+  for (void* q : {(void*)t->samples, (void*)t->p, (void*)t->m, (void*)t->v, (void*)t->w,
+                  (void*)t->partial, (void*)t->loss})
+    if (q) (void)hipFree(q);
+  if (t->stream) (void)hipStreamDestroy(t->stream);
+  delete t;
+  return PD_OK;
+}
+
+const char* pd_trainer_last_error(const pd_trainer* t) {
+  return t ? t->err.c_str() : g_last_error.c_str();
+}
+
+int pd_trainer_add(pd_trainer* t, pd_ctx* c, uint64_t seed, int32_t stride, int64_t* added) {
+  if (!t || !c) return tfail(t, PD_ERR_ARG, "null trainer or context");
+  if (c->device != t->device) return tfail(t, PD_ERR_ARG, "trainer and context on different devices");
+  if (c->mat_prec < 0) return tfail(t, PD_ERR_STATE, "no matrix assembled in the context");
+  if (stride < 1) return tfail(t, PD_ERR_ARG, "stride must be >= 1");
+  THIPCHK(t, hipSetDevice(c->device));
+  const int32_t n = c->n;
+  uint64_t h = seed;
+  h = (h ^ (h >> 30)) * 0xBF58476D1CE4E5B9ull;
+  h = (h ^ (h >> 27)) * 0x94D049BB133111EBull;
+  h ^= h >> 31;
+  const int64_t off = (int64_t)(h % (uint64_t)stride);
+  const int64_t rows = off < n ? (n - 1 - off) / stride + 1 : 0;
+  if (t->count + rows > t->cap) return tfail(t, PD_ERR_ARG, "trainer capacity exceeded");
+  if (!c->pc_feat)
+    if (int e = dalloc(c, &c->pc_feat, (size_t)std::max(n, 1) * PC_NIN))
+      return tfail(t, e, c->err);
+  pd::PcFeatArgs fa;
+  fa.n = n;
+  fa.slice_off = c->slice_off;
+  fa.top = c->top;
+  fa.cap = c->cap;
+  fa.col = c->col;
+  fa.x = c->x;
+  fa.diag = c->diag;
+  fa.bond_stats = c->bond_stats;
+  fa.act = c->act;
+  fa.body = c->body;
+  fa.mat = c->dmat;
+  fa.feat = c->pc_feat;
+  pd::launch_pc_features(fa, c->stream);
+  pd::launch_pc_probe(n, seed, c->z, c->stream);
+  if (c->mat_prec == 1)
+    pd::launch_spmv_wide(mx_args(c, 1), c->z, c->y, c->stream);
+  else
+    pd::launch_spmv_plain(spmv_view(c), c->z, c->y, c->stream);
+  pd::PcSampleArgs sa;
+  sa.n = n;
+  sa.feat = c->pc_feat;
+  sa.diag = c->diag;
+  sa.Ae = c->y;
+  sa.e = c->z;
+  sa.act = c->act;
+  sa.bc = c->bc;
+  sa.seed = seed;
+  sa.keep = (uint32_t)stride;
+  sa.out = t->samples + t->count * PC_SAMPLE;
+  sa.capacity = t->cap - t->count;
+  pd::launch_pc_samples(sa, c->stream);
+  THIPCHK(t, hipGetLastError());
+  THIPCHK(t, hipStreamSynchronize(c->stream));
+  t->count += rows;
+  if (added) *added = rows;
+  return PD_OK;
+}
+
+int pd_trainer_fit(pd_trainer* t, int32_t steps, int32_t batch, float lr, uint64_t seed,
+                   float* losses) {
+  if (!t) return tfail(nullptr, PD_ERR_ARG, "null trainer");
+  if (t->count < 1) return tfail(t, PD_ERR_STATE, "no samples");
+  if (steps < 1 || batch < 128 || batch % 128 != 0)
+    return tfail(t, PD_ERR_ARG, "steps >= 1 and batch a positive multiple of 128");
+  THIPCHK(t, hipSetDevice(t->device));
+  const int32_t waves = batch / 32;
+  if (waves > t->partial_waves) {
+    if (t->partial) (void)hipFree(t->partial);
+    t->partial = nullptr;
+    THIPCHK(t, hipMalloc((void**)&t->partial, sizeof(float) * (PC_NWEIGHT + 1) * waves));
+    t->partial_waves = waves;
+  }
+  if (steps > t->loss_cap) {
+    if (t->loss) (void)hipFree(t->loss);
+    t->loss = nullptr;
+    THIPCHK(t, hipMalloc((void**)&t->loss, sizeof(float) * steps));
+    t->loss_cap = steps;
+  }
+  if (!t->normed) {
+    pd::launch_pc_norm(t->samples, t->count, t->p, t->stream);
+    t->normed = true;
+  }
+  for (int32_t k = 0; k < steps; ++k) {
+    pd::PcTrainArgs a;
+    a.samples = t->samples;
+    a.nsamples = t->count;
+    a.batch = batch;
+    a.seed = seed;
+    a.step = t->steps_done;
+    a.w = t->w;
+    a.p = t->p;
+    a.partial = t->partial;
+    pd::launch_pc_train(a, t->stream);
+    pd::launch_pc_adam(t->partial, waves, t->p, t->m, t->v, t->w, t->steps_done, lr, t->loss + k,
+                       t->stream);
+    ++t->steps_done;
+  }
+  THIPCHK(t, hipGetLastError());
+  if (losses)
+    THIPCHK(t, hipMemcpyAsync(losses, t->loss, sizeof(float) * steps, hipMemcpyDeviceToHost,
+                              t->stream));
+  THIPCHK(t, hipStreamSynchronize(t->stream));
+  if (losses)
+    for (int32_t k = 0; k < steps; ++k) losses[k] /= (float)batch;  // mean |M v - e|^2
+  return PD_OK;
+}
+
+int pd_trainer_params(pd_trainer* t, float* params, int32_t count) {
+  if (!t || !params || count != PC_NPARAM) return tfail(t, PD_ERR_ARG, "bad arguments");
+  THIPCHK(t, hipSetDevice(t->device));
+  THIPCHK(t, hipMemcpyAsync(params, t->p, sizeof(float) * PC_NPARAM, hipMemcpyDeviceToHost,
+                            t->stream));
+  THIPCHK(t, hipStreamSynchronize(t->stream));
+  return PD_OK;
+}
+
+int pd_trainer_samples(pd_trainer* t, int64_t first, int64_t count, float* out) {
+  if (!t || !out || first < 0 || count < 0 || first + count > t->count)
+    return tfail(t, PD_ERR_ARG, "bad sample range");
+  THIPCHK(t, hipSetDevice(t->device));
+  if (count)
+    THIPCHK(t, hipMemcpy(out, t->samples + first * PC_SAMPLE, sizeof(float) * PC_SAMPLE * count,
+                         hipMemcpyDeviceToHost));
   return PD_OK;
 }
 

@@ -52,6 +52,7 @@ struct AsmArgs {
   int32_t contact;
   int32_t track_gap;
   const pd_step_consts* consts;  // non-null: dt, dt2, clock, gravity from here
+  float2* bond_stats;            // non-null: (mean, max) stretch of the live bonds
   double* gap_val;   // [n] when track_gap
   int32_t* gap_q;    // [n] when track_gap
   pd_step_counters* counters;
@@ -133,6 +134,91 @@ struct MxArgs {
   int32_t precond;
   int32_t unroll;  // slots per trip of the SpMV row loops: 2, 4 or 8
 };
+
+// Learned block preconditioner: a 16-64-64-8 MLP (BF16 weights, FP32
+// accumulation on MFMA) maps per-particle features to S_p = L L^T + eps I,
+// and M_p = C_p^{-T} S_p C_p^{-1} with D_p = C_p C_p^T (S = I is block Jacobi).
+#define PC_NIN 16
+#define PC_NH 64
+#define PC_NOUT 8
+// flat parameter vector: W1[NH][NIN] b1[NH] W2[NH][NH] b2[NH] W3[NOUT][NH]
+// b3[NOUT] fmean[NIN] fistd[NIN] eps
+#define PC_OFF_W1 0
+#define PC_OFF_B1 (PC_OFF_W1 + PC_NH * PC_NIN)
+#define PC_OFF_W2 (PC_OFF_B1 + PC_NH)
+#define PC_OFF_B2 (PC_OFF_W2 + PC_NH * PC_NH)
+#define PC_OFF_W3 (PC_OFF_B2 + PC_NH)
+#define PC_OFF_B3 (PC_OFF_W3 + PC_NOUT * PC_NH)
+#define PC_OFF_MEAN (PC_OFF_B3 + PC_NOUT)
+#define PC_OFF_ISTD (PC_OFF_MEAN + PC_NIN)
+#define PC_OFF_EPS (PC_OFF_ISTD + PC_NIN)
+#define PC_NPARAM (PC_OFF_EPS + 1)
+#define PC_NWEIGHT PC_OFF_MEAN  // trainable entries (weights and biases)
+
+struct PcNet {
+  const __bf16* w;     // [PC_NWEIGHT] bf16 copy (biases unused in this copy)
+  const float* p;      // [PC_NPARAM] FP32 parameters
+};
+
+// per-particle raw features (float[n][PC_NIN]) of the assembled system
+struct PcFeatArgs {
+  int32_t n;
+  const int64_t* slice_off;
+  const int32_t* top;
+  const int32_t* cap;
+  const int32_t* col;
+  const double4* x;
+  const double* diag;  // FP64 planes
+  const float2* bond_stats;
+  const int32_t* act;
+  const int32_t* body;
+  const pd_mat_dev* mat;
+  float* feat;
+};
+void launch_pc_features(const PcFeatArgs& a, hipStream_t s);
+// M_p from the network for every visible row: minv (FP64 planes) and, when
+// minv32 is non-null, its float copy
+void launch_pc_infer(int32_t n, const float* feat, const PcNet& net, const double* diag,
+                     const int32_t* act, double* minv, float* minv32, hipStream_t s);
+// block-Jacobi inverse of every visible row's diagonal block (the fallback)
+void launch_block_jacobi(int32_t n, const double* diag, const int32_t* act, double* minv,
+                         float* minv32, hipStream_t s);
+
+// training samples: feat[NIN], D (6, float), v = (A e)_p (3), e_p (3)
+#define PC_SAMPLE 28
+struct PcSampleArgs {
+  int32_t n;
+  const float* feat;
+  const double* diag;
+  const double4* Ae;  // A e
+  const double4* e;
+  const int32_t* act;
+  const int32_t* bc;
+  uint64_t seed;
+  uint32_t keep;  // stride: rows p with p % keep == (hash of seed) % keep
+  float* out;     // [capacity][PC_SAMPLE], sample p / keep
+  int64_t capacity;
+};
+void launch_pc_probe(int32_t n, uint64_t seed, double4* e, hipStream_t s);
+void launch_pc_samples(const PcSampleArgs& a, hipStream_t s);
+// one optimisation step on `batch` samples drawn with replacement
+struct PcTrainArgs {
+  const float* samples;
+  int64_t nsamples;
+  int32_t batch;  // multiple of 32
+  uint64_t seed;
+  int32_t step;
+  const __bf16* w;  // bf16 weights
+  const float* p;   // FP32 parameters (biases, normalisation, eps)
+  float* partial;   // [batch / 32][PC_NWEIGHT + 1] per-wave gradients + loss
+};
+void launch_pc_train(const PcTrainArgs& a, hipStream_t s);
+// fold the partial gradients, Adam update of p (weights and biases), refresh w
+void launch_pc_adam(const float* partial, int32_t nwaves, float* p, float* m, float* v,
+                    __bf16* w, int32_t step, float lr, float* loss_out, hipStream_t s);
+void launch_pc_to_bf16(const float* p, __bf16* w, hipStream_t s);
+// per-feature mean and 1/std over the samples' raw features into p
+void launch_pc_norm(const float* samples, int64_t nsamples, float* p, hipStream_t s);
 
 #define PD_MAX_ZONE_BODIES 8
 

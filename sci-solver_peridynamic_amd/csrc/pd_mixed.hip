@@ -154,7 +154,7 @@ __device__ __forceinline__ void row_f32(const MxArgs& A, const float4* __restric
 
 __device__ __forceinline__ void precond_f(const float* minv, int pc, int p, float r0, float r1,
                                           float r2, float& z0, float& z1, float& z2) {
-  if (pc == 1) {
+  if (pc != 0) {  // block Jacobi (1) or the learned block (2)
     const float* m = minv + (int64_t)(p >> 6) * 6 * 64 + (p & 63);
     const float m00 = m[0], m01 = m[64], m02 = m[128], m11 = m[192], m12 = m[256],
                 m22 = m[320];

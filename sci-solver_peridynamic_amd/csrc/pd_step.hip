@@ -46,6 +46,7 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
     const double stretch = a.s[p];
     const float stretch_f = (float)stretch;
     double smin = 0.0;
+    double ssum = 0.0, smax = -INFINITY;
     double k00 = 0.0, k01 = 0.0, k02 = 0.0, k11 = 0.0, k12 = 0.0, k22 = 0.0;
     double gmin = INFINITY;
     int gq = -1;
@@ -110,6 +111,8 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
       double vscale = 1.0;
       if (d0 > hr) vscale = 0.5 + (mt.horizon - d0) / 2.0 / mt.radius;
       if (drdivd < smin) smin = drdivd;
+      ssum += drdivd;
+      smax = drdivd > smax ? drdivd : smax;
       const double kk = drdivd * vscale;
       f0 += kk * e0;
       f1 += kk * e1;
@@ -241,6 +244,9 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
       broken = (unsigned long long)(otop - top);
     }
     live = (unsigned long long)(top + 1);
+    if (a.bond_stats)
+      a.bond_stats[p] = make_float2(top >= 0 ? (float)(ssum / (top + 1)) : 0.0f,
+                                    top >= 0 ? (float)smax : 0.0f);
     if (a.track_gap) {
       a.gap_val[p] = gmin;
       a.gap_q[p] = gq;
@@ -462,7 +468,7 @@ void launch_spmv_plain(const SpmvView& A, const double4* x, double4* y, hipStrea
 __device__ __forceinline__ void precond(const double* minv, int pc, int p, double r0,
                                         double r1, double r2, double& z0, double& z1,
                                         double& z2) {
-  if (pc == 1) {
+  if (pc != 0) {  // block Jacobi (1) or the learned block (2)
     const double* m = minv + (int64_t)(p >> 6) * 6 * 64 + (p & 63);
     const double m00 = m[0], m01 = m[64], m02 = m[128], m11 = m[192], m12 = m[256],
                  m22 = m[320];

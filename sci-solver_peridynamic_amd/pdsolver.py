@@ -61,6 +61,7 @@ class StepParams(C.Structure):
         ("precision", C.c_int32), ("max_refine", C.c_int32),
         ("inner_rtol", C.c_double), ("inner_max_iter", C.c_int32),
         ("contact_mode", C.c_int32), ("spmv_unroll", C.c_int32), ("graph", C.c_int32),
+        ("learned_fallback", C.c_double),
     ]
 
 
@@ -74,7 +75,7 @@ class StepStats(C.Structure):
         ("ms_update", C.c_double), ("ms_spmv_total", C.c_double),
         ("spmv_launches", C.c_int32), ("host_syncs", C.c_int32),
         ("refinements", C.c_int32), ("ms_resid_total", C.c_double),
-        ("resid_launches", C.c_int32),
+        ("resid_launches", C.c_int32), ("precond_fallbacks", C.c_int32),
     ]
 
     def as_dict(self):
@@ -97,7 +98,8 @@ DEFAULT_PARAMS = dict(dt=1e-4, clock_scale=1.0, gravity=9.8, integrator=0, preco
                       bmax=(1e30,) * 3, wall_radius=0.0, restitution=0.001,
                       contact_radius=0.0, contact_stiffness=0.0, contact_volume=1.0,
                       contact_cell=0.0, track_gap=0, timing=0, precision=0, max_refine=20,
-                      inner_rtol=1e-5, contact_mode=0, spmv_unroll=0, graph=0)
+                      inner_rtol=1e-5, contact_mode=0, spmv_unroll=0, graph=0,
+                      learned_fallback=0.0)
 
 
 def make_params(**kw) -> StepParams:
@@ -152,13 +154,25 @@ def lib():
         L.pd_spmv_bytes.argtypes = [vp, C.POINTER(C.c_double)]
         L.pd_set_specimens.argtypes = [vp, C.c_int32, vp, vp]
         L.pd_tune_spmv.argtypes = [vp, C.c_int32, vp, C.POINTER(C.c_int32)]
+        L.pd_learned_set.argtypes = [vp, vp, C.c_int32]
+        L.pd_get_precond.argtypes = [vp, vp]
+        L.pd_trainer_create.argtypes = [C.c_int32, C.c_int64, C.c_uint64, C.POINTER(vp)]
+        L.pd_trainer_destroy.argtypes = [vp]
+        L.pd_trainer_last_error.argtypes = [vp]
+        L.pd_trainer_last_error.restype = C.c_char_p
+        L.pd_trainer_add.argtypes = [vp, vp, C.c_uint64, C.c_int32, C.POINTER(C.c_int64)]
+        L.pd_trainer_fit.argtypes = [vp, C.c_int32, C.c_int32, C.c_float, C.c_uint64, vp]
+        L.pd_trainer_params.argtypes = [vp, vp, C.c_int32]
+        L.pd_trainer_samples.argtypes = [vp, C.c_int64, C.c_int64, vp]
         L.pd_get_specimen_stats.argtypes = [vp, C.c_int32, C.POINTER(SpecimenStats)]
         for name in ("pd_ctx_create", "pd_ctx_destroy", "pd_upload", "pd_download",
                      "pd_build_bonds", "pd_set_bonds", "pd_get_bonds", "pd_get_matrix",
                      "pd_step", "pd_assemble", "pd_solve", "pd_update", "pd_commit_stretch",
                      "pd_spmv", "pd_backup", "pd_restore", "pd_synchronize", "pd_bench_spmv",
                      "pd_spmv_bytes", "pd_set_specimens", "pd_get_specimen_stats",
-                     "pd_tune_spmv"):
+                     "pd_tune_spmv", "pd_learned_set", "pd_get_precond", "pd_trainer_create",
+                     "pd_trainer_destroy", "pd_trainer_add", "pd_trainer_fit",
+                     "pd_trainer_params", "pd_trainer_samples"):
             getattr(L, name).restype = C.c_int
         _lib = L
     return _lib
@@ -284,6 +298,15 @@ class Context:
         self._check(lib().pd_tune_spmv(self._h, iters, _ptr(ms), C.byref(best)))
         return ms, best.value
 
+    def learned_set(self, params):
+        params = np.ascontiguousarray(params, np.float32)
+        self._check(lib().pd_learned_set(self._h, _ptr(params), len(params)))
+
+    def get_precond(self):
+        out = np.zeros((self.n, 9))
+        self._check(lib().pd_get_precond(self._h, _ptr(out)))
+        return out
+
     def set_specimens(self, first, count):
         first = np.ascontiguousarray(first, np.int32)
         count = np.ascontiguousarray(count, np.int32)
@@ -326,6 +349,57 @@ class Context:
 HOST_LIB_PATH = os.path.join(HERE, "libpdhost.so")
 CONFIG_DIR = os.path.join(HERE, "configs")
 _hlib = None
+
+
+LEARNED_PARAMS = 5801
+LEARNED_LAYOUT = dict(W1=(0, (64, 16)), b1=(1024, (64,)), W2=(1088, (64, 64)), b2=(5184, (64,)),
+                      W3=(5248, (8, 64)), b3=(5760, (8,)), mean=(5768, (16,)),
+                      istd=(5784, (16,)), eps=(5800, ()))
+
+
+class Trainer:
+    """pd_trainer: samples of assembled systems and the network fit."""
+
+    def __init__(self, capacity, device=0, seed=1):
+        self._h = C.c_void_p()
+        rc = lib().pd_trainer_create(device, int(capacity), seed, C.byref(self._h))
+        if rc != PD_OK:
+            raise PDError(rc, lib().pd_trainer_last_error(None).decode())
+
+    def _check(self, rc):
+        if rc != PD_OK:
+            raise PDError(rc, lib().pd_trainer_last_error(self._h).decode())
+
+    def add(self, ctx, seed, stride):
+        n = C.c_int64()
+        self._check(lib().pd_trainer_add(self._h, ctx._h, seed, stride, C.byref(n)))
+        return n.value
+
+    def fit(self, steps, batch=8192, lr=1e-3, seed=7):
+        losses = np.zeros(steps, np.float32)
+        self._check(lib().pd_trainer_fit(self._h, steps, batch, lr, seed, _ptr(losses)))
+        return losses
+
+    def params(self):
+        out = np.zeros(LEARNED_PARAMS, np.float32)
+        self._check(lib().pd_trainer_params(self._h, _ptr(out), LEARNED_PARAMS))
+        return out
+
+    def samples(self, first, count):
+        out = np.zeros((count, 28), np.float32)
+        self._check(lib().pd_trainer_samples(self._h, first, count, _ptr(out)))
+        return out
+
+    def close(self):
+        if self._h:
+            lib().pd_trainer_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def host_lib():
