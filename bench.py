@@ -67,17 +67,28 @@ def step_params(desc, step, precision, **kw):
     return pds.make_params(**dict(pds.step_kwargs(desc, step), precision=precision, **kw))
 
 
-def gpu_run(steps, warmup, precision=0, copy_calibration=0):
+NETWORK = os.path.join(PKG, "networks", "impact_precond.txt")
+
+
+def gpu_run(steps, warmup, precision=0, copy_calibration=0, precond=None, network=None):
     desc = pds.describe(CONFIG)
     sc = pds.scene(CONFIG)
     n = desc["num_pd_particle"]
     ctx, bst = new_context(desc, sc)
-    log(f"[{'mixed' if precision else 'fp64'}] particles {n}, directed bonds {bst.total_bonds} "
+    extra = {}
+    if precond is not None:
+        extra["precond"] = precond
+    if network is not None:
+        ctx.learned_set(network)
+    log(f"[{'mixed' if precision else 'fp64'} precond {extra.get('precond', desc['precond'])}] "
+        f"particles {n}, directed bonds {bst.total_bonds} "
         f"({bst.min_bonds}..{bst.max_bonds} per particle), bond build {bst.ms_build:.1f} ms")
     step = 0
     for _ in range(warmup):
         step += 1
-        ctx.step(step_params(desc, step, precision))
+        # block Jacobi first: the learned solve's fallback budget comes from it
+        kw = dict(extra, precond=1) if step == 1 else extra
+        ctx.step(step_params(desc, step, precision, **kw))
     ctx.synchronize()
     # the autotuner picks the SpMV variant on the resident matrix (outside
     # the timed region); spmv_unroll = 0 then uses it
@@ -88,12 +99,14 @@ def gpu_run(steps, warmup, precision=0, copy_calibration=0):
     resid_n = 0
     iters = []
     refinements = []
+    fallbacks = 0
     broken = 0
     phase = dict(contact=0.0, assemble=0.0, solve=0.0, update=0.0)
     t0 = time.perf_counter()
     for _ in range(steps):
         step += 1
-        st = ctx.step(step_params(desc, step, precision, timing=1))
+        st = ctx.step(step_params(desc, step, precision, timing=1, **extra))
+        fallbacks += st.precond_fallbacks
         spmv_ms += st.ms_spmv_total
         spmv_n += st.spmv_launches
         resid_ms += st.ms_resid_total
@@ -122,7 +135,8 @@ def gpu_run(steps, warmup, precision=0, copy_calibration=0):
                 spmv_ms=spmv_ms, spmv_n=spmv_n, iters=iters, broken=int(broken),
                 spmv_bytes=spmv_bytes, phase=phase, build_ms=bst.ms_build,
                 plain_ms=plain_ms, plain_bytes=plain_bytes, resid_ms=resid_ms, resid_n=resid_n,
-                refinements=refinements, unroll=unroll, tune_ms=[float(t) for t in tune_ms])
+                refinements=refinements, unroll=unroll, tune_ms=[float(t) for t in tune_ms],
+                fallbacks=fallbacks)
 
 
 BATCH_CONFIG = os.path.join(PKG, "configs", "batch_64.txt")
@@ -272,6 +286,8 @@ def main():
                     help="skip the mixed-precision report of the fp64 line")
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the batched-specimen report")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip the learned-preconditioner report")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="after the run, N rounds of device copies of known size")
@@ -320,8 +336,10 @@ def main():
     if prec:
         out["config"]["refinements_mean"] = float(np.mean(r["refinements"]))
         out["config"]["resid_spmv_ms_mean"] = r["resid_ms"] / max(r["resid_n"], 1)
+    m_run = None
     if not prec and not a.no_mixed:
         m = gpu_run(a.steps, a.warmup, 1)
+        m_run = m
         m_rate = m["steps"] / m["elapsed"]
         out["mixed_precision"] = {
             "value": m_rate,
@@ -337,6 +355,28 @@ def main():
             "spmv_unroll": m["unroll"],
             "spmv_tune_ms_2_4_8": m["tune_ms"],
             "position_error_vs_fp64": mixed_error(a.warmup + a.steps),
+        }
+    if not a.no_learned and os.path.exists(NETWORK):
+        net = np.loadtxt(NETWORK, dtype=np.float32)
+        rows = {}
+        for prec in (0, 1):
+            L = gpu_run(a.steps, a.warmup, prec, precond=2, network=net)
+            base = r if prec == 0 or m_run is None else m_run
+            rows["fp64" if prec == 0 else "mixed"] = {
+                "value": L["steps"] / L["elapsed"],
+                "unit": "steps/s",
+                "speedup_vs_fp64_block_jacobi": (L["steps"] / L["elapsed"]) / out["value"],
+                "cg_iterations_mean": float(np.mean(L["iters"])),
+                "block_jacobi_iterations_mean": float(np.mean(base["iters"])),
+                "fallbacks": L["fallbacks"],
+                "phase_ms_per_step": {k: v / L["steps"] for k, v in L["phase"].items()},
+            }
+        out["learned_preconditioner"] = {
+            "network": "16-64-64-8 MLP, BF16 weights on MFMA, FP32 accumulation; trained by "
+                       "train_precond.py on samples of 20 impact_100 runs with scene seeds "
+                       "101-120; evaluated here on the shipped seeds (1973 / 2024)",
+            "fallback": "block Jacobi after 1.5x the last block-Jacobi iteration count",
+            **rows,
         }
     if not a.no_batched:
         bw = max(1, min(a.warmup, 5))

@@ -80,6 +80,7 @@ RunningParameters::RunningParameters()
       batch_s0_min(0.01),
       batch_s0_max(0.03) {
   obj_file[0] = 0;
+  learned_network[0] = 0;
   saving_path[0] = 0;
 }
 
@@ -186,7 +187,11 @@ void SystemParameter::loadParametersFromFile(const char* _paramFile) {
     if (name == "pd_gravity") s.pd_gravity = atof(v);
     if (name == "pd_phase_in") s.pd_phase_in = atoi(v);
     if (name == "cg_preconditioner")
-      r.cg_preconditioner = (value == "block_jacobi" || value == "1") ? 1 : 0;
+      r.cg_preconditioner = (value == "block_jacobi" || value == "1")
+                                ? 1
+                                : ((value == "learned" || value == "2") ? 2 : 0);
+    if (name == "learned_network")
+      snprintf(r.learned_network, sizeof(r.learned_network), "%s", v);
     if (name == "cg_relative_tolerance") r.cg_relative_tolerance = atof(v);
     if (name == "cg_max_iterations") r.cg_max_iterations = atoi(v);
     if (name == "cg_precision") r.cg_precision = (value == "mixed" || value == "1") ? 1 : 0;

@@ -124,7 +124,8 @@ struct RunningParameters {
   Integrator integrator;
 
   // ---- added ----
-  int cg_preconditioner;        // 0 identity (reference), 1 block Jacobi
+  int cg_preconditioner;        // 0 identity (reference), 1 block Jacobi, 2 learned
+  char learned_network[512];    // parameter file of the learned preconditioner
   double cg_relative_tolerance; // CG_RELATIVE_TOLERANCE
   int cg_max_iterations;        // MAX_CG_ITERATION
   int cg_precision;             // 0 FP64, 1 float inner solve + FP64 refinement

@@ -118,6 +118,16 @@ void Simulator::initPeridynamicsBonds() {
   if (!check(pd_upload(ctx_, PD_FIELD_BC_VELOCITY, bc_velocity_.data(), v4), "upload vbc")) return;
   if (!check(pd_upload(ctx_, PD_FIELD_SOLUTION, zero.data(), v4), "upload solution")) return;
   if (!check(pd_build_bonds(ctx_, &bond_stats_), "pd_build_bonds")) return;
+  if (runningParams_.cg_preconditioner == 2) {
+    // the learned preconditioner's parameters: one float per line
+    std::vector<float> net;
+    if (FILE* fp = fopen(runningParams_.learned_network, "r")) {
+      float v;
+      while (fscanf(fp, "%f", &v) == 1) net.push_back(v);
+      fclose(fp);
+    }
+    if (!check(pd_learned_set(ctx_, net.data(), (int32_t)net.size()), "learned_network")) return;
+  }
   hostSimParams_.pd_max_num_bonds = bond_stats_.max_bonds;
   if (!spec_first_.empty() &&
       !check(pd_set_specimens(ctx_, (int32_t)spec_first_.size(), spec_first_.data(),
