@@ -761,7 +761,8 @@ void orc_cg(orc_system* sys, const orc_step_params* prm, orc_step_stats* st) {
  * Higham, "Accuracy and Stability of Numerical Algorithms", ch. 12) with a
  * PCG inner solver:
  *   A32 = the assembled off-diagonal blocks rounded to float, FP64 diagonal
- *   loop:  r = S(b - A32 x) in FP64 (float blocks widened exactly)
+ *   loop:  r = S(b - A32 x) in FP64 (float blocks widened exactly; bond
+ *          terms summed first, the diagonal term added last)
  *          stop if ||r|| <= rtol ||S b|| (FP64 norms)
  *          inner PCG on float vectors and float arithmetic for A32 d = r,
  *            diagonal block and block-Jacobi inverse rounded to float,
@@ -785,13 +786,13 @@ static inline void blk_mul_wide(const float* A, const double* x, double* o) {
   o[2] = (double)A[6] * x[0] + (double)A[7] * x[1] + (double)A[8] * x[2];
 }
 
-/* y = A32 x in FP64: FP64 diagonal, float bonds widened, FP64 x */
+/* y = A32 x in FP64: FP64 diagonal, float bonds widened, FP64 x; the bond
+ * terms are summed first (from 0.0, list order), the diagonal term last */
 void orc_spmv_wide(const orc_system* sys, const double* x4, double* y4) {
   const int32_t n = sys->n;
 #pragma omp parallel for schedule(static)
   for (int32_t p = 0; p < n; ++p) {
-    double acc[3], t[3];
-    blk_mul(sys->diag + 9 * (int64_t)p, x4 + 4 * (int64_t)p, acc);
+    double acc[3] = {0.0, 0.0, 0.0}, t[3], d[3];
     const int64_t rs = sys->row_start[p];
     for (int32_t bnd = 0; bnd <= sys->top[p]; ++bnd) {
       const int32_t q = sys->col[rs + bnd];
@@ -800,10 +801,11 @@ void orc_spmv_wide(const orc_system* sys, const double* x4, double* y4) {
       acc[1] += t[1];
       acc[2] += t[2];
     }
+    blk_mul(sys->diag + 9 * (int64_t)p, x4 + 4 * (int64_t)p, d);
     double* y = y4 + 4 * (int64_t)p;
-    y[0] = acc[0];
-    y[1] = acc[1];
-    y[2] = acc[2];
+    y[0] = d[0] + acc[0];
+    y[1] = d[1] + acc[1];
+    y[2] = d[2] + acc[2];
     y[3] = 0.0;
   }
 }

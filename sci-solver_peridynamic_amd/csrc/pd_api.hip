@@ -531,12 +531,32 @@ int enqueue_contact(pd_ctx* c, const pd_step_params* prm) {
   return PD_OK;
 }
 
+// The mixed solve with block Jacobi can take its first residual from the
+// assembly pass (the bond blocks and the inverse are in registers there).
+bool fused_resid(const pd_step_params* prm) {
+  return prm->precision == 1 && prm->precond == 1;
+}
+
+// fuse: also produce the mixed solve's first residual and inner start
+// (fused_resid(prm) must hold; the next solve is run with fused = true)
 int enqueue_assemble(pd_ctx* c, const pd_step_params* prm,
-                     const pd_step_consts* consts = nullptr) {
+                     const pd_step_consts* consts = nullptr, bool fuse = false) {
   if (prm->precision == 1)
     if (int e = ensure_mixed(c)) return e;
   pd::launch_reset_counters(c->counters, c->stream);
+  const double dt2 = prm->integrator == 1 ? (prm->dt * prm->dt) / 4.0 : prm->dt * prm->dt;
+  if (fuse) {
+    pd::launch_set_dirichlet(c->n, c->bc, c->act, c->vbc, prm->integrator, prm->dt, consts,
+                             c->sol, c->stream);
+    CHECK_LAUNCH(c, "dirichlet values");
+  }
   pd::AsmArgs a;
+  a.fuse = fuse ? 1 : 0;
+  a.xs = c->sol;
+  a.r32 = c->r32;
+  a.p32 = c->p32;
+  a.d32 = c->d32;
+  a.part = c->part;
   a.mixed = prm->precision == 1 ? 1 : 0;
   a.val32 = c->val32;
   a.diag32 = c->diag32;
@@ -564,7 +584,7 @@ int enqueue_assemble(pd_ctx* c, const pd_step_params* prm,
   a.bc = c->bc;
   a.mat = c->dmat;
   a.dt = prm->dt;
-  a.dt2 = prm->integrator == 1 ? (prm->dt * prm->dt) / 4.0 : prm->dt * prm->dt;
+  a.dt2 = dt2;
   a.clock = prm->clock_scale;
   a.gravity = prm->gravity;
   a.integrator = prm->integrator;
@@ -611,14 +631,17 @@ void read_assemble_stats(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st
 }
 
 // CG on the device; the host waits once per chunk of iterations.
-int run_solve_mixed(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st, int resumed);
+int run_solve_mixed(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st, int resumed,
+                    bool fused);
 
 // resumed > 0: a captured step already ran the start and `resumed`
 // iterations; the device state is in h_st / h_mx.  No per-launch timing then.
-int run_solve(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st, int resumed = 0) {
+// fused: the assembly produced the mixed solve's first residual partials.
+int run_solve(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st, int resumed = 0,
+              bool fused = false) {
   if (c->mat_prec != prm->precision)
     return fail(c, PD_ERR_STATE, "the matrix was not assembled with this precision");
-  if (prm->precision == 1) return run_solve_mixed(c, prm, st, resumed);
+  if (prm->precision == 1) return run_solve_mixed(c, prm, st, resumed, fused);
   const pd::CgArgs a = cg_args(c, prm->precond);
   const int ns = c->nspec;
   if (!resumed) {
@@ -703,7 +726,8 @@ int run_solve(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st, int resum
 // iterations, then the correction, the FP64 residual and its test; every
 // kernel returns at once unless the device-side phase asks for it, so the
 // host only predicts how many inner iterations to enqueue before it waits.
-int run_solve_mixed(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st, int resumed) {
+int run_solve_mixed(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st, int resumed,
+                    bool fused) {
   const pd::MxArgs a = mx_args(c, prm->precond);
   const MixedLimits lim = mixed_limits(prm);
   const bool timing = prm->timing != 0 && !resumed;
@@ -721,12 +745,16 @@ int run_solve_mixed(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st, int
   if (!resumed) {
     pd::launch_mx_setup(c->mx, ns, prm->rtol, prm->max_iter, lim.inner_rtol, lim.inner_max,
                         lim.max_refine, c->stream);
-    if (timing) {
-      if (int e = pair(resid_ev)) return e;
-      HIPCHK(c, hipEventRecord(c->ev[resid_ev.back()], c->stream));
+    if (fused) {
+      pd::launch_mx_resid_fold(a, c->stream);
+    } else {
+      if (timing) {
+        if (int e = pair(resid_ev)) return e;
+        HIPCHK(c, hipEventRecord(c->ev[resid_ev.back()], c->stream));
+      }
+      pd::launch_mx_resid(a, c->stream);
+      if (timing) HIPCHK(c, hipEventRecord(c->ev[resid_ev.back() + 1], c->stream));
     }
-    pd::launch_mx_resid(a, c->stream);
-    if (timing) HIPCHK(c, hipEventRecord(c->ev[resid_ev.back() + 1], c->stream));
     CHECK_LAUNCH(c, "mixed residual");
   }
   int chunk = resumed ? 1 : std::max(1, c->last_mx_iters);
@@ -791,7 +819,7 @@ int run_solve_mixed(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st, int
       for (int k = 0; k < ran; ++k)
         spmv_ms += elapsed(c->ev[chunk_spmv[k]], c->ev[chunk_spmv[k] + 1]);
       spmv_ran += ran;
-      if (syncs == 1 && !resumed) {
+      if (syncs == 1 && !resumed && !fused) {
         resid_ms += elapsed(c->ev[resid_ev[0]], c->ev[resid_ev[0] + 1]);
         ++resid_ran;
       }
@@ -887,9 +915,10 @@ int apply_learned(pd_ctx* c, const pd_step_params* prm) {
 // iteration count; the learned preconditioner gets at most
 // learned_fallback x that many iterations (when known), after which the
 // block-Jacobi inverse is restored and the solve goes on from its iterate.
-int run_solve_pc(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st) {
+int run_solve_pc(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st,
+                 bool fused = false) {
   if (prm->precond != 2) {
-    if (int e = run_solve(c, prm, st)) return e;
+    if (int e = run_solve(c, prm, st, 0, fused)) return e;
     if (prm->precond == 1) c->bj_iters = prm->precision == 1 ? c->last_mx_iters : c->last_iters;
     return PD_OK;
   }
@@ -982,14 +1011,18 @@ int capture_step(pd_ctx* c, const pd_step_params* prm, int K) {
   destroy_graphs(c);
   const int ns = c->nspec;
   int e = capture(c, &c->g_solve, [&]() -> int {
-    if (int e2 = enqueue_assemble(c, prm, c->consts)) return e2;
+    const bool fuse = fused_resid(prm);
+    if (int e2 = enqueue_assemble(c, prm, c->consts, fuse)) return e2;
     pd::launch_copy(c->s, c->s_new, c->n, c->stream);  // core/simulator.cuh:2612
     if (prm->precision == 1) {
       const pd::MxArgs a = mx_args(c, prm->precond);
       const MixedLimits lim = mixed_limits(prm);
       pd::launch_mx_setup(c->mx, ns, prm->rtol, prm->max_iter, lim.inner_rtol, lim.inner_max,
                           lim.max_refine, c->stream);
-      pd::launch_mx_resid(a, c->stream);
+      if (fuse)
+        pd::launch_mx_resid_fold(a, c->stream);
+      else
+        pd::launch_mx_resid(a, c->stream);
       for (int i = 0; i < K; ++i) {
         pd::launch_mx_spmv(a, c->stream);
         pd::launch_mx_update(a, c->stream);
@@ -1519,10 +1552,11 @@ int pd_step(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st) {
   if (prm->contact_radius > 0.0)
     if (int e = enqueue_contact(c, prm)) return e;
   if (timing) HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
-  if (int e = enqueue_assemble(c, prm)) return e;
+  const bool fuse = fused_resid(prm);
+  if (int e = enqueue_assemble(c, prm, nullptr, fuse)) return e;
   if (timing) HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
   std::swap(c->s, c->s_new);  // core/simulator.cuh:2612
-  if (int e = run_solve_pc(c, prm, st)) return e;
+  if (int e = run_solve_pc(c, prm, st, fuse)) return e;
   read_assemble_stats(c, prm, st);  // the solve's waits covered the copies
   if (timing) HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
   if (int e = enqueue_update(c, prm)) return e;

@@ -53,6 +53,17 @@ struct AsmArgs {
   int32_t track_gap;
   const pd_step_consts* consts;  // non-null: dt, dt2, clock, gravity from here
   float2* bond_stats;            // non-null: (mean, max) stretch of the live bonds
+  // mixed only, fuse = 1: the solve's first residual r = S(b - A32 xs) (bond
+  // terms first, then the diagonal term, as the mixed residual kernel) and
+  // the inner start r32, p32 = M32 r32, d32 = 0 with the four tile partials
+  // <r,r>, <r32,z>, <r32,r32>, <Sb,Sb> in part; Dirichlet rows of xs must
+  // hold their prescribed values already (launch_set_dirichlet)
+  int32_t fuse;
+  const double4* xs;
+  float4* r32;
+  float4* p32;
+  float4* d32;
+  double* part;
   double* gap_val;   // [n] when track_gap
   int32_t* gap_q;    // [n] when track_gap
   pd_step_counters* counters;
@@ -248,6 +259,8 @@ void launch_spmv_plain(const SpmvView& A, const double4* x, double4* y, hipStrea
 void launch_mx_setup(pd_mx_state* st, int32_t nspec, double rtol, int32_t max_iter,
                      double inner_rtol, int32_t inner_max, int32_t max_refine, hipStream_t s);
 void launch_mx_resid(const MxArgs& a, hipStream_t s);  // residual + its fold
+// the fold alone, after an assembly that produced the residual partials
+void launch_mx_resid_fold(const MxArgs& a, hipStream_t s);
 void launch_mx_spmv(const MxArgs& a, hipStream_t s);
 void launch_mx_update(const MxArgs& a, hipStream_t s);  // alpha, update, beta
 void launch_mx_dir(const MxArgs& a, hipStream_t s);
@@ -261,6 +274,11 @@ void launch_update(int32_t n, double4* x, double4* v, const double4* sol,
                    const double* lo, const double* hi, double restitution,
                    const pd_step_consts* consts, hipStream_t s);
 void launch_copy(double* dst, const double* src, int32_t n, hipStream_t s);
+// sol of the Dirichlet rows: vbc (implicit Euler) or dt vbc (Newmark-beta),
+// the values the assembly writes; dt from consts when non-null
+void launch_set_dirichlet(int32_t n, const int32_t* bc, const int32_t* act, const double4* vbc,
+                          int32_t integrator, double dt, const pd_step_consts* consts,
+                          double4* sol, hipStream_t s);
 // contact forces of the m candidates cand[0..m) (particles 0..m when cand
 // is null); fc of every other particle is set to zero
 void launch_contact(int32_t n, const double4* x, const int32_t* body,

@@ -6,8 +6,9 @@
 // the residual, its norm and every dot product in FP64) and moves the
 // bandwidth-bound inner iterations to 4-byte values and vectors:
 //   residual  r = S(b - A32 x): FP64 diagonal, float bond blocks widened
-//             exactly, FP64 x; fused with the inner start d = 0, z = M32 r32,
-//             p = z (r32 = float(r))
+//             exactly, FP64 x (bond terms summed first in list order, the
+//             diagonal term added last); fused with the inner start d = 0,
+//             z = M32 r32, p = z (r32 = float(r))
 //   inner     y = A32 p in float arithmetic (diagonal first, bonds in list
 //             order, (a0 x0 + a1 x1) + a2 x2 per block row), alpha and beta
 //             in FP64 from FP64 sums of widened float products, rounded to
@@ -56,14 +57,11 @@ __device__ __forceinline__ void row_wide(const MxArgs& A, const double4* __restr
   const int s = p >> 6;
   const int lane = p & 63;
   const double* dg = A.diag + (int64_t)s * 6 * 64 + lane;
-  {
-    const double4 xp = x[p];
-    const double a00 = dg[0], a01 = dg[64], a02 = dg[128], a11 = dg[192], a12 = dg[256],
-                 a22 = dg[320];
-    y0 = a00 * xp.x + a01 * xp.y + a02 * xp.z;
-    y1 = a01 * xp.x + a11 * xp.y + a12 * xp.z;
-    y2 = a02 * xp.x + a12 * xp.y + a22 * xp.z;
-  }
+  // the bond terms first, from 0.0 in list order; the diagonal term is
+  // added last (the order the assembly can produce in the same pass)
+  y0 = 0.0;
+  y1 = 0.0;
+  y2 = 0.0;
   const int64_t base = A.slice_off[s];
   const int top = A.top[p];
   const int* __restrict__ cp = A.col + base * 64 + lane;
@@ -101,6 +99,14 @@ __device__ __forceinline__ void row_wide(const MxArgs& A, const double4* __restr
     y0 += t0;
     y1 += t1;
     y2 += t2;
+  }
+  {
+    const double4 xp = x[p];
+    const double a00 = dg[0], a01 = dg[64], a02 = dg[128], a11 = dg[192], a12 = dg[256],
+                 a22 = dg[320];
+    y0 = (a00 * xp.x + a01 * xp.y + a02 * xp.z) + y0;
+    y1 = (a01 * xp.x + a11 * xp.y + a12 * xp.z) + y1;
+    y2 = (a02 * xp.x + a12 * xp.y + a22 * xp.z) + y2;
   }
 }
 
@@ -449,6 +455,11 @@ void launch_mx_resid(const MxArgs& a, hipStream_t s) {
     default: hipLaunchKernelGGL(k_mx_resid<4>, dim3(g), dim3(PD_BLOCK), 0, s, a);
   }
   hipLaunchKernelGGL(k_mx_resid_fold, dim3(a.batch.nspec), dim3(PD_BLOCK), 0, s, a, g);
+}
+void launch_mx_resid_fold(const MxArgs& a, hipStream_t s) {
+  if (a.n)
+    hipLaunchKernelGGL(k_mx_resid_fold, dim3(a.batch.nspec), dim3(PD_BLOCK), 0, s, a,
+                       tiles_of(a.n));
 }
 void launch_mx_spmv(const MxArgs& a, hipStream_t s) {
   if (!a.n) return;

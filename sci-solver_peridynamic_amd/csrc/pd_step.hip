@@ -33,6 +33,7 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int p = tile * PD_BLOCK + threadIdx.x;
   unsigned long long broken = 0, live = 0;
+  double t_rr = 0.0, t_rz = 0.0, t_r32 = 0.0, t_bb = 0.0;  // fused residual partials
   if (p < a.n && a.act[p] != PD_INVISIBLE) {
     const pd_mat_dev mt = a.mat[a.body[p]];
     const int s = p >> 6;
@@ -47,6 +48,7 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
     const float stretch_f = (float)stretch;
     double smin = 0.0;
     double ssum = 0.0, smax = -INFINITY;
+    double rb0 = 0.0, rb1 = 0.0, rb2 = 0.0;  // fused residual: bond terms
     double k00 = 0.0, k01 = 0.0, k02 = 0.0, k11 = 0.0, k12 = 0.0, k22 = 0.0;
     double gmin = INFINITY;
     int gq = -1;
@@ -133,13 +135,23 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
       k12 += F12;
       k22 += F22;
       if constexpr (MIXED) {
+        const float g00 = (float)((F00 * -1.0) * a.dt2), g01 = (float)((F01 * -1.0) * a.dt2),
+                    g02 = (float)((F02 * -1.0) * a.dt2), g11 = (float)((F11 * -1.0) * a.dt2),
+                    g12 = (float)((F12 * -1.0) * a.dt2), g22 = (float)((F22 * -1.0) * a.dt2);
         float* vo = a.val32 + slot * 6 * 64 + lane;
-        __builtin_nontemporal_store((float)((F00 * -1.0) * a.dt2), vo + 0 * 64);
-        __builtin_nontemporal_store((float)((F01 * -1.0) * a.dt2), vo + 1 * 64);
-        __builtin_nontemporal_store((float)((F02 * -1.0) * a.dt2), vo + 2 * 64);
-        __builtin_nontemporal_store((float)((F11 * -1.0) * a.dt2), vo + 3 * 64);
-        __builtin_nontemporal_store((float)((F12 * -1.0) * a.dt2), vo + 4 * 64);
-        __builtin_nontemporal_store((float)((F22 * -1.0) * a.dt2), vo + 5 * 64);
+        __builtin_nontemporal_store(g00, vo + 0 * 64);
+        __builtin_nontemporal_store(g01, vo + 1 * 64);
+        __builtin_nontemporal_store(g02, vo + 2 * 64);
+        __builtin_nontemporal_store(g11, vo + 3 * 64);
+        __builtin_nontemporal_store(g12, vo + 4 * 64);
+        __builtin_nontemporal_store(g22, vo + 5 * 64);
+        if (a.fuse) {
+          const double4 u = a.xs[q];
+          const double b00 = g00, b01 = g01, b02 = g02, b11 = g11, b12 = g12, b22 = g22;
+          rb0 += b00 * u.x + b01 * u.y + b02 * u.z;
+          rb1 += b01 * u.x + b11 * u.y + b12 * u.z;
+          rb2 += b02 * u.x + b12 * u.y + b22 * u.z;
+        }
       } else {
         double* vo = a.val + slot * 6 * 64 + lane;
         __builtin_nontemporal_store((F00 * -1.0) * a.dt2, vo + 0 * 64);
@@ -175,6 +187,7 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
       dgf[5 * 64] = (float)d22;
     }
     // block-Jacobi inverse by cofactors (six distinct entries)
+    float m32[6];
     {
       const double c00 = d11 * d22 - d12 * d12;
       const double c01 = d02 * d12 - d01 * d22;
@@ -185,13 +198,15 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
       const double det = (d00 * c00 + d01 * c01) + d02 * c02;
       const double id = 1.0 / det;
       if constexpr (MIXED) {
+        m32[0] = (float)(c00 * id);
+        m32[1] = (float)(c01 * id);
+        m32[2] = (float)(c02 * id);
+        m32[3] = (float)(c11 * id);
+        m32[4] = (float)(c12 * id);
+        m32[5] = (float)(c22 * id);
         float* mo = a.minv32 + (int64_t)s * 6 * 64 + lane;
-        mo[0 * 64] = (float)(c00 * id);
-        mo[1 * 64] = (float)(c01 * id);
-        mo[2 * 64] = (float)(c02 * id);
-        mo[3 * 64] = (float)(c11 * id);
-        mo[4 * 64] = (float)(c12 * id);
-        mo[5 * 64] = (float)(c22 * id);
+#pragma unroll
+        for (int c = 0; c < 6; ++c) mo[c * 64] = m32[c];
       } else {
         double* mo = a.minv + (int64_t)s * 6 * 64 + lane;
         mo[0 * 64] = c00 * id;
@@ -232,7 +247,33 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
     }
     bo.w = 0.0;
     a.b[p] = bo;
-    if (a.bc[p]) {
+    if constexpr (MIXED) {
+      if (a.fuse) {
+        // y = D x_p + (bond terms); r = S(b - y); inner start
+        const double4 xp = a.xs[p];
+        const double y0 = (d00 * xp.x + d01 * xp.y + d02 * xp.z) + rb0;
+        const double y1 = (d01 * xp.x + d11 * xp.y + d12 * xp.z) + rb1;
+        const double y2 = (d02 * xp.x + d12 * xp.y + d22 * xp.z) + rb2;
+        const bool fr = a.bc[p] == 0;
+        const double r0 = fr ? bo.x - y0 : 0.0;
+        const double r1 = fr ? bo.y - y1 : 0.0;
+        const double r2 = fr ? bo.z - y2 : 0.0;
+        const double sb0 = fr ? bo.x : 0.0, sb1 = fr ? bo.y : 0.0, sb2 = fr ? bo.z : 0.0;
+        const float h0 = (float)r0, h1 = (float)r1, h2 = (float)r2;
+        // z = M32 r32, the float block-Jacobi inverse just written
+        const float z0 = m32[0] * h0 + m32[1] * h1 + m32[2] * h2;
+        const float z1 = m32[1] * h0 + m32[3] * h1 + m32[4] * h2;
+        const float z2 = m32[2] * h0 + m32[4] * h1 + m32[5] * h2;
+        a.r32[p] = make_float4(h0, h1, h2, 0.0f);
+        a.p32[p] = make_float4(z0, z1, z2, 0.0f);
+        a.d32[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        t_rr = (r0 * r0 + r1 * r1) + r2 * r2;
+        t_rz = ((double)h0 * (double)z0 + (double)h1 * (double)z1) + (double)h2 * (double)z2;
+        t_r32 = ((double)h0 * (double)h0 + (double)h1 * (double)h1) + (double)h2 * (double)h2;
+        t_bb = (sb0 * sb0 + sb1 * sb1) + sb2 * sb2;
+      }
+    }
+    if (a.bc[p] && !(MIXED && a.fuse)) {  // fused: launch_set_dirichlet wrote it
       const double4 vb = a.vbc[p];
       if (a.integrator == 1)
         a.sol[p] = make_double4(a.dt * vb.x, a.dt * vb.y, a.dt * vb.z, 0.0);
@@ -251,9 +292,35 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
       a.gap_val[p] = gmin;
       a.gap_q[p] = gq;
     }
-  } else if (p < a.n && a.track_gap) {
-    a.gap_val[p] = INFINITY;
-    a.gap_q[p] = -1;
+  } else if (p < a.n) {
+    if (a.track_gap) {
+      a.gap_val[p] = INFINITY;
+      a.gap_q[p] = -1;
+    }
+    if constexpr (MIXED) {
+      if (a.fuse) {
+        const float4 z4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        a.r32[p] = z4;
+        a.p32[p] = z4;
+        a.d32[p] = z4;
+      }
+    }
+  }
+  if constexpr (MIXED) {
+    if (a.fuse) {  // the residual kernel's tile partials, all threads
+      __shared__ double lds4[4];
+      const int ntiles = gridDim.x;
+      const double v0 = tile_sum(t_rr, lds4);
+      const double v1 = tile_sum(t_rz, lds4);
+      const double v2 = tile_sum(t_r32, lds4);
+      const double v3 = tile_sum(t_bb, lds4);
+      if (threadIdx.x == 0) {
+        a.part[tile] = v0;
+        a.part[ntiles + tile] = v1;
+        a.part[2 * (int64_t)ntiles + tile] = v2;
+        a.part[3 * (int64_t)ntiles + tile] = v3;
+      }
+    }
   }
   // integer counters: order-free
 #pragma unroll
@@ -809,6 +876,31 @@ void launch_update(int32_t n, double4* x, double4* v, const double4* sol,
   a.rest = restitution;
   const int nb = (n + PD_BLOCK - 1) / PD_BLOCK;
   if (nb) hipLaunchKernelGGL(k_update, dim3(nb), dim3(PD_BLOCK), 0, s, a);
+}
+
+__global__ __launch_bounds__(PD_BLOCK) void k_set_dirichlet(int n, const int32_t* bc,
+                                                            const int32_t* act,
+                                                            const double4* vbc, int integrator,
+                                                            double dt,
+                                                            const pd_step_consts* consts,
+                                                            double4* sol) {
+  const int p = blockIdx.x * PD_BLOCK + threadIdx.x;
+  if (p >= n || !bc[p] || act[p] == PD_INVISIBLE) return;
+  if (consts) dt = consts->dt;
+  const double4 vb = vbc[p];
+  if (integrator == 1)
+    sol[p] = make_double4(dt * vb.x, dt * vb.y, dt * vb.z, 0.0);
+  else
+    sol[p] = make_double4(vb.x, vb.y, vb.z, 0.0);
+}
+
+void launch_set_dirichlet(int32_t n, const int32_t* bc, const int32_t* act, const double4* vbc,
+                          int32_t integrator, double dt, const pd_step_consts* consts,
+                          double4* sol, hipStream_t s) {
+  const int nb = (n + PD_BLOCK - 1) / PD_BLOCK;
+  if (nb)
+    hipLaunchKernelGGL(k_set_dirichlet, dim3(nb), dim3(PD_BLOCK), 0, s, n, bc, act, vbc,
+                       integrator, dt, consts, sol);
 }
 
 __global__ __launch_bounds__(PD_BLOCK) void k_copy(double* dst, const double* src, int n) {
