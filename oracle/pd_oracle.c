@@ -611,6 +611,31 @@ void orc_spmv(const orc_system* sys, const double* x4, double* y4) {
   }
 }
 
+/* y = A x for the solver's first residual: the bond terms summed first,
+ * from 0.0 in list order, the diagonal term added last (the order in which
+ * the GPU assembly pass can form it while it writes the blocks) */
+static void orc_spmv_resid(const orc_system* sys, const double* x4, double* y4) {
+  const int32_t n = sys->n;
+#pragma omp parallel for schedule(static)
+  for (int32_t p = 0; p < n; ++p) {
+    double acc[3] = {0.0, 0.0, 0.0}, t[3], d[3];
+    const int64_t rs = sys->row_start[p];
+    for (int32_t bnd = 0; bnd <= sys->top[p]; ++bnd) {
+      const int32_t q = sys->col[rs + bnd];
+      blk_mul(sys->val + 9 * (rs + bnd), x4 + 4 * (int64_t)q, t);
+      acc[0] += t[0];
+      acc[1] += t[1];
+      acc[2] += t[2];
+    }
+    blk_mul(sys->diag + 9 * (int64_t)p, x4 + 4 * (int64_t)p, d);
+    double* y = y4 + 4 * (int64_t)p;
+    y[0] = d[0] + acc[0];
+    y[1] = d[1] + acc[1];
+    y[2] = d[2] + acc[2];
+    y[3] = 0.0;
+  }
+}
+
 /* Inverse of the symmetric 3x3 diagonal block by cofactors; the six
  * distinct entries are computed once and mirrored so the inverse is
  * exactly symmetric. */
@@ -647,6 +672,8 @@ void orc_block_jacobi(orc_system* sys) {
  * vendored, the loop below is the textbook PCG of Saad Alg. 9.1 with the
  * CUSP monitor test ||r|| <= rtol ||b|| ahead of every iteration):
  *   y = A x; r = b - y; z = M r; p = z; rz = <r,z>
+ *     (this first product sums each row's bond terms first, then the
+ *     diagonal term: orc_spmv_resid)
  *   while !(||r|| <= tol or it >= max):
  *     y = A p; alpha = rz / <y,p>; x += alpha p; r -= alpha y;
  *     z = M r; rz' = <r,z>; beta = rz'/rz; p = z + beta p; ++it
@@ -692,7 +719,7 @@ void orc_cg(orc_system* sys, const orc_step_params* prm, orc_step_stats* st) {
       x[4 * (int64_t)i + 3] = 0.0;
     }
   }
-  orc_spmv(sys, x, y);
+  orc_spmv_resid(sys, x, y);
 #pragma omp parallel for schedule(static)
   for (int32_t i = 0; i < n; ++i) {
     const int64_t o = 4 * (int64_t)i;

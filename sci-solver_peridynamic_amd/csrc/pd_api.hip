@@ -531,14 +531,13 @@ int enqueue_contact(pd_ctx* c, const pd_step_params* prm) {
   return PD_OK;
 }
 
-// The mixed solve with block Jacobi can take its first residual from the
-// assembly pass (the bond blocks and the inverse are in registers there).
-bool fused_resid(const pd_step_params* prm) {
-  return prm->precision == 1 && prm->precond == 1;
-}
+// The solve can take its first residual and start from the assembly pass
+// (the blocks and the block-Jacobi inverse are in registers there); the
+// learned preconditioner's blocks are only formed after the assembly.
+bool fused_resid(const pd_step_params* prm) { return prm->precond != 2; }
 
-// fuse: also produce the mixed solve's first residual and inner start
-// (fused_resid(prm) must hold; the next solve is run with fused = true)
+// fuse: also produce the solve's first residual and start (fused_resid(prm)
+// must hold; the next solve is run with fused = true)
 int enqueue_assemble(pd_ctx* c, const pd_step_params* prm,
                      const pd_step_consts* consts = nullptr, bool fuse = false) {
   if (prm->precision == 1)
@@ -552,7 +551,10 @@ int enqueue_assemble(pd_ctx* c, const pd_step_params* prm,
   }
   pd::AsmArgs a;
   a.fuse = fuse ? 1 : 0;
+  a.pc = prm->precond;
   a.xs = c->sol;
+  a.r64 = c->r;
+  a.p64 = c->p;
   a.r32 = c->r32;
   a.p32 = c->p32;
   a.d32 = c->d32;
@@ -646,7 +648,10 @@ int run_solve(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st, int resum
   const int ns = c->nspec;
   if (!resumed) {
     pd::launch_cg_setup(c->st, ns, prm->rtol, prm->max_iter, c->stream);
-    pd::launch_cg_init(a, c->stream);
+    if (fused)
+      pd::launch_cg_init_fold(a, c->stream);
+    else
+      pd::launch_cg_init(a, c->stream);
     CHECK_LAUNCH(c, "cg init");
   }
   int launched = resumed;
@@ -1035,7 +1040,10 @@ int capture_step(pd_ctx* c, const pd_step_params* prm, int K) {
     } else {
       const pd::CgArgs a = cg_args(c, prm->precond);
       pd::launch_cg_setup(c->st, ns, prm->rtol, prm->max_iter, c->stream);
-      pd::launch_cg_init(a, c->stream);
+      if (fuse)
+        pd::launch_cg_init_fold(a, c->stream);
+      else
+        pd::launch_cg_init(a, c->stream);
       for (int i = 0; i < K; ++i) {
         pd::launch_cg_spmv(a, c->stream);
         pd::launch_cg_update(a, c->stream);

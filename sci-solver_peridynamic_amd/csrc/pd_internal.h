@@ -53,13 +53,19 @@ struct AsmArgs {
   int32_t track_gap;
   const pd_step_consts* consts;  // non-null: dt, dt2, clock, gravity from here
   float2* bond_stats;            // non-null: (mean, max) stretch of the live bonds
-  // mixed only, fuse = 1: the solve's first residual r = S(b - A32 xs) (bond
-  // terms first, then the diagonal term, as the mixed residual kernel) and
-  // the inner start r32, p32 = M32 r32, d32 = 0 with the four tile partials
-  // <r,r>, <r32,z>, <r32,r32>, <Sb,Sb> in part; Dirichlet rows of xs must
-  // hold their prescribed values already (launch_set_dirichlet)
+  // fuse = 1: the solve's first residual r = S(b - A xs), each row's bond
+  // terms summed first and the diagonal term last, with the solver's start
+  // and its tile partials in part (Dirichlet rows of xs must hold their
+  // prescribed values already: launch_set_dirichlet).  z = r (pc = 0) or
+  // the block-Jacobi inverse just formed times r (pc = 1).
+  //   FP64:  r64 = r, p64 = z; partials <r,z>, <r,r>, <Sb,Sb>
+  //   mixed: r32 = float(r), p32 = z, d32 = 0 (A32 with float bond blocks);
+  //          partials <r,r>, <r32,z>, <r32,r32>, <Sb,Sb>
   int32_t fuse;
+  int32_t pc;
   const double4* xs;
+  double4* r64;
+  double4* p64;
   float4* r32;
   float4* p32;
   float4* d32;
@@ -251,6 +257,8 @@ void launch_gap_reduce(const double* gap_val, const int32_t* gap_q, int32_t n,
 void launch_cg_setup(pd_cg_state* st, int32_t nspec, double rtol, int32_t max_iter,
                      hipStream_t s);
 void launch_cg_init(const CgArgs& a, hipStream_t s);
+// the fold alone, after an assembly that produced the start's partials
+void launch_cg_init_fold(const CgArgs& a, hipStream_t s);
 void launch_cg_spmv(const CgArgs& a, hipStream_t s);
 void launch_cg_update(const CgArgs& a, hipStream_t s);
 void launch_cg_dir(const CgArgs& a, hipStream_t s);

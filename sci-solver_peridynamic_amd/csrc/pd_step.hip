@@ -153,13 +153,22 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
           rb2 += b02 * u.x + b12 * u.y + b22 * u.z;
         }
       } else {
+        const double b00 = (F00 * -1.0) * a.dt2, b01 = (F01 * -1.0) * a.dt2,
+                     b02 = (F02 * -1.0) * a.dt2, b11 = (F11 * -1.0) * a.dt2,
+                     b12 = (F12 * -1.0) * a.dt2, b22 = (F22 * -1.0) * a.dt2;
         double* vo = a.val + slot * 6 * 64 + lane;
-        __builtin_nontemporal_store((F00 * -1.0) * a.dt2, vo + 0 * 64);
-        __builtin_nontemporal_store((F01 * -1.0) * a.dt2, vo + 1 * 64);
-        __builtin_nontemporal_store((F02 * -1.0) * a.dt2, vo + 2 * 64);
-        __builtin_nontemporal_store((F11 * -1.0) * a.dt2, vo + 3 * 64);
-        __builtin_nontemporal_store((F12 * -1.0) * a.dt2, vo + 4 * 64);
-        __builtin_nontemporal_store((F22 * -1.0) * a.dt2, vo + 5 * 64);
+        __builtin_nontemporal_store(b00, vo + 0 * 64);
+        __builtin_nontemporal_store(b01, vo + 1 * 64);
+        __builtin_nontemporal_store(b02, vo + 2 * 64);
+        __builtin_nontemporal_store(b11, vo + 3 * 64);
+        __builtin_nontemporal_store(b12, vo + 4 * 64);
+        __builtin_nontemporal_store(b22, vo + 5 * 64);
+        if (a.fuse) {
+          const double4 u = a.xs[q];
+          rb0 += b00 * u.x + b01 * u.y + b02 * u.z;
+          rb1 += b01 * u.x + b11 * u.y + b12 * u.z;
+          rb2 += b02 * u.x + b12 * u.y + b22 * u.z;
+        }
       }
       ++bnd;
     }
@@ -188,6 +197,7 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
     }
     // block-Jacobi inverse by cofactors (six distinct entries)
     float m32[6];
+    double m64[6];
     {
       const double c00 = d11 * d22 - d12 * d12;
       const double c01 = d02 * d12 - d01 * d22;
@@ -208,13 +218,15 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
 #pragma unroll
         for (int c = 0; c < 6; ++c) mo[c * 64] = m32[c];
       } else {
+        m64[0] = c00 * id;
+        m64[1] = c01 * id;
+        m64[2] = c02 * id;
+        m64[3] = c11 * id;
+        m64[4] = c12 * id;
+        m64[5] = c22 * id;
         double* mo = a.minv + (int64_t)s * 6 * 64 + lane;
-        mo[0 * 64] = c00 * id;
-        mo[1 * 64] = c01 * id;
-        mo[2 * 64] = c02 * id;
-        mo[3 * 64] = c11 * id;
-        mo[4 * 64] = c12 * id;
-        mo[5 * 64] = c22 * id;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) mo[c * 64] = m64[c];
       }
     }
     f0 *= mt.c_bond;
@@ -247,33 +259,45 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
     }
     bo.w = 0.0;
     a.b[p] = bo;
-    if constexpr (MIXED) {
-      if (a.fuse) {
-        // y = D x_p + (bond terms); r = S(b - y); inner start
-        const double4 xp = a.xs[p];
-        const double y0 = (d00 * xp.x + d01 * xp.y + d02 * xp.z) + rb0;
-        const double y1 = (d01 * xp.x + d11 * xp.y + d12 * xp.z) + rb1;
-        const double y2 = (d02 * xp.x + d12 * xp.y + d22 * xp.z) + rb2;
-        const bool fr = a.bc[p] == 0;
-        const double r0 = fr ? bo.x - y0 : 0.0;
-        const double r1 = fr ? bo.y - y1 : 0.0;
-        const double r2 = fr ? bo.z - y2 : 0.0;
-        const double sb0 = fr ? bo.x : 0.0, sb1 = fr ? bo.y : 0.0, sb2 = fr ? bo.z : 0.0;
+    if (a.fuse) {
+      // y = D x_p + (bond terms); r = S(b - y); the solver's start
+      const double4 xp = a.xs[p];
+      const double y0 = (d00 * xp.x + d01 * xp.y + d02 * xp.z) + rb0;
+      const double y1 = (d01 * xp.x + d11 * xp.y + d12 * xp.z) + rb1;
+      const double y2 = (d02 * xp.x + d12 * xp.y + d22 * xp.z) + rb2;
+      const bool fr = a.bc[p] == 0;
+      const double r0 = fr ? bo.x - y0 : 0.0;
+      const double r1 = fr ? bo.y - y1 : 0.0;
+      const double r2 = fr ? bo.z - y2 : 0.0;
+      const double sb0 = fr ? bo.x : 0.0, sb1 = fr ? bo.y : 0.0, sb2 = fr ? bo.z : 0.0;
+      t_rr = (r0 * r0 + r1 * r1) + r2 * r2;
+      t_bb = (sb0 * sb0 + sb1 * sb1) + sb2 * sb2;
+      if constexpr (MIXED) {
         const float h0 = (float)r0, h1 = (float)r1, h2 = (float)r2;
-        // z = M32 r32, the float block-Jacobi inverse just written
-        const float z0 = m32[0] * h0 + m32[1] * h1 + m32[2] * h2;
-        const float z1 = m32[1] * h0 + m32[3] * h1 + m32[4] * h2;
-        const float z2 = m32[2] * h0 + m32[4] * h1 + m32[5] * h2;
+        float z0 = h0, z1 = h1, z2 = h2;
+        if (a.pc) {  // z = M32 r32, the float block-Jacobi inverse just written
+          z0 = m32[0] * h0 + m32[1] * h1 + m32[2] * h2;
+          z1 = m32[1] * h0 + m32[3] * h1 + m32[4] * h2;
+          z2 = m32[2] * h0 + m32[4] * h1 + m32[5] * h2;
+        }
         a.r32[p] = make_float4(h0, h1, h2, 0.0f);
         a.p32[p] = make_float4(z0, z1, z2, 0.0f);
         a.d32[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        t_rr = (r0 * r0 + r1 * r1) + r2 * r2;
         t_rz = ((double)h0 * (double)z0 + (double)h1 * (double)z1) + (double)h2 * (double)z2;
         t_r32 = ((double)h0 * (double)h0 + (double)h1 * (double)h1) + (double)h2 * (double)h2;
-        t_bb = (sb0 * sb0 + sb1 * sb1) + sb2 * sb2;
+      } else {
+        double z0 = r0, z1 = r1, z2 = r2;
+        if (a.pc) {  // z = M r, the block-Jacobi inverse just written
+          z0 = m64[0] * r0 + m64[1] * r1 + m64[2] * r2;
+          z1 = m64[1] * r0 + m64[3] * r1 + m64[4] * r2;
+          z2 = m64[2] * r0 + m64[4] * r1 + m64[5] * r2;
+        }
+        a.r64[p] = make_double4(r0, r1, r2, 0.0);
+        a.p64[p] = make_double4(z0, z1, z2, 0.0);
+        t_rz = (r0 * z0 + r1 * z1) + r2 * z2;
       }
     }
-    if (a.bc[p] && !(MIXED && a.fuse)) {  // fused: launch_set_dirichlet wrote it
+    if (a.bc[p] && !a.fuse) {  // fused: launch_set_dirichlet wrote it
       const double4 vb = a.vbc[p];
       if (a.integrator == 1)
         a.sol[p] = make_double4(a.dt * vb.x, a.dt * vb.y, a.dt * vb.z, 0.0);
@@ -297,19 +321,23 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
       a.gap_val[p] = INFINITY;
       a.gap_q[p] = -1;
     }
-    if constexpr (MIXED) {
-      if (a.fuse) {
+    if (a.fuse) {
+      if constexpr (MIXED) {
         const float4 z4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         a.r32[p] = z4;
         a.p32[p] = z4;
         a.d32[p] = z4;
+      } else {
+        const double4 z4 = make_double4(0.0, 0.0, 0.0, 0.0);
+        a.r64[p] = z4;
+        a.p64[p] = z4;
       }
     }
   }
-  if constexpr (MIXED) {
-    if (a.fuse) {  // the residual kernel's tile partials, all threads
-      __shared__ double lds4[4];
-      const int ntiles = gridDim.x;
+  if (a.fuse) {  // the start kernel's tile partials, all threads
+    __shared__ double lds4[4];
+    const int ntiles = gridDim.x;
+    if constexpr (MIXED) {
       const double v0 = tile_sum(t_rr, lds4);
       const double v1 = tile_sum(t_rz, lds4);
       const double v2 = tile_sum(t_r32, lds4);
@@ -319,6 +347,15 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
         a.part[ntiles + tile] = v1;
         a.part[2 * (int64_t)ntiles + tile] = v2;
         a.part[3 * (int64_t)ntiles + tile] = v3;
+      }
+    } else {
+      const double v0 = tile_sum(t_rz, lds4);
+      const double v1 = tile_sum(t_rr, lds4);
+      const double v2 = tile_sum(t_bb, lds4);
+      if (threadIdx.x == 0) {
+        a.part[tile] = v0;
+        a.part[ntiles + tile] = v1;
+        a.part[2 * (int64_t)ntiles + tile] = v2;
       }
     }
   }
@@ -453,14 +490,22 @@ __device__ __forceinline__ double ntl(const double* p) {
 __device__ __forceinline__ int ntl(const int* p) { return __builtin_nontemporal_load(p); }
 
 // U slots per trip: all their loads are issued before any product (the
-// summation order does not depend on U)
-template <int U>
+// summation order does not depend on U).  DIAG_LAST: the bond terms are
+// summed from 0.0 and the diagonal term is added last (the solver's first
+// residual, the order the assembly pass forms it in).
+template <int U, bool DIAG_LAST = false>
 __device__ __forceinline__ void spmv_row(const SpmvView& A, const double4* __restrict__ x,
                                          int p, double& y0, double& y1, double& y2) {
   const int s = p >> 6;
   const int lane = p & 63;
   const double* dg = A.diag + (int64_t)s * 6 * 64 + lane;
-  blk6(dg[0], dg[64], dg[128], dg[192], dg[256], dg[320], x[p], y0, y1, y2);
+  if constexpr (DIAG_LAST) {
+    y0 = 0.0;
+    y1 = 0.0;
+    y2 = 0.0;
+  } else {
+    blk6(dg[0], dg[64], dg[128], dg[192], dg[256], dg[320], x[p], y0, y1, y2);
+  }
   const int64_t base = A.slice_off[s];
   const int top = A.top[p];
   const int* __restrict__ cp = A.col + base * 64 + lane;
@@ -498,6 +543,13 @@ __device__ __forceinline__ void spmv_row(const SpmvView& A, const double4* __res
     y0 += t0;
     y1 += t1;
     y2 += t2;
+  }
+  if constexpr (DIAG_LAST) {
+    double d0, d1, d2;
+    blk6(dg[0], dg[64], dg[128], dg[192], dg[256], dg[320], x[p], d0, d1, d2);
+    y0 = d0 + y0;
+    y1 = d1 + y1;
+    y2 = d2 + y2;
   }
 }
 
@@ -589,7 +641,7 @@ __global__ __launch_bounds__(PD_BLOCK) void k_cg_init(CgArgs a) {
   double t_rz = 0.0, t_rr = 0.0, t_bb = 0.0;
   if (p < a.A.n) {
     double y0, y1, y2;
-    spmv_row<U>(a.A, a.x, p, y0, y1, y2);
+    spmv_row<U, true>(a.A, a.x, p, y0, y1, y2);
     const double4 bp = a.b[p];
     const bool fr = a.bc[p] == 0;
     const double r0 = fr ? bp.x - y0 : 0.0;
@@ -770,6 +822,11 @@ void launch_cg_init(const CgArgs& a, hipStream_t s) {
     default: hipLaunchKernelGGL(k_cg_init<4>, dim3(g), dim3(PD_BLOCK), 0, s, a);
   }
   hipLaunchKernelGGL(k_cg_init_fold, dim3(a.batch.nspec), dim3(PD_BLOCK), 0, s, a, g);
+}
+void launch_cg_init_fold(const CgArgs& a, hipStream_t s) {
+  if (a.A.n)
+    hipLaunchKernelGGL(k_cg_init_fold, dim3(a.batch.nspec), dim3(PD_BLOCK), 0, s, a,
+                       tiles_of(a.A.n));
 }
 void launch_cg_spmv(const CgArgs& a, hipStream_t s) {
   if (!a.A.n) return;
