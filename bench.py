@@ -36,17 +36,19 @@ import pdsolver as pds  # noqa: E402
 
 CONFIG = os.path.join(PKG, "configs", "impact_100.txt")
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "cg_spmv_impact_100_traffic.txt")
+TRAFFIC_FILES = {0: os.path.join(ROOT, "profiles", "cg_spmv_impact_100_traffic.txt"),
+                 1: os.path.join(ROOT, "profiles", "mx_spmv_impact_100_traffic.txt")}
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def read_traffic():
-    """HBM bytes per CG SpMV launch from the committed rocprof counter runs."""
+def read_traffic(precision=0):
+    """HBM bytes per CG SpMV launch (FP64 k_cg_spmv, or the float inner
+    k_mx_spmv for precision 1) from the committed rocprof counter runs."""
     try:
-        with open(TRAFFIC_FILE) as f:
+        with open(TRAFFIC_FILES[precision]) as f:
             for line in f:
                 if line.startswith("hbm_bytes_per_launch"):
                     return float(line.split()[1])
@@ -331,7 +333,7 @@ def main():
             "spmv_unroll": r["unroll"],
             "spmv_tune_ms_2_4_8": r["tune_ms"],
         },
-        "roofline": spmv_roofline(r, kernel, None if prec else read_traffic()),
+        "roofline": spmv_roofline(r, kernel, read_traffic(prec)),
     }
     if prec:
         out["config"]["refinements_mean"] = float(np.mean(r["refinements"]))
@@ -349,7 +351,7 @@ def main():
             "cg_iterations_mean": float(np.mean(m["iters"])),
             "refinements_mean": float(np.mean(m["refinements"])),
             "phase_ms_per_step": {k: v / m["steps"] for k, v in m["phase"].items()},
-            "roofline": spmv_roofline(m, "k_mx_spmv (float inner CG SpMV)", None),
+            "roofline": spmv_roofline(m, "k_mx_spmv (float inner CG SpMV)", read_traffic(1)),
             "resid_spmv_ms_mean": m["resid_ms"] / max(m["resid_n"], 1),
             "plain_spmv_gbs": m["plain_bytes"] / (m["plain_ms"] * 1e-3) / 1e9,
             "spmv_unroll": m["unroll"],

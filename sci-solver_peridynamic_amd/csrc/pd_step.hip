@@ -54,27 +54,36 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
     int gq = -1;
     const double hr = mt.horizon - mt.radius;
     int bnd = 0;
-    int q_n = 0;
+    // software pipeline: the neighbour data of slot bnd + 1 and the column
+    // of slot bnd + 2 are in flight while slot bnd is computed, so neither
+    // the column load nor the gathers it feeds wait in line
+    int q_n = 0, q_nn = 0;
     double4 xq_n = make_double4(0, 0, 0, 0), oq_n = make_double4(0, 0, 0, 0);
+    double4 us_n = make_double4(0, 0, 0, 0);  // fused residual: xs of the neighbour
     double sq_n = 0.0;
     if (top >= 0) {
       q_n = a.col[base * 64 + lane];
       xq_n = a.x[q_n];
       oq_n = a.x0[q_n];
       sq_n = a.s[q_n];
+      if (a.fuse) us_n = a.xs[q_n];
     }
+    if (top >= 1) q_nn = a.col[(base + 1) * 64 + lane];
     while (bnd <= top) {
       const int64_t slot = base + bnd;
       const int q = q_n;
       const double4 xq = xq_n;
       const double4 oq = oq_n;
       const double sqv = sq_n;
+      const double4 u = us_n;
       if (bnd + 1 <= top) {
-        q_n = a.col[(slot + 1) * 64 + lane];
+        q_n = q_nn;
         xq_n = a.x[q_n];
         oq_n = a.x0[q_n];
         sq_n = a.s[q_n];
+        if (a.fuse) us_n = a.xs[q_n];
       }
+      if (bnd + 2 <= top) q_nn = a.col[(slot + 2) * 64 + lane];
       double e0 = xq.x - pos_p.x, e1 = xq.y - pos_p.y, e2 = xq.z - pos_p.z;
       const double dij = sqrt(e0 * e0 + e1 * e1 + e2 * e2);
       // cg_solver.cuh:256: divide by (float)max(dij, SMALL)
@@ -104,10 +113,13 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
         a.col[slot * 64 + lane] = last;
         a.col[last_slot] = q;
         --top;
+        // slot bnd again, now holding `last`: restart the pipeline there
         q_n = last;
         xq_n = a.x[last];
         oq_n = a.x0[last];
         sq_n = a.s[last];
+        if (a.fuse) us_n = a.xs[last];
+        if (bnd + 1 <= top) q_nn = a.col[(slot + 1) * 64 + lane];
         continue;
       }
       double vscale = 1.0;
@@ -146,7 +158,6 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
         __builtin_nontemporal_store(g12, vo + 4 * 64);
         __builtin_nontemporal_store(g22, vo + 5 * 64);
         if (a.fuse) {
-          const double4 u = a.xs[q];
           const double b00 = g00, b01 = g01, b02 = g02, b11 = g11, b12 = g12, b22 = g22;
           rb0 += b00 * u.x + b01 * u.y + b02 * u.z;
           rb1 += b01 * u.x + b11 * u.y + b12 * u.z;
@@ -164,7 +175,6 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
         __builtin_nontemporal_store(b12, vo + 4 * 64);
         __builtin_nontemporal_store(b22, vo + 5 * 64);
         if (a.fuse) {
-          const double4 u = a.xs[q];
           rb0 += b00 * u.x + b01 * u.y + b02 * u.z;
           rb1 += b01 * u.x + b11 * u.y + b12 * u.z;
           rb2 += b02 * u.x + b12 * u.y + b22 * u.z;
