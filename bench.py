@@ -21,6 +21,7 @@ hipGraph-captured and eager).
 from __future__ import annotations
 
 import argparse
+import hashlib
 import json
 import os
 import sys
@@ -72,7 +73,8 @@ def step_params(desc, step, precision, **kw):
 NETWORK = os.path.join(PKG, "networks", "impact_precond.txt")
 
 
-def gpu_run(steps, warmup, precision=0, copy_calibration=0, precond=None, network=None):
+def gpu_run(steps, warmup, precision=0, copy_calibration=0, precond=None, network=None,
+            matrix_free=0):
     desc = pds.describe(CONFIG)
     sc = pds.scene(CONFIG)
     n = desc["num_pd_particle"]
@@ -80,9 +82,12 @@ def gpu_run(steps, warmup, precision=0, copy_calibration=0, precond=None, networ
     extra = {}
     if precond is not None:
         extra["precond"] = precond
+    if matrix_free:
+        extra["matrix_free"] = 1
     if network is not None:
         ctx.learned_set(network)
-    log(f"[{'mixed' if precision else 'fp64'} precond {extra.get('precond', desc['precond'])}] "
+    log(f"[{'mixed' if precision else 'fp64'}{' matrix-free' if matrix_free else ''} "
+        f"precond {extra.get('precond', desc['precond'])}] "
         f"particles {n}, directed bonds {bst.total_bonds} "
         f"({bst.min_bonds}..{bst.max_bonds} per particle), bond build {bst.ms_build:.1f} ms")
     step = 0
@@ -94,7 +99,9 @@ def gpu_run(steps, warmup, precision=0, copy_calibration=0, precond=None, networ
     ctx.synchronize()
     # the autotuner picks the SpMV variant on the resident matrix (outside
     # the timed region); spmv_unroll = 0 then uses it
-    tune_ms, unroll = ctx.tune_spmv(10) if warmup > 0 else (np.zeros(3), 4)
+    # (the matrix-free operator has no stored-block variants to tune)
+    tune_ms, unroll = (ctx.tune_spmv(10) if warmup > 0 and not matrix_free
+                       else (np.zeros(3), 0))
     spmv_ms = 0.0
     spmv_n = 0
     resid_ms = 0.0
@@ -124,6 +131,9 @@ def gpu_run(steps, warmup, precision=0, copy_calibration=0, precond=None, networ
     elapsed = time.perf_counter() - t0
     spmv_bytes = ctx.spmv_bytes()
     live = st.live_bonds
+    # final state digest: equal digests = bit-identical trajectories
+    digest = hashlib.sha256(ctx.download("position").tobytes() +
+                            ctx.download("velocity").tobytes()).hexdigest()[:16]
     # after the timed region: the plain SpMV kernel y = A x alone
     plain_ms, plain_bytes = ctx.bench_spmv(20)
     for _ in range(copy_calibration):
@@ -138,7 +148,7 @@ def gpu_run(steps, warmup, precision=0, copy_calibration=0, precond=None, networ
                 spmv_bytes=spmv_bytes, phase=phase, build_ms=bst.ms_build,
                 plain_ms=plain_ms, plain_bytes=plain_bytes, resid_ms=resid_ms, resid_n=resid_n,
                 refinements=refinements, unroll=unroll, tune_ms=[float(t) for t in tune_ms],
-                fallbacks=fallbacks)
+                fallbacks=fallbacks, digest=digest)
 
 
 BATCH_CONFIG = os.path.join(PKG, "configs", "batch_64.txt")
@@ -238,6 +248,34 @@ def spmv_roofline(r, kernel, traffic):
     }
 
 
+# FP64 operations per live bond of the matrix-free SpMV (bond_geom 21,
+# bond_jac 27, the dt^2 scaling 12, the 3x3 product 15, the row sum 3;
+# a division or square root counts as one) and per row (diagonal product
+# 15, the <y, p> term 5)
+MF_FLOPS_BOND = 78.0
+MF_FLOPS_ROW = 20.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def mf_roofline(r):
+    avg_ms = r["spmv_ms"] / max(r["spmv_n"], 1)
+    flops = MF_FLOPS_BOND * r["live"] + MF_FLOPS_ROW * r["n"]
+    achieved = flops / (avg_ms * 1e-3) / 1e12
+    return {
+        "kernel": "k_cg_spmv<0> (matrix-free CG SpMV: blocks recomputed from positions)",
+        "bound": "fp64-valu",
+        "achieved": achieved,
+        "peak": FP64_PEAK_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": achieved / FP64_PEAK_TFLOPS,
+        "flops_per_launch": flops,
+        "hbm_bytes_per_launch": r["spmv_bytes"],
+        "hbm_gbs": r["spmv_bytes"] / (avg_ms * 1e-3) / 1e9,
+        "avg_launch_ms": avg_ms,
+        "launches": r["spmv_n"],
+    }
+
+
 def cpu_baseline(sample_steps=2):
     """The CPU oracle on the same workload: kd-tree bonds per body (not
     timed), then `sample_steps` steps timed with a wall clock."""
@@ -290,6 +328,8 @@ def main():
                     help="skip the batched-specimen report")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the learned-preconditioner report")
+    ap.add_argument("--no-mf", action="store_true",
+                    help="skip the matrix-free FP64 operator report")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="after the run, N rounds of device copies of known size")
@@ -357,6 +397,20 @@ def main():
             "spmv_unroll": m["unroll"],
             "spmv_tune_ms_2_4_8": m["tune_ms"],
             "position_error_vs_fp64": mixed_error(a.warmup + a.steps),
+        }
+    if not prec and not a.no_mf:
+        f = gpu_run(a.steps, a.warmup, 0, matrix_free=1)
+        f_rate = f["steps"] / f["elapsed"]
+        out["matrix_free"] = {
+            "value": f_rate,
+            "unit": "steps/s",
+            "speedup_vs_stored": f_rate / out["value"],
+            "ms_per_step": 1000.0 * f["elapsed"] / f["steps"],
+            "identical_to_stored": f["digest"] == r["digest"],
+            "final_state_digest": f["digest"],
+            "cg_iterations_mean": float(np.mean(f["iters"])),
+            "phase_ms_per_step": {k: v / f["steps"] for k, v in f["phase"].items()},
+            "roofline": mf_roofline(f),
         }
     if not a.no_learned and os.path.exists(NETWORK):
         net = np.loadtxt(NETWORK, dtype=np.float32)

@@ -126,6 +126,15 @@ typedef struct pd_step_params {
    * iterations of this context's last block-Jacobi solve (0: 1.5), then
    * continues with block Jacobi from its iterate */
   double learned_fallback;
+  /* Operator of the FP64 solve (not in the reference, which stores A).
+   * 0: the assembly stores every bond block (3x3 BSR) and the SpMV streams
+   * them.  1: matrix-free: the assembly stores the diagonal, b and the
+   * block-Jacobi inverse only, and every SpMV recomputes each live bond's
+   * block from the step's positions with the assembly's own arithmetic,
+   * so every product, and every result, is identical to 0.  Needs
+   * precision 0 and precond 0 or 1; pd_get_matrix then fails (no stored
+   * bond blocks) and pd_spmv applies the recomputed operator. */
+  int32_t matrix_free;
 } pd_step_params;
 
 typedef struct pd_step_stats {
@@ -273,14 +282,16 @@ int pd_bench_spmv(pd_ctx* ctx, int32_t iters, double* ms_per_launch,
 /* Algorithmic HBM bytes of one CG SpMV launch on the current bonds:
  * FP64, live blocks * (6*8 + 4) + N * (6*8 + 4 + 2*24); after an assembly
  * with precision 1 (float inner SpMV), live blocks * (6*4 + 4) +
- * N * (6*4 + 4 + 2*12). */
+ * N * (6*4 + 4 + 2*12); after a matrix-free assembly, live bonds * 4 +
+ * N * (6*8 + 4 + 24 + 24 + 4 + 2*24) (columns, diagonal, top, positions,
+ * reference positions, body, x and y: the blocks are recomputed). */
 int pd_spmv_bytes(pd_ctx* ctx, double* bytes);
 
 /* Autotuner (no counterpart in the reference): times `iters` launches of
  * each SpMV variant (2, 4 and 8 slots per trip) on the resident matrix of
  * the last assembly's precision, remembers the fastest for that precision
  * (used when spmv_unroll = 0) and returns the mean times in ms3[3] and the
- * choice in *best. */
+ * choice in *best.  Fails after a matrix-free assembly (no stored blocks). */
 int pd_tune_spmv(pd_ctx* ctx, int32_t iters, double* ms3, int32_t* best);
 
 /* Learned block preconditioner (no counterpart in the reference).  The

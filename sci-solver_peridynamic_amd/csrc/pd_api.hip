@@ -130,6 +130,11 @@ struct pd_ctx {
   hipGraphExec_t g_solve = nullptr, g_update = nullptr;
   std::string graph_key;
   int graph_k = 0;
+
+  // matrix-free operator: set by the last assembly
+  int mf = 0;                                // 1: bond blocks not stored
+  double asm_dt2 = 0.0;                      // its dt^2 scale
+  const pd_step_consts* asm_consts = nullptr;  // captured step: dt2 read here
 };
 
 struct pd_trainer {
@@ -258,6 +263,10 @@ int check_params(pd_ctx* c, const pd_step_params* prm) {
     c->unroll[k] = prm->spmv_unroll ? prm->spmv_unroll : (c->tuned[k] ? c->tuned[k] : 4);
   if (prm->precision != 0 && prm->precision != 1)
     return fail(c, PD_ERR_ARG, "precision must be 0 (FP64) or 1 (float inner solve + FP64 refinement)");
+  if (prm->matrix_free != 0 && prm->matrix_free != 1)
+    return fail(c, PD_ERR_ARG, "matrix_free must be 0 or 1");
+  if (prm->matrix_free && (prm->precision != 0 || prm->precond == 2))
+    return fail(c, PD_ERR_ARG, "matrix_free needs precision 0 and precond 0 or 1");
   if (prm->precision == 1 && (prm->inner_rtol < 0.0 || prm->max_refine < 0 || prm->inner_max_iter < 0))
     return fail(c, PD_ERR_ARG, "inner_rtol, max_refine and inner_max_iter must be >= 0");
   if (!c->bonds_ready) return fail(c, PD_ERR_STATE, "bonds not built (pd_build_bonds / pd_set_bonds)");
@@ -399,6 +408,13 @@ pd::SpmvView spmv_view(pd_ctx* c) {
   A.val = c->val;
   A.diag = c->diag;
   A.unroll = c->unroll[0];
+  A.mf = c->mf;
+  A.pos = c->x;
+  A.pos0 = c->x0;
+  A.body = c->body;
+  A.mat = c->dmat;
+  A.dt2 = c->asm_dt2;
+  A.consts = c->asm_consts;
   return A;
 }
 
@@ -550,6 +566,10 @@ int enqueue_assemble(pd_ctx* c, const pd_step_params* prm,
     CHECK_LAUNCH(c, "dirichlet values");
   }
   pd::AsmArgs a;
+  a.store_blocks = prm->matrix_free ? 0 : 1;
+  c->mf = prm->matrix_free ? 1 : 0;
+  c->asm_dt2 = dt2;
+  c->asm_consts = consts;
   a.fuse = fuse ? 1 : 0;
   a.pc = prm->precond;
   a.xs = c->sol;
@@ -984,8 +1004,9 @@ std::string graph_key(pd_ctx* c, const pd_step_params* prm, int K) {
   const MixedLimits lim = mixed_limits(prm);
   char buf[1024];
   snprintf(buf, sizeof(buf),
-           "%d %d %d %a %d %a %d %d %d %a %a %a %a %a %a %a %a %d %d %d %p %p %p %p %p %p %p",
-           prm->precision, prm->precond, prm->integrator, prm->rtol, prm->max_iter,
+           "%d %d %d %d %a %d %a %d %d %d %a %a %a %a %a %a %a %a %d %d %d %p %p %p %p %p %p %p",
+           prm->matrix_free, prm->precision, prm->precond, prm->integrator, prm->rtol,
+           prm->max_iter,
            lim.inner_rtol, lim.inner_max, lim.max_refine, prm->walls, prm->bmin[0],
            prm->bmin[1], prm->bmin[2], prm->bmax[0], prm->bmax[1], prm->bmax[2],
            prm->wall_radius, prm->restitution, c->unroll[prm->precision], c->nspec, K,
@@ -1468,6 +1489,8 @@ int pd_get_bonds(pd_ctx* c, int32_t* capacity, int32_t* lists, int32_t stride) {
 int pd_get_matrix(pd_ctx* c, double* diag9, double* vals9, int32_t stride) {
   if (!c) return fail(nullptr, PD_ERR_ARG, "null context");
   if (!c->bonds_ready) return fail(c, PD_ERR_STATE, "bonds not built");
+  if (c->mf)
+    return fail(c, PD_ERR_STATE, "the last assembly was matrix-free: no stored bond blocks");
   const int32_t n = c->n;
   if (n == 0) return PD_OK;
   if (!diag9 || !vals9) return fail(c, PD_ERR_ARG, "null output");
@@ -1632,6 +1655,10 @@ int pd_synchronize(pd_ctx* c) {
 
 int pd_spmv_bytes(pd_ctx* c, double* bytes) {
   if (!c || !bytes) return fail(c, PD_ERR_ARG, "null argument");
+  if (c->mf) {  // columns; diagonal, top, positions, reference positions, body, x, y
+    *bytes = (double)c->live_bonds * 4.0 + (double)c->n * (48.0 + 4.0 + 24.0 + 24.0 + 4.0 + 48.0);
+    return PD_OK;
+  }
   const double w = c->mat_prec == 1 ? 4.0 : 8.0;  // bytes per value and vector component
   *bytes = (double)c->live_bonds * (6.0 * w + 4.0) + (double)c->n * (6.0 * w + 4.0 + 2.0 * 3.0 * w);
   return PD_OK;
@@ -1774,6 +1801,7 @@ int pd_get_specimen_stats(pd_ctx* c, int32_t count, pd_specimen_stats* out) {
 int pd_tune_spmv(pd_ctx* c, int32_t iters, double* ms3, int32_t* best) {
   if (!c) return fail(nullptr, PD_ERR_ARG, "null context");
   if (c->mat_prec < 0) return fail(c, PD_ERR_STATE, "no matrix assembled");
+  if (c->mf) return fail(c, PD_ERR_STATE, "matrix-free operator: no stored-block SpMV variants");
   if (iters < 1) return fail(c, PD_ERR_ARG, "iters must be >= 1");
   HIPCHK(c, hipSetDevice(c->device));
   if (int e = ensure_events(c, 2)) return e;

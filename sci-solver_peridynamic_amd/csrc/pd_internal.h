@@ -53,6 +53,7 @@ struct AsmArgs {
   int32_t track_gap;
   const pd_step_consts* consts;  // non-null: dt, dt2, clock, gravity from here
   float2* bond_stats;            // non-null: (mean, max) stretch of the live bonds
+  int32_t store_blocks;          // FP64: 0 = matrix-free solve, bond blocks not stored
   // fuse = 1: the solve's first residual r = S(b - A xs), each row's bond
   // terms summed first and the diagonal term last, with the solver's start
   // and its tile partials in part (Dirichlet rows of xs must hold their
@@ -83,6 +84,16 @@ struct SpmvView {
   const double* val;
   const double* diag;
   int32_t unroll;  // slots per trip of the SpMV row loop: 2, 4 or 8
+  // mf = 1: matrix-free operator.  val is not read; each live bond's block
+  // is recomputed from the step's positions (pos, pos0, the row's material
+  // and dt2, from consts when non-null) exactly as the assembly forms it.
+  int32_t mf;
+  const double4* pos;
+  const double4* pos0;
+  const int32_t* body;
+  const pd_mat_dev* mat;
+  double dt2;
+  const pd_step_consts* consts;
 };
 
 // Independent specimens solved together: specimen k owns the 256-row

@@ -11,6 +11,42 @@
 
 namespace pd {
 
+// Geometry of bond (p, q), cg_solver.cuh:253-258: the unit vector e (the
+// difference divided by the float-rounded length max(|d|, SMALL)), the
+// current length dij and the reference length d0.
+__device__ __forceinline__ void bond_geom(const double4 pos_p, const double4 xq,
+                                          const double4 opos_p, const double4 oq, double& e0,
+                                          double& e1, double& e2, double& dij, double& d0) {
+  e0 = xq.x - pos_p.x;
+  e1 = xq.y - pos_p.y;
+  e2 = xq.z - pos_p.z;
+  dij = sqrt(e0 * e0 + e1 * e1 + e2 * e2);
+  const float dij_f = (float)dij;
+  const float small_f = (float)1.0e-10;
+  const double fd = (double)(dij_f > small_f ? dij_f : small_f);
+  e0 /= fd;
+  e1 /= fd;
+  e2 /= fd;
+  const double o0 = opos_p.x - oq.x, o1 = opos_p.y - oq.y, o2 = opos_p.z - oq.z;
+  d0 = sqrt(o0 * o0 + o1 * o1 + o2 * o2);
+}
+
+// The bond's 3x3 Jacobian F (six distinct entries), cg_solver.cuh:292-299:
+// (c / d0) [(1 - d0/dij) I + (d0/dij) e e^T]
+__device__ __forceinline__ void bond_jac(double e0, double e1, double e2, double dij, double d0,
+                                         double c_bond, double& F00, double& F01, double& F02,
+                                         double& F11, double& F12, double& F22) {
+  const double ratio = d0 / dij;
+  const double dg = 1.0 - ratio;
+  const double sc = c_bond / d0;
+  F00 = (dg + (e0 * e0) * ratio) * sc;
+  F01 = (0.0 + (e0 * e1) * ratio) * sc;
+  F02 = (0.0 + (e0 * e2) * ratio) * sc;
+  F11 = (dg + (e1 * e1) * ratio) * sc;
+  F12 = (0.0 + (e1 * e2) * ratio) * sc;
+  F22 = (dg + (e2 * e2) * ratio) * sc;
+}
+
 // ---------------------------------------------------------------------------
 // Fused assembly: core/cg_solver.cuh:200-339 (calculatePDForceDerivative)
 // + core/implicit_euler.cuh:19-86 / core/newmark_beta.cuh:19-86 (fill A, b)
@@ -84,17 +120,8 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
         if (a.fuse) us_n = a.xs[q_n];
       }
       if (bnd + 2 <= top) q_nn = a.col[(slot + 2) * 64 + lane];
-      double e0 = xq.x - pos_p.x, e1 = xq.y - pos_p.y, e2 = xq.z - pos_p.z;
-      const double dij = sqrt(e0 * e0 + e1 * e1 + e2 * e2);
-      // cg_solver.cuh:256: divide by (float)max(dij, SMALL)
-      const float dij_f = (float)dij;
-      const float small_f = (float)1.0e-10;
-      const double fd = (double)(dij_f > small_f ? dij_f : small_f);
-      e0 /= fd;
-      e1 /= fd;
-      e2 /= fd;
-      const double o0 = opos_p.x - oq.x, o1 = opos_p.y - oq.y, o2 = opos_p.z - oq.z;
-      const double d0 = sqrt(o0 * o0 + o1 * o1 + o2 * o2);
+      double e0, e1, e2, dij, d0;
+      bond_geom(pos_p, xq, opos_p, oq, e0, e1, e2, dij, d0);
       const double drdivd = dij / d0 - 1.0;
       // cg_solver.cuh:264: threshold (float)max(s_p, s_q)
       const float sq_f = (float)sqv;
@@ -131,15 +158,8 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
       f0 += kk * e0;
       f1 += kk * e1;
       f2 += kk * e2;
-      const double ratio = d0 / dij;
-      const double dg = 1.0 - ratio;
-      const double sc = mt.c_bond / d0;
-      const double F00 = (dg + (e0 * e0) * ratio) * sc;
-      const double F01 = (0.0 + (e0 * e1) * ratio) * sc;
-      const double F02 = (0.0 + (e0 * e2) * ratio) * sc;
-      const double F11 = (dg + (e1 * e1) * ratio) * sc;
-      const double F12 = (0.0 + (e1 * e2) * ratio) * sc;
-      const double F22 = (dg + (e2 * e2) * ratio) * sc;
+      double F00, F01, F02, F11, F12, F22;
+      bond_jac(e0, e1, e2, dij, d0, mt.c_bond, F00, F01, F02, F11, F12, F22);
       k00 += F00;
       k01 += F01;
       k02 += F02;
@@ -167,13 +187,15 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
         const double b00 = (F00 * -1.0) * a.dt2, b01 = (F01 * -1.0) * a.dt2,
                      b02 = (F02 * -1.0) * a.dt2, b11 = (F11 * -1.0) * a.dt2,
                      b12 = (F12 * -1.0) * a.dt2, b22 = (F22 * -1.0) * a.dt2;
-        double* vo = a.val + slot * 6 * 64 + lane;
-        __builtin_nontemporal_store(b00, vo + 0 * 64);
-        __builtin_nontemporal_store(b01, vo + 1 * 64);
-        __builtin_nontemporal_store(b02, vo + 2 * 64);
-        __builtin_nontemporal_store(b11, vo + 3 * 64);
-        __builtin_nontemporal_store(b12, vo + 4 * 64);
-        __builtin_nontemporal_store(b22, vo + 5 * 64);
+        if (a.store_blocks) {
+          double* vo = a.val + slot * 6 * 64 + lane;
+          __builtin_nontemporal_store(b00, vo + 0 * 64);
+          __builtin_nontemporal_store(b01, vo + 1 * 64);
+          __builtin_nontemporal_store(b02, vo + 2 * 64);
+          __builtin_nontemporal_store(b11, vo + 3 * 64);
+          __builtin_nontemporal_store(b12, vo + 4 * 64);
+          __builtin_nontemporal_store(b22, vo + 5 * 64);
+        }
         if (a.fuse) {
           rb0 += b00 * u.x + b01 * u.y + b02 * u.z;
           rb1 += b01 * u.x + b11 * u.y + b12 * u.z;
@@ -563,6 +585,75 @@ __device__ __forceinline__ void spmv_row(const SpmvView& A, const double4* __res
   }
 }
 
+// Row p of A x, matrix-free: each live bond's block is recomputed from the
+// step's positions with bond_geom / bond_jac and scaled as the assembly
+// scales it, so the blocks, and the products, equal the stored ones bit
+// for bit (same summation order as spmv_row).  The neighbour's positions
+// and vector entry for slot bnd + 1 are in flight while slot bnd is
+// computed.  Bound by FP64 arithmetic (about 170 FP64 instructions per
+// bond), not by HBM.
+template <bool DIAG_LAST = false>
+__device__ __forceinline__ void spmv_row_mf(const SpmvView& A, const double4* __restrict__ x,
+                                            int p, double& y0, double& y1, double& y2) {
+  const int s = p >> 6;
+  const int lane = p & 63;
+  const double* dg = A.diag + (int64_t)s * 6 * 64 + lane;
+  if constexpr (DIAG_LAST) {
+    y0 = 0.0;
+    y1 = 0.0;
+    y2 = 0.0;
+  } else {
+    blk6(dg[0], dg[64], dg[128], dg[192], dg[256], dg[320], x[p], y0, y1, y2);
+  }
+  const int top = A.top[p];
+  if (top >= 0) {
+    const int64_t base = A.slice_off[s];
+    const int* __restrict__ cp = A.col + base * 64 + lane;
+    const double4 pos_p = A.pos[p];
+    const double4 opos_p = A.pos0[p];
+    const double cb = A.mat[A.body[p]].c_bond;
+    const double dt2 = A.consts ? A.consts->dt2 : A.dt2;
+    int qn = ntl(cp);
+    double4 xn = A.pos[qn], on = A.pos0[qn], vn = x[qn];
+    for (int bnd = 0; bnd <= top; ++bnd) {
+      const double4 xq = xn, oq = on, vq = vn;
+      if (bnd + 1 <= top) {
+        qn = ntl(cp + (int64_t)(bnd + 1) * 64);
+        xn = A.pos[qn];
+        on = A.pos0[qn];
+        vn = x[qn];
+      }
+      double e0, e1, e2, dij, d0;
+      bond_geom(pos_p, xq, opos_p, oq, e0, e1, e2, dij, d0);
+      double F00, F01, F02, F11, F12, F22;
+      bond_jac(e0, e1, e2, dij, d0, cb, F00, F01, F02, F11, F12, F22);
+      double t0, t1, t2;
+      blk6((F00 * -1.0) * dt2, (F01 * -1.0) * dt2, (F02 * -1.0) * dt2, (F11 * -1.0) * dt2,
+           (F12 * -1.0) * dt2, (F22 * -1.0) * dt2, vq, t0, t1, t2);
+      y0 += t0;
+      y1 += t1;
+      y2 += t2;
+    }
+  }
+  if constexpr (DIAG_LAST) {
+    double d0, d1, d2;
+    blk6(dg[0], dg[64], dg[128], dg[192], dg[256], dg[320], x[p], d0, d1, d2);
+    y0 = d0 + y0;
+    y1 = d1 + y1;
+    y2 = d2 + y2;
+  }
+}
+
+// U slots per trip of the stored-block row; U = 0: the matrix-free row
+template <int U, bool DIAG_LAST = false>
+__device__ __forceinline__ void row_of(const SpmvView& A, const double4* __restrict__ x, int p,
+                                       double& y0, double& y1, double& y2) {
+  if constexpr (U == 0)
+    spmv_row_mf<DIAG_LAST>(A, x, p, y0, y1, y2);
+  else
+    spmv_row<U, DIAG_LAST>(A, x, p, y0, y1, y2);
+}
+
 template <int U>
 __global__ __launch_bounds__(PD_BLOCK) void k_spmv_plain(SpmvView A,
                                                         const double4* __restrict__ x,
@@ -571,13 +662,17 @@ __global__ __launch_bounds__(PD_BLOCK) void k_spmv_plain(SpmvView A,
   const int p = tile * PD_BLOCK + threadIdx.x;
   if (p >= A.n) return;
   double y0, y1, y2;
-  spmv_row<U>(A, x, p, y0, y1, y2);
+  row_of<U>(A, x, p, y0, y1, y2);
   y[p] = make_double4(y0, y1, y2, 0.0);
 }
 
 void launch_spmv_plain(const SpmvView& A, const double4* x, double4* y, hipStream_t s) {
   const int nb = (A.n + PD_BLOCK - 1) / PD_BLOCK;
   if (!nb) return;
+  if (A.mf) {
+    hipLaunchKernelGGL(k_spmv_plain<0>, dim3(nb), dim3(PD_BLOCK), 0, s, A, x, y);
+    return;
+  }
   switch (A.unroll) {
     case 2: hipLaunchKernelGGL(k_spmv_plain<2>, dim3(nb), dim3(PD_BLOCK), 0, s, A, x, y); break;
     case 8: hipLaunchKernelGGL(k_spmv_plain<8>, dim3(nb), dim3(PD_BLOCK), 0, s, A, x, y); break;
@@ -651,7 +746,7 @@ __global__ __launch_bounds__(PD_BLOCK) void k_cg_init(CgArgs a) {
   double t_rz = 0.0, t_rr = 0.0, t_bb = 0.0;
   if (p < a.A.n) {
     double y0, y1, y2;
-    spmv_row<U, true>(a.A, a.x, p, y0, y1, y2);
+    row_of<U, true>(a.A, a.x, p, y0, y1, y2);
     const double4 bp = a.b[p];
     const bool fr = a.bc[p] == 0;
     const double r0 = fr ? bp.x - y0 : 0.0;
@@ -715,7 +810,7 @@ __global__ __launch_bounds__(PD_BLOCK) void k_cg_spmv(CgArgs a) {
   double t = 0.0;
   if (p < a.A.n) {
     double y0, y1, y2;
-    spmv_row<U>(a.A, a.p, p, y0, y1, y2);
+    row_of<U>(a.A, a.p, p, y0, y1, y2);
     a.y[p] = make_double4(y0, y1, y2, 0.0);
     const double4 pp = a.p[p];
     t = term3(y0, y1, y2, pp.x, pp.y, pp.z);
@@ -826,10 +921,14 @@ static inline int tiles_of(int n) { return (n + PD_BLOCK - 1) / PD_BLOCK; }
 void launch_cg_init(const CgArgs& a, hipStream_t s) {
   if (!a.A.n) return;
   const int g = tiles_of(a.A.n);
-  switch (a.A.unroll) {
-    case 2: hipLaunchKernelGGL(k_cg_init<2>, dim3(g), dim3(PD_BLOCK), 0, s, a); break;
-    case 8: hipLaunchKernelGGL(k_cg_init<8>, dim3(g), dim3(PD_BLOCK), 0, s, a); break;
-    default: hipLaunchKernelGGL(k_cg_init<4>, dim3(g), dim3(PD_BLOCK), 0, s, a);
+  if (a.A.mf) {
+    hipLaunchKernelGGL(k_cg_init<0>, dim3(g), dim3(PD_BLOCK), 0, s, a);
+  } else {
+    switch (a.A.unroll) {
+      case 2: hipLaunchKernelGGL(k_cg_init<2>, dim3(g), dim3(PD_BLOCK), 0, s, a); break;
+      case 8: hipLaunchKernelGGL(k_cg_init<8>, dim3(g), dim3(PD_BLOCK), 0, s, a); break;
+      default: hipLaunchKernelGGL(k_cg_init<4>, dim3(g), dim3(PD_BLOCK), 0, s, a);
+    }
   }
   hipLaunchKernelGGL(k_cg_init_fold, dim3(a.batch.nspec), dim3(PD_BLOCK), 0, s, a, g);
 }
@@ -841,6 +940,10 @@ void launch_cg_init_fold(const CgArgs& a, hipStream_t s) {
 void launch_cg_spmv(const CgArgs& a, hipStream_t s) {
   if (!a.A.n) return;
   const int g = tiles_of(a.A.n);
+  if (a.A.mf) {
+    hipLaunchKernelGGL(k_cg_spmv<0>, dim3(g), dim3(PD_BLOCK), 0, s, a);
+    return;
+  }
   switch (a.A.unroll) {
     case 2: hipLaunchKernelGGL(k_cg_spmv<2>, dim3(g), dim3(PD_BLOCK), 0, s, a); break;
     case 8: hipLaunchKernelGGL(k_cg_spmv<8>, dim3(g), dim3(PD_BLOCK), 0, s, a); break;

@@ -61,7 +61,7 @@ class StepParams(C.Structure):
         ("precision", C.c_int32), ("max_refine", C.c_int32),
         ("inner_rtol", C.c_double), ("inner_max_iter", C.c_int32),
         ("contact_mode", C.c_int32), ("spmv_unroll", C.c_int32), ("graph", C.c_int32),
-        ("learned_fallback", C.c_double),
+        ("learned_fallback", C.c_double), ("matrix_free", C.c_int32),
     ]
 
 
@@ -99,7 +99,7 @@ DEFAULT_PARAMS = dict(dt=1e-4, clock_scale=1.0, gravity=9.8, integrator=0, preco
                       contact_radius=0.0, contact_stiffness=0.0, contact_volume=1.0,
                       contact_cell=0.0, track_gap=0, timing=0, precision=0, max_refine=20,
                       inner_rtol=1e-5, contact_mode=0, spmv_unroll=0, graph=0,
-                      learned_fallback=0.0)
+                      learned_fallback=0.0, matrix_free=0)
 
 
 def make_params(**kw) -> StepParams:
