@@ -132,7 +132,7 @@ struct pd_ctx {
   int graph_k = 0;
 
   // matrix-free operator: set by the last assembly
-  int mf = 0;                                // 1: bond blocks not stored
+  int mf = 0;  // 1: the last assembly stored no bond blocks (matrix-free solve)
   double asm_dt2 = 0.0;                      // its dt^2 scale
   const pd_step_consts* asm_consts = nullptr;  // captured step: dt2 read here
 };
@@ -264,7 +264,7 @@ int check_params(pd_ctx* c, const pd_step_params* prm) {
   if (prm->precision != 0 && prm->precision != 1)
     return fail(c, PD_ERR_ARG, "precision must be 0 (FP64) or 1 (float inner solve + FP64 refinement)");
   if (prm->matrix_free != 0 && prm->matrix_free != 1)
-    return fail(c, PD_ERR_ARG, "matrix_free must be 0 or 1");
+    return fail(c, PD_ERR_ARG, "matrix_free must be 0 (stored blocks) or 1 (recomputed blocks)");
   if (prm->matrix_free && (prm->precision != 0 || prm->precond == 2))
     return fail(c, PD_ERR_ARG, "matrix_free needs precision 0 and precond 0 or 1");
   if (prm->precision == 1 && (prm->inner_rtol < 0.0 || prm->max_refine < 0 || prm->inner_max_iter < 0))
@@ -1490,7 +1490,7 @@ int pd_get_matrix(pd_ctx* c, double* diag9, double* vals9, int32_t stride) {
   if (!c) return fail(nullptr, PD_ERR_ARG, "null context");
   if (!c->bonds_ready) return fail(c, PD_ERR_STATE, "bonds not built");
   if (c->mf)
-    return fail(c, PD_ERR_STATE, "the last assembly was matrix-free: no stored bond blocks");
+    return fail(c, PD_ERR_STATE, "the last assembly was matrix-free: bond blocks not stored");
   const int32_t n = c->n;
   if (n == 0) return PD_OK;
   if (!diag9 || !vals9) return fail(c, PD_ERR_ARG, "null output");

@@ -53,7 +53,7 @@ struct AsmArgs {
   int32_t track_gap;
   const pd_step_consts* consts;  // non-null: dt, dt2, clock, gravity from here
   float2* bond_stats;            // non-null: (mean, max) stretch of the live bonds
-  int32_t store_blocks;          // FP64: 0 = matrix-free solve, bond blocks not stored
+  int32_t store_blocks;  // FP64: 0 = matrix-free solve, bond blocks not written
   // fuse = 1: the solve's first residual r = S(b - A xs), each row's bond
   // terms summed first and the diagonal term last, with the solver's start
   // and its tile partials in part (Dirichlet rows of xs must hold their

@@ -592,7 +592,7 @@ __device__ __forceinline__ void spmv_row(const SpmvView& A, const double4* __res
 // and vector entry for slot bnd + 1 are in flight while slot bnd is
 // computed.  Bound by FP64 arithmetic (about 170 FP64 instructions per
 // bond), not by HBM.
-template <bool DIAG_LAST = false>
+template <int V, bool DIAG_LAST = false>
 __device__ __forceinline__ void spmv_row_mf(const SpmvView& A, const double4* __restrict__ x,
                                             int p, double& y0, double& y1, double& y2) {
   const int s = p >> 6;
@@ -613,23 +613,46 @@ __device__ __forceinline__ void spmv_row_mf(const SpmvView& A, const double4* __
     const double4 opos_p = A.pos0[p];
     const double cb = A.mat[A.body[p]].c_bond;
     const double dt2 = A.consts ? A.consts->dt2 : A.dt2;
-    int qn = ntl(cp);
-    double4 xn = A.pos[qn], on = A.pos0[qn], vn = x[qn];
-    for (int bnd = 0; bnd <= top; ++bnd) {
-      const double4 xq = xn, oq = on, vq = vn;
-      if (bnd + 1 <= top) {
-        qn = ntl(cp + (int64_t)(bnd + 1) * 64);
-        xn = A.pos[qn];
-        on = A.pos0[qn];
-        vn = x[qn];
+    // V slots per trip: their blocks are independent chains of FP64
+    // arithmetic (divisions, square roots) the scheduler interleaves
+    int bnd = 0;
+    for (; bnd + V - 1 <= top; bnd += V) {
+      int q[V];
+#pragma unroll
+      for (int u = 0; u < V; ++u) q[u] = ntl(cp + (int64_t)(bnd + u) * 64);
+      double4 xq[V], oq[V], vq[V];
+#pragma unroll
+      for (int u = 0; u < V; ++u) {
+        xq[u] = A.pos[q[u]];
+        oq[u] = A.pos0[q[u]];
+        vq[u] = x[q[u]];
       }
+      double t[V][3];
+#pragma unroll
+      for (int u = 0; u < V; ++u) {
+        double e0, e1, e2, dij, d0;
+        bond_geom(pos_p, xq[u], opos_p, oq[u], e0, e1, e2, dij, d0);
+        double F00, F01, F02, F11, F12, F22;
+        bond_jac(e0, e1, e2, dij, d0, cb, F00, F01, F02, F11, F12, F22);
+        blk6((F00 * -1.0) * dt2, (F01 * -1.0) * dt2, (F02 * -1.0) * dt2, (F11 * -1.0) * dt2,
+             (F12 * -1.0) * dt2, (F22 * -1.0) * dt2, vq[u], t[u][0], t[u][1], t[u][2]);
+      }
+#pragma unroll
+      for (int u = 0; u < V; ++u) {
+        y0 += t[u][0];
+        y1 += t[u][1];
+        y2 += t[u][2];
+      }
+    }
+    for (; bnd <= top; ++bnd) {
+      const int q0 = ntl(cp + (int64_t)bnd * 64);
       double e0, e1, e2, dij, d0;
-      bond_geom(pos_p, xq, opos_p, oq, e0, e1, e2, dij, d0);
+      bond_geom(pos_p, A.pos[q0], opos_p, A.pos0[q0], e0, e1, e2, dij, d0);
       double F00, F01, F02, F11, F12, F22;
       bond_jac(e0, e1, e2, dij, d0, cb, F00, F01, F02, F11, F12, F22);
       double t0, t1, t2;
       blk6((F00 * -1.0) * dt2, (F01 * -1.0) * dt2, (F02 * -1.0) * dt2, (F11 * -1.0) * dt2,
-           (F12 * -1.0) * dt2, (F22 * -1.0) * dt2, vq, t0, t1, t2);
+           (F12 * -1.0) * dt2, (F22 * -1.0) * dt2, x[q0], t0, t1, t2);
       y0 += t0;
       y1 += t1;
       y2 += t2;
@@ -644,12 +667,13 @@ __device__ __forceinline__ void spmv_row_mf(const SpmvView& A, const double4* __
   }
 }
 
-// U slots per trip of the stored-block row; U = 0: the matrix-free row
+// U > 0: every block stored, U slots per trip; U = 0: every block
+// recomputed, three slots per trip
 template <int U, bool DIAG_LAST = false>
 __device__ __forceinline__ void row_of(const SpmvView& A, const double4* __restrict__ x, int p,
                                        double& y0, double& y1, double& y2) {
   if constexpr (U == 0)
-    spmv_row_mf<DIAG_LAST>(A, x, p, y0, y1, y2);
+    spmv_row_mf<3, DIAG_LAST>(A, x, p, y0, y1, y2);
   else
     spmv_row<U, DIAG_LAST>(A, x, p, y0, y1, y2);
 }

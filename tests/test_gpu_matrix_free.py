@@ -24,8 +24,9 @@ def say(*a):
     sys.stdout.flush()
 
 
-def test_mf_spmv_equals_stored_and_oracle():
-    say("case mf spmv")
+@pytest.mark.parametrize("mode", [1])
+def test_mf_spmv_equals_stored_and_oracle(mode):
+    say("case mf spmv mode", mode)
     sys_, ctx = deformed_pair()
     sys_.assemble(orc.make_params())
     rng = np.random.default_rng(11)
@@ -33,22 +34,23 @@ def test_mf_spmv_equals_stored_and_oracle():
     x[:, :3] = rng.normal(size=(sys_.n, 3))
     ctx.assemble(pds.make_params())
     y_stored = ctx.spmv(x)
-    ctx.assemble(pds.make_params(matrix_free=1))
+    ctx.assemble(pds.make_params(matrix_free=mode))
     y_mf = ctx.spmv(x)
     assert np.array_equal(y_mf[:, :3], y_stored[:, :3])
     assert np.array_equal(y_mf[:, :3], sys_.spmv(x)[:, :3])
     ctx.close()
 
 
+@pytest.mark.parametrize("mode", [1])
 @pytest.mark.parametrize("precond", [0, 1])
-def test_mf_cg_bit_exact(precond):
-    say("case mf cg precond", precond)
+def test_mf_cg_bit_exact(precond, mode):
+    say("case mf cg precond", precond, "mode", mode)
     sys_, ctx = deformed_pair(s_fixed=10.0)
     kw = dict(dt=2e-4, precond=precond, rtol=1e-10)
     sys_.assemble(orc.make_params(**kw))
-    ctx.assemble(pds.make_params(matrix_free=1, **kw))
+    ctx.assemble(pds.make_params(matrix_free=mode, **kw))
     st_o = sys_.cg(orc.make_params(**kw))
-    st_g = ctx.solve(pds.make_params(matrix_free=1, **kw))
+    st_g = ctx.solve(pds.make_params(matrix_free=mode, **kw))
     assert st_o.converged and st_g.converged
     assert st_g.cg_iterations == st_o.cg_iterations
     assert st_g.residual_norm == st_o.residual_norm
@@ -56,11 +58,11 @@ def test_mf_cg_bit_exact(precond):
     ctx.close()
 
 
-@pytest.mark.parametrize("integrator", [0, 1])
-def test_mf_steps_equal_stored_and_oracle(integrator):
+@pytest.mark.parametrize("mode,integrator", [(1, 0), (1, 1)])
+def test_mf_steps_equal_stored_and_oracle(mode, integrator):
     """Tension run with bond breaking: the matrix-free context, the stored
     context and the oracle side by side."""
-    say("case mf steps integrator", integrator)
+    say("case mf steps mode", mode, "integrator", integrator)
     x4, bc, vbc, v4 = config0_state(8, 8, 10)
     mats = [material(0.01)]
     sys_, mf = make_pair(x4, mats, v4=v4, bc=bc, vbc=vbc)
@@ -71,7 +73,7 @@ def test_mf_steps_equal_stored_and_oracle(integrator):
                   integrator=integrator)
         so = sys_.step(orc.make_params(**kw))
         ss = stored.step(pds.make_params(**kw))
-        sm = mf.step(pds.make_params(matrix_free=1, **kw))
+        sm = mf.step(pds.make_params(matrix_free=mode, **kw))
         assert (sm.cg_iterations, sm.broken_bonds, sm.residual_norm) == \
             (ss.cg_iterations, ss.broken_bonds, ss.residual_norm), k
         assert sm.cg_iterations == so.cg_iterations and sm.broken_bonds == so.broken_bonds, k
@@ -101,17 +103,18 @@ def batch_context():
     return desc, ex, ctx
 
 
-def test_mf_batch_graph_equals_stored_eager():
+@pytest.mark.parametrize("mode", [1])
+def test_mf_batch_graph_equals_stored_eager(mode):
     """Batched specimens: matrix-free steps replayed from hipGraphs against
     stored-block eager steps."""
-    say("case mf batch graph")
+    say("case mf batch graph mode", mode)
     desc, ex, stored = batch_context()
     _, _, mf = batch_context()
     nspec = len(ex["first"])
     for k in range(1, 31):
         kw = dict(pds.step_kwargs(desc, k), precision=0)
         ss = stored.step(pds.make_params(**kw))
-        sm = mf.step(pds.make_params(matrix_free=1, graph=1, **kw))
+        sm = mf.step(pds.make_params(matrix_free=mode, graph=1, **kw))
         assert (sm.cg_iterations, sm.broken_bonds, sm.live_bonds) == \
             (ss.cg_iterations, ss.broken_bonds, ss.live_bonds), k
         assert stored.specimen_stats(nspec) == mf.specimen_stats(nspec), k
@@ -123,7 +126,7 @@ def test_mf_batch_graph_equals_stored_eager():
 def test_mf_argument_and_state_errors():
     say("case mf errors")
     sys_, ctx = deformed_pair()
-    for bad in (dict(matrix_free=1, precision=1), dict(matrix_free=2)):
+    for bad in (dict(matrix_free=1, precision=1), dict(matrix_free=2), dict(matrix_free=-1)):
         with pytest.raises(pds.PDError) as e:
             ctx.assemble(pds.make_params(**bad))
         assert e.value.code == pds.PD_ERR_ARG
