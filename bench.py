@@ -248,11 +248,11 @@ def spmv_roofline(r, kernel, traffic):
     }
 
 
-# FP64 operations per live bond of the matrix-free SpMV (bond_geom 21,
+# FP64 operations per live bond of the matrix-free SpMV (bond_dir 12,
 # bond_jac 27, the dt^2 scaling 12, the 3x3 product 15, the row sum 3;
 # a division or square root counts as one) and per row (diagonal product
 # 15, the <y, p> term 5)
-MF_FLOPS_BOND = 78.0
+MF_FLOPS_BOND = 69.0
 MF_FLOPS_ROW = 20.0
 FP64_PEAK_TFLOPS = 78.6
 

@@ -282,9 +282,9 @@ int pd_bench_spmv(pd_ctx* ctx, int32_t iters, double* ms_per_launch,
 /* Algorithmic HBM bytes of one CG SpMV launch on the current bonds:
  * FP64, live blocks * (6*8 + 4) + N * (6*8 + 4 + 2*24); after an assembly
  * with precision 1 (float inner SpMV), live blocks * (6*4 + 4) +
- * N * (6*4 + 4 + 2*12); after a matrix-free assembly, live bonds * 4 +
- * N * (6*8 + 4 + 24 + 24 + 4 + 2*24) (columns, diagonal, top, positions,
- * reference positions, body, x and y: the blocks are recomputed). */
+ * N * (6*4 + 4 + 2*12); after a matrix-free assembly, live bonds * (4 + 8)
+ * + N * (6*8 + 4 + 24 + 4 + 2*24) (columns and reference lengths;
+ * diagonal, top, positions, body, x and y: the blocks are recomputed). */
 int pd_spmv_bytes(pd_ctx* ctx, double* bytes);
 
 /* Autotuner (no counterpart in the reference): times `iters` launches of

@@ -133,6 +133,10 @@ struct pd_ctx {
 
   // matrix-free operator: set by the last assembly
   int mf = 0;  // 1: the last assembly stored no bond blocks (matrix-free solve)
+  // reference bond lengths |x0_p - x0_q| per slot, moved with the columns by
+  // the swap-removes; recomputed when bonds or reference positions change
+  double* d0 = nullptr;
+  bool d0_valid = false;
   double asm_dt2 = 0.0;                      // its dt^2 scale
   const pd_step_consts* asm_consts = nullptr;  // captured step: dt2 read here
 };
@@ -383,8 +387,11 @@ int ensure_slots(pd_ctx* c, int64_t slots) {
   if (slots <= c->slots_cap) return PD_OK;
   dfree(c->col);
   dfree(c->val);
+  dfree(c->d0);
+  c->d0_valid = false;
   if (int e = dalloc(c, &c->col, (size_t)slots * 64)) return e;
   if (int e = dalloc(c, &c->val, (size_t)slots * 64 * 6)) return e;
+  if (int e = dalloc(c, &c->d0, (size_t)slots * 64)) return e;
   c->slots_cap = slots;
   return PD_OK;
 }
@@ -410,11 +417,11 @@ pd::SpmvView spmv_view(pd_ctx* c) {
   A.unroll = c->unroll[0];
   A.mf = c->mf;
   A.pos = c->x;
-  A.pos0 = c->x0;
   A.body = c->body;
   A.mat = c->dmat;
   A.dt2 = c->asm_dt2;
   A.consts = c->asm_consts;
+  A.d0 = c->d0;
   return A;
 }
 
@@ -565,7 +572,13 @@ int enqueue_assemble(pd_ctx* c, const pd_step_params* prm,
                              c->sol, c->stream);
     CHECK_LAUNCH(c, "dirichlet values");
   }
+  if (prm->matrix_free && !c->d0_valid) {
+    pd::launch_bond_d0(c->n, c->slice_off, c->cap, c->col, c->x0, c->d0, c->stream);
+    CHECK_LAUNCH(c, "reference bond lengths");
+    c->d0_valid = true;
+  }
   pd::AsmArgs a;
+  a.d0 = c->d0_valid ? c->d0 : nullptr;
   a.store_blocks = prm->matrix_free ? 0 : 1;
   c->mf = prm->matrix_free ? 1 : 0;
   c->asm_dt2 = dt2;
@@ -1036,6 +1049,11 @@ int capture(pd_ctx* c, hipGraphExec_t* out, const std::function<int()>& body) {
 int capture_step(pd_ctx* c, const pd_step_params* prm, int K) {
   destroy_graphs(c);
   const int ns = c->nspec;
+  if (prm->matrix_free && !c->d0_valid) {  // once, outside the captured step
+    pd::launch_bond_d0(c->n, c->slice_off, c->cap, c->col, c->x0, c->d0, c->stream);
+    CHECK_LAUNCH(c, "reference bond lengths");
+    c->d0_valid = true;
+  }
   int e = capture(c, &c->g_solve, [&]() -> int {
     const bool fuse = fused_resid(prm);
     if (int e2 = enqueue_assemble(c, prm, c->consts, fuse)) return e2;
@@ -1278,7 +1296,7 @@ int pd_ctx_destroy(pd_ctx* c) {
   dfree(c->x); dfree(c->v); dfree(c->x0); dfree(c->sol); dfree(c->f); dfree(c->b);
   dfree(c->fc); dfree(c->vbc); dfree(c->s); dfree(c->s_new); dfree(c->s00);
   dfree(c->act); dfree(c->body); dfree(c->bc); dfree(c->top); dfree(c->cap);
-  dfree(c->slice_off); dfree(c->col); dfree(c->val); dfree(c->diag); dfree(c->minv);
+  dfree(c->slice_off); dfree(c->col); dfree(c->val); dfree(c->d0); dfree(c->diag); dfree(c->minv);
   dfree(c->r); dfree(c->z); dfree(c->p); dfree(c->y); dfree(c->part); dfree(c->st);
   dfree(c->counters); dfree(c->dmat); dfree(c->dhorizon);
   dfree(c->gap_val); dfree(c->gap_q); dfree(c->gpart_v); dfree(c->gpart_p); dfree(c->gpart_q);
@@ -1320,6 +1338,7 @@ int pd_upload(pd_ctx* c, int32_t field, const void* src, size_t bytes) {
       if (t[i] < -1 || t[i] >= c->h_cap[i])
         return fail(c, PD_ERR_ARG, "bond top outside [-1, capacity)");
   }
+  if (field == PD_FIELD_ORIGINAL_POSITION) c->d0_valid = false;
   if (field == PD_FIELD_BODY) {
     const int32_t* bt = (const int32_t*)src;
     for (int32_t i = 0; i < c->n; ++i)
@@ -1407,6 +1426,7 @@ int pd_build_bonds(pd_ctx* c, pd_bond_stats* stats) {
   c->mat_prec = -1;
   c->last_iters = 0;
   c->h_cap = counts;
+  c->d0_valid = false;
   if (stats) {
     stats->total_bonds = total;
     stats->min_bonds = mn;
@@ -1460,6 +1480,7 @@ int pd_set_bonds(pd_ctx* c, const int32_t* counts, const int32_t* lists, int32_t
   c->mat_prec = -1;
   c->last_iters = 0;
   c->h_cap = cap;
+  c->d0_valid = false;
   return PD_OK;
 }
 
@@ -1655,8 +1676,8 @@ int pd_synchronize(pd_ctx* c) {
 
 int pd_spmv_bytes(pd_ctx* c, double* bytes) {
   if (!c || !bytes) return fail(c, PD_ERR_ARG, "null argument");
-  if (c->mf) {  // columns; diagonal, top, positions, reference positions, body, x, y
-    *bytes = (double)c->live_bonds * 4.0 + (double)c->n * (48.0 + 4.0 + 24.0 + 24.0 + 4.0 + 48.0);
+  if (c->mf) {  // columns, reference lengths; diagonal, top, positions, body, x, y
+    *bytes = (double)c->live_bonds * 12.0 + (double)c->n * (48.0 + 4.0 + 24.0 + 4.0 + 48.0);
     return PD_OK;
   }
   const double w = c->mat_prec == 1 ? 4.0 : 8.0;  // bytes per value and vector component

@@ -54,6 +54,7 @@ struct AsmArgs {
   const pd_step_consts* consts;  // non-null: dt, dt2, clock, gravity from here
   float2* bond_stats;            // non-null: (mean, max) stretch of the live bonds
   int32_t store_blocks;  // FP64: 0 = matrix-free solve, bond blocks not written
+  double* d0;            // non-null: per-slot reference lengths, swapped with col
   // fuse = 1: the solve's first residual r = S(b - A xs), each row's bond
   // terms summed first and the diagonal term last, with the solver's start
   // and its tile partials in part (Dirichlet rows of xs must hold their
@@ -85,15 +86,16 @@ struct SpmvView {
   const double* diag;
   int32_t unroll;  // slots per trip of the SpMV row loop: 2, 4 or 8
   // mf = 1: matrix-free operator.  val is not read; each live bond's block
-  // is recomputed from the step's positions (pos, pos0, the row's material
-  // and dt2, from consts when non-null) exactly as the assembly forms it.
+  // is recomputed from the step's positions (pos), the slot's reference
+  // length (d0), the row's material and dt2 (from consts when non-null)
+  // exactly as the assembly forms it.
   int32_t mf;
   const double4* pos;
-  const double4* pos0;
   const int32_t* body;
   const pd_mat_dev* mat;
   double dt2;
   const pd_step_consts* consts;
+  const double* d0;  // mf: reference bond length per slot
 };
 
 // Independent specimens solved together: specimen k owns the 256-row
@@ -293,6 +295,9 @@ void launch_update(int32_t n, double4* x, double4* v, const double4* sol,
                    const double* lo, const double* hi, double restitution,
                    const pd_step_consts* consts, hipStream_t s);
 void launch_copy(double* dst, const double* src, int32_t n, hipStream_t s);
+// d0[slot] = |x0_p - x0_col[slot]| for every slot below the row's capacity
+void launch_bond_d0(int32_t n, const int64_t* slice_off, const int32_t* cap, const int32_t* col,
+                    const double4* x0, double* d0, hipStream_t s);
 // sol of the Dirichlet rows: vbc (implicit Euler) or dt vbc (Newmark-beta),
 // the values the assembly writes; dt from consts when non-null
 void launch_set_dirichlet(int32_t n, const int32_t* bc, const int32_t* act, const double4* vbc,

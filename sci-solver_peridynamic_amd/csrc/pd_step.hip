@@ -11,12 +11,11 @@
 
 namespace pd {
 
-// Geometry of bond (p, q), cg_solver.cuh:253-258: the unit vector e (the
-// difference divided by the float-rounded length max(|d|, SMALL)), the
-// current length dij and the reference length d0.
-__device__ __forceinline__ void bond_geom(const double4 pos_p, const double4 xq,
-                                          const double4 opos_p, const double4 oq, double& e0,
-                                          double& e1, double& e2, double& dij, double& d0) {
+// Current geometry of bond (p, q), cg_solver.cuh:253-256: the unit vector e
+// (the difference divided by the float-rounded length max(|d|, SMALL)) and
+// the current length dij.
+__device__ __forceinline__ void bond_dir(const double4 pos_p, const double4 xq, double& e0,
+                                         double& e1, double& e2, double& dij) {
   e0 = xq.x - pos_p.x;
   e1 = xq.y - pos_p.y;
   e2 = xq.z - pos_p.z;
@@ -27,8 +26,19 @@ __device__ __forceinline__ void bond_geom(const double4 pos_p, const double4 xq,
   e0 /= fd;
   e1 /= fd;
   e2 /= fd;
+}
+
+// Reference length of bond (p, q), cg_solver.cuh:258
+__device__ __forceinline__ double ref_len(const double4 opos_p, const double4 oq) {
   const double o0 = opos_p.x - oq.x, o1 = opos_p.y - oq.y, o2 = opos_p.z - oq.z;
-  d0 = sqrt(o0 * o0 + o1 * o1 + o2 * o2);
+  return sqrt(o0 * o0 + o1 * o1 + o2 * o2);
+}
+
+__device__ __forceinline__ void bond_geom(const double4 pos_p, const double4 xq,
+                                          const double4 opos_p, const double4 oq, double& e0,
+                                          double& e1, double& e2, double& dij, double& d0) {
+  bond_dir(pos_p, xq, e0, e1, e2, dij);
+  d0 = ref_len(opos_p, oq);
 }
 
 // The bond's 3x3 Jacobian F (six distinct entries), cg_solver.cuh:292-299:
@@ -139,6 +149,11 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
         const int last = a.col[last_slot];
         a.col[slot * 64 + lane] = last;
         a.col[last_slot] = q;
+        if (a.d0) {  // the reference lengths move with their columns
+          const double dl = a.d0[last_slot];
+          a.d0[last_slot] = a.d0[slot * 64 + lane];
+          a.d0[slot * 64 + lane] = dl;
+        }
         --top;
         // slot bnd again, now holding `last`: restart the pipeline there
         q_n = last;
@@ -586,12 +601,10 @@ __device__ __forceinline__ void spmv_row(const SpmvView& A, const double4* __res
 }
 
 // Row p of A x, matrix-free: each live bond's block is recomputed from the
-// step's positions with bond_geom / bond_jac and scaled as the assembly
-// scales it, so the blocks, and the products, equal the stored ones bit
-// for bit (same summation order as spmv_row).  The neighbour's positions
-// and vector entry for slot bnd + 1 are in flight while slot bnd is
-// computed.  Bound by FP64 arithmetic (about 170 FP64 instructions per
-// bond), not by HBM.
+// step's positions (bond_dir), the slot's stored reference length and
+// bond_jac, and scaled as the assembly scales it, so the blocks, and the
+// products, equal the stored ones bit for bit (same summation order as
+// spmv_row).  Bound by FP64 arithmetic and its latency, not by HBM.
 template <int V, bool DIAG_LAST = false>
 __device__ __forceinline__ void spmv_row_mf(const SpmvView& A, const double4* __restrict__ x,
                                             int p, double& y0, double& y1, double& y2) {
@@ -610,30 +623,33 @@ __device__ __forceinline__ void spmv_row_mf(const SpmvView& A, const double4* __
     const int64_t base = A.slice_off[s];
     const int* __restrict__ cp = A.col + base * 64 + lane;
     const double4 pos_p = A.pos[p];
-    const double4 opos_p = A.pos0[p];
     const double cb = A.mat[A.body[p]].c_bond;
     const double dt2 = A.consts ? A.consts->dt2 : A.dt2;
+    const double* __restrict__ lp = A.d0 + base * 64 + lane;
     // V slots per trip: their blocks are independent chains of FP64
     // arithmetic (divisions, square roots) the scheduler interleaves
     int bnd = 0;
     for (; bnd + V - 1 <= top; bnd += V) {
       int q[V];
+      double l0[V];
 #pragma unroll
-      for (int u = 0; u < V; ++u) q[u] = ntl(cp + (int64_t)(bnd + u) * 64);
-      double4 xq[V], oq[V], vq[V];
+      for (int u = 0; u < V; ++u) {
+        q[u] = ntl(cp + (int64_t)(bnd + u) * 64);
+        l0[u] = ntl(lp + (int64_t)(bnd + u) * 64);
+      }
+      double4 xq[V], vq[V];
 #pragma unroll
       for (int u = 0; u < V; ++u) {
         xq[u] = A.pos[q[u]];
-        oq[u] = A.pos0[q[u]];
         vq[u] = x[q[u]];
       }
       double t[V][3];
 #pragma unroll
       for (int u = 0; u < V; ++u) {
-        double e0, e1, e2, dij, d0;
-        bond_geom(pos_p, xq[u], opos_p, oq[u], e0, e1, e2, dij, d0);
+        double e0, e1, e2, dij;
+        bond_dir(pos_p, xq[u], e0, e1, e2, dij);
         double F00, F01, F02, F11, F12, F22;
-        bond_jac(e0, e1, e2, dij, d0, cb, F00, F01, F02, F11, F12, F22);
+        bond_jac(e0, e1, e2, dij, l0[u], cb, F00, F01, F02, F11, F12, F22);
         blk6((F00 * -1.0) * dt2, (F01 * -1.0) * dt2, (F02 * -1.0) * dt2, (F11 * -1.0) * dt2,
              (F12 * -1.0) * dt2, (F22 * -1.0) * dt2, vq[u], t[u][0], t[u][1], t[u][2]);
       }
@@ -646,10 +662,10 @@ __device__ __forceinline__ void spmv_row_mf(const SpmvView& A, const double4* __
     }
     for (; bnd <= top; ++bnd) {
       const int q0 = ntl(cp + (int64_t)bnd * 64);
-      double e0, e1, e2, dij, d0;
-      bond_geom(pos_p, A.pos[q0], opos_p, A.pos0[q0], e0, e1, e2, dij, d0);
+      double e0, e1, e2, dij;
+      bond_dir(pos_p, A.pos[q0], e0, e1, e2, dij);
       double F00, F01, F02, F11, F12, F22;
-      bond_jac(e0, e1, e2, dij, d0, cb, F00, F01, F02, F11, F12, F22);
+      bond_jac(e0, e1, e2, dij, ntl(lp + (int64_t)bnd * 64), cb, F00, F01, F02, F11, F12, F22);
       double t0, t1, t2;
       blk6((F00 * -1.0) * dt2, (F01 * -1.0) * dt2, (F02 * -1.0) * dt2, (F11 * -1.0) * dt2,
            (F12 * -1.0) * dt2, (F22 * -1.0) * dt2, x[q0], t0, t1, t2);
@@ -1095,6 +1111,27 @@ void launch_set_dirichlet(int32_t n, const int32_t* bc, const int32_t* act, cons
   if (nb)
     hipLaunchKernelGGL(k_set_dirichlet, dim3(nb), dim3(PD_BLOCK), 0, s, n, bc, act, vbc,
                        integrator, dt, consts, sol);
+}
+
+__global__ __launch_bounds__(PD_BLOCK) void k_bond_d0(int n, const int64_t* slice_off,
+                                                      const int32_t* cap, const int32_t* col,
+                                                      const double4* x0, double* d0) {
+  const int p = blockIdx.x * PD_BLOCK + threadIdx.x;
+  if (p >= n) return;
+  const int64_t base = slice_off[p >> 6];
+  const int lane = p & 63;
+  const double4 op = x0[p];
+  for (int k = 0; k < cap[p]; ++k) {
+    const int64_t i = (base + k) * 64 + lane;
+    d0[i] = ref_len(op, x0[col[i]]);
+  }
+}
+
+void launch_bond_d0(int32_t n, const int64_t* slice_off, const int32_t* cap, const int32_t* col,
+                    const double4* x0, double* d0, hipStream_t s) {
+  const int nb = (n + PD_BLOCK - 1) / PD_BLOCK;
+  if (nb)
+    hipLaunchKernelGGL(k_bond_d0, dim3(nb), dim3(PD_BLOCK), 0, s, n, slice_off, cap, col, x0, d0);
 }
 
 __global__ __launch_bounds__(PD_BLOCK) void k_copy(double* dst, const double* src, int n) {
