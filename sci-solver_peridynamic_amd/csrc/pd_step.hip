@@ -34,13 +34,6 @@ __device__ __forceinline__ double ref_len(const double4 opos_p, const double4 oq
   return sqrt(o0 * o0 + o1 * o1 + o2 * o2);
 }
 
-__device__ __forceinline__ void bond_geom(const double4 pos_p, const double4 xq,
-                                          const double4 opos_p, const double4 oq, double& e0,
-                                          double& e1, double& e2, double& dij, double& d0) {
-  bond_dir(pos_p, xq, e0, e1, e2, dij);
-  d0 = ref_len(opos_p, oq);
-}
-
 // The bond's 3x3 Jacobian F (six distinct entries), cg_solver.cuh:292-299:
 // (c / d0) [(1 - d0/dij) I + (d0/dij) e e^T]
 __device__ __forceinline__ void bond_jac(double e0, double e1, double e2, double dij, double d0,
@@ -107,10 +100,14 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
     double4 xq_n = make_double4(0, 0, 0, 0), oq_n = make_double4(0, 0, 0, 0);
     double4 us_n = make_double4(0, 0, 0, 0);  // fused residual: xs of the neighbour
     double sq_n = 0.0;
+    double l_n = 0.0;  // stored reference length (a.d0), else from x0
     if (top >= 0) {
       q_n = a.col[base * 64 + lane];
       xq_n = a.x[q_n];
-      oq_n = a.x0[q_n];
+      if (a.d0)
+        l_n = a.d0[base * 64 + lane];
+      else
+        oq_n = a.x0[q_n];
       sq_n = a.s[q_n];
       if (a.fuse) us_n = a.xs[q_n];
     }
@@ -120,18 +117,23 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
       const int q = q_n;
       const double4 xq = xq_n;
       const double4 oq = oq_n;
+      const double lq = l_n;
       const double sqv = sq_n;
       const double4 u = us_n;
       if (bnd + 1 <= top) {
         q_n = q_nn;
         xq_n = a.x[q_n];
-        oq_n = a.x0[q_n];
+        if (a.d0)
+          l_n = a.d0[(slot + 1) * 64 + lane];
+        else
+          oq_n = a.x0[q_n];
         sq_n = a.s[q_n];
         if (a.fuse) us_n = a.xs[q_n];
       }
       if (bnd + 2 <= top) q_nn = a.col[(slot + 2) * 64 + lane];
-      double e0, e1, e2, dij, d0;
-      bond_geom(pos_p, xq, opos_p, oq, e0, e1, e2, dij, d0);
+      double e0, e1, e2, dij;
+      bond_dir(pos_p, xq, e0, e1, e2, dij);
+      const double d0 = a.d0 ? lq : ref_len(opos_p, oq);
       const double drdivd = dij / d0 - 1.0;
       // cg_solver.cuh:264: threshold (float)max(s_p, s_q)
       const float sq_f = (float)sqv;
@@ -158,7 +160,10 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
         // slot bnd again, now holding `last`: restart the pipeline there
         q_n = last;
         xq_n = a.x[last];
-        oq_n = a.x0[last];
+        if (a.d0)
+          l_n = a.d0[slot * 64 + lane];
+        else
+          oq_n = a.x0[last];
         sq_n = a.s[last];
         if (a.fuse) us_n = a.xs[last];
         if (bnd + 1 <= top) q_nn = a.col[(slot + 1) * 64 + lane];
