@@ -572,13 +572,13 @@ int enqueue_assemble(pd_ctx* c, const pd_step_params* prm,
                              c->sol, c->stream);
     CHECK_LAUNCH(c, "dirichlet values");
   }
-  if (prm->matrix_free && !c->d0_valid) {
+  if (!c->d0_valid) {  // the assembly and the matrix-free SpMV read them
     pd::launch_bond_d0(c->n, c->slice_off, c->cap, c->col, c->x0, c->d0, c->stream);
     CHECK_LAUNCH(c, "reference bond lengths");
     c->d0_valid = true;
   }
   pd::AsmArgs a;
-  a.d0 = c->d0_valid ? c->d0 : nullptr;
+  a.d0 = c->d0;
   a.store_blocks = prm->matrix_free ? 0 : 1;
   c->mf = prm->matrix_free ? 1 : 0;
   c->asm_dt2 = dt2;
@@ -604,7 +604,6 @@ int enqueue_assemble(pd_ctx* c, const pd_step_params* prm,
   a.diag = c->diag;
   a.minv = c->minv;
   a.x = c->x;
-  a.x0 = c->x0;
   a.v = c->v;
   a.fc = c->fc;
   a.vbc = c->vbc;
@@ -1049,7 +1048,7 @@ int capture(pd_ctx* c, hipGraphExec_t* out, const std::function<int()>& body) {
 int capture_step(pd_ctx* c, const pd_step_params* prm, int K) {
   destroy_graphs(c);
   const int ns = c->nspec;
-  if (prm->matrix_free && !c->d0_valid) {  // once, outside the captured step
+  if (!c->d0_valid) {  // once, outside the captured step
     pd::launch_bond_d0(c->n, c->slice_off, c->cap, c->col, c->x0, c->d0, c->stream);
     CHECK_LAUNCH(c, "reference bond lengths");
     c->d0_valid = true;

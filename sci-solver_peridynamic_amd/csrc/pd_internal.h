@@ -30,7 +30,6 @@ struct AsmArgs {
   float* minv32;
   int32_t mixed;
   const double4* x;
-  const double4* x0;
   const double4* v;
   const double4* fc;
   const double4* vbc;
@@ -54,7 +53,7 @@ struct AsmArgs {
   const pd_step_consts* consts;  // non-null: dt, dt2, clock, gravity from here
   float2* bond_stats;            // non-null: (mean, max) stretch of the live bonds
   int32_t store_blocks;  // FP64: 0 = matrix-free solve, bond blocks not written
-  double* d0;            // non-null: per-slot reference lengths, swapped with col
+  double* d0;            // per-slot reference lengths: read, and swapped with col
   // fuse = 1: the solve's first residual r = S(b - A xs), each row's bond
   // terms summed first and the diagonal term last, with the solver's start
   // and its tile partials in part (Dirichlet rows of xs must hold their

@@ -79,7 +79,6 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
     const int lane = p & 63;
     const int64_t base = a.slice_off[s];
     const double4 pos_p = a.x[p];
-    const double4 opos_p = a.x0[p];
     double f0 = 0.0, f1 = 0.0, f2 = 0.0;
     int top = a.top[p];
     const int otop = top;
@@ -97,17 +96,14 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
     // of slot bnd + 2 are in flight while slot bnd is computed, so neither
     // the column load nor the gathers it feeds wait in line
     int q_n = 0, q_nn = 0;
-    double4 xq_n = make_double4(0, 0, 0, 0), oq_n = make_double4(0, 0, 0, 0);
+    double4 xq_n = make_double4(0, 0, 0, 0);
     double4 us_n = make_double4(0, 0, 0, 0);  // fused residual: xs of the neighbour
     double sq_n = 0.0;
-    double l_n = 0.0;  // stored reference length (a.d0), else from x0
+    double l_n = 0.0;  // the slot's reference length
     if (top >= 0) {
       q_n = a.col[base * 64 + lane];
       xq_n = a.x[q_n];
-      if (a.d0)
-        l_n = a.d0[base * 64 + lane];
-      else
-        oq_n = a.x0[q_n];
+      l_n = a.d0[base * 64 + lane];
       sq_n = a.s[q_n];
       if (a.fuse) us_n = a.xs[q_n];
     }
@@ -116,24 +112,19 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
       const int64_t slot = base + bnd;
       const int q = q_n;
       const double4 xq = xq_n;
-      const double4 oq = oq_n;
-      const double lq = l_n;
+      const double d0 = l_n;
       const double sqv = sq_n;
       const double4 u = us_n;
       if (bnd + 1 <= top) {
         q_n = q_nn;
         xq_n = a.x[q_n];
-        if (a.d0)
-          l_n = a.d0[(slot + 1) * 64 + lane];
-        else
-          oq_n = a.x0[q_n];
+        l_n = a.d0[(slot + 1) * 64 + lane];
         sq_n = a.s[q_n];
         if (a.fuse) us_n = a.xs[q_n];
       }
       if (bnd + 2 <= top) q_nn = a.col[(slot + 2) * 64 + lane];
       double e0, e1, e2, dij;
       bond_dir(pos_p, xq, e0, e1, e2, dij);
-      const double d0 = a.d0 ? lq : ref_len(opos_p, oq);
       const double drdivd = dij / d0 - 1.0;
       // cg_solver.cuh:264: threshold (float)max(s_p, s_q)
       const float sq_f = (float)sqv;
@@ -151,7 +142,7 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
         const int last = a.col[last_slot];
         a.col[slot * 64 + lane] = last;
         a.col[last_slot] = q;
-        if (a.d0) {  // the reference lengths move with their columns
+        {  // the reference lengths move with their columns
           const double dl = a.d0[last_slot];
           a.d0[last_slot] = a.d0[slot * 64 + lane];
           a.d0[slot * 64 + lane] = dl;
@@ -160,10 +151,7 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
         // slot bnd again, now holding `last`: restart the pipeline there
         q_n = last;
         xq_n = a.x[last];
-        if (a.d0)
-          l_n = a.d0[slot * 64 + lane];
-        else
-          oq_n = a.x0[last];
+        l_n = a.d0[slot * 64 + lane];
         sq_n = a.s[last];
         if (a.fuse) us_n = a.xs[last];
         if (bnd + 1 <= top) q_nn = a.col[(slot + 1) * 64 + lane];
