@@ -330,6 +330,9 @@ def main():
                     help="skip the learned-preconditioner report")
     ap.add_argument("--no-mf", action="store_true",
                     help="skip the matrix-free FP64 operator report")
+    ap.add_argument("--operator", choices=["stored", "mf"], default="stored",
+                    help="FP64 operator of the timed line (mf: bond blocks recomputed in "
+                         "every SpMV, results identical)")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="after the run, N rounds of device copies of known size")
@@ -340,7 +343,8 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     prec = 1 if a.precision == "mixed" else 0
-    r = gpu_run(a.steps, a.warmup, prec, a.copy_calibration)
+    mf = a.operator == "mf" and not prec
+    r = gpu_run(a.steps, a.warmup, prec, a.copy_calibration, matrix_free=1 if mf else 0)
     ms_step = 1000.0 * r["elapsed"] / r["steps"]
     kernel = ("k_mx_spmv (float inner CG SpMV y = A32 p + <y,p> partial)" if prec else
               "k_cg_spmv (CG SpMV y = A p + <y,p> partial)")
@@ -360,7 +364,8 @@ def main():
         "config": {
             "workload": "impact_100: 100^3 jittered lattice + 5000-particle projectile at "
                         "20 m/s, implicit Euler dt 5e-5 s, block-Jacobi CG rtol 1e-6, " +
-                        ("float inner CG with FP64 residual / refinement" if prec else "FP64"),
+                        ("float inner CG with FP64 residual / refinement" if prec else
+                         "FP64, matrix-free operator" if mf else "FP64"),
             "particles": r["n"],
             "directed_bonds": r["bonds"],
             "live_bonds_end": r["live"],
@@ -373,7 +378,7 @@ def main():
             "spmv_unroll": r["unroll"],
             "spmv_tune_ms_2_4_8": r["tune_ms"],
         },
-        "roofline": spmv_roofline(r, kernel, read_traffic(prec)),
+        "roofline": mf_roofline(r) if mf else spmv_roofline(r, kernel, read_traffic(prec)),
     }
     if prec:
         out["config"]["refinements_mean"] = float(np.mean(r["refinements"]))
@@ -398,7 +403,7 @@ def main():
             "spmv_tune_ms_2_4_8": m["tune_ms"],
             "position_error_vs_fp64": mixed_error(a.warmup + a.steps),
         }
-    if not prec and not a.no_mf:
+    if not prec and not mf and not a.no_mf:
         f = gpu_run(a.steps, a.warmup, 0, matrix_free=1)
         f_rate = f["steps"] / f["elapsed"]
         out["matrix_free"] = {
