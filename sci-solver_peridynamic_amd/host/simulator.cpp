@@ -78,6 +78,7 @@ pd_step_params Simulator::stepParams() const {
   p.track_gap = 0;
   p.timing = 1;
   p.precision = r.cg_precision;
+  p.matrix_free = r.cg_precision == 0 ? r.cg_matrix_free : 0;
   p.inner_rtol = r.cg_inner_rtol;
   p.max_refine = r.cg_max_refine;
   p.inner_max_iter = 0;

@@ -553,7 +553,8 @@ def step_kwargs(desc, step):
                 contact_volume=desc["contact_volume"], contact_cell=desc["contact_radius"],
                 precision=desc.get("precision", 0), inner_rtol=desc.get("inner_rtol", 1e-5),
                 max_refine=desc.get("max_refine", 20), inner_max_iter=desc["max_iter"],
-                contact_mode=desc.get("contact_mode", 0))
+                contact_mode=desc.get("contact_mode", 0),
+                matrix_free=desc.get("matrix_free", 0))
 
 
 def materials_of(desc):

@@ -49,3 +49,15 @@ def test_batch_small_has_no_cross_specimen_bonds():
     for r in range(len(idx)):
         nb = idx[lists[r, :counts[r]]]
         assert np.all(spec[nb] == spec[idx[r]])
+
+
+def test_cg_operator_key(tmp_path):
+    """cg_operator matrix_free reaches the step parameters (FP64 only)."""
+    base = open(os.path.join(CFG, "tension_20.txt")).read()
+    for text, want in ((base, 0), (base + "\ncg_operator matrix_free\n", 1),
+                       (base + "\ncg_operator matrix_free\ncg_precision mixed\n", 0)):
+        p = tmp_path / "params.txt"
+        p.write_text(text)
+        desc = pds.describe(str(p))
+        assert desc["matrix_free"] == want
+        assert pds.make_params(**pds.step_kwargs(desc, 1)).matrix_free == want

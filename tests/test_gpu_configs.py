@@ -49,14 +49,16 @@ def setup_pair(cfg_path):
     return desc, sc, S, ctx
 
 
-@pytest.mark.parametrize("cfg,steps,precision", [("tension_20.txt", 100, 0),
-                                                 ("impact_30.txt", 60, 0),
-                                                 ("tension_20.txt", 100, 1),
-                                                 ("impact_30.txt", 60, 1)])
-def test_config_run_parity(cfg, steps, precision):
+@pytest.mark.parametrize("cfg,steps,precision,mf", [("tension_20.txt", 100, 0, 0),
+                                                    ("impact_30.txt", 60, 0, 0),
+                                                    ("tension_20.txt", 100, 1, 0),
+                                                    ("impact_30.txt", 60, 1, 0),
+                                                    ("impact_30.txt", 60, 0, 1)])
+def test_config_run_parity(cfg, steps, precision, mf):
     """precision 1 runs the mixed-precision solver in both the oracle and
-    the engine: the same bar applies."""
-    say("case", cfg, steps, "precision", precision)
+    the engine: the same bar applies.  mf 1: the engine's matrix-free FP64
+    operator against the same oracle."""
+    say("case", cfg, steps, "precision", precision, "matrix_free", mf)
     desc, sc, S, ctx = setup_pair(os.path.join(CFG, cfg))
     first_near = None
     worst_rel = 0.0
@@ -65,7 +67,7 @@ def test_config_run_parity(cfg, steps, precision):
     for k in range(1, steps + 1):
         kw = dict(pds.step_kwargs(desc, k), precision=precision)
         so = S.step(orc.make_params(**kw))
-        sg = ctx.step(pds.make_params(track_gap=1, **kw))
+        sg = ctx.step(pds.make_params(track_gap=1, matrix_free=mf, **kw))
         xg = ctx.download("position")
         d = np.abs(xg[:, :3] - S.x[:, :3]).max()
         rel = d / np.abs(S.x[:, :3]).max()
