@@ -199,6 +199,9 @@ def make_params(**kw) -> StepParams:
                     contact_cell=0.0, precision=0, inner_rtol=1e-5, max_refine=20,
                     contact_mode=0)
     defaults.update(kw)
+    # the engine's operator choice (stored or recomputed blocks) gives the
+    # same results: nothing of it to restate
+    defaults.pop("matrix_free", None)
     defaults.setdefault("inner_max_iter", defaults["max_iter"])
     for k, v in defaults.items():
         if k in ("bmin", "bmax"):

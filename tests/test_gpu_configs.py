@@ -67,7 +67,7 @@ def test_config_run_parity(cfg, steps, precision, mf):
     for k in range(1, steps + 1):
         kw = dict(pds.step_kwargs(desc, k), precision=precision)
         so = S.step(orc.make_params(**kw))
-        sg = ctx.step(pds.make_params(track_gap=1, matrix_free=mf, **kw))
+        sg = ctx.step(pds.make_params(**dict(kw, track_gap=1, matrix_free=mf)))
         xg = ctx.download("position")
         d = np.abs(xg[:, :3] - S.x[:, :3]).max()
         rel = d / np.abs(S.x[:, :3]).max()

@@ -114,7 +114,7 @@ def test_mf_batch_graph_equals_stored_eager(mode):
     for k in range(1, 31):
         kw = dict(pds.step_kwargs(desc, k), precision=0)
         ss = stored.step(pds.make_params(**kw))
-        sm = mf.step(pds.make_params(matrix_free=mode, graph=1, **kw))
+        sm = mf.step(pds.make_params(**dict(kw, matrix_free=mode, graph=1)))
         assert (sm.cg_iterations, sm.broken_bonds, sm.live_bonds) == \
             (ss.cg_iterations, ss.broken_bonds, ss.live_bonds), k
         assert stored.specimen_stats(nspec) == mf.specimen_stats(nspec), k
