@@ -326,6 +326,17 @@ int read_bbox(pd_ctx* c, int32_t sel, double* out6) {
   return PD_OK;
 }
 
+// bounding boxes of bodies 0 .. nbody - 1, one pass and one wait
+int read_bboxes(pd_ctx* c, double (*out)[6]) {
+  pd::launch_bbox_bodies(c->x, c->body, c->n, c->bbox_part, c->bbox, c->stream);
+  CHECK_LAUNCH(c, "bbox");
+  HIPCHK(c, hipMemcpyAsync(c->h_bbox, c->bbox, 6 * PD_MAX_BODIES * sizeof(double),
+                           hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int b = 0; b < c->nbody; ++b) memcpy(out[b], c->h_bbox + 6 * b, 6 * sizeof(double));
+  return PD_OK;
+}
+
 // Uniform grid with origin mn and edge `cell`: the sub-grid of cells
 // c0 .. c0 + dims - 1 over the m particles subset[0..m) (all n particles
 // when subset is null), sorted by cell with a stable radix sort so each
@@ -488,8 +499,8 @@ int enqueue_contact(pd_ctx* c, const pd_step_params* prm) {
   const int nb = c->nbody;
   double bb[PD_MAX_BODIES][6];
   double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+  if (int e = read_bboxes(c, bb)) return e;
   for (int b = 0; b < nb; ++b) {
-    if (int e = read_bbox(c, b, bb[b])) return e;
     for (int k = 0; k < 3; ++k) {
       mn[k] = std::min(mn[k], bb[b][k]);
       mx[k] = std::max(mx[k], bb[b][3 + k]);
@@ -1253,8 +1264,8 @@ int pd_ctx_create(const pd_config* cfg, pd_ctx** out) {
   ALLOC(c->idx_a, n);
   ALLOC(c->idx_b, n);
   ALLOC(c->hist, pd::radix_hist_words(c->n));
-  ALLOC(c->bbox_part, 256 * 6);
-  ALLOC(c->bbox, 6);
+  ALLOC(c->bbox_part, 256 * 6 * PD_MAX_BODIES);
+  ALLOC(c->bbox, 6 * PD_MAX_BODIES);
   ALLOC(c->counts, n);
   ALLOC(c->zone_cnt, pd::zone_tile_words(c->n));
   ALLOC(c->cand, n);
@@ -1266,7 +1277,7 @@ int pd_ctx_create(const pd_config* cfg, pd_ctx** out) {
 #undef ALLOC
   if (hipHostMalloc((void**)&c->h_st, sizeof(pd_cg_state), hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc((void**)&c->h_cnt, sizeof(pd_step_counters), hipHostMallocDefault) != hipSuccess ||
-      hipHostMalloc((void**)&c->h_bbox, 6 * sizeof(double), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&c->h_bbox, 6 * PD_MAX_BODIES * sizeof(double), hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc((void**)&c->h_gap, sizeof(double), hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc((void**)&c->h_gap_pq, 2 * sizeof(int32_t), hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc((void**)&c->h_zone_count, sizeof(uint32_t), hipHostMallocDefault) != hipSuccess)

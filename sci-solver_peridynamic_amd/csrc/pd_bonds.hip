@@ -10,6 +10,7 @@
 #include <math.h>
 
 #include "pd_internal.h"
+#include "pd_solver.h"  // PD_MAX_BODIES
 
 namespace pd {
 
@@ -67,6 +68,87 @@ __global__ __launch_bounds__(PD_BLOCK) void k_bbox_final(const double* part, int
 }
 
 static const int kBboxBlocks = 256;
+
+// Bounding boxes of all bodies in one pass: per block (lo xyz, hi xyz) for
+// each body b < PD_MAX_BODIES at part[(block * PD_MAX_BODIES + b) * 6 + k].
+// min / max are exact, so the result does not depend on the order.
+__global__ __launch_bounds__(PD_BLOCK) void k_bbox_bodies_partial(const double4* x,
+                                                                 const int32_t* body, int n,
+                                                                 double* part) {
+  double v[PD_MAX_BODIES][6];
+#pragma unroll
+  for (int b = 0; b < PD_MAX_BODIES; ++b)
+#pragma unroll
+    for (int k = 0; k < 6; ++k) v[b][k] = k < 3 ? INFINITY : -INFINITY;
+  for (int i = blockIdx.x * PD_BLOCK + threadIdx.x; i < n; i += gridDim.x * PD_BLOCK) {
+    const int bi = body[i];
+    const double4 p = x[i];
+#pragma unroll
+    for (int b = 0; b < PD_MAX_BODIES; ++b)
+      if (b == bi) {
+        v[b][0] = fmin(v[b][0], p.x);
+        v[b][1] = fmin(v[b][1], p.y);
+        v[b][2] = fmin(v[b][2], p.z);
+        v[b][3] = fmax(v[b][3], p.x);
+        v[b][4] = fmax(v[b][4], p.y);
+        v[b][5] = fmax(v[b][5], p.z);
+      }
+  }
+  __shared__ double s[PD_BLOCK / 64][PD_MAX_BODIES * 6];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int b = 0; b < PD_MAX_BODIES; ++b)
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      double t = v[b][k];
+#pragma unroll
+      for (int m = 32; m >= 1; m >>= 1) {
+        const double o = __shfl_xor(t, m, 64);
+        t = k < 3 ? fmin(t, o) : fmax(t, o);
+      }
+      if (lane == 0) s[wave][b * 6 + k] = t;
+    }
+  __syncthreads();
+  const int t = threadIdx.x;
+  if (t < PD_MAX_BODIES * 6) {
+    const bool lo = (t % 6) < 3;
+    double r = s[0][t];
+    for (int w = 1; w < PD_BLOCK / 64; ++w) r = lo ? fmin(r, s[w][t]) : fmax(r, s[w][t]);
+    part[blockIdx.x * PD_MAX_BODIES * 6 + t] = r;
+  }
+}
+
+// one thread per block partial (g <= PD_BLOCK), every value folded by wave
+// butterflies and across the four waves
+__global__ __launch_bounds__(PD_BLOCK) void k_bbox_bodies_final(const double* part, int g,
+                                                               double* out) {
+  __shared__ double s[PD_BLOCK / 64][PD_MAX_BODIES * 6];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  for (int k = 0; k < PD_MAX_BODIES * 6; ++k) {
+    const bool lo = (k % 6) < 3;
+    double r = t < g ? part[t * PD_MAX_BODIES * 6 + k] : (lo ? INFINITY : -INFINITY);
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+      const double o = __shfl_xor(r, m, 64);
+      r = lo ? fmin(r, o) : fmax(r, o);
+    }
+    if (lane == 0) s[wave][k] = r;
+  }
+  __syncthreads();
+  if (t < PD_MAX_BODIES * 6) {
+    const bool lo = (t % 6) < 3;
+    double r = s[0][t];
+    for (int w = 1; w < PD_BLOCK / 64; ++w) r = lo ? fmin(r, s[w][t]) : fmax(r, s[w][t]);
+    out[t] = r;
+  }
+}
+
+void launch_bbox_bodies(const double4* x, const int32_t* body, int32_t n, double* part,
+                        double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_bbox_bodies_partial, dim3(kBboxBlocks), dim3(PD_BLOCK), 0, s, x, body, n,
+                     part);
+  hipLaunchKernelGGL(k_bbox_bodies_final, dim3(1), dim3(PD_BLOCK), 0, s, part, kBboxBlocks, out);
+}
 
 void launch_bbox(const double4* x, const int32_t* body, int32_t n, int32_t body_sel,
                  double* part, double* out6, hipStream_t s) {

@@ -1161,9 +1161,16 @@ __global__ __launch_bounds__(PD_BLOCK) void k_zero4(double4* v, int n) {
   if (i < n) v[i] = make_double4(0.0, 0.0, 0.0, 0.0);
 }
 
+// One wave per candidate p.  The members of its 27 cells are numbered in
+// the serial order (cells in z, y, x order, members ascending); each lane
+// evaluates one member's term at a time, and the terms are added into f in
+// that order (broadcast lane by lane), so f equals the serial sum bit for
+// bit.  The member loads and square roots of a candidate run in parallel
+// instead of one after another.
 __global__ __launch_bounds__(PD_BLOCK) void k_contact(ContactArgs a) {
-  const int i = blockIdx.x * PD_BLOCK + threadIdx.x;
-  if (i >= a.m) return;
+  const int lane = threadIdx.x & 63;
+  const int i = blockIdx.x * (PD_BLOCK / 64) + (threadIdx.x >> 6);
+  if (i >= a.m) return;  // whole waves
   const int p = a.cand ? a.cand[i] : i;
   if (a.act[p] == PD_INVISIBLE) return;  // fc[p] stays zero
   const double4 pp = a.x[p];
@@ -1171,35 +1178,64 @@ __global__ __launch_bounds__(PD_BLOCK) void k_contact(ContactArgs a) {
   const int cx = (int)floor((pp.x - a.g.mn[0]) / a.g.cell) - a.g.c0[0];
   const int cy = (int)floor((pp.y - a.g.mn[1]) / a.g.cell) - a.g.c0[1];
   const int cz = (int)floor((pp.z - a.g.mn[2]) / a.g.cell) - a.g.c0[2];
+  // lane j < 27: cell j's member range (dx fastest, then dy, then dz)
+  int st = 0, cnt = 0;
+  if (lane < 27) {
+    const int ix = cx + lane % 3 - 1, iy = cy + (lane / 3) % 3 - 1, iz = cz + lane / 9 - 1;
+    if (ix >= 0 && iy >= 0 && iz >= 0 && ix < a.g.dims[0] && iy < a.g.dims[1] &&
+        iz < a.g.dims[2]) {
+      const int64_t cell = ((int64_t)iz * a.g.dims[1] + iy) * a.g.dims[0] + ix;
+      st = a.g.cell_start[cell];
+      cnt = a.g.cell_end[cell] - st;
+    }
+  }
+  int incl = cnt;  // inclusive prefix sum of the counts over the lanes
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int v = __shfl_up(incl, d, 64);
+    if (lane >= d) incl += v;
+  }
+  const int excl = incl - cnt;
+  const int total = __shfl(incl, 63, 64);
   double f0 = 0.0, f1 = 0.0, f2 = 0.0;
-  for (int dz = -1; dz <= 1; ++dz)
-    for (int dy = -1; dy <= 1; ++dy)
-      for (int dx = -1; dx <= 1; ++dx) {
-        const int ix = cx + dx, iy = cy + dy, iz = cz + dz;
-        if (ix < 0 || iy < 0 || iz < 0 || ix >= a.g.dims[0] || iy >= a.g.dims[1] ||
-            iz >= a.g.dims[2])
-          continue;
-        const int64_t cell = ((int64_t)iz * a.g.dims[1] + iy) * a.g.dims[0] + ix;
-        const int e = a.g.cell_end[cell];
-        for (int m = a.g.cell_start[cell]; m < e; ++m) {
-          const int q = a.g.sorted_idx[m];
-          if (q == p) continue;
-          if (!a.all_pairs && a.body[q] == bp) continue;
-          if (a.act[q] == PD_INVISIBLE) continue;
-          const double4 xq = a.x[q];
-          const double q0 = pp.x - xq.x, q1 = pp.y - xq.y, q2 = pp.z - xq.z;
-          const double r = sqrt(q0 * q0 + q1 * q1 + q2 * q2);
-          if (r >= a.Rc) continue;
-          if (r == 0.0) continue;
+  for (int base = 0; base < total; base += 64) {
+    const int t = base + lane;
+    int m = -1;
+    for (int j = 0; j < 27; ++j) {
+      const int ej = __shfl(excl, j, 64), cj = __shfl(cnt, j, 64), sj = __shfl(st, j, 64);
+      if (t >= ej && t < ej + cj) m = sj + (t - ej);
+    }
+    int use = 0;
+    double t0 = 0.0, t1 = 0.0, t2 = 0.0;
+    if (t < total && m >= 0) {
+      const int q = a.g.sorted_idx[m];
+      if (q != p && (a.all_pairs || a.body[q] != bp) && a.act[q] != PD_INVISIBLE) {
+        const double4 xq = a.x[q];
+        const double q0 = pp.x - xq.x, q1 = pp.y - xq.y, q2 = pp.z - xq.z;
+        const double r = sqrt(q0 * q0 + q1 * q1 + q2 * q2);
+        if (r < a.Rc && r != 0.0) {
           const double howfar = (r - a.Rc);
           const double mag = a.kc * (howfar * howfar);
           const double coef = mag / a.vol;
-          f0 += coef * (q0 / r);
-          f1 += coef * (q1 / r);
-          f2 += coef * (q2 / r);
+          t0 = coef * (q0 / r);
+          t1 = coef * (q1 / r);
+          t2 = coef * (q2 / r);
+          use = 1;
         }
       }
-  a.fc[p] = make_double4(f0, f1, f2, 0.0);
+    }
+    const int nk = min(64, total - base);
+    for (int k = 0; k < nk; ++k) {  // the serial order
+      const int u = __shfl(use, k, 64);
+      const double a0 = __shfl(t0, k, 64), a1 = __shfl(t1, k, 64), a2 = __shfl(t2, k, 64);
+      if (u) {
+        f0 += a0;
+        f1 += a1;
+        f2 += a2;
+      }
+    }
+  }
+  if (lane == 0) a.fc[p] = make_double4(f0, f1, f2, 0.0);
 }
 
 void launch_contact(int32_t n, const double4* x, const int32_t* body, const int32_t* act,
@@ -1221,7 +1257,8 @@ void launch_contact(int32_t n, const double4* x, const int32_t* body, const int3
   a.vol = vol;
   a.all_pairs = all_pairs;
   a.fc = fc;
-  const int mb = (m + PD_BLOCK - 1) / PD_BLOCK;
+  const int per = PD_BLOCK / 64;  // candidates per workgroup, one wave each
+  const int mb = (m + per - 1) / per;
   hipLaunchKernelGGL(k_contact, dim3(mb), dim3(PD_BLOCK), 0, s, a);
 }
 
