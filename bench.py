@@ -420,10 +420,12 @@ def main():
     if not a.no_learned and os.path.exists(NETWORK):
         net = np.loadtxt(NETWORK, dtype=np.float32)
         rows = {}
-        for prec in (0, 1):
-            L = gpu_run(a.steps, a.warmup, prec, precond=2, network=net)
+        for prec, mfree, name in ((0, 0, "fp64"), (1, 0, "mixed"), (0, 1, "fp64_matrix_free")):
+            if mfree and (mf or a.no_mf):
+                continue
+            L = gpu_run(a.steps, a.warmup, prec, precond=2, network=net, matrix_free=mfree)
             base = r if prec == 0 or m_run is None else m_run
-            rows["fp64" if prec == 0 else "mixed"] = {
+            rows[name] = {
                 "value": L["steps"] / L["elapsed"],
                 "unit": "steps/s",
                 "speedup_vs_fp64_block_jacobi": (L["steps"] / L["elapsed"]) / out["value"],

@@ -132,8 +132,9 @@ typedef struct pd_step_params {
    * block-Jacobi inverse only, and every SpMV recomputes each live bond's
    * block from the step's positions with the assembly's own arithmetic,
    * so every product, and every result, is identical to 0.  Needs
-   * precision 0 and precond 0 or 1; pd_get_matrix then fails (no stored
-   * bond blocks) and pd_spmv applies the recomputed operator. */
+   * precision 0; pd_get_matrix then fails (no stored bond blocks), and
+   * pd_spmv, the learned preconditioner's trainer and every solve apply
+   * the recomputed operator. */
   int32_t matrix_free;
 } pd_step_params;
 

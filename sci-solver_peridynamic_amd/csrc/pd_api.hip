@@ -269,8 +269,8 @@ int check_params(pd_ctx* c, const pd_step_params* prm) {
     return fail(c, PD_ERR_ARG, "precision must be 0 (FP64) or 1 (float inner solve + FP64 refinement)");
   if (prm->matrix_free != 0 && prm->matrix_free != 1)
     return fail(c, PD_ERR_ARG, "matrix_free must be 0 (stored blocks) or 1 (recomputed blocks)");
-  if (prm->matrix_free && (prm->precision != 0 || prm->precond == 2))
-    return fail(c, PD_ERR_ARG, "matrix_free needs precision 0 and precond 0 or 1");
+  if (prm->matrix_free && prm->precision != 0)
+    return fail(c, PD_ERR_ARG, "matrix_free needs precision 0 (the float solve streams float blocks)");
   if (prm->precision == 1 && (prm->inner_rtol < 0.0 || prm->max_refine < 0 || prm->inner_max_iter < 0))
     return fail(c, PD_ERR_ARG, "inner_rtol, max_refine and inner_max_iter must be >= 0");
   if (!c->bonds_ready) return fail(c, PD_ERR_STATE, "bonds not built (pd_build_bonds / pd_set_bonds)");

@@ -139,6 +139,27 @@ def test_learned_solve_matches_oracle_given_blocks(precision):
     ctx.close()
 
 
+def test_learned_solve_matrix_free_equals_stored():
+    """The learned preconditioner over the matrix-free operator: the same
+    blocks, iterations and solution as over the stored blocks."""
+    say("case learned solve matrix-free")
+    _, a = deformed_pair(s_fixed=10.0)
+    _, b = deformed_pair(s_fixed=10.0)
+    net = random_net(5)
+    a.learned_set(net)
+    b.learned_set(net)
+    kw = dict(dt=2e-4, precond=2, rtol=1e-10)
+    a.assemble(pds.make_params(**kw))
+    b.assemble(pds.make_params(matrix_free=1, **kw))
+    sa = a.solve(pds.make_params(**kw))
+    sb = b.solve(pds.make_params(matrix_free=1, **kw))
+    assert np.array_equal(a.get_precond(), b.get_precond())
+    assert (sa.cg_iterations, sa.residual_norm) == (sb.cg_iterations, sb.residual_norm)
+    assert np.array_equal(a.download("solution"), b.download("solution"))
+    a.close()
+    b.close()
+
+
 def test_fallback_to_block_jacobi():
     say("case learned fallback")
     x4, bc, vbc, v4 = config0_state(8, 8, 10)
