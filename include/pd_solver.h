@@ -280,6 +280,14 @@ int pd_synchronize(pd_ctx* ctx);
 int pd_bench_spmv(pd_ctx* ctx, int32_t iters, double* ms_per_launch,
                   double* bytes_per_launch);
 
+/* Self-check (no counterpart in the reference) of the matrix-free
+ * operator's shared-reciprocal quotients: on `samples` random operand pairs
+ * spanning the ranges under which the operator uses them (a coordinate
+ * difference over a float bond length; a bond stiffness over a reference
+ * length), counts in *mismatches the quotients whose bits differ from IEEE
+ * division on this device.  0 is the expected answer. */
+int pd_check_division(pd_ctx* ctx, int64_t samples, uint64_t seed, int64_t* mismatches);
+
 /* Algorithmic HBM bytes of one CG SpMV launch on the current bonds:
  * FP64, live blocks * (6*8 + 4) + N * (6*8 + 4 + 2*24); after an assembly
  * with precision 1 (float inner SpMV), live blocks * (6*4 + 4) +

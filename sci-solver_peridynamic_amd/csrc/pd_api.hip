@@ -139,6 +139,12 @@ struct pd_ctx {
   bool d0_valid = false;
   double asm_dt2 = 0.0;                      // its dt^2 scale
   const pd_step_consts* asm_consts = nullptr;  // captured step: dt2 read here
+  // per particle the position and the CG direction side by side (6 doubles,
+  // the matrix-free SpMV's gather record), allocated on first use
+  double* rec = nullptr;
+  // true from an assembly until positions change (update, upload, restore,
+  // new bonds): the assembly's range flag and record describe the positions
+  bool asm_fresh = false;
 };
 
 struct pd_trainer {
@@ -400,6 +406,7 @@ int ensure_slots(pd_ctx* c, int64_t slots) {
   dfree(c->val);
   dfree(c->d0);
   c->d0_valid = false;
+  c->asm_fresh = false;
   if (int e = dalloc(c, &c->col, (size_t)slots * 64)) return e;
   if (int e = dalloc(c, &c->val, (size_t)slots * 64 * 6)) return e;
   if (int e = dalloc(c, &c->d0, (size_t)slots * 64)) return e;
@@ -433,6 +440,10 @@ pd::SpmvView spmv_view(pd_ctx* c) {
   A.dt2 = c->asm_dt2;
   A.consts = c->asm_consts;
   A.d0 = c->d0;
+  A.range_bad = c->asm_fresh ? (const unsigned int*)((const char*)c->counters +
+                                                     offsetof(pd_step_counters, range_bad))
+                             : nullptr;
+  A.rec = c->asm_fresh && c->mf ? c->rec : nullptr;
   return A;
 }
 
@@ -565,6 +576,12 @@ int enqueue_contact(pd_ctx* c, const pd_step_params* prm) {
   return PD_OK;
 }
 
+int ensure_rec(pd_ctx* c) {
+  if (!c->rec)
+    if (int e = dalloc(c, &c->rec, (size_t)c->n * 6)) return e;
+  return PD_OK;
+}
+
 // The solve can take its first residual and start from the assembly pass
 // (the blocks and the block-Jacobi inverse are in registers there); the
 // learned preconditioner's blocks are only formed after the assembly.
@@ -588,8 +605,11 @@ int enqueue_assemble(pd_ctx* c, const pd_step_params* prm,
     CHECK_LAUNCH(c, "reference bond lengths");
     c->d0_valid = true;
   }
+  if (prm->matrix_free)
+    if (int e = ensure_rec(c)) return e;
   pd::AsmArgs a;
   a.d0 = c->d0;
+  a.rec = prm->matrix_free ? c->rec : nullptr;
   a.store_blocks = prm->matrix_free ? 0 : 1;
   c->mf = prm->matrix_free ? 1 : 0;
   c->asm_dt2 = dt2;
@@ -643,6 +663,7 @@ int enqueue_assemble(pd_ctx* c, const pd_step_params* prm,
   pd::launch_assemble(a, c->stream);
   CHECK_LAUNCH(c, "assemble");
   c->mat_prec = prm->precision;
+  c->asm_fresh = true;
   if (prm->track_gap) {
     pd::launch_gap_reduce(c->gap_val, c->gap_q, c->n, c->gpart_v, c->gpart_p, c->gpart_q,
                           c->gap_out, c->gap_pq, c->stream);
@@ -928,6 +949,7 @@ int enqueue_update(pd_ctx* c, const pd_step_params* prm,
   pd::launch_update(c->n, c->x, c->v, c->sol, c->act, prm->integrator, prm->dt, prm->walls, lo, hi,
                     prm->restitution, consts, c->stream);
   CHECK_LAUNCH(c, "update");
+  c->asm_fresh = false;
   return PD_OK;
 }
 
@@ -1027,14 +1049,14 @@ std::string graph_key(pd_ctx* c, const pd_step_params* prm, int K) {
   const MixedLimits lim = mixed_limits(prm);
   char buf[1024];
   snprintf(buf, sizeof(buf),
-           "%d %d %d %d %a %d %a %d %d %d %a %a %a %a %a %a %a %a %d %d %d %p %p %p %p %p %p %p",
+           "%d %d %d %d %a %d %a %d %d %d %a %a %a %a %a %a %a %a %d %d %d %p %p %p %p %p %p %p %p",
            prm->matrix_free, prm->precision, prm->precond, prm->integrator, prm->rtol,
            prm->max_iter,
            lim.inner_rtol, lim.inner_max, lim.max_refine, prm->walls, prm->bmin[0],
            prm->bmin[1], prm->bmin[2], prm->bmax[0], prm->bmax[1], prm->bmax[2],
            prm->wall_radius, prm->restitution, c->unroll[prm->precision], c->nspec, K,
            (void*)c->s, (void*)c->s_new, (void*)c->val, (void*)c->col, (void*)c->val32,
-           (void*)c->tile_spec, (void*)c->st);
+           (void*)c->tile_spec, (void*)c->st, (void*)c->rec);
   return std::string(buf);
 }
 
@@ -1064,6 +1086,8 @@ int capture_step(pd_ctx* c, const pd_step_params* prm, int K) {
     CHECK_LAUNCH(c, "reference bond lengths");
     c->d0_valid = true;
   }
+  if (prm->matrix_free)  // no allocation inside the capture
+    if (int e = ensure_rec(c)) return e;
   int e = capture(c, &c->g_solve, [&]() -> int {
     const bool fuse = fused_resid(prm);
     if (int e2 = enqueue_assemble(c, prm, c->consts, fuse)) return e2;
@@ -1143,10 +1167,12 @@ int run_step_graph(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st) {
   HIPCHK(c, hipGraphLaunch(c->g_solve, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->mat_prec = prm->precision;
+  c->asm_fresh = true;  // the graph's assembly ran
   if (int e = run_solve(c, prm, st, K)) return e;
   read_assemble_stats(c, prm, st);
   if (timing) HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
   HIPCHK(c, hipGraphLaunch(c->g_update, c->stream));
+  c->asm_fresh = false;
   if (timing) HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (st) {
@@ -1308,7 +1334,7 @@ int pd_ctx_destroy(pd_ctx* c) {
   dfree(c->act); dfree(c->body); dfree(c->bc); dfree(c->top); dfree(c->cap);
   dfree(c->slice_off); dfree(c->col); dfree(c->val); dfree(c->d0); dfree(c->diag); dfree(c->minv);
   dfree(c->r); dfree(c->z); dfree(c->p); dfree(c->y); dfree(c->part); dfree(c->st);
-  dfree(c->counters); dfree(c->dmat); dfree(c->dhorizon);
+  dfree(c->counters); dfree(c->dmat); dfree(c->dhorizon); dfree(c->rec);
   dfree(c->gap_val); dfree(c->gap_q); dfree(c->gpart_v); dfree(c->gpart_p); dfree(c->gpart_q);
   dfree(c->gap_out); dfree(c->gap_pq);
   dfree(c->keys_a); dfree(c->keys_b); dfree(c->idx_a); dfree(c->idx_b); dfree(c->hist);
@@ -1349,6 +1375,7 @@ int pd_upload(pd_ctx* c, int32_t field, const void* src, size_t bytes) {
         return fail(c, PD_ERR_ARG, "bond top outside [-1, capacity)");
   }
   if (field == PD_FIELD_ORIGINAL_POSITION) c->d0_valid = false;
+  c->asm_fresh = false;
   if (field == PD_FIELD_BODY) {
     const int32_t* bt = (const int32_t*)src;
     for (int32_t i = 0; i < c->n; ++i)
@@ -1437,6 +1464,7 @@ int pd_build_bonds(pd_ctx* c, pd_bond_stats* stats) {
   c->last_iters = 0;
   c->h_cap = counts;
   c->d0_valid = false;
+  c->asm_fresh = false;
   if (stats) {
     stats->total_bonds = total;
     stats->min_bonds = mn;
@@ -1491,6 +1519,7 @@ int pd_set_bonds(pd_ctx* c, const int32_t* counts, const int32_t* lists, int32_t
   c->last_iters = 0;
   c->h_cap = cap;
   c->d0_valid = false;
+  c->asm_fresh = false;
   return PD_OK;
 }
 
@@ -1672,6 +1701,7 @@ int pd_restore(pd_ctx* c) {
   if (n == 0) return PD_OK;
   HIPCHK(c, hipMemcpyAsync(c->top, c->top_bak, n * 4, hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->x, c->x_bak, n * 32, hipMemcpyDeviceToDevice, c->stream));
+  c->asm_fresh = false;
   HIPCHK(c, hipMemcpyAsync(c->v, c->v_bak, n * 32, hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return PD_OK;
@@ -1692,6 +1722,27 @@ int pd_spmv_bytes(pd_ctx* c, double* bytes) {
   }
   const double w = c->mat_prec == 1 ? 4.0 : 8.0;  // bytes per value and vector component
   *bytes = (double)c->live_bonds * (6.0 * w + 4.0) + (double)c->n * (6.0 * w + 4.0 + 2.0 * 3.0 * w);
+  return PD_OK;
+}
+
+int pd_check_division(pd_ctx* c, int64_t samples, uint64_t seed, int64_t* mismatches) {
+  if (!c) return fail(nullptr, PD_ERR_ARG, "null context");
+  if (!mismatches) return fail(c, PD_ERR_ARG, "null output");
+  if (samples < 0 || samples > ((int64_t)1 << 36)) return fail(c, PD_ERR_ARG, "samples outside [0, 2^36]");
+  HIPCHK(c, hipSetDevice(c->device));
+  unsigned long long* bad = nullptr;
+  if (int e = dalloc(c, &bad, 1)) return e;
+  unsigned long long h = 0;
+  hipError_t err = hipMemcpyAsync(bad, &h, sizeof(h), hipMemcpyHostToDevice, c->stream);
+  if (err == hipSuccess) {
+    pd::launch_check_division(samples, seed, bad, c->stream);
+    err = hipGetLastError();
+  }
+  if (err == hipSuccess) err = hipMemcpyAsync(&h, bad, sizeof(h), hipMemcpyDeviceToHost, c->stream);
+  if (err == hipSuccess) err = hipStreamSynchronize(c->stream);
+  (void)hipFree(bad);
+  if (err != hipSuccess) return fail(c, PD_ERR_HIP, std::string("division check: ") + hipGetErrorString(err));
+  *mismatches = (int64_t)h;
   return PD_OK;
 }
 

@@ -86,6 +86,11 @@ struct pd_step_consts {
 struct pd_step_counters {
   unsigned long long broken;
   unsigned long long live;
+  // nonzero: some position, reference length or bond stiffness of the
+  // assembled bonds lies outside the ranges of the shared-reciprocal
+  // quotients, so the matrix-free operator divides one quotient at a time
+  unsigned int range_bad;
+  unsigned int pad;
 };
 
 __device__ __forceinline__ double4 ld4(const double4* p) { return *p; }

@@ -74,6 +74,7 @@ struct AsmArgs {
   double* gap_val;   // [n] when track_gap
   int32_t* gap_q;    // [n] when track_gap
   pd_step_counters* counters;
+  double* rec;  // non-null: write the position and p parts of the gather record
 };
 
 struct SpmvView {
@@ -95,6 +96,13 @@ struct SpmvView {
   double dt2;
   const pd_step_consts* consts;
   const double* d0;  // mf: reference bond length per slot
+  // mf, both set only between an assembly and the next position change:
+  // range_bad non-null: the assembly's range flag (0: the bond quotients
+  // may share reciprocals); rec non-null: per particle the position and
+  // the CG direction p side by side (6 doubles), kept by the assembly and
+  // every kernel that writes p, so one gather serves both
+  const unsigned int* range_bad;
+  double* rec;
 };
 
 // Independent specimens solved together: specimen k owns the 256-row
@@ -294,6 +302,10 @@ void launch_update(int32_t n, double4* x, double4* v, const double4* sol,
                    const double* lo, const double* hi, double restitution,
                    const pd_step_consts* consts, hipStream_t s);
 void launch_copy(double* dst, const double* src, int32_t n, hipStream_t s);
+// Compares shared-reciprocal quotients with IEEE division on `samples`
+// random operand pairs; *bad counts the differing ones.
+void launch_check_division(int64_t samples, uint64_t seed, unsigned long long* bad,
+                           hipStream_t s);
 // d0[slot] = |x0_p - x0_col[slot]| for every slot below the row's capacity
 void launch_bond_d0(int32_t n, const int64_t* slice_off, const int32_t* cap, const int32_t* col,
                     const double4* x0, double* d0, hipStream_t s);

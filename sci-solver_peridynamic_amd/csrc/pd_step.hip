@@ -11,9 +11,51 @@
 
 namespace pd {
 
+// Division by a reciprocal shared between quotients.  gfx950 divides a / b
+// in FP64 as: bs = div_scale(b), as = div_scale(a); y = rcp(bs) refined by
+// two Newton steps y += y (1 - bs y); q = as y; the Markstein correction
+// fma(as - bs q, y, q) (div_fmas); div_fixup for special operands and the
+// sign.  When no operand needs scaling (div_scale leaves both unchanged and
+// div_fmas applies no 2^64 factor) and no special case applies, that is
+// exactly rcp_nr(b) then div_rcp(a, b, y) below, so quotients by one
+// denominator can share y.  No scaling happens when b and 1/b are normal,
+// a is 0 or |a| >= 2^-969, a / b is normal and exponent(a) - exponent(b) <
+// 768; a = 0 gives a zero of the quotient's sign (copysign restores it, as
+// div_fixup does).  range_ok_* state sufficient ranges; callers take this
+// path only when the assembly found every operand inside them.
+__device__ __forceinline__ double rcp_nr(double b) {
+  double y = __builtin_amdgcn_rcp(b);
+  double t = __builtin_fma(-b, y, 1.0);
+  y = __builtin_fma(y, t, y);
+  t = __builtin_fma(-b, y, 1.0);
+  y = __builtin_fma(y, t, y);
+  return y;
+}
+__device__ __forceinline__ double div_rcp(double a, double b, double y) {
+  const double q = a * y;
+  const double r = __builtin_fma(-b, q, a);
+  return __builtin_copysign(__builtin_fma(r, y, q), q);
+}
+// A coordinate 0 or of magnitude in [2^-500, 2^100]: a difference of two
+// such values is 0 or of magnitude in [2^-552, 2^101], so it divided by the
+// bond length fd = max(|d|, 1e-10) (at most 2^102, a float) needs no scaling.
+__device__ __forceinline__ bool range_ok_coord(double v) {
+  const double m = fabs(v);
+  return m == 0.0 || (m >= 0x1p-500 && m <= 0x1p100);
+}
+// c_bond / d0 with c_bond in [2^-300, 2^300] and d0 in [2^-400, 2^100].
+__device__ __forceinline__ bool range_ok_len(double d0) {
+  return d0 >= 0x1p-400 && d0 <= 0x1p100;
+}
+__device__ __forceinline__ bool range_ok_stiff(double c) {
+  return c >= 0x1p-300 && c <= 0x1p300;
+}
+
 // Current geometry of bond (p, q), cg_solver.cuh:253-256: the unit vector e
 // (the difference divided by the float-rounded length max(|d|, SMALL)) and
-// the current length dij.
+// the current length dij.  SHARED: the three quotients share one
+// reciprocal (identical results inside the ranges above).
+template <bool SHARED = false>
 __device__ __forceinline__ void bond_dir(const double4 pos_p, const double4 xq, double& e0,
                                          double& e1, double& e2, double& dij) {
   e0 = xq.x - pos_p.x;
@@ -23,9 +65,16 @@ __device__ __forceinline__ void bond_dir(const double4 pos_p, const double4 xq, 
   const float dij_f = (float)dij;
   const float small_f = (float)1.0e-10;
   const double fd = (double)(dij_f > small_f ? dij_f : small_f);
-  e0 /= fd;
-  e1 /= fd;
-  e2 /= fd;
+  if constexpr (SHARED) {
+    const double y = rcp_nr(fd);
+    e0 = div_rcp(e0, fd, y);
+    e1 = div_rcp(e1, fd, y);
+    e2 = div_rcp(e2, fd, y);
+  } else {
+    e0 /= fd;
+    e1 /= fd;
+    e2 /= fd;
+  }
 }
 
 // Reference length of bond (p, q), cg_solver.cuh:258
@@ -35,13 +84,19 @@ __device__ __forceinline__ double ref_len(const double4 opos_p, const double4 oq
 }
 
 // The bond's 3x3 Jacobian F (six distinct entries), cg_solver.cuh:292-299:
-// (c / d0) [(1 - d0/dij) I + (d0/dij) e e^T]
+// (c / d0) [(1 - d0/dij) I + (d0/dij) e e^T].  SHARED: c / d0 through
+// rcp_nr (identical results inside the ranges above).
+template <bool SHARED = false>
 __device__ __forceinline__ void bond_jac(double e0, double e1, double e2, double dij, double d0,
                                          double c_bond, double& F00, double& F01, double& F02,
                                          double& F11, double& F12, double& F22) {
   const double ratio = d0 / dij;
   const double dg = 1.0 - ratio;
-  const double sc = c_bond / d0;
+  double sc;
+  if constexpr (SHARED)
+    sc = div_rcp(c_bond, d0, rcp_nr(d0));
+  else
+    sc = c_bond / d0;
   F00 = (dg + (e0 * e0) * ratio) * sc;
   F01 = (0.0 + (e0 * e1) * ratio) * sc;
   F02 = (0.0 + (e0 * e2) * ratio) * sc;
@@ -72,6 +127,7 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int p = tile * PD_BLOCK + threadIdx.x;
   unsigned long long broken = 0, live = 0;
+  bool range_bad = false;
   double t_rr = 0.0, t_rz = 0.0, t_r32 = 0.0, t_bb = 0.0;  // fused residual partials
   if (p < a.n && a.act[p] != PD_INVISIBLE) {
     const pd_mat_dev mt = a.mat[a.body[p]];
@@ -79,6 +135,15 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
     const int lane = p & 63;
     const int64_t base = a.slice_off[s];
     const double4 pos_p = a.x[p];
+    // operands of the matrix-free operator's shared-reciprocal quotients
+    range_bad = !(range_ok_coord(pos_p.x) && range_ok_coord(pos_p.y) &&
+                  range_ok_coord(pos_p.z) && range_ok_stiff(mt.c_bond));
+    if (a.rec) {  // the matrix-free operator's gather record: position part
+      double* rp = a.rec + 6 * (int64_t)p;
+      rp[0] = pos_p.x;
+      rp[1] = pos_p.y;
+      rp[2] = pos_p.z;
+    }
     double f0 = 0.0, f1 = 0.0, f2 = 0.0;
     int top = a.top[p];
     const int otop = top;
@@ -123,6 +188,7 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
         if (a.fuse) us_n = a.xs[q_n];
       }
       if (bnd + 2 <= top) q_nn = a.col[(slot + 2) * 64 + lane];
+      range_bad |= !range_ok_len(d0);
       double e0, e1, e2, dij;
       bond_dir(pos_p, xq, e0, e1, e2, dij);
       const double drdivd = dij / d0 - 1.0;
@@ -334,6 +400,12 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
         }
         a.r64[p] = make_double4(r0, r1, r2, 0.0);
         a.p64[p] = make_double4(z0, z1, z2, 0.0);
+        if (a.rec) {
+          double* rp = a.rec + 6 * (int64_t)p;
+          rp[3] = z0;
+          rp[4] = z1;
+          rp[5] = z2;
+        }
         t_rz = (r0 * z0 + r1 * z1) + r2 * z2;
       }
     }
@@ -409,6 +481,7 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
     if (broken) atomicAdd(&a.counters->broken, broken);
     if (live) atomicAdd(&a.counters->live, live);
   }
+  if (range_bad) atomicOr(&a.counters->range_bad, 1u);
 }
 
 void launch_assemble(const AsmArgs& a, hipStream_t s) {
@@ -424,6 +497,7 @@ __global__ void k_reset_counters(pd_step_counters* c) {
   if (threadIdx.x == 0) {
     c->broken = 0;
     c->live = 0;
+    c->range_bad = 0;
   }
 }
 
@@ -597,8 +671,28 @@ __device__ __forceinline__ void spmv_row(const SpmvView& A, const double4* __res
 // step's positions (bond_dir), the slot's stored reference length and
 // bond_jac, and scaled as the assembly scales it, so the blocks, and the
 // products, equal the stored ones bit for bit (same summation order as
-// spmv_row).  Bound by FP64 arithmetic and its latency, not by HBM.
-template <int V, bool DIAG_LAST = false>
+// spmv_row).  Bound by the vector memory pipeline (gathers) and FP64
+// arithmetic, not by HBM.  REC: the neighbour's position and x come from
+// A.rec (x must be the CG direction p the record holds): three 16-byte
+// loads instead of four.  SHARED: quotients share reciprocals (the
+// assembly found every operand in range).
+typedef double dbl2v __attribute__((ext_vector_type(2)));
+
+template <bool REC>
+__device__ __forceinline__ void gather_mf(const SpmvView& A, const double4* __restrict__ x,
+                                          int q, double4& xq, double4& vq) {
+  if constexpr (REC) {
+    const dbl2v* r = (const dbl2v*)(A.rec + 6 * (int64_t)q);
+    const dbl2v r0 = r[0], r1 = r[1], r2 = r[2];
+    xq = make_double4(r0.x, r0.y, r1.x, 0.0);
+    vq = make_double4(r1.y, r2.x, r2.y, 0.0);
+  } else {
+    xq = A.pos[q];
+    vq = x[q];
+  }
+}
+
+template <int V, bool DIAG_LAST, bool REC, bool SHARED>
 __device__ __forceinline__ void spmv_row_mf(const SpmvView& A, const double4* __restrict__ x,
                                             int p, double& y0, double& y1, double& y2) {
   const int s = p >> 6;
@@ -632,17 +726,14 @@ __device__ __forceinline__ void spmv_row_mf(const SpmvView& A, const double4* __
       }
       double4 xq[V], vq[V];
 #pragma unroll
-      for (int u = 0; u < V; ++u) {
-        xq[u] = A.pos[q[u]];
-        vq[u] = x[q[u]];
-      }
+      for (int u = 0; u < V; ++u) gather_mf<REC>(A, x, q[u], xq[u], vq[u]);
       double t[V][3];
 #pragma unroll
       for (int u = 0; u < V; ++u) {
         double e0, e1, e2, dij;
-        bond_dir(pos_p, xq[u], e0, e1, e2, dij);
+        bond_dir<SHARED>(pos_p, xq[u], e0, e1, e2, dij);
         double F00, F01, F02, F11, F12, F22;
-        bond_jac(e0, e1, e2, dij, l0[u], cb, F00, F01, F02, F11, F12, F22);
+        bond_jac<SHARED>(e0, e1, e2, dij, l0[u], cb, F00, F01, F02, F11, F12, F22);
         blk6((F00 * -1.0) * dt2, (F01 * -1.0) * dt2, (F02 * -1.0) * dt2, (F11 * -1.0) * dt2,
              (F12 * -1.0) * dt2, (F22 * -1.0) * dt2, vq[u], t[u][0], t[u][1], t[u][2]);
       }
@@ -655,13 +746,16 @@ __device__ __forceinline__ void spmv_row_mf(const SpmvView& A, const double4* __
     }
     for (; bnd <= top; ++bnd) {
       const int q0 = ntl(cp + (int64_t)bnd * 64);
+      double4 xq, vq;
+      gather_mf<REC>(A, x, q0, xq, vq);
       double e0, e1, e2, dij;
-      bond_dir(pos_p, A.pos[q0], e0, e1, e2, dij);
+      bond_dir<SHARED>(pos_p, xq, e0, e1, e2, dij);
       double F00, F01, F02, F11, F12, F22;
-      bond_jac(e0, e1, e2, dij, ntl(lp + (int64_t)bnd * 64), cb, F00, F01, F02, F11, F12, F22);
+      bond_jac<SHARED>(e0, e1, e2, dij, ntl(lp + (int64_t)bnd * 64), cb, F00, F01, F02, F11,
+                       F12, F22);
       double t0, t1, t2;
       blk6((F00 * -1.0) * dt2, (F01 * -1.0) * dt2, (F02 * -1.0) * dt2, (F11 * -1.0) * dt2,
-           (F12 * -1.0) * dt2, (F22 * -1.0) * dt2, x[q0], t0, t1, t2);
+           (F12 * -1.0) * dt2, (F22 * -1.0) * dt2, vq, t0, t1, t2);
       y0 += t0;
       y1 += t1;
       y2 += t2;
@@ -677,14 +771,23 @@ __device__ __forceinline__ void spmv_row_mf(const SpmvView& A, const double4* __
 }
 
 // U > 0: every block stored, U slots per trip; U = 0: every block
-// recomputed, three slots per trip
-template <int U, bool DIAG_LAST = false>
+// recomputed, three slots per trip, with the gathers from the record when
+// REC and, when SHARED_OK and the assembly's range flag allows (a
+// wave-uniform choice), shared reciprocals.  The CG SpMV takes SHARED_OK;
+// in the plain SpMV and the solver start a kernel holding both quotient
+// variants ran 20 % slower than either alone, so those divide one quotient
+// at a time.
+template <int U, bool DIAG_LAST = false, bool REC = false, bool SHARED_OK = false>
 __device__ __forceinline__ void row_of(const SpmvView& A, const double4* __restrict__ x, int p,
                                        double& y0, double& y1, double& y2) {
-  if constexpr (U == 0)
-    spmv_row_mf<3, DIAG_LAST>(A, x, p, y0, y1, y2);
-  else
+  if constexpr (U == 0) {
+    if (SHARED_OK && A.range_bad && *A.range_bad == 0)
+      spmv_row_mf<3, DIAG_LAST, REC, true>(A, x, p, y0, y1, y2);
+    else
+      spmv_row_mf<3, DIAG_LAST, REC, false>(A, x, p, y0, y1, y2);
+  } else {
     spmv_row<U, DIAG_LAST>(A, x, p, y0, y1, y2);
+  }
 }
 
 template <int U>
@@ -697,6 +800,52 @@ __global__ __launch_bounds__(PD_BLOCK) void k_spmv_plain(SpmvView A,
   double y0, y1, y2;
   row_of<U>(A, x, p, y0, y1, y2);
   y[p] = make_double4(y0, y1, y2, 0.0);
+}
+
+// Self-check of the shared-reciprocal quotients against IEEE division over
+// random operands spanning the ranges the range flag admits (odd samples:
+// a coordinate difference over a float bond length; even samples: a bond
+// stiffness over a reference length); the exponent ends are drawn often.
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z += 0x9e3779b97f4a7c15ull;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+__device__ __forceinline__ double draw_f64(uint64_t h, int elo, int ehi) {
+  const int span = ehi - elo + 1;
+  const int pick = (int)((h >> 52) & 7);
+  const int e = pick == 0 ? elo : pick == 1 ? ehi : elo + (int)((h >> 12) % (uint64_t)span);
+  const uint64_t bits = ((h >> 63) << 63) | ((uint64_t)(e + 1023) << 52) |
+                        (mix64(h) & 0xfffffffffffffull);
+  return __longlong_as_double((long long)bits);
+}
+__global__ __launch_bounds__(PD_BLOCK) void k_check_division(int64_t samples, uint64_t seed,
+                                                             unsigned long long* bad) {
+  const int64_t i = (int64_t)blockIdx.x * PD_BLOCK + threadIdx.x;
+  if (i >= samples) return;
+  const uint64_t h1 = mix64(seed ^ (2 * (uint64_t)i)), h2 = mix64(seed ^ (2 * (uint64_t)i + 1));
+  double a, b;
+  if (i & 1) {
+    a = (h1 & 15) == 0 ? ((h1 >> 4) & 1 ? -0.0 : 0.0) : draw_f64(h1, -552, 100);
+    const float small_f = (float)1.0e-10;
+    float bf = (h2 & 15) == 0 ? small_f : fabsf((float)draw_f64(h2, -34, 101));
+    if (!(bf >= small_f)) bf = small_f;
+    b = (double)bf;
+  } else {
+    a = fabs(draw_f64(h1, -300, 299));
+    b = fabs(draw_f64(h2, -400, 99));
+  }
+  const double ieee = a / b;
+  const double shared = div_rcp(a, b, rcp_nr(b));
+  if (__double_as_longlong(ieee) != __double_as_longlong(shared)) atomicAdd(bad, 1ull);
+}
+
+void launch_check_division(int64_t samples, uint64_t seed, unsigned long long* bad,
+                           hipStream_t s) {
+  const int64_t nb = (samples + PD_BLOCK - 1) / PD_BLOCK;
+  if (nb > 0) hipLaunchKernelGGL(k_check_division, dim3((unsigned)nb), dim3(PD_BLOCK), 0, s,
+                                 samples, seed, bad);
 }
 
 void launch_spmv_plain(const SpmvView& A, const double4* x, double4* y, hipStream_t s) {
@@ -769,6 +918,16 @@ void launch_cg_setup(pd_cg_state* st, int32_t nspec, double rtol, int32_t max_it
   hipLaunchKernelGGL(k_cg_setup, dim3(1), dim3(256), 0, s, st, nspec, rtol, max_iter);
 }
 
+// The CG direction's part of the matrix-free gather record (rec non-null)
+__device__ __forceinline__ void rec_p(double* rec, int p, double z0, double z1, double z2) {
+  if (rec) {
+    double* rp = rec + 6 * (int64_t)p;
+    rp[3] = z0;
+    rp[4] = z1;
+    rp[5] = z2;
+  }
+}
+
 // y = A x0; r = S(b - y); z = M r; p = z; partials <r,z>, <r,r>, <Sb,Sb>
 template <int U>
 __global__ __launch_bounds__(PD_BLOCK) void k_cg_init(CgArgs a) {
@@ -790,6 +949,7 @@ __global__ __launch_bounds__(PD_BLOCK) void k_cg_init(CgArgs a) {
     precond(a.minv, a.precond, p, r0, r1, r2, z0, z1, z2);
     a.r[p] = make_double4(r0, r1, r2, 0.0);
     a.p[p] = make_double4(z0, z1, z2, 0.0);
+    rec_p(a.A.rec, p, z0, z1, z2);
     t_rz = term3(r0, r1, r2, z0, z1, z2);
     t_rr = term3(r0, r1, r2, r0, r1, r2);
     t_bb = term3(sb0, sb1, sb2, sb0, sb1, sb2);
@@ -843,7 +1003,10 @@ __global__ __launch_bounds__(PD_BLOCK) void k_cg_spmv(CgArgs a) {
   double t = 0.0;
   if (p < a.A.n) {
     double y0, y1, y2;
-    row_of<U>(a.A, a.p, p, y0, y1, y2);
+    if (U == 0 && a.A.rec)
+      row_of<U, false, true, true>(a.A, a.p, p, y0, y1, y2);
+    else
+      row_of<U, false, false, true>(a.A, a.p, p, y0, y1, y2);
     a.y[p] = make_double4(y0, y1, y2, 0.0);
     const double4 pp = a.p[p];
     t = term3(y0, y1, y2, pp.x, pp.y, pp.z);
@@ -947,6 +1110,7 @@ __global__ __launch_bounds__(PD_BLOCK) void k_cg_dir(CgArgs a) {
   pp.y = zz.y + beta * pp.y;
   pp.z = zz.z + beta * pp.z;
   a.p[p] = pp;
+  rec_p(a.A.rec, p, pp.x, pp.y, pp.z);
 }
 
 static inline int tiles_of(int n) { return (n + PD_BLOCK - 1) / PD_BLOCK; }

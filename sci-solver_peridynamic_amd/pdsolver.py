@@ -152,6 +152,7 @@ def lib():
         L.pd_synchronize.argtypes = [vp]
         L.pd_bench_spmv.argtypes = [vp, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.pd_spmv_bytes.argtypes = [vp, C.POINTER(C.c_double)]
+        L.pd_check_division.argtypes = [vp, C.c_int64, C.c_uint64, C.POINTER(C.c_int64)]
         L.pd_set_specimens.argtypes = [vp, C.c_int32, vp, vp]
         L.pd_tune_spmv.argtypes = [vp, C.c_int32, vp, C.POINTER(C.c_int32)]
         L.pd_learned_set.argtypes = [vp, vp, C.c_int32]
@@ -169,7 +170,7 @@ def lib():
                      "pd_build_bonds", "pd_set_bonds", "pd_get_bonds", "pd_get_matrix",
                      "pd_step", "pd_assemble", "pd_solve", "pd_update", "pd_commit_stretch",
                      "pd_spmv", "pd_backup", "pd_restore", "pd_synchronize", "pd_bench_spmv",
-                     "pd_spmv_bytes", "pd_set_specimens", "pd_get_specimen_stats",
+                     "pd_spmv_bytes", "pd_check_division", "pd_set_specimens", "pd_get_specimen_stats",
                      "pd_tune_spmv", "pd_learned_set", "pd_get_precond", "pd_trainer_create",
                      "pd_trainer_destroy", "pd_trainer_add", "pd_trainer_fit",
                      "pd_trainer_params", "pd_trainer_samples"):
@@ -339,6 +340,13 @@ class Context:
         by = C.c_double()
         self._check(lib().pd_bench_spmv(self._h, iters, C.byref(ms), C.byref(by)))
         return ms.value, by.value
+
+    def check_division(self, samples, seed=1):
+        """Bitwise mismatches of the matrix-free operator's shared-reciprocal
+        quotients against IEEE division on `samples` random operand pairs."""
+        bad = C.c_int64()
+        self._check(lib().pd_check_division(self._h, samples, seed, C.byref(bad)))
+        return bad.value
 
     def spmv_bytes(self):
         by = C.c_double()

@@ -90,6 +90,46 @@ def test_mf_steps_equal_stored_and_oracle(mode, integrator):
     stored.close()
 
 
+def test_shared_reciprocal_division_matches_ieee():
+    """The operator's quotients that share one reciprocal equal IEEE division
+    bit for bit over the operand ranges its range flag admits."""
+    say("case shared-reciprocal division")
+    _, ctx = deformed_pair(nx=4, ny=4, nz=4)
+    for seed in (1, 2, 3):
+        assert ctx.check_division(1 << 24, seed) == 0, seed
+    ctx.close()
+
+
+@pytest.mark.parametrize("case", ["aligned", "tiny_coordinate"])
+def test_mf_range_cases_equal_stored(case):
+    """An unjittered lattice (coordinate differences exactly zero, quotients
+    through shared reciprocals) and a coordinate of 1e-200 (outside the
+    shared-reciprocal range: the assembly's flag sends every quotient
+    through IEEE division): the matrix-free steps equal the stored ones and
+    the oracle."""
+    say("case mf range", case)
+    x4, bc, vbc, v4 = config0_state(6, 6, 8, jitter=0.0 if case == "aligned" else 0.05)
+    if case == "tiny_coordinate":
+        x4[0, 0] = 1e-200
+    mats = [material(0.01)]
+    sys_, mf = make_pair(x4, mats, v4=v4, bc=bc, vbc=vbc)
+    _, stored = make_pair(x4, mats, v4=v4, bc=bc, vbc=vbc)
+    for k in range(1, 13):
+        kw = dict(dt=1e-4, clock_scale=min(k, 30) / 30.0, precond=1, rtol=1e-8)
+        so = sys_.step(orc.make_params(**kw))
+        ss = stored.step(pds.make_params(**kw))
+        sm = mf.step(pds.make_params(matrix_free=1, **kw))
+        assert (sm.cg_iterations, sm.broken_bonds, sm.residual_norm) == \
+            (ss.cg_iterations, ss.broken_bonds, ss.residual_norm), k
+        assert sm.cg_iterations == so.cg_iterations, k
+        xm = mf.download("position")
+        assert np.array_equal(xm, stored.download("position")), k
+        rel = np.abs(xm[:, :3] - sys_.x[:, :3]).max() / np.abs(sys_.x[:, :3]).max()
+        assert rel <= 1e-9, (k, rel)
+    mf.close()
+    stored.close()
+
+
 def batch_context():
     desc = pds.describe(BATCH)
     sc = pds.scene(BATCH)
