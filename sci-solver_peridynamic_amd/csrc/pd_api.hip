@@ -1585,9 +1585,16 @@ int pd_get_matrix(pd_ctx* c, double* diag9, double* vals9, int32_t stride) {
     unpack(dg.data() + (size_t)s * 384 + lane, 64, diag9 + 9 * (size_t)p);
     if (top[p] + 1 > stride) return fail(c, PD_ERR_ARG, "stride smaller than a row");
     const int64_t base = c->h_slice_off[s];
-    for (int32_t k = 0; k <= top[p]; ++k)
-      unpack(vv.data() + (size_t)(base + k) * 384 + lane, 64,
-             vals9 + 9 * ((size_t)p * stride + k));
+    for (int32_t k = 0; k <= top[p]; ++k) {
+      double* out = vals9 + 9 * ((size_t)p * stride + k);
+      if (c->mat_prec == 1) {  // float component planes
+        unpack(vv.data() + (size_t)(base + k) * 384 + lane, 64, out);
+      } else {  // FP64 planes of pairs: value i at (i / 2) * 128 + 2 lane + i % 2
+        const double* sl = vv.data() + (size_t)(base + k) * 384 + 2 * lane;
+        const double six[6] = {sl[0], sl[1], sl[128], sl[129], sl[256], sl[257]};
+        unpack(six, 1, out);
+      }
+    }
   }
   return PD_OK;
 }

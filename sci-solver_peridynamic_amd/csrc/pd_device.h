@@ -8,8 +8,11 @@
 //     row p = 64 s + lane lives at (slice_off[s] + k) * 64 + lane, so the
 //     64 lanes of a wave touch 64 consecutive words per slot;
 //   * matrix values: the 3x3 blocks are symmetric bit for bit, so six
-//     values (a00 a01 a02 a11 a12 a22) are stored in component planes:
-//     value c of slot k of row p at ((slice_off[s] + k) * 6 + c) * 64 + lane;
+//     values (a00 a01 a02 a11 a12 a22) are stored.  FP64: in three planes
+//     of pairs, so a lane reads 16 bytes per load: value c of slot k of row
+//     p at ((slice_off[s] + k) * 3 + c / 2) * 128 + 2 * lane + c % 2.
+//     Float copy (val32): in component planes, value c at
+//     ((slice_off[s] + k) * 6 + c) * 64 + lane;
 //   * diagonal and block-Jacobi inverse blocks: (s * 6 + c) * 64 + lane;
 //   * mixed precision keeps float copies in the same positions (val32,
 //     diag32, minv32) and float4 vectors (16 B) for the inner solve.

@@ -11,6 +11,9 @@
 
 namespace pd {
 
+// two FP64 block values of one lane (the pair planes of val)
+typedef double vpair __attribute__((ext_vector_type(2)));
+
 // Division by a reciprocal shared between quotients.  gfx950 divides a / b
 // in FP64 as: bs = div_scale(b), as = div_scale(a); y = rcp(bs) refined by
 // two Newton steps y += y (1 - bs y); q = as y; the Markstein correction
@@ -262,13 +265,11 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
                      b02 = (F02 * -1.0) * a.dt2, b11 = (F11 * -1.0) * a.dt2,
                      b12 = (F12 * -1.0) * a.dt2, b22 = (F22 * -1.0) * a.dt2;
         if (a.store_blocks) {
-          double* vo = a.val + slot * 6 * 64 + lane;
-          __builtin_nontemporal_store(b00, vo + 0 * 64);
-          __builtin_nontemporal_store(b01, vo + 1 * 64);
-          __builtin_nontemporal_store(b02, vo + 2 * 64);
-          __builtin_nontemporal_store(b11, vo + 3 * 64);
-          __builtin_nontemporal_store(b12, vo + 4 * 64);
-          __builtin_nontemporal_store(b22, vo + 5 * 64);
+          // three planes of value pairs, 16 bytes per lane
+          vpair* vo = (vpair*)(a.val + slot * 6 * 64) + lane;
+          __builtin_nontemporal_store(vpair{b00, b01}, vo);
+          __builtin_nontemporal_store(vpair{b02, b11}, vo + 64);
+          __builtin_nontemporal_store(vpair{b12, b22}, vo + 128);
         }
         if (a.fuse) {
           rb0 += b00 * u.x + b01 * u.y + b02 * u.z;
@@ -623,7 +624,8 @@ __device__ __forceinline__ void spmv_row(const SpmvView& A, const double4* __res
   const int64_t base = A.slice_off[s];
   const int top = A.top[p];
   const int* __restrict__ cp = A.col + base * 64 + lane;
-  const double* __restrict__ vp = A.val + base * 6 * 64 + lane;
+  // three planes of value pairs per slot (16 bytes per lane and load)
+  const vpair* __restrict__ vp = (const vpair*)(A.val + base * 6 * 64) + lane;
   int bnd = 0;
   for (; bnd + U - 1 <= top; bnd += U) {
     int q[U];
@@ -632,9 +634,13 @@ __device__ __forceinline__ void spmv_row(const SpmvView& A, const double4* __res
     for (int u = 0; u < U; ++u) q[u] = ntl(cp + (int64_t)(bnd + u) * 64);
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const double* vv = vp + (int64_t)(bnd + u) * 384;
+      const vpair* vv = vp + (int64_t)(bnd + u) * 192;
 #pragma unroll
-      for (int c = 0; c < 6; ++c) v[u][c] = ntl(vv + c * 64);
+      for (int c = 0; c < 3; ++c) {
+        const vpair t = __builtin_nontemporal_load(vv + c * 64);
+        v[u][2 * c] = t.x;
+        v[u][2 * c + 1] = t.y;
+      }
     }
     double4 xx[U];
 #pragma unroll
@@ -650,10 +656,11 @@ __device__ __forceinline__ void spmv_row(const SpmvView& A, const double4* __res
   }
   for (; bnd <= top; ++bnd) {
     const int q0 = ntl(cp + (int64_t)bnd * 64);
-    const double* v0 = vp + (int64_t)bnd * 384;
+    const vpair* v0 = vp + (int64_t)bnd * 192;
+    const vpair a0 = __builtin_nontemporal_load(v0), a1 = __builtin_nontemporal_load(v0 + 64),
+                a2 = __builtin_nontemporal_load(v0 + 128);
     double t0, t1, t2;
-    blk6(ntl(v0), ntl(v0 + 64), ntl(v0 + 128), ntl(v0 + 192), ntl(v0 + 256), ntl(v0 + 320),
-         x[q0], t0, t1, t2);
+    blk6(a0.x, a0.y, a1.x, a1.y, a2.x, a2.y, x[q0], t0, t1, t2);
     y0 += t0;
     y1 += t1;
     y2 += t2;
