@@ -14,9 +14,11 @@ baseline (the CPU oracle on a bounded sample of the same workload).
 
 The FP64 line also carries a "mixed_precision" object (the same run with the
 float inner CG and FP64 refinement, its SpMV roofline, and its per-step
-position error against the FP64 path) and a "batched_specimens" object
-(64 independent tension specimens in one context, aggregate particle-steps/s,
-hipGraph-captured and eager).
+position error against the FP64 path), a "matrix_free" object (the same FP64
+run with the blocks recomputed in every SpMV: identical results), the learned
+preconditioner rows and a "batched_specimens" object (64 independent tension
+specimens in one context, aggregate particle-steps/s: FP64 with the
+matrix-free operator, and the mixed solver captured and eager).
 """
 from __future__ import annotations
 
@@ -154,9 +156,10 @@ def gpu_run(steps, warmup, precision=0, copy_calibration=0, precond=None, networ
 BATCH_CONFIG = os.path.join(PKG, "configs", "batch_64.txt")
 
 
-def batched_run(steps, warmup, graph):
-    """batch_64: 64 independent tension specimens in one context, float
-    inner CG with FP64 refinement; aggregate particle-steps/s over the
+def batched_run(steps, warmup, graph, **over):
+    """batch_64: 64 independent tension specimens in one context (the
+    config's float inner CG with FP64 refinement unless `over` sets
+    precision / matrix_free); aggregate particle-steps/s over the
     specimens' particles (padding rows excluded)."""
     desc = pds.describe(BATCH_CONFIG)
     sc = pds.scene(BATCH_CONFIG)
@@ -172,14 +175,16 @@ def batched_run(steps, warmup, graph):
     step = 0
     for _ in range(warmup):
         step += 1
-        ctx.step(pds.make_params(**dict(pds.step_kwargs(desc, step), graph=graph)))
+        ctx.step(pds.make_params(**dict(pds.step_kwargs(desc, step), graph=graph, **over)))
     ctx.synchronize()
-    tune_ms, unroll = ctx.tune_spmv(10)
+    # the matrix-free operator has no stored-block variants to tune
+    tune_ms, unroll = (ctx.tune_spmv(10) if not over.get("matrix_free")
+                       else (np.zeros(3), 0))
     its, spec_its, broken, syncs = [], [], 0, 0
     t0 = time.perf_counter()
     for _ in range(steps):
         step += 1
-        st = ctx.step(pds.make_params(**dict(pds.step_kwargs(desc, step), graph=graph)))
+        st = ctx.step(pds.make_params(**dict(pds.step_kwargs(desc, step), graph=graph, **over)))
         its.append(st.cg_iterations)
         broken += st.broken_bonds
         syncs += st.host_syncs
@@ -188,7 +193,7 @@ def batched_run(steps, warmup, graph):
     per = ctx.specimen_stats(nspec)
     spec_its = [p["cg_iterations"] for p in per]
     ctx.close()
-    log(f"[batch graph={graph}] {nspec} specimens, {real} particles ({n} rows), "
+    log(f"[batch graph={graph} {over}] {nspec} specimens, {real} particles ({n} rows), "
         f"{steps / elapsed:.1f} steps/s")
     return dict(specimens=nspec, particles=real, rows=n, bonds=int(bst.total_bonds),
                 steps=steps, elapsed=elapsed, value=real * steps / elapsed,
@@ -443,19 +448,21 @@ def main():
         }
     if not a.no_batched:
         bw = max(1, min(a.warmup, 5))
-        eager = batched_run(95, bw, 0)
+        mf = batched_run(95, bw, 1, precision=0, matrix_free=1)
         graph = batched_run(95, bw, 1)
+        eager = batched_run(95, bw, 0)
         out["batched_specimens"] = {
             "metric": "aggregate particle-steps/s",
-            "value": graph["value"],
+            "value": mf["value"],
             "unit": "particle-steps/s",
             "workload": "batch_64: 64 independent tension specimens in one context, counts "
                         "log-uniform in [5000, 60000], aspect 1..4, s0 U[0.01, 0.03], dt 1e-4 s, "
-                        "block-Jacobi CG rtol 1e-6 with float inner solve and FP64 refinement, "
-                        "one launch per phase for all specimens, hipGraph-captured steps, "
-                        "autotuned SpMV slots per trip",
-            "graph": graph,
-            "eager": eager,
+                        "block-Jacobi CG rtol 1e-6, one launch per phase for all specimens, "
+                        "hipGraph-captured steps; value: FP64 with the matrix-free operator",
+            "fp64_matrix_free_graph": mf,
+            "mixed_graph": dict(graph, solver="float inner CG with FP64 refinement, "
+                                              "autotuned SpMV slots per trip"),
+            "mixed_eager": eager,
         }
     if not a.no_cpu_baseline:
         log("cpu baseline: CPU oracle on the same workload ...")
