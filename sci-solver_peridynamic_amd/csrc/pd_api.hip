@@ -334,7 +334,7 @@ int read_bbox(pd_ctx* c, int32_t sel, double* out6) {
 
 // bounding boxes of bodies 0 .. nbody - 1, one pass and one wait
 int read_bboxes(pd_ctx* c, double (*out)[6]) {
-  pd::launch_bbox_bodies(c->x, c->body, c->n, c->bbox_part, c->bbox, c->stream);
+  pd::launch_bbox_bodies(c->x, c->body, c->n, c->nbody, c->bbox_part, c->bbox, c->stream);
   CHECK_LAUNCH(c, "bbox");
   HIPCHK(c, hipMemcpyAsync(c->h_bbox, c->bbox, 6 * PD_MAX_BODIES * sizeof(double),
                            hipMemcpyDeviceToHost, c->stream));
@@ -1290,7 +1290,7 @@ int pd_ctx_create(const pd_config* cfg, pd_ctx** out) {
   ALLOC(c->idx_a, n);
   ALLOC(c->idx_b, n);
   ALLOC(c->hist, pd::radix_hist_words(c->n));
-  ALLOC(c->bbox_part, 256 * 6 * PD_MAX_BODIES);
+  ALLOC(c->bbox_part, 1024 * 6 * PD_MAX_BODIES);
   ALLOC(c->bbox, 6 * PD_MAX_BODIES);
   ALLOC(c->counts, n);
   ALLOC(c->zone_cnt, pd::zone_tile_words(c->n));

@@ -68,13 +68,15 @@ __global__ __launch_bounds__(PD_BLOCK) void k_bbox_final(const double* part, int
 }
 
 static const int kBboxBlocks = 256;
+// all-body boxes: four workgroups per CU keep the position stream in flight
+static const int kBodyBboxBlocks = 1024;
 
 // Bounding boxes of all bodies in one pass: per block (lo xyz, hi xyz) for
 // each body b < PD_MAX_BODIES at part[(block * PD_MAX_BODIES + b) * 6 + k].
 // min / max are exact, so the result does not depend on the order.
 __global__ __launch_bounds__(PD_BLOCK) void k_bbox_bodies_partial(const double4* x,
                                                                  const int32_t* body, int n,
-                                                                 double* part) {
+                                                                 int nbody, double* part) {
   double v[PD_MAX_BODIES][6];
 #pragma unroll
   for (int b = 0; b < PD_MAX_BODIES; ++b)
@@ -97,7 +99,8 @@ __global__ __launch_bounds__(PD_BLOCK) void k_bbox_bodies_partial(const double4*
   __shared__ double s[PD_BLOCK / 64][PD_MAX_BODIES * 6];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
-  for (int b = 0; b < PD_MAX_BODIES; ++b)
+  for (int b = 0; b < PD_MAX_BODIES; ++b) {
+    if (b >= nbody) break;  // uniform: only the bodies that exist are folded
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
       double t = v[b][k];
@@ -108,9 +111,10 @@ __global__ __launch_bounds__(PD_BLOCK) void k_bbox_bodies_partial(const double4*
       }
       if (lane == 0) s[wave][b * 6 + k] = t;
     }
+  }
   __syncthreads();
   const int t = threadIdx.x;
-  if (t < PD_MAX_BODIES * 6) {
+  if (t < nbody * 6) {
     const bool lo = (t % 6) < 3;
     double r = s[0][t];
     for (int w = 1; w < PD_BLOCK / 64; ++w) r = lo ? fmin(r, s[w][t]) : fmax(r, s[w][t]);
@@ -118,15 +122,20 @@ __global__ __launch_bounds__(PD_BLOCK) void k_bbox_bodies_partial(const double4*
   }
 }
 
-// one thread per block partial (g <= PD_BLOCK), every value folded by wave
-// butterflies and across the four waves
+// each thread folds the block partials t, t + 256, ... (g <= 4 PD_BLOCK),
+// then every value of the bodies that exist is folded by wave butterflies
+// and across the four waves; the other bodies' entries are left unwritten
 __global__ __launch_bounds__(PD_BLOCK) void k_bbox_bodies_final(const double* part, int g,
-                                                               double* out) {
+                                                               int nbody, double* out) {
   __shared__ double s[PD_BLOCK / 64][PD_MAX_BODIES * 6];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  for (int k = 0; k < PD_MAX_BODIES * 6; ++k) {
+  for (int k = 0; k < nbody * 6; ++k) {
     const bool lo = (k % 6) < 3;
-    double r = t < g ? part[t * PD_MAX_BODIES * 6 + k] : (lo ? INFINITY : -INFINITY);
+    double r = lo ? INFINITY : -INFINITY;
+    for (int i = t; i < g; i += PD_BLOCK) {
+      const double v = part[i * PD_MAX_BODIES * 6 + k];
+      r = lo ? fmin(r, v) : fmax(r, v);
+    }
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) {
       const double o = __shfl_xor(r, m, 64);
@@ -135,7 +144,7 @@ __global__ __launch_bounds__(PD_BLOCK) void k_bbox_bodies_final(const double* pa
     if (lane == 0) s[wave][k] = r;
   }
   __syncthreads();
-  if (t < PD_MAX_BODIES * 6) {
+  if (t < nbody * 6) {
     const bool lo = (t % 6) < 3;
     double r = s[0][t];
     for (int w = 1; w < PD_BLOCK / 64; ++w) r = lo ? fmin(r, s[w][t]) : fmax(r, s[w][t]);
@@ -143,11 +152,12 @@ __global__ __launch_bounds__(PD_BLOCK) void k_bbox_bodies_final(const double* pa
   }
 }
 
-void launch_bbox_bodies(const double4* x, const int32_t* body, int32_t n, double* part,
-                        double* out, hipStream_t s) {
-  hipLaunchKernelGGL(k_bbox_bodies_partial, dim3(kBboxBlocks), dim3(PD_BLOCK), 0, s, x, body, n,
-                     part);
-  hipLaunchKernelGGL(k_bbox_bodies_final, dim3(1), dim3(PD_BLOCK), 0, s, part, kBboxBlocks, out);
+void launch_bbox_bodies(const double4* x, const int32_t* body, int32_t n, int32_t nbody,
+                        double* part, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_bbox_bodies_partial, dim3(kBodyBboxBlocks), dim3(PD_BLOCK), 0, s, x,
+                     body, n, nbody, part);
+  hipLaunchKernelGGL(k_bbox_bodies_final, dim3(1), dim3(PD_BLOCK), 0, s, part, kBodyBboxBlocks,
+                     nbody, out);
 }
 
 void launch_bbox(const double4* x, const int32_t* body, int32_t n, int32_t body_sel,
@@ -161,7 +171,7 @@ void launch_bbox(const double4* x, const int32_t* body, int32_t n, int32_t body_
 // Stable selection of the particles inside any zone box of another body:
 // per-tile counts, an exclusive scan of the counts, then each tile writes
 // its selected indices in ascending order.
-#define CP_ITEMS 16
+#define CP_ITEMS 4
 #define CP_TILE (PD_BLOCK * CP_ITEMS)
 
 __device__ __forceinline__ bool in_zone(const double4 p, int b, const ZoneBoxes& z) {

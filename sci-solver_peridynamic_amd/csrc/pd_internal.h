@@ -332,10 +332,11 @@ struct ZoneBoxes {
 
 // pd_bonds.hip
 // bounds of the particles of body `body_sel` (all particles if < 0)
-// the bounding boxes of all bodies: out[b * 6 + k] (lo xyz, hi xyz; +inf /
-// -inf for bodies without particles); part holds 256 * PD_MAX_BODIES * 6
-void launch_bbox_bodies(const double4* x, const int32_t* body, int32_t n, double* part,
-                        double* out, hipStream_t s);
+// the bounding boxes of bodies 0 .. nbody - 1: out[b * 6 + k] (lo xyz, hi
+// xyz; +inf / -inf for bodies without particles); part holds
+// 1024 * PD_MAX_BODIES * 6
+void launch_bbox_bodies(const double4* x, const int32_t* body, int32_t n, int32_t nbody,
+                        double* part, double* out, hipStream_t s);
 void launch_bbox(const double4* x, const int32_t* body, int32_t n, int32_t body_sel,
                  double* part, double* out6, hipStream_t s);
 // stable (ascending) list of contact candidates; the count lands in
