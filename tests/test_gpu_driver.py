@@ -103,12 +103,15 @@ def last_frame(out):
     return np.fromfile(os.path.join(d, f"frame.{k}")).reshape(-1, 4)
 
 
+@pytest.mark.parametrize("operator", ["stored", "matrix_free"])
 @pytest.mark.parametrize("integrator", ["implicit_euler", "newmark_beta"])
-def test_adaptive_step_control_matches_oracle(integrator):
-    say("case adaptive", integrator)
+def test_adaptive_step_control_matches_oracle(integrator, operator):
+    """Rollbacks re-assemble from restored positions; the matrix-free
+    operator must follow them exactly as the stored blocks do."""
+    say("case adaptive", integrator, operator)
     steps = 60
     text = edited(final_step=steps - 1, adaptive_integration=1, integrator=integrator,
-                  steps_per_frame=1000)
+                  steps_per_frame=1000, cg_operator=operator)
     desc = pds.describe(CFG)
     desc["integrator"] = 0 if integrator == "implicit_euler" else 1
     with tempfile.TemporaryDirectory() as out:
