@@ -268,6 +268,9 @@ def mf_roofline(r):
     achieved = flops / (avg_ms * 1e-3) / 1e12
     return {
         "kernel": "k_cg_spmv<0> (matrix-free CG SpMV: blocks recomputed from positions)",
+        # FP64 arithmetic and the gathers' latency share the limit (counters:
+        # VALU about 70 % busy, texture units about 55 %); the FP64 peak is the
+        # arithmetic ceiling it is quoted against
         "bound": "fp64-valu",
         "achieved": achieved,
         "peak": FP64_PEAK_TFLOPS,
