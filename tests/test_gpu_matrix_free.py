@@ -180,4 +180,8 @@ def test_mf_argument_and_state_errors():
     ctx.assemble(pds.make_params())  # stored blocks again
     diag, vals = ctx.get_matrix()
     assert np.isfinite(vals).all()
+    with pytest.raises(pds.PDError) as e:
+        ctx.check_division(-1)
+    assert e.value.code == pds.PD_ERR_ARG
+    assert ctx.check_division(0) == 0
     ctx.close()
