@@ -253,6 +253,24 @@ def spmv_roofline(r, kernel, traffic):
     }
 
 
+def step_roofline(r):
+    """Whole-step algorithmic HBM bytes over the step time, stored FP64
+    blocks: B_step = B_asm + iterations x (B_spmv + B_vec) + N x 96 with
+    B_asm = live blocks x (6 x 8 + 4) + N x (24 + 24 + 8 + 8 + 24 + 8) (blocks
+    and columns written / read; x, x0, s, s00 read, f, s_new written) and
+    B_vec = N x (10 x 24 + 72) per iteration (the fused CG vector work)."""
+    n, live = r["n"], r["live"]
+    its = float(np.mean(r["iters"]))
+    b_asm = live * (6 * 8 + 4) + n * (24 + 24 + 8 + 8 + 24 + 8)
+    b_vec = n * (10 * 24 + 72)
+    b_step = b_asm + its * (r["spmv_bytes"] + b_vec) + n * 96
+    t_step = r["elapsed"] / r["steps"]
+    achieved = b_step / t_step / 1e9
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "bytes_per_step": b_step,
+            "ms_per_step": 1e3 * t_step, "cg_iterations_mean": its}
+
+
 # FP64 operations per live bond of the matrix-free SpMV (bond_dir 12,
 # bond_jac 27, the dt^2 scaling 12, the 3x3 product 15, the row sum 3;
 # a division or square root counts as one) and per row (diagonal product
@@ -388,6 +406,8 @@ def main():
         },
         "roofline": mf_roofline(r) if mf else spmv_roofline(r, kernel, read_traffic(prec)),
     }
+    if not prec and not mf:
+        out["step_roofline"] = step_roofline(r)
     if prec:
         out["config"]["refinements_mean"] = float(np.mean(r["refinements"]))
         out["config"]["resid_spmv_ms_mean"] = r["resid_ms"] / max(r["resid_n"], 1)
