@@ -153,13 +153,17 @@ typedef struct pd_step_stats {
   double ms_assemble;
   double ms_solve;
   double ms_update;
-  double ms_spmv_total; /* sum over the CG SpMV launches that ran */
+  double ms_spmv_total; /* sum over the CG SpMV launches that ran, streaming
+                           stored blocks or recomputing them (matrix-free) */
   int32_t spmv_launches;
   int32_t host_syncs;
   int32_t refinements;    /* precision 1: FP64 corrections applied */
   double ms_resid_total;  /* precision 1: FP64 residual launches that ran */
   int32_t resid_launches;
   int32_t precond_fallbacks; /* precond 2: 1 if block Jacobi took over */
+  /* stored-block FP64 steps: the solve's first product, which also writes
+   * the bond blocks the assembly left to it (not in ms_spmv_total) */
+  double ms_spmv_store;
 } pd_step_stats;
 
 /* Solver result of one specimen (pd_set_specimens). */

@@ -145,6 +145,10 @@ struct pd_ctx {
   // true from an assembly until positions change (update, upload, restore,
   // new bonds): the assembly's range flag and record describe the positions
   bool asm_fresh = false;
+  // a stored-block FP64 step's assembly left the blocks to the solve's first
+  // product (launch_cg_spmv_store); cleared when that product or
+  // ensure_blocks has written them
+  bool blocks_deferred = false;
 };
 
 struct pd_trainer {
@@ -498,6 +502,8 @@ pd::CgArgs cg_args(pd_ctx* c, int precond) {
   a.part = c->part;
   a.st = c->st;
   a.precond = precond;
+  a.vals = c->val;
+  a.rec = c->rec;
   return a;
 }
 
@@ -582,6 +588,20 @@ int ensure_rec(pd_ctx* c) {
   return PD_OK;
 }
 
+bool defer_blocks(const pd_step_params* prm, bool fuse) {
+  return fuse && !prm->matrix_free && prm->precision == 0;
+}
+
+// Deferred blocks written from the assembly's positions (for callers that
+// read the stored blocks without a solve having run).
+int ensure_blocks(pd_ctx* c) {
+  if (!c->blocks_deferred) return PD_OK;
+  pd::launch_write_blocks(spmv_view(c), c->val, c->stream);
+  CHECK_LAUNCH(c, "write blocks");
+  c->blocks_deferred = false;
+  return PD_OK;
+}
+
 // The solve can take its first residual and start from the assembly pass
 // (the blocks and the block-Jacobi inverse are in registers there); the
 // learned preconditioner's blocks are only formed after the assembly.
@@ -605,13 +625,18 @@ int enqueue_assemble(pd_ctx* c, const pd_step_params* prm,
     CHECK_LAUNCH(c, "reference bond lengths");
     c->d0_valid = true;
   }
-  if (prm->matrix_free)
+  // FP64 stored blocks with the solve's start fused: the blocks are written
+  // by the solve's first product instead (it recomputes them at a lower
+  // cost than the assembly's stores add)
+  const bool defer = defer_blocks(prm, fuse);
+  if (prm->matrix_free || defer)
     if (int e = ensure_rec(c)) return e;
   pd::AsmArgs a;
   a.d0 = c->d0;
-  a.rec = prm->matrix_free ? c->rec : nullptr;
-  a.store_blocks = prm->matrix_free ? 0 : 1;
+  a.rec = prm->matrix_free || defer ? c->rec : nullptr;
+  a.store_blocks = prm->matrix_free || defer ? 0 : 1;
   c->mf = prm->matrix_free ? 1 : 0;
+  c->blocks_deferred = defer;
   c->asm_dt2 = dt2;
   c->asm_consts = consts;
   a.fuse = fuse ? 1 : 0;
@@ -720,6 +745,7 @@ int run_solve(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st, int resum
   }
   int launched = resumed;
   int syncs = 0;
+  int store_at = -1;  // the launch that also wrote the blocks
   // launch one iteration fewer than the previous solve needed, then single
   // iterations, then doubling chunks: launches after convergence are rare
   int chunk = resumed ? 1 : std::max(1, c->last_iters - 1);
@@ -736,7 +762,13 @@ int run_solve(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st, int resum
       if (int e = ensure_events(c, 2 * (size_t)(launched + chunk) + 8)) return e;
     for (int i = 0; i < chunk; ++i) {
       if (timing) HIPCHK(c, hipEventRecord(c->ev[8 + 2 * (launched + i)], c->stream));
-      pd::launch_cg_spmv(a, c->stream);
+      if (c->blocks_deferred) {  // the first product writes the blocks
+        pd::launch_cg_spmv_store(a, c->stream);
+        c->blocks_deferred = false;
+        store_at = launched + i;
+      } else {
+        pd::launch_cg_spmv(a, c->stream);
+      }
       if (timing) HIPCHK(c, hipEventRecord(c->ev[8 + 2 * (launched + i) + 1], c->stream));
       pd::launch_cg_update(a, c->stream);
       pd::launch_cg_dir(a, c->stream);
@@ -782,10 +814,19 @@ int run_solve(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st, int resum
     st->host_syncs += syncs;
     if (timing) {
       double tot = 0.0;
+      int cnt = 0;
       const int ran = std::min(it, launched);
-      for (int k = 0; k < ran; ++k) tot += elapsed(c->ev[8 + 2 * k], c->ev[8 + 2 * k + 1]);
+      for (int k = 0; k < ran; ++k) {
+        const double ms = elapsed(c->ev[8 + 2 * k], c->ev[8 + 2 * k + 1]);
+        if (k == store_at) {
+          st->ms_spmv_store = ms;
+        } else {
+          tot += ms;
+          ++cnt;
+        }
+      }
       st->ms_spmv_total = tot;
-      st->spmv_launches = ran;
+      st->spmv_launches = cnt;
     }
   }
   return PD_OK;
@@ -941,6 +982,7 @@ int run_solve_mixed(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st, int
 
 int enqueue_update(pd_ctx* c, const pd_step_params* prm,
                    const pd_step_consts* consts = nullptr) {
+  if (int e = ensure_blocks(c)) return e;  // from the positions they belong to
   double lo[3], hi[3];
   for (int k = 0; k < 3; ++k) {
     lo[k] = prm->bmin[k] + prm->wall_radius;
@@ -1086,7 +1128,7 @@ int capture_step(pd_ctx* c, const pd_step_params* prm, int K) {
     CHECK_LAUNCH(c, "reference bond lengths");
     c->d0_valid = true;
   }
-  if (prm->matrix_free)  // no allocation inside the capture
+  if (prm->matrix_free || defer_blocks(prm, fused_resid(prm)))  // no allocation inside the capture
     if (int e = ensure_rec(c)) return e;
   int e = capture(c, &c->g_solve, [&]() -> int {
     const bool fuse = fused_resid(prm);
@@ -1118,7 +1160,12 @@ int capture_step(pd_ctx* c, const pd_step_params* prm, int K) {
       else
         pd::launch_cg_init(a, c->stream);
       for (int i = 0; i < K; ++i) {
-        pd::launch_cg_spmv(a, c->stream);
+        if (c->blocks_deferred) {  // the first product writes the blocks
+          pd::launch_cg_spmv_store(a, c->stream);
+          c->blocks_deferred = false;
+        } else {
+          pd::launch_cg_spmv(a, c->stream);
+        }
         pd::launch_cg_update(a, c->stream);
         pd::launch_cg_dir(a, c->stream);
       }
@@ -1191,6 +1238,7 @@ void zero_timing(pd_step_stats* st) {
   if (!st) return;
   st->ms_contact = st->ms_assemble = st->ms_solve = st->ms_update = 0.0;
   st->ms_spmv_total = 0.0;
+  st->ms_spmv_store = 0.0;
   st->spmv_launches = 0;
   st->host_syncs = 0;
   st->refinements = 0;
@@ -1555,6 +1603,8 @@ int pd_get_matrix(pd_ctx* c, double* diag9, double* vals9, int32_t stride) {
   if (n == 0) return PD_OK;
   if (!diag9 || !vals9) return fail(c, PD_ERR_ARG, "null output");
   HIPCHK(c, hipSetDevice(c->device));
+  if (int e = ensure_blocks(c)) return e;
+  HIPCHK(c, hipSetDevice(c->device));
   std::vector<double> dg((size_t)c->nslices * 64 * 6), vv((size_t)c->slots * 64 * 6);
   std::vector<int32_t> top(n);
   HIPCHK(c, hipMemcpyAsync(dg.data(), c->diag, dg.size() * sizeof(double), hipMemcpyDeviceToHost,
@@ -1622,6 +1672,7 @@ int pd_solve(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st) {
   if (!c) return fail(nullptr, PD_ERR_ARG, "null context");
   if (int e = check_params(c, prm)) return e;
   HIPCHK(c, hipSetDevice(c->device));
+  if (int e = ensure_blocks(c)) return e;
   if (int e = run_solve_pc(c, prm, st)) return e;
   if (!c->last_converged)
     return fail(c, PD_ERR_NOT_CONVERGED, "CG reached the iteration limit before rtol");
@@ -1677,6 +1728,7 @@ int pd_spmv(pd_ctx* c, const double* x4, double* y4) {
   if (c->n == 0) return PD_OK;
   if (!x4 || !y4) return fail(c, PD_ERR_ARG, "null vector");
   HIPCHK(c, hipSetDevice(c->device));
+  if (int e = ensure_blocks(c)) return e;
   const size_t bytes = (size_t)c->n * 32;
   HIPCHK(c, hipMemcpyAsync(c->z, x4, bytes, hipMemcpyHostToDevice, c->stream));
   if (c->mat_prec == 1)
@@ -1758,6 +1810,7 @@ int pd_bench_spmv(pd_ctx* c, int32_t iters, double* ms_per_launch, double* bytes
   if (!c->bonds_ready) return fail(c, PD_ERR_STATE, "bonds not built");
   if (iters < 1) return fail(c, PD_ERR_ARG, "iters must be >= 1");
   HIPCHK(c, hipSetDevice(c->device));
+  if (int e = ensure_blocks(c)) return e;
   if (int e = ensure_events(c, 2)) return e;
   const pd::SpmvView A = spmv_view(c);
   const bool f32 = c->mat_prec == 1;
@@ -1893,6 +1946,7 @@ int pd_tune_spmv(pd_ctx* c, int32_t iters, double* ms3, int32_t* best) {
   if (c->mf) return fail(c, PD_ERR_STATE, "matrix-free operator: no stored-block SpMV variants");
   if (iters < 1) return fail(c, PD_ERR_ARG, "iters must be >= 1");
   HIPCHK(c, hipSetDevice(c->device));
+  if (int e = ensure_blocks(c)) return e;
   if (int e = ensure_events(c, 2)) return e;
   const int prec = c->mat_prec;
   const int variants[3] = {2, 4, 8};
@@ -2049,6 +2103,7 @@ int pd_trainer_add(pd_trainer* t, pd_ctx* c, uint64_t seed, int32_t stride, int6
   if (c->mat_prec < 0) return tfail(t, PD_ERR_STATE, "no matrix assembled in the context");
   if (stride < 1) return tfail(t, PD_ERR_ARG, "stride must be >= 1");
   THIPCHK(t, hipSetDevice(c->device));
+  if (ensure_blocks(c) != PD_OK) return tfail(t, PD_ERR_HIP, "writing the deferred blocks failed");
   const int32_t n = c->n;
   uint64_t h = seed;
   h = (h ^ (h >> 30)) * 0xBF58476D1CE4E5B9ull;

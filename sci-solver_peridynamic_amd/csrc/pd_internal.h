@@ -144,6 +144,12 @@ struct CgArgs {
   double* part;  // [3 * ntiles]
   pd_cg_state* st;  // [batch.nspec]
   int32_t precond;
+  // stored-block FP64 steps whose assembly left the blocks to the first
+  // product: that product (launch_cg_spmv_store) recomputes every live
+  // bond's block from the gather record rec (positions and p) and writes it
+  // to vals, which the later products stream
+  double* vals;
+  const double* rec;
 };
 
 // Mixed-precision solve: float matrix copy, float inner vectors, FP64
@@ -280,6 +286,13 @@ void launch_cg_init(const CgArgs& a, hipStream_t s);
 // the fold alone, after an assembly that produced the start's partials
 void launch_cg_init_fold(const CgArgs& a, hipStream_t s);
 void launch_cg_spmv(const CgArgs& a, hipStream_t s);
+// the first CG product of a step whose assembly did not write the blocks:
+// y = A p with the blocks recomputed and written to a.vals (for every row,
+// converged specimens included), <y,p> partials as launch_cg_spmv
+void launch_cg_spmv_store(const CgArgs& a, hipStream_t s);
+// the blocks of the current bond lists written to vals from the positions
+// (a step's deferred blocks when no product wrote them)
+void launch_write_blocks(const SpmvView& A, double* vals, hipStream_t s);
 void launch_cg_update(const CgArgs& a, hipStream_t s);
 void launch_cg_dir(const CgArgs& a, hipStream_t s);
 void launch_spmv_plain(const SpmvView& A, const double4* x, double4* y, hipStream_t s);

@@ -685,11 +685,22 @@ __device__ __forceinline__ void spmv_row(const SpmvView& A, const double4* __res
 // assembly found every operand in range).
 typedef double dbl2v __attribute__((ext_vector_type(2)));
 
+// One bond block into the pair planes of vals (the assembly's layout).
+__device__ __forceinline__ void store_block(double* vals, int64_t slot, int lane, double b00,
+                                            double b01, double b02, double b11, double b12,
+                                            double b22) {
+  vpair* vo = (vpair*)(vals + slot * 6 * 64) + lane;
+  __builtin_nontemporal_store(vpair{b00, b01}, vo);
+  __builtin_nontemporal_store(vpair{b02, b11}, vo + 64);
+  __builtin_nontemporal_store(vpair{b12, b22}, vo + 128);
+}
+
 template <bool REC>
-__device__ __forceinline__ void gather_mf(const SpmvView& A, const double4* __restrict__ x,
-                                          int q, double4& xq, double4& vq) {
+__device__ __forceinline__ void gather_mf(const SpmvView& A, const double* __restrict__ rec,
+                                          const double4* __restrict__ x, int q, double4& xq,
+                                          double4& vq) {
   if constexpr (REC) {
-    const dbl2v* r = (const dbl2v*)(A.rec + 6 * (int64_t)q);
+    const dbl2v* r = (const dbl2v*)(rec + 6 * (int64_t)q);
     const dbl2v r0 = r[0], r1 = r[1], r2 = r[2];
     xq = make_double4(r0.x, r0.y, r1.x, 0.0);
     vq = make_double4(r1.y, r2.x, r2.y, 0.0);
@@ -699,9 +710,12 @@ __device__ __forceinline__ void gather_mf(const SpmvView& A, const double4* __re
   }
 }
 
-template <int V, bool DIAG_LAST, bool REC, bool SHARED>
+// STORE: each recomputed block is also written to vals (pair planes).
+template <int V, bool DIAG_LAST, bool REC, bool SHARED, bool STORE = false>
 __device__ __forceinline__ void spmv_row_mf(const SpmvView& A, const double4* __restrict__ x,
-                                            int p, double& y0, double& y1, double& y2) {
+                                            int p, double& y0, double& y1, double& y2,
+                                            const double* __restrict__ rec = nullptr,
+                                            double* __restrict__ vals = nullptr) {
   const int s = p >> 6;
   const int lane = p & 63;
   const double* dg = A.diag + (int64_t)s * 6 * 64 + lane;
@@ -733,7 +747,7 @@ __device__ __forceinline__ void spmv_row_mf(const SpmvView& A, const double4* __
       }
       double4 xq[V], vq[V];
 #pragma unroll
-      for (int u = 0; u < V; ++u) gather_mf<REC>(A, x, q[u], xq[u], vq[u]);
+      for (int u = 0; u < V; ++u) gather_mf<REC>(A, rec, x, q[u], xq[u], vq[u]);
       double t[V][3];
 #pragma unroll
       for (int u = 0; u < V; ++u) {
@@ -741,8 +755,11 @@ __device__ __forceinline__ void spmv_row_mf(const SpmvView& A, const double4* __
         bond_dir<SHARED>(pos_p, xq[u], e0, e1, e2, dij);
         double F00, F01, F02, F11, F12, F22;
         bond_jac<SHARED>(e0, e1, e2, dij, l0[u], cb, F00, F01, F02, F11, F12, F22);
-        blk6((F00 * -1.0) * dt2, (F01 * -1.0) * dt2, (F02 * -1.0) * dt2, (F11 * -1.0) * dt2,
-             (F12 * -1.0) * dt2, (F22 * -1.0) * dt2, vq[u], t[u][0], t[u][1], t[u][2]);
+        const double b00 = (F00 * -1.0) * dt2, b01 = (F01 * -1.0) * dt2,
+                     b02 = (F02 * -1.0) * dt2, b11 = (F11 * -1.0) * dt2,
+                     b12 = (F12 * -1.0) * dt2, b22 = (F22 * -1.0) * dt2;
+        if constexpr (STORE) store_block(vals, base + bnd + u, lane, b00, b01, b02, b11, b12, b22);
+        blk6(b00, b01, b02, b11, b12, b22, vq[u], t[u][0], t[u][1], t[u][2]);
       }
 #pragma unroll
       for (int u = 0; u < V; ++u) {
@@ -754,15 +771,17 @@ __device__ __forceinline__ void spmv_row_mf(const SpmvView& A, const double4* __
     for (; bnd <= top; ++bnd) {
       const int q0 = ntl(cp + (int64_t)bnd * 64);
       double4 xq, vq;
-      gather_mf<REC>(A, x, q0, xq, vq);
+      gather_mf<REC>(A, rec, x, q0, xq, vq);
       double e0, e1, e2, dij;
       bond_dir<SHARED>(pos_p, xq, e0, e1, e2, dij);
       double F00, F01, F02, F11, F12, F22;
       bond_jac<SHARED>(e0, e1, e2, dij, ntl(lp + (int64_t)bnd * 64), cb, F00, F01, F02, F11,
                        F12, F22);
+      const double b00 = (F00 * -1.0) * dt2, b01 = (F01 * -1.0) * dt2, b02 = (F02 * -1.0) * dt2,
+                   b11 = (F11 * -1.0) * dt2, b12 = (F12 * -1.0) * dt2, b22 = (F22 * -1.0) * dt2;
+      if constexpr (STORE) store_block(vals, base + bnd, lane, b00, b01, b02, b11, b12, b22);
       double t0, t1, t2;
-      blk6((F00 * -1.0) * dt2, (F01 * -1.0) * dt2, (F02 * -1.0) * dt2, (F11 * -1.0) * dt2,
-           (F12 * -1.0) * dt2, (F22 * -1.0) * dt2, vq, t0, t1, t2);
+      blk6(b00, b01, b02, b11, b12, b22, vq, t0, t1, t2);
       y0 += t0;
       y1 += t1;
       y2 += t2;
@@ -789,9 +808,9 @@ __device__ __forceinline__ void row_of(const SpmvView& A, const double4* __restr
                                        double& y0, double& y1, double& y2) {
   if constexpr (U == 0) {
     if (SHARED_OK && A.range_bad && *A.range_bad == 0)
-      spmv_row_mf<3, DIAG_LAST, REC, true>(A, x, p, y0, y1, y2);
+      spmv_row_mf<3, DIAG_LAST, REC, true>(A, x, p, y0, y1, y2, A.rec);
     else
-      spmv_row_mf<3, DIAG_LAST, REC, false>(A, x, p, y0, y1, y2);
+      spmv_row_mf<3, DIAG_LAST, REC, false>(A, x, p, y0, y1, y2, A.rec);
   } else {
     spmv_row<U, DIAG_LAST>(A, x, p, y0, y1, y2);
   }
@@ -1022,6 +1041,58 @@ __global__ __launch_bounds__(PD_BLOCK) void k_cg_spmv(CgArgs a) {
   if (threadIdx.x == 0) a.part[tile] = v;
 }
 
+// The first CG product of a stored-block FP64 step whose assembly left the
+// blocks out: every block is recomputed from the gather record (positions
+// and p side by side) exactly as the assembly forms it, written to its slot
+// and multiplied; the later products stream the written blocks.  Tiles of
+// converged specimens write their blocks too but no product or partial.
+__global__ __launch_bounds__(PD_BLOCK) void k_cg_spmv_store(CgArgs a) {
+  __shared__ double lds4[4];
+  const int ntiles = gridDim.x;
+  const int tile = xcd_tile(blockIdx.x, ntiles);
+  const bool done = a.st[spec_of(a.batch, tile)].done;
+  const int p = tile * PD_BLOCK + threadIdx.x;
+  double t = 0.0;
+  if (p < a.A.n) {
+    double y0, y1, y2;
+    if (a.A.range_bad && *a.A.range_bad == 0)
+      spmv_row_mf<3, false, true, true, true>(a.A, a.p, p, y0, y1, y2, a.rec, a.vals);
+    else
+      spmv_row_mf<3, false, true, false, true>(a.A, a.p, p, y0, y1, y2, a.rec, a.vals);
+    if (!done) {
+      a.y[p] = make_double4(y0, y1, y2, 0.0);
+      const double4 pp = a.p[p];
+      t = term3(y0, y1, y2, pp.x, pp.y, pp.z);
+    }
+  }
+  if (done) return;  // uniform over the tile
+  const double v = tile_sum(t, lds4);
+  if (threadIdx.x == 0) a.part[tile] = v;
+}
+
+// The blocks of the current bond lists from the positions, written to vals.
+__global__ __launch_bounds__(PD_BLOCK) void k_write_blocks(SpmvView A, double* vals) {
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int p = tile * PD_BLOCK + threadIdx.x;
+  if (p >= A.n) return;
+  const int top = A.top[p];
+  if (top < 0) return;
+  const int s = p >> 6, lane = p & 63;
+  const int64_t base = A.slice_off[s];
+  const double4 pos_p = A.pos[p];
+  const double cb = A.mat[A.body[p]].c_bond;
+  const double dt2 = A.consts ? A.consts->dt2 : A.dt2;
+  for (int bnd = 0; bnd <= top; ++bnd) {
+    const int64_t i = (base + bnd) * 64 + lane;
+    double e0, e1, e2, dij;
+    bond_dir(pos_p, A.pos[A.col[i]], e0, e1, e2, dij);
+    double F00, F01, F02, F11, F12, F22;
+    bond_jac(e0, e1, e2, dij, A.d0[i], cb, F00, F01, F02, F11, F12, F22);
+    store_block(vals, base + bnd, lane, (F00 * -1.0) * dt2, (F01 * -1.0) * dt2,
+                (F02 * -1.0) * dt2, (F11 * -1.0) * dt2, (F12 * -1.0) * dt2, (F22 * -1.0) * dt2);
+  }
+}
+
 // alpha = rz / <y,p>
 __global__ __launch_bounds__(PD_BLOCK) void k_cg_alpha(CgArgs a, int ntiles) {
   pd_cg_state* st = a.st + blockIdx.x;
@@ -1153,6 +1224,14 @@ void launch_cg_spmv(const CgArgs& a, hipStream_t s) {
     case 8: hipLaunchKernelGGL(k_cg_spmv<8>, dim3(g), dim3(PD_BLOCK), 0, s, a); break;
     default: hipLaunchKernelGGL(k_cg_spmv<4>, dim3(g), dim3(PD_BLOCK), 0, s, a);
   }
+}
+void launch_cg_spmv_store(const CgArgs& a, hipStream_t s) {
+  if (!a.A.n) return;
+  hipLaunchKernelGGL(k_cg_spmv_store, dim3(tiles_of(a.A.n)), dim3(PD_BLOCK), 0, s, a);
+}
+void launch_write_blocks(const SpmvView& A, double* vals, hipStream_t s) {
+  if (!A.n) return;
+  hipLaunchKernelGGL(k_write_blocks, dim3(tiles_of(A.n)), dim3(PD_BLOCK), 0, s, A, vals);
 }
 void launch_cg_update(const CgArgs& a, hipStream_t s) {
   if (!a.A.n) return;

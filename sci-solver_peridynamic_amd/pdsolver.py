@@ -76,6 +76,7 @@ class StepStats(C.Structure):
         ("spmv_launches", C.c_int32), ("host_syncs", C.c_int32),
         ("refinements", C.c_int32), ("ms_resid_total", C.c_double),
         ("resid_launches", C.c_int32), ("precond_fallbacks", C.c_int32),
+        ("ms_spmv_store", C.c_double),
     ]
 
     def as_dict(self):

@@ -1,0 +1,52 @@
+"""Stored-block FP64 steps leave the bond blocks to the solve's first
+product, which recomputes and writes them.  A context stepped with pd_step
+and one driven through pd_assemble (its assembly stores the blocks),
+pd_commit_stretch, pd_solve and pd_update must hold the same matrix, bit
+for bit, after every step, and the same trajectory; also when the solve
+converges at its start and the first product only writes blocks."""
+import sys
+
+import numpy as np
+import pytest
+
+from pdhelpers import config0_state, make_pair, material
+
+pytestmark = pytest.mark.gpu
+pds = pytest.importorskip("pdsolver")
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("rtol", [1e-8, 1e10])
+def test_step_matrix_equals_assembled_matrix(rtol):
+    say("case first product writes blocks, rtol", rtol)
+    x4, bc, vbc, v4 = config0_state(8, 8, 10)
+    mats = [material(0.01)]
+    _, stepped = make_pair(x4, mats, v4=v4, bc=bc, vbc=vbc)
+    _, split = make_pair(x4, mats, v4=v4, bc=bc, vbc=vbc)
+    broken = 0
+    for k in range(1, 31):
+        kw = dict(dt=1e-4, clock_scale=k / 30.0, precond=1, rtol=rtol)
+        prm = pds.make_params(**kw)
+        st = stepped.step(prm)
+        sa = split.assemble(prm)
+        split.commit_stretch()
+        ss = split.solve(prm)
+        split.update(prm)
+        assert st.cg_iterations == ss.cg_iterations and st.broken_bonds == sa.broken_bonds, k
+        if rtol > 1.0:
+            assert st.cg_iterations == 0 and st.converged
+        ds, vs = stepped.get_matrix()
+        da, va = split.get_matrix()
+        assert np.array_equal(ds, da), k
+        assert np.array_equal(vs, va), k
+        assert np.array_equal(stepped.download("position"), split.download("position")), k
+        broken += st.broken_bonds
+    say(dict(broken=broken))
+    if rtol < 1.0:
+        assert broken > 0, "the tension case must break bonds"
+    stepped.close()
+    split.close()
