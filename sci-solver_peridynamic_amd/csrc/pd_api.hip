@@ -588,8 +588,11 @@ int ensure_rec(pd_ctx* c) {
   return PD_OK;
 }
 
-bool defer_blocks(const pd_step_params* prm, bool fuse) {
-  return fuse && !prm->matrix_free && prm->precision == 0;
+// FP64 stored blocks: the solve's first product writes them (the CG
+// product when the start is fused into the assembly, else the start's
+// y = A x0), at a lower cost than the assembly's stores add
+bool defer_blocks(const pd_step_params* prm) {
+  return !prm->matrix_free && prm->precision == 0;
 }
 
 // Deferred blocks written from the assembly's positions (for callers that
@@ -625,16 +628,12 @@ int enqueue_assemble(pd_ctx* c, const pd_step_params* prm,
     CHECK_LAUNCH(c, "reference bond lengths");
     c->d0_valid = true;
   }
-  // FP64 stored blocks with the solve's start fused: the blocks are written
-  // by the solve's first product instead (it recomputes them at a lower
-  // cost than the assembly's stores add)
-  const bool defer = defer_blocks(prm, fuse);
+  const bool defer = defer_blocks(prm);
   if (prm->matrix_free || defer)
     if (int e = ensure_rec(c)) return e;
   pd::AsmArgs a;
   a.d0 = c->d0;
   a.rec = prm->matrix_free || defer ? c->rec : nullptr;
-  a.store_blocks = prm->matrix_free || defer ? 0 : 1;
   c->mf = prm->matrix_free ? 1 : 0;
   c->blocks_deferred = defer;
   c->asm_dt2 = dt2;
@@ -737,10 +736,14 @@ int run_solve(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st, int resum
   const int ns = c->nspec;
   if (!resumed) {
     pd::launch_cg_setup(c->st, ns, prm->rtol, prm->max_iter, c->stream);
-    if (fused)
+    if (fused) {
       pd::launch_cg_init_fold(a, c->stream);
-    else
+    } else if (c->blocks_deferred) {  // the start's product writes the blocks
+      pd::launch_cg_init_store(a, c->stream);
+      c->blocks_deferred = false;
+    } else {
       pd::launch_cg_init(a, c->stream);
+    }
     CHECK_LAUNCH(c, "cg init");
   }
   int launched = resumed;
@@ -1128,7 +1131,7 @@ int capture_step(pd_ctx* c, const pd_step_params* prm, int K) {
     CHECK_LAUNCH(c, "reference bond lengths");
     c->d0_valid = true;
   }
-  if (prm->matrix_free || defer_blocks(prm, fused_resid(prm)))  // no allocation inside the capture
+  if (prm->matrix_free || defer_blocks(prm))  // no allocation inside the capture
     if (int e = ensure_rec(c)) return e;
   int e = capture(c, &c->g_solve, [&]() -> int {
     const bool fuse = fused_resid(prm);
@@ -1155,10 +1158,14 @@ int capture_step(pd_ctx* c, const pd_step_params* prm, int K) {
     } else {
       const pd::CgArgs a = cg_args(c, prm->precond);
       pd::launch_cg_setup(c->st, ns, prm->rtol, prm->max_iter, c->stream);
-      if (fuse)
+      if (fuse) {
         pd::launch_cg_init_fold(a, c->stream);
-      else
+      } else if (c->blocks_deferred) {
+        pd::launch_cg_init_store(a, c->stream);
+        c->blocks_deferred = false;
+      } else {
         pd::launch_cg_init(a, c->stream);
+      }
       for (int i = 0; i < K; ++i) {
         if (c->blocks_deferred) {  // the first product writes the blocks
           pd::launch_cg_spmv_store(a, c->stream);
@@ -1672,7 +1679,6 @@ int pd_solve(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st) {
   if (!c) return fail(nullptr, PD_ERR_ARG, "null context");
   if (int e = check_params(c, prm)) return e;
   HIPCHK(c, hipSetDevice(c->device));
-  if (int e = ensure_blocks(c)) return e;
   if (int e = run_solve_pc(c, prm, st)) return e;
   if (!c->last_converged)
     return fail(c, PD_ERR_NOT_CONVERGED, "CG reached the iteration limit before rtol");

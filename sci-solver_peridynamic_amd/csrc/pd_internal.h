@@ -52,7 +52,6 @@ struct AsmArgs {
   int32_t track_gap;
   const pd_step_consts* consts;  // non-null: dt, dt2, clock, gravity from here
   float2* bond_stats;            // non-null: (mean, max) stretch of the live bonds
-  int32_t store_blocks;  // FP64: 0 = matrix-free solve, bond blocks not written
   double* d0;            // per-slot reference lengths: read, and swapped with col
   // fuse = 1: the solve's first residual r = S(b - A xs), each row's bond
   // terms summed first and the diagonal term last, with the solver's start
@@ -290,6 +289,9 @@ void launch_cg_spmv(const CgArgs& a, hipStream_t s);
 // y = A p with the blocks recomputed and written to a.vals (for every row,
 // converged specimens included), <y,p> partials as launch_cg_spmv
 void launch_cg_spmv_store(const CgArgs& a, hipStream_t s);
+// launch_cg_init when the assembly left the blocks: they are recomputed
+// from the positions, written to a.vals and multiplied by x0
+void launch_cg_init_store(const CgArgs& a, hipStream_t s);
 // the blocks of the current bond lists written to vals from the positions
 // (a step's deferred blocks when no product wrote them)
 void launch_write_blocks(const SpmvView& A, double* vals, hipStream_t s);

@@ -114,11 +114,14 @@ __device__ __forceinline__ void bond_jac(double e0, double e1, double e2, double
 // + the block-Jacobi inverse of A_pp.  One lane per particle; a wave is one
 // 64-row slice, so every slot read / write is 64 consecutive words.  The
 // next bond's column and neighbour data are fetched while the current bond
-// is computed; the matrix values are written with non-temporal stores (they
-// are read back once per CG iteration, never soon enough to gain from L2).
+// is computed.  FP64 bond blocks are not written here: the solve's first
+// product recomputes and writes them (writing them here cost more than
+// that recomputation, the stores adding to the arithmetic rather than
+// overlapping it), or every matrix-free product recomputes them.
 // ---------------------------------------------------------------------------
-// MIXED: the bond blocks, the diagonal and its inverse are also (or, for
-// the bonds and the inverse, only) written rounded to float.
+// MIXED: the bond blocks and the block-Jacobi inverse are written rounded
+// to float (non-temporal stores: they are read back once per inner
+// iteration, never soon enough to gain from L2), the diagonal in both.
 template <bool MIXED>
 __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
   if (a.consts) {
@@ -261,16 +264,12 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
           rb2 += b02 * u.x + b12 * u.y + b22 * u.z;
         }
       } else {
+        // FP64: the blocks are written by the solve's first product
+        // (k_cg_spmv_store, or k_cg_init's storing variant) or, matrix-free,
+        // recomputed by every product; here they feed the residual only
         const double b00 = (F00 * -1.0) * a.dt2, b01 = (F01 * -1.0) * a.dt2,
                      b02 = (F02 * -1.0) * a.dt2, b11 = (F11 * -1.0) * a.dt2,
                      b12 = (F12 * -1.0) * a.dt2, b22 = (F22 * -1.0) * a.dt2;
-        if (a.store_blocks) {
-          // three planes of value pairs, 16 bytes per lane
-          vpair* vo = (vpair*)(a.val + slot * 6 * 64) + lane;
-          __builtin_nontemporal_store(vpair{b00, b01}, vo);
-          __builtin_nontemporal_store(vpair{b02, b11}, vo + 64);
-          __builtin_nontemporal_store(vpair{b12, b22}, vo + 128);
-        }
         if (a.fuse) {
           rb0 += b00 * u.x + b01 * u.y + b02 * u.z;
           rb1 += b01 * u.x + b11 * u.y + b12 * u.z;
@@ -954,8 +953,10 @@ __device__ __forceinline__ void rec_p(double* rec, int p, double z0, double z1, 
   }
 }
 
-// y = A x0; r = S(b - y); z = M r; p = z; partials <r,z>, <r,r>, <Sb,Sb>
-template <int U>
+// y = A x0; r = S(b - y); z = M r; p = z; partials <r,z>, <r,r>, <Sb,Sb>.
+// STORE: a stored-block FP64 solve whose assembly left the blocks to it:
+// every block is recomputed from the positions, written to a.vals and used.
+template <int U, bool STORE = false>
 __global__ __launch_bounds__(PD_BLOCK) void k_cg_init(CgArgs a) {
   __shared__ double lds4[4];
   const int ntiles = gridDim.x;
@@ -964,7 +965,10 @@ __global__ __launch_bounds__(PD_BLOCK) void k_cg_init(CgArgs a) {
   double t_rz = 0.0, t_rr = 0.0, t_bb = 0.0;
   if (p < a.A.n) {
     double y0, y1, y2;
-    row_of<U, true>(a.A, a.x, p, y0, y1, y2);
+    if constexpr (STORE)
+      spmv_row_mf<3, true, false, false, true>(a.A, a.x, p, y0, y1, y2, nullptr, a.vals);
+    else
+      row_of<U, true>(a.A, a.x, p, y0, y1, y2);
     const double4 bp = a.b[p];
     const bool fr = a.bc[p] == 0;
     const double r0 = fr ? bp.x - y0 : 0.0;
@@ -1205,6 +1209,12 @@ void launch_cg_init(const CgArgs& a, hipStream_t s) {
       default: hipLaunchKernelGGL(k_cg_init<4>, dim3(g), dim3(PD_BLOCK), 0, s, a);
     }
   }
+  hipLaunchKernelGGL(k_cg_init_fold, dim3(a.batch.nspec), dim3(PD_BLOCK), 0, s, a, g);
+}
+void launch_cg_init_store(const CgArgs& a, hipStream_t s) {
+  if (!a.A.n) return;
+  const int g = tiles_of(a.A.n);
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cg_init<0, true>), dim3(g), dim3(PD_BLOCK), 0, s, a);
   hipLaunchKernelGGL(k_cg_init_fold, dim3(a.batch.nspec), dim3(PD_BLOCK), 0, s, a, g);
 }
 void launch_cg_init_fold(const CgArgs& a, hipStream_t s) {

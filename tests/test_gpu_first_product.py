@@ -1,9 +1,11 @@
-"""Stored-block FP64 steps leave the bond blocks to the solve's first
-product, which recomputes and writes them.  A context stepped with pd_step
-and one driven through pd_assemble (its assembly stores the blocks),
-pd_commit_stretch, pd_solve and pd_update must hold the same matrix, bit
-for bit, after every step, and the same trajectory; also when the solve
-converges at its start and the first product only writes blocks."""
+"""Stored-block FP64 assemblies leave the bond blocks to the solve's first
+product, which recomputes and writes them: the first CG product when the
+solve's start is fused into the assembly (pd_step), the start's y = A x0
+otherwise (pd_assemble then pd_solve).  The two contexts must hold the same
+matrix, bit for bit, after every step, and the same trajectory; also when
+the solve converges at its start and the CG product only writes blocks.
+(The blocks that pd_get_matrix writes itself after a bare pd_assemble are
+compared with the oracle's by the assembly parity test.)"""
 import sys
 
 import numpy as np
