@@ -410,6 +410,7 @@ int ensure_slots(pd_ctx* c, int64_t slots) {
   dfree(c->val);
   dfree(c->d0);
   c->d0_valid = false;
+  c->blocks_deferred = false;  // new bond lists: no matrix to keep
   c->asm_fresh = false;
   if (int e = dalloc(c, &c->col, (size_t)slots * 64)) return e;
   if (int e = dalloc(c, &c->val, (size_t)slots * 64 * 6)) return e;
@@ -1422,6 +1423,11 @@ int pd_upload(pd_ctx* c, int32_t field, const void* src, size_t bytes) {
   if (bytes != fi.elem * (size_t)c->n) return fail(c, PD_ERR_ARG, "upload size mismatch");
   if (c->n == 0) return PD_OK;
   if (!src) return fail(c, PD_ERR_ARG, "null source");
+  if (field == PD_FIELD_POSITION || field == PD_FIELD_BOND_TOP || field == PD_FIELD_BODY) {
+    // blocks a solve has not written yet belong to the state being replaced
+    HIPCHK(c, hipSetDevice(c->device));
+    if (int e = ensure_blocks(c)) return e;
+  }
   if (field == PD_FIELD_BOND_TOP) {
     if (!c->bonds_ready) return fail(c, PD_ERR_STATE, "bonds not built");
     const int32_t* t = (const int32_t*)src;
@@ -1519,6 +1525,7 @@ int pd_build_bonds(pd_ctx* c, pd_bond_stats* stats) {
   c->last_iters = 0;
   c->h_cap = counts;
   c->d0_valid = false;
+  c->blocks_deferred = false;  // new bond lists: no matrix to keep
   c->asm_fresh = false;
   if (stats) {
     stats->total_bonds = total;
@@ -1574,6 +1581,7 @@ int pd_set_bonds(pd_ctx* c, const int32_t* counts, const int32_t* lists, int32_t
   c->last_iters = 0;
   c->h_cap = cap;
   c->d0_valid = false;
+  c->blocks_deferred = false;  // new bond lists: no matrix to keep
   c->asm_fresh = false;
   return PD_OK;
 }
@@ -1764,6 +1772,7 @@ int pd_restore(pd_ctx* c) {
   HIPCHK(c, hipSetDevice(c->device));
   const size_t n = (size_t)c->n;
   if (n == 0) return PD_OK;
+  if (int e = ensure_blocks(c)) return e;  // from the state being replaced
   HIPCHK(c, hipMemcpyAsync(c->top, c->top_bak, n * 4, hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->x, c->x_bak, n * 32, hipMemcpyDeviceToDevice, c->stream));
   c->asm_fresh = false;

@@ -52,3 +52,24 @@ def test_step_matrix_equals_assembled_matrix(rtol):
         assert broken > 0, "the tension case must break bonds"
     stepped.close()
     split.close()
+
+
+def test_blocks_follow_the_assembly_not_later_uploads():
+    """After a bare pd_assemble the blocks are still unwritten; replacing the
+    positions must not change the matrix the assembly defined."""
+    say("case deferred blocks and uploads")
+    x4, bc, vbc, v4 = config0_state(6, 6, 8)
+    mats = [material(0.01)]
+    _, a = make_pair(x4, mats, v4=v4, bc=bc, vbc=vbc)
+    _, b = make_pair(x4, mats, v4=v4, bc=bc, vbc=vbc)
+    prm = pds.make_params(dt=1e-4, clock_scale=0.5, precond=1)
+    a.assemble(prm)
+    b.assemble(prm)
+    moved = a.download("position")
+    moved[:, :3] += 1e-4
+    a.upload("position", moved)
+    da, va = a.get_matrix()
+    db, vb = b.get_matrix()
+    assert np.array_equal(da, db) and np.array_equal(va, vb)
+    a.close()
+    b.close()
