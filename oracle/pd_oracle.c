@@ -788,8 +788,8 @@ void orc_cg(orc_system* sys, const orc_step_params* prm, orc_step_stats* st) {
  * Higham, "Accuracy and Stability of Numerical Algorithms", ch. 12) with a
  * PCG inner solver:
  *   A32 = the assembled off-diagonal blocks rounded to float, FP64 diagonal
- *   loop:  r = S(b - A32 x) in FP64 (float blocks widened exactly; bond
- *          terms summed first, the diagonal term added last)
+ *   loop:  r = S(b - A x) in FP64 with the FP64 system (orc_spmv_resid:
+ *          bond terms summed first, the diagonal term added last)
  *          stop if ||r|| <= rtol ||S b|| (FP64 norms)
  *          inner PCG on float vectors and float arithmetic for A32 d = r,
  *            diagonal block and block-Jacobi inverse rounded to float,
@@ -924,8 +924,9 @@ void orc_cg_mixed(orc_system* sys, const orc_step_params* prm, orc_step_stats* s
   int32_t it = 0, ref = 0, converged = 0, first = 1;
   double bnorm = 0.0, tol = 0.0, rnorm = 0.0;
   for (;;) {
-    /* FP64 residual, then the inner solve's start: d = 0, z = M r, p = z */
-    orc_spmv_wide(sys, x, y);
+    /* FP64 residual of the FP64 system, then the inner solve's start:
+     * d = 0, z = M r, p = z */
+    orc_spmv_resid(sys, x, y);
 #pragma omp parallel for schedule(static)
     for (int32_t i = 0; i < n; ++i) {
       const int64_t o = 4 * (int64_t)i;

@@ -485,6 +485,7 @@ pd::MxArgs mx_args(pd_ctx* c, int precond) {
   a.st = c->mx;
   a.precond = precond;
   a.unroll = c->unroll[1];
+  a.A64 = spmv_view(c);
   return a;
 }
 

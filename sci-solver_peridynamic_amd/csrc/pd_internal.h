@@ -175,6 +175,9 @@ struct MxArgs {
   pd_mx_state* st;  // [batch.nspec]
   int32_t precond;
   int32_t unroll;  // slots per trip of the SpMV row loops: 2, 4 or 8
+  // the FP64 system of the residual: its bond blocks recomputed from the
+  // positions as the assembly forms them (matrix-free), FP64 diagonal
+  SpmvView A64;
 };
 
 // Learned block preconditioner: a 16-64-64-8 MLP (BF16 weights, FP32

@@ -5,10 +5,13 @@
 // keeps its system and stopping rule (||S(b - A x)|| <= rtol ||S b||, with
 // the residual, its norm and every dot product in FP64) and moves the
 // bandwidth-bound inner iterations to 4-byte values and vectors:
-//   residual  r = S(b - A32 x): FP64 diagonal, float bond blocks widened
-//             exactly, FP64 x (bond terms summed first in list order, the
-//             diagonal term added last); fused with the inner start d = 0,
-//             z = M32 r32, p = z (r32 = float(r))
+//   residual  r = S(b - A x) of the FP64 system: every FP64 bond block
+//             recomputed from the positions with the assembly's arithmetic
+//             (no FP64 blocks are stored), FP64 diagonal and x (bond terms
+//             summed first in list order, the diagonal term added last);
+//             fused with the inner start d = 0, z = M32 r32, p = z
+//             (r32 = float(r)).  Refinement therefore converges to the
+//             FP64 system's solution; A32 only serves the inner solves
 //   inner     y = A32 p in float arithmetic (diagonal first, bonds in list
 //             order, (a0 x0 + a1 x1) + a2 x2 per block row), alpha and beta
 //             in FP64 from FP64 sums of widened float products, rounded to
@@ -22,15 +25,12 @@
 // reproducible run to run and equal bit for bit to the oracle's restatement.
 #include <math.h>
 
-#include "pd_internal.h"
+#include "pd_bond.h"
 
 namespace pd {
 
 namespace {
 
-__device__ __forceinline__ double ntl(const double* p) { return __builtin_nontemporal_load(p); }
-__device__ __forceinline__ float ntl(const float* p) { return __builtin_nontemporal_load(p); }
-__device__ __forceinline__ int ntl(const int* p) { return __builtin_nontemporal_load(p); }
 
 __device__ __forceinline__ void blk6w(float a00, float a01, float a02, float a11, float a12,
                                       float a22, const double4 x, double& o0, double& o1,
@@ -214,9 +214,9 @@ __global__ void k_mx_setup(pd_mx_state* states, int nspec, double rtol, int max_
   }
 }
 
-// r = S(b - A32 x) in FP64; r32 = float(r); z = M32 r32; p = z; d = 0;
-// partials <r,r>, <r32,z>, <r32,r32>, <Sb,Sb>
-template <int U>
+// r = S(b - A x) in FP64 (the FP64 system, blocks recomputed); r32 =
+// float(r); z = M32 r32; p = z; d = 0; partials <r,r>, <r32,z>, <r32,r32>,
+// <Sb,Sb>
 __global__ __launch_bounds__(PD_BLOCK) void k_mx_resid(MxArgs a) {
   __shared__ double lds4[4];
   const int ntiles = gridDim.x;
@@ -226,7 +226,7 @@ __global__ __launch_bounds__(PD_BLOCK) void k_mx_resid(MxArgs a) {
   double t_rr = 0.0, t_rz = 0.0, t_r32 = 0.0, t_bb = 0.0;
   if (p < a.n) {
     double y0, y1, y2;
-    row_wide<U>(a, a.x, p, y0, y1, y2);
+    spmv_row_mf<3, true, false, false>(a.A64, a.x, p, y0, y1, y2);
     const double4 bp = a.b[p];
     const bool fr = a.bc[p] == 0;
     const double r0 = fr ? bp.x - y0 : 0.0;
@@ -449,11 +449,7 @@ void launch_mx_setup(pd_mx_state* st, int32_t nspec, double rtol, int32_t max_it
 void launch_mx_resid(const MxArgs& a, hipStream_t s) {
   if (!a.n) return;
   const int g = tiles_of(a.n);
-  switch (a.unroll) {
-    case 2: hipLaunchKernelGGL(k_mx_resid<2>, dim3(g), dim3(PD_BLOCK), 0, s, a); break;
-    case 8: hipLaunchKernelGGL(k_mx_resid<8>, dim3(g), dim3(PD_BLOCK), 0, s, a); break;
-    default: hipLaunchKernelGGL(k_mx_resid<4>, dim3(g), dim3(PD_BLOCK), 0, s, a);
-  }
+  hipLaunchKernelGGL(k_mx_resid, dim3(g), dim3(PD_BLOCK), 0, s, a);
   hipLaunchKernelGGL(k_mx_resid_fold, dim3(a.batch.nspec), dim3(PD_BLOCK), 0, s, a, g);
 }
 void launch_mx_resid_fold(const MxArgs& a, hipStream_t s) {
