@@ -99,10 +99,10 @@ typedef struct pd_step_params {
   /* Solver precision (not in the reference, which is FP64 only).
    * 0: FP64 CG.  1: the assembly also rounds the bond blocks, the diagonal
    * and its block-Jacobi inverse to float, and the solve is iterative
-   * refinement: r = S(b - A32 x) in FP64 (diagonal FP64, float bond blocks
-   * widened), stop when ||r|| <= rtol ||S b||, else a PCG on float vectors
-   * solves A32 d = r until ||r32|| <= max(inner_rtol ||r||, tol / 2) and
-   * x += d in FP64.  Dot products and norms are FP64 throughout. */
+   * refinement: r = S(b - A x) of the FP64 system in FP64 (its bond blocks
+   * recomputed), stop when ||r|| <= rtol ||S b||, else a PCG on float
+   * vectors solves A32 d = r until ||r32|| <= max(inner_rtol ||r||, tol / 2)
+   * and x += d in FP64.  Dot products and norms are FP64 throughout. */
   int32_t precision;
   int32_t max_refine;     /* precision 1: refinement limit (0: 20) */
   double inner_rtol;      /* precision 1: inner reduction (0: 1e-5) */

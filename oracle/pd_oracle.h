@@ -28,9 +28,11 @@
  *   orc_update          free-fall closed form, wall-clamp closed form
  *   orc_contact         two-particle closed form, symmetry
  *   orc_dot             exact rational sum on integer-valued data
- *   orc_cg_mixed        numpy dense solve of the float-rounded system,
- *                       FP64 stopping rule on the true residual, and the
- *                       float inner iterations replayed in numpy float32
+ *   orc_cg_mixed        numpy dense solves of the FP64 system (the refined
+ *                       solution) and of the float-rounded one (which it
+ *                       must not reach), FP64 stopping rule on the true
+ *                       residual, the float accuracy floor of one inner
+ *                       solve
  */
 #ifndef PD_ORACLE_H
 #define PD_ORACLE_H
@@ -165,9 +167,10 @@ void orc_block_jacobi(orc_system* sys);
 /* core/simulator.cu:735-799 (CUSP krylov::cg structure), projected onto
  * the free rows when bc[] marks Dirichlet particles. */
 void orc_cg(orc_system* sys, const orc_step_params* prm, orc_step_stats* st);
-/* Mixed-precision solve (precision = 1): iterative refinement in FP64
- * around a block-Jacobi PCG on FP32 vectors and FP32 blocks whose dot
- * products and norms are FP64 (orc_dot's tree over float operands). */
+/* Mixed-precision solve (precision = 1): iterative refinement in FP64 with
+ * the FP64 system's residual around a block-Jacobi PCG on FP32 vectors and
+ * FP32 blocks whose dot products and norms are FP64 (orc_dot's tree over
+ * float operands). */
 void orc_cg_mixed(orc_system* sys, const orc_step_params* prm, orc_step_stats* st);
 /* block-Jacobi inverse, then val32 / diag32 / minv32 = float(val / diag / minv) */
 void orc_mixed_matrix(orc_system* sys);

@@ -311,7 +311,8 @@ void launch_mx_spmv(const MxArgs& a, hipStream_t s);
 void launch_mx_update(const MxArgs& a, hipStream_t s);  // alpha, update, beta
 void launch_mx_dir(const MxArgs& a, hipStream_t s);
 void launch_mx_apply(const MxArgs& a, hipStream_t s);
-// y = A32 x with FP64 arithmetic (the residual operator), and y32 = A32 x32
+// y = A32 x with FP64 arithmetic (float bond blocks widened; pd_spmv after a
+// mixed assembly), and y32 = A32 x32
 void launch_spmv_wide(const MxArgs& a, const double4* x, double4* y, hipStream_t s);
 void launch_spmv_f32(const MxArgs& a, const float4* x, float4* y, hipStream_t s);
 // consts non-null: dt from there
