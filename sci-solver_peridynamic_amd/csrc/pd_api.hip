@@ -370,17 +370,30 @@ int grid_over(pd_ctx* c, double cell, const double* mn, const int32_t* c0, const
     if (int e = dalloc(c, &c->cell_end, (size_t)cap)) return e;
     c->cell_cap = cap;
   }
-  pd::launch_cell_keys(c->x, subset, m, mn, cell, c0, dims, c->keys_a, c->idx_a, c->stream);
-  CHECK_LAUNCH(c, "cell keys");
   int bits = 1;
   while (bits < 32 && ((int64_t)1 << bits) < ncell) ++bits;
-  const int where = pd::radix_sort(c->keys_a, c->idx_a, c->keys_b, c->idx_b, m, bits, c->hist,
-                                   c->stream);
-  CHECK_LAUNCH(c, "radix sort");
+  // a candidate subset of contact size: the whole build in one launch
+  const bool small = subset && m > 0 && m <= PD_SMALL_GRID_MAX_M &&
+                     ncell <= PD_SMALL_GRID_MAX_CELLS;
+  int where;
+  if (small) {
+    where = pd::launch_small_grid(c->x, subset, m, mn, cell, c0, dims, (int32_t)ncell, bits,
+                                  c->keys_a, c->idx_a, c->keys_b, c->idx_b, c->cell_start,
+                                  c->cell_end, c->stream);
+    CHECK_LAUNCH(c, "small grid");
+  } else {
+    pd::launch_cell_keys(c->x, subset, m, mn, cell, c0, dims, c->keys_a, c->idx_a, c->stream);
+    CHECK_LAUNCH(c, "cell keys");
+    where = pd::radix_sort(c->keys_a, c->idx_a, c->keys_b, c->idx_b, m, bits, c->hist,
+                           c->stream);
+    CHECK_LAUNCH(c, "radix sort");
+  }
   const uint32_t* keys = where ? c->keys_b : c->keys_a;
   const int32_t* idx = where ? c->idx_b : c->idx_a;
-  pd::launch_cell_ranges(keys, m, c->cell_start, c->cell_end, ncell, c->stream);
-  CHECK_LAUNCH(c, "cell ranges");
+  if (!small) {
+    pd::launch_cell_ranges(keys, m, c->cell_start, c->cell_end, ncell, c->stream);
+    CHECK_LAUNCH(c, "cell ranges");
+  }
   g->sorted_key = keys;
   g->sorted_idx = idx;
   g->cell_start = c->cell_start;
@@ -556,6 +569,28 @@ int enqueue_contact(pd_ctx* c, const pd_step_params* prm) {
       ulo[k] = std::min(ulo[k], z.lo[b][k]);
       uhi[k] = std::max(uhi[k], z.hi[b][k]);
     }
+  // A candidate of body o lies in o's own box and in the zone of another
+  // body b, so every candidate's cell lies in the box over those
+  // intersections; the sub-grid spans just that box.  Its cells keep their
+  // full-grid coordinates and the cells it leaves out hold no candidate, so
+  // each contact sum runs over the same members in the same order.
+  double glo[3] = {INFINITY, INFINITY, INFINITY}, ghi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int b = 0; b < nb; ++b)
+    for (int o = 0; o < nb; ++o) {
+      if (o == b) continue;
+      double lo[3], hi[3];
+      bool meet = true;
+      for (int k = 0; k < 3; ++k) {
+        lo[k] = std::max(z.lo[b][k], bb[o][k]);
+        hi[k] = std::min(z.hi[b][k], bb[o][3 + k]);
+        meet = meet && lo[k] <= hi[k];
+      }
+      if (!meet) continue;
+      for (int k = 0; k < 3; ++k) {
+        glo[k] = std::min(glo[k], lo[k]);
+        ghi[k] = std::max(ghi[k], hi[k]);
+      }
+    }
   const size_t tw = pd::zone_tile_words(c->n);
   pd::launch_zone_select(c->x, c->body, c->n, z, c->zone_cnt, c->cand, c->stream);
   CHECK_LAUNCH(c, "contact candidates");
@@ -570,8 +605,9 @@ int enqueue_contact(pd_ctx* c, const pd_step_params* prm) {
   int32_t c0[3] = {0, 0, 0}, dims[3] = {1, 1, 1};
   if (m > 0) {
     for (int k = 0; k < 3; ++k) {
-      const double lo = std::max(ulo[k], mn[k]);
-      const double hi = std::min(uhi[k], mx[k]);
+      const bool tight = glo[k] <= ghi[k];  // always, when a candidate exists
+      const double lo = tight ? glo[k] : std::max(ulo[k], mn[k]);
+      const double hi = tight ? ghi[k] : std::min(uhi[k], mx[k]);
       c0[k] = cell_index(lo, mn[k], cell);
       dims[k] = cell_index(hi, mn[k], cell) - c0[k] + 1;
       if (dims[k] < 1) dims[k] = 1;

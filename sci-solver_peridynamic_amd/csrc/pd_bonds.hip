@@ -467,6 +467,159 @@ void launch_cell_ranges(const uint32_t* keys, int32_t n, int32_t* start, int32_t
   if (nb) hipLaunchKernelGGL(k_cell_ranges, dim3(nb), dim3(PD_BLOCK), 0, s, keys, n, start, end);
 }
 
+// ------------------------------------------------------- small sub-grids
+// The per-step contact grid holds a few thousand candidates over a few
+// thousand cells.  One 1024-thread workgroup then does the whole build that
+// k_cell_keys, the radix passes and the cell-range kernels do in eleven or
+// more launches: cell start / end cleared, keys, the same stable LSD radix
+// sort (8-bit digits; each chunk of 1024 keys ranked by digit with eight
+// ballots per wave and per-wave digit counts in LDS), then the ranges.
+// Results equal the multi-launch build's.  Ping-pong buffers are global
+// (L2-resident at these sizes); __syncthreads orders their accesses.
+#define SG_THREADS 1024
+#define SG_WAVES (SG_THREADS / 64)
+
+struct SmallGridArgs {
+  const double4* x;
+  const int32_t* subset;
+  int m;
+  double mn[3];
+  double cell;
+  int c0[3];
+  int dims[3];
+  int ncell;
+  int passes;
+  uint32_t* keys_a;
+  int32_t* idx_a;
+  uint32_t* keys_b;
+  int32_t* idx_b;
+  int32_t* start;
+  int32_t* end;
+};
+
+__global__ __launch_bounds__(SG_THREADS) void k_small_grid(SmallGridArgs a) {
+  __shared__ uint32_t base[256];
+  __shared__ uint32_t run[256];
+  __shared__ uint32_t wcnt[SG_WAVES][256];
+  const int t = threadIdx.x;
+  const int w = t >> 6;
+  const int lane = t & 63;
+  const unsigned long long lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+  for (int c = t; c < a.ncell; c += SG_THREADS) {
+    a.start[c] = 0;
+    a.end[c] = 0;
+  }
+  for (int i = t; i < a.m; i += SG_THREADS) {
+    const int p = a.subset[i];
+    int cx, cy, cz;
+    cell_of(a.x[p], a.mn, a.cell, cx, cy, cz);
+    cx -= a.c0[0];
+    cy -= a.c0[1];
+    cz -= a.c0[2];
+    a.keys_a[i] = (uint32_t)(((int64_t)cz * a.dims[1] + cy) * a.dims[0] + cx);
+    a.idx_a[i] = p;
+  }
+  uint32_t* kin = a.keys_a;
+  int32_t* vin = a.idx_a;
+  uint32_t* kout = a.keys_b;
+  int32_t* vout = a.idx_b;
+  for (int ps = 0; ps < a.passes; ++ps) {
+    const int shift = 8 * ps;
+    if (t < 256) {
+      base[t] = 0;
+      run[t] = 0;
+    }
+    __syncthreads();
+    for (int i = t; i < a.m; i += SG_THREADS) atomicAdd(&base[(kin[i] >> shift) & 255u], 1u);
+    __syncthreads();
+    if (w == 0) {  // exclusive scan of the 256 digit counts, four per lane
+      const uint32_t h0 = base[4 * lane], h1 = base[4 * lane + 1], h2 = base[4 * lane + 2],
+                     h3 = base[4 * lane + 3];
+      const uint32_t sum = h0 + h1 + h2 + h3;
+      uint32_t incl = sum;
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t v = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += v;
+      }
+      const uint32_t ex = incl - sum;
+      base[4 * lane] = ex;
+      base[4 * lane + 1] = ex + h0;
+      base[4 * lane + 2] = ex + h0 + h1;
+      base[4 * lane + 3] = ex + h0 + h1 + h2;
+    }
+    for (int chunk = 0; chunk < a.m; chunk += SG_THREADS) {
+      const int i = chunk + t;
+      const bool valid = i < a.m;
+      const uint32_t k = valid ? kin[i] : 0u;
+      const int32_t v = valid ? vin[i] : 0;
+      const uint32_t d = (k >> shift) & 255u;
+      for (int j = t; j < SG_WAVES * 256; j += SG_THREADS) (&wcnt[0][0])[j] = 0u;
+      __syncthreads();
+      unsigned long long peers = __ballot(valid);
+#pragma unroll
+      for (int b = 0; b < 8; ++b) {
+        const bool bit = (d >> b) & 1u;
+        const unsigned long long msk = __ballot(bit);
+        peers &= bit ? msk : ~msk;
+      }
+      const uint32_t rank = (uint32_t)__popcll(peers & lt);
+      if (valid && rank == 0) wcnt[w][d] = (uint32_t)__popcll(peers);
+      __syncthreads();
+      if (valid) {
+        uint32_t pos = base[d] + run[d] + rank;
+        for (int ww = 0; ww < w; ++ww) pos += wcnt[ww][d];
+        kout[pos] = k;
+        vout[pos] = v;
+      }
+      __syncthreads();
+      if (t < 256) {
+        uint32_t add = 0;
+        for (int ww = 0; ww < SG_WAVES; ++ww) add += wcnt[ww][t];
+        run[t] += add;
+      }
+      __syncthreads();  // wcnt is cleared next; run and kout are read next
+    }
+    uint32_t* tk = kin;
+    kin = kout;
+    kout = tk;
+    int32_t* tv = vin;
+    vin = vout;
+    vout = tv;
+  }
+  for (int i = t; i < a.m; i += SG_THREADS) {
+    const uint32_t k = kin[i];
+    if (i == 0 || kin[i - 1] != k) a.start[k] = i;
+    if (i == a.m - 1 || kin[i + 1] != k) a.end[k] = i + 1;
+  }
+}
+
+int launch_small_grid(const double4* x, const int32_t* subset, int32_t m, const double* mn,
+                      double cell, const int32_t* c0, const int32_t* dims, int32_t ncell,
+                      int32_t bits, uint32_t* keys_a, int32_t* idx_a, uint32_t* keys_b,
+                      int32_t* idx_b, int32_t* start, int32_t* end, hipStream_t s) {
+  SmallGridArgs a;
+  a.x = x;
+  a.subset = subset;
+  a.m = m;
+  for (int k = 0; k < 3; ++k) {
+    a.mn[k] = mn[k];
+    a.c0[k] = c0[k];
+    a.dims[k] = dims[k];
+  }
+  a.cell = cell;
+  a.ncell = ncell;
+  a.passes = (bits + 7) / 8;
+  a.keys_a = keys_a;
+  a.idx_a = idx_a;
+  a.keys_b = keys_b;
+  a.idx_b = idx_b;
+  a.start = start;
+  a.end = end;
+  hipLaunchKernelGGL(k_small_grid, dim3(1), dim3(SG_THREADS), 0, s, a);
+  return a.passes & 1;
+}
+
 // ---------------------------------------------------------- neighbour test
 // core/kdtree.h:66-79 dist() with target p: dx = x_p - x_q
 __device__ __forceinline__ bool in_horizon(const double4 a, const double4 b, double h) {

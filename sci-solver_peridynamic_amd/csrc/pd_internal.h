@@ -377,6 +377,16 @@ int radix_sort(uint32_t* keys_a, int32_t* idx_a, uint32_t* keys_b, int32_t* idx_
 size_t radix_hist_words(int32_t n);
 void launch_cell_ranges(const uint32_t* keys, int32_t n, int32_t* start,
                         int32_t* end, int64_t ncell, hipStream_t s);
+// Bounds of the one-workgroup build below
+#define PD_SMALL_GRID_MAX_M 16384
+#define PD_SMALL_GRID_MAX_CELLS 65536
+// keys, stable sort and cell start / end of subset[0..m) in one launch
+// (m <= PD_SMALL_GRID_MAX_M, ncell <= PD_SMALL_GRID_MAX_CELLS); returns which
+// buffer pair (0: a, 1: b) holds the sorted keys and indices
+int launch_small_grid(const double4* x, const int32_t* subset, int32_t m, const double* mn,
+                      double cell, const int32_t* c0, const int32_t* dims, int32_t ncell,
+                      int32_t bits, uint32_t* keys_a, int32_t* idx_a, uint32_t* keys_b,
+                      int32_t* idx_b, int32_t* start, int32_t* end, hipStream_t s);
 // INVISIBLE particles take no bonds and are no one's neighbour
 void launch_bond_count(int32_t n, const double4* x, const int32_t* body, const int32_t* act,
                        const double* horizon, const GridView& g, int32_t* counts,

@@ -180,16 +180,42 @@ def test_tension_steps_small():
     ctx.close()
 
 
-def test_contact_bit_exact():
-    say("case contact")
-    a = lattice(8, 8, 4, 0.01, 0.05, 1)
-    b = lattice(4, 4, 4, 0.0047, 0.05, 2, origin=(0.02, 0.02, 0.0335))
-    x4 = np.concatenate([a, b])
-    body = np.r_[np.zeros(len(a)), np.ones(len(b))].astype(np.int32)
-    mats = [material(0.01), material(0.0047)]
+def contact_scene(case):
+    """(x4, body, materials, contact kwargs) of a contact case.  two_bodies and
+    three_bodies build their per-step grid in one workgroup (few candidates,
+    few cells); split_far's grid spans more than 65536 cells and interleaved
+    selects more than 16384 candidates, so both take the multi-launch build."""
+    if case == "two_bodies":
+        a = lattice(8, 8, 4, 0.01, 0.05, 1)
+        b = lattice(4, 4, 4, 0.0047, 0.05, 2, origin=(0.02, 0.02, 0.0335))
+        parts, mats, rc = [a, b], [material(0.01), material(0.0047)], 0.0075
+    elif case in ("three_bodies", "split_far"):
+        # a slab with a small block just above one corner and another just
+        # below the opposite corner (no bonds: horizon under the spacing)
+        a = lattice(30, 30, 10, 0.01, 0.05, 4)
+        b = lattice(3, 3, 3, 0.01, 0.05, 5, origin=(0.0, 0.0, 0.093))
+        c = lattice(3, 3, 3, 0.01, 0.05, 6, origin=(0.27, 0.27, -0.023))
+        parts = [a, b, c]
+        mats = [material(0.01, horizon_factor=0.5)] * 3
+        rc = 0.0075 if case == "three_bodies" else 0.004
+    else:  # interleaved: two bodies on the two sublattices of one lattice
+        x4 = lattice(28, 28, 28, 0.01, 0.05, 3)
+        i = np.arange(x4.shape[0])
+        body = ((i % 28 + (i // 28) % 28 + i // 784) % 2).astype(np.int32)
+        kw = dict(contact_radius=0.012, contact_stiffness=2.5e8, contact_volume=1.0,
+                  contact_cell=0.012)
+        return x4, body, [material(0.01, horizon_factor=0.5)] * 2, kw
+    x4 = np.concatenate(parts)
+    body = np.concatenate([np.full(len(p), k) for k, p in enumerate(parts)]).astype(np.int32)
+    kw = dict(contact_radius=rc, contact_stiffness=2.5e8, contact_volume=1.0, contact_cell=rc)
+    return x4, body, mats, kw
+
+
+@pytest.mark.parametrize("case", ["two_bodies", "three_bodies", "split_far", "interleaved"])
+def test_contact_bit_exact(case):
+    say("case contact", case)
+    x4, body, mats, kw = contact_scene(case)
     sys_, ctx = make_pair(x4, mats, body=body)
-    kw = dict(contact_radius=0.0075, contact_stiffness=2.5e8, contact_volume=1.0,
-              contact_cell=0.0075)
     sys_.contact(orc.make_params(**kw))
     ctx.assemble(pds.make_params(**kw))
     fc = ctx.download("contact_force")
