@@ -474,10 +474,14 @@ void launch_cell_ranges(const uint32_t* keys, int32_t n, int32_t* start, int32_t
 // more launches: cell start / end cleared, keys, the same stable LSD radix
 // sort (8-bit digits; each chunk of 1024 keys ranked by digit with eight
 // ballots per wave and per-wave digit counts in LDS), then the ranges.
-// Results equal the multi-launch build's.  Ping-pong buffers are global
-// (L2-resident at these sizes); __syncthreads orders their accesses.
+// Results equal the multi-launch build's.  Each thread keeps its elements
+// of the current order in registers; the order a pass writes goes through
+// global ping-pong buffers (L2-resident at these sizes), which
+// __syncthreads orders.
 #define SG_THREADS 1024
 #define SG_WAVES (SG_THREADS / 64)
+static_assert(PD_SMALL_GRID_MAX_M % (4 * SG_THREADS) == 0, "elements are handled four at a time");
+static_assert(PD_SMALL_GRID_MAX_CELLS <= 65536, "at most two 8-bit passes");
 
 struct SmallGridArgs {
   const double4* x;
@@ -497,6 +501,8 @@ struct SmallGridArgs {
   int32_t* end;
 };
 
+#define SG_PER (PD_SMALL_GRID_MAX_M / SG_THREADS)  // elements per thread
+
 __global__ __launch_bounds__(SG_THREADS) void k_small_grid(SmallGridArgs a) {
   __shared__ uint32_t base[256];
   __shared__ uint32_t run[256];
@@ -509,15 +515,31 @@ __global__ __launch_bounds__(SG_THREADS) void k_small_grid(SmallGridArgs a) {
     a.start[c] = 0;
     a.end[c] = 0;
   }
-  for (int i = t; i < a.m; i += SG_THREADS) {
-    const int p = a.subset[i];
-    int cx, cy, cz;
-    cell_of(a.x[p], a.mn, a.cell, cx, cy, cz);
-    cx -= a.c0[0];
-    cy -= a.c0[1];
-    cz -= a.c0[2];
-    a.keys_a[i] = (uint32_t)(((int64_t)cz * a.dims[1] + cy) * a.dims[0] + cx);
-    a.idx_a[i] = p;
+  // thread t holds elements j * SG_THREADS + t of the current order in
+  // registers, so every chunk below is ranked without a load in its path
+  uint32_t kr[SG_PER];
+  int32_t vr[SG_PER];
+#pragma unroll
+  for (int h = 0; h < SG_PER; h += 4) {  // keys, four gathers in flight
+    int pp[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int i = (h + j) * SG_THREADS + t;
+      pp[j] = i < a.m ? a.subset[i] : -1;
+    }
+    double4 xx[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) xx[j] = pp[j] >= 0 ? a.x[pp[j]] : make_double4(0.0, 0.0, 0.0, 0.0);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      int cx, cy, cz;
+      cell_of(xx[j], a.mn, a.cell, cx, cy, cz);
+      cx -= a.c0[0];
+      cy -= a.c0[1];
+      cz -= a.c0[2];
+      kr[h + j] = pp[j] >= 0 ? (uint32_t)(((int64_t)cz * a.dims[1] + cy) * a.dims[0] + cx) : 0u;
+      vr[h + j] = pp[j];
+    }
   }
   uint32_t* kin = a.keys_a;
   int32_t* vin = a.idx_a;
@@ -525,12 +547,22 @@ __global__ __launch_bounds__(SG_THREADS) void k_small_grid(SmallGridArgs a) {
   int32_t* vout = a.idx_b;
   for (int ps = 0; ps < a.passes; ++ps) {
     const int shift = 8 * ps;
+    if (ps > 0) {
+#pragma unroll
+      for (int j = 0; j < SG_PER; ++j) {
+        const int i = j * SG_THREADS + t;
+        kr[j] = i < a.m ? kin[i] : 0u;
+        vr[j] = i < a.m ? vin[i] : 0;
+      }
+    }
     if (t < 256) {
       base[t] = 0;
       run[t] = 0;
     }
     __syncthreads();
-    for (int i = t; i < a.m; i += SG_THREADS) atomicAdd(&base[(kin[i] >> shift) & 255u], 1u);
+#pragma unroll
+    for (int j = 0; j < SG_PER; ++j)
+      if (j * SG_THREADS + t < a.m) atomicAdd(&base[(kr[j] >> shift) & 255u], 1u);
     __syncthreads();
     if (w == 0) {  // exclusive scan of the 256 digit counts, four per lane
       const uint32_t h0 = base[4 * lane], h1 = base[4 * lane + 1], h2 = base[4 * lane + 2],
@@ -548,13 +580,13 @@ __global__ __launch_bounds__(SG_THREADS) void k_small_grid(SmallGridArgs a) {
       base[4 * lane + 2] = ex + h0 + h1;
       base[4 * lane + 3] = ex + h0 + h1 + h2;
     }
-    for (int chunk = 0; chunk < a.m; chunk += SG_THREADS) {
-      const int i = chunk + t;
-      const bool valid = i < a.m;
-      const uint32_t k = valid ? kin[i] : 0u;
-      const int32_t v = valid ? vin[i] : 0;
+#pragma unroll
+    for (int j = 0; j < SG_PER; ++j) {
+      if (j * SG_THREADS >= a.m) break;  // uniform
+      const bool valid = j * SG_THREADS + t < a.m;
+      const uint32_t k = kr[j];
       const uint32_t d = (k >> shift) & 255u;
-      for (int j = t; j < SG_WAVES * 256; j += SG_THREADS) (&wcnt[0][0])[j] = 0u;
+      for (int q = t; q < SG_WAVES * 256; q += SG_THREADS) (&wcnt[0][0])[q] = 0u;
       __syncthreads();
       unsigned long long peers = __ballot(valid);
 #pragma unroll
@@ -570,7 +602,7 @@ __global__ __launch_bounds__(SG_THREADS) void k_small_grid(SmallGridArgs a) {
         uint32_t pos = base[d] + run[d] + rank;
         for (int ww = 0; ww < w; ++ww) pos += wcnt[ww][d];
         kout[pos] = k;
-        vout[pos] = v;
+        vout[pos] = vr[j];
       }
       __syncthreads();
       if (t < 256) {
@@ -587,10 +619,24 @@ __global__ __launch_bounds__(SG_THREADS) void k_small_grid(SmallGridArgs a) {
     vin = vout;
     vout = tv;
   }
-  for (int i = t; i < a.m; i += SG_THREADS) {
-    const uint32_t k = kin[i];
-    if (i == 0 || kin[i - 1] != k) a.start[k] = i;
-    if (i == a.m - 1 || kin[i + 1] != k) a.end[k] = i + 1;
+#pragma unroll
+  for (int h = 0; h < SG_PER; h += 4) {  // cell ranges, the loads of four elements in flight
+    uint32_t kc[4], kp[4], kn[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int i = (h + j) * SG_THREADS + t;
+      kc[j] = i < a.m ? kin[i] : 0u;
+      kp[j] = i > 0 && i < a.m ? kin[i - 1] : 0u;
+      kn[j] = i + 1 < a.m ? kin[i + 1] : 0u;
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int i = (h + j) * SG_THREADS + t;
+      if (i < a.m) {
+        if (i == 0 || kp[j] != kc[j]) a.start[kc[j]] = i;
+        if (i == a.m - 1 || kn[j] != kc[j]) a.end[kc[j]] = i + 1;
+      }
+    }
   }
 }
 
