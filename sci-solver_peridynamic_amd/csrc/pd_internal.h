@@ -118,15 +118,20 @@ struct Batch {
 __device__ __forceinline__ int spec_of(const Batch& b, int tile) {
   return b.tile_spec ? b.tile_spec[tile] : 0;
 }
-// tile range of the specimen folded by this workgroup (blockIdx.x)
-__device__ __forceinline__ void spec_range(const Batch& b, int ntiles, int& t0, int& nt) {
+// tile range of specimen s
+__device__ __forceinline__ void spec_range_of(const Batch& b, int s, int ntiles, int& t0,
+                                              int& nt) {
   if (b.spec_tile0) {
-    t0 = b.spec_tile0[blockIdx.x];
-    nt = b.spec_ntiles[blockIdx.x];
+    t0 = b.spec_tile0[s];
+    nt = b.spec_ntiles[s];
   } else {
     t0 = 0;
     nt = ntiles;
   }
+}
+// tile range of the specimen folded by this workgroup (blockIdx.x)
+__device__ __forceinline__ void spec_range(const Batch& b, int ntiles, int& t0, int& nt) {
+  spec_range_of(b, blockIdx.x, ntiles, t0, nt);
 }
 
 struct CgArgs {
@@ -140,7 +145,7 @@ struct CgArgs {
   double4* z;
   double4* p;
   double4* y;
-  double* part;  // [3 * ntiles]
+  double* part;  // [4 * ntiles]
   pd_cg_state* st;  // [batch.nspec]
   int32_t precond;
   // stored-block FP64 steps whose assembly left the blocks to the first

@@ -316,36 +316,42 @@ __global__ __launch_bounds__(PD_BLOCK) void k_mx_spmv(MxArgs a) {
   if (threadIdx.x == 0) a.part[tile] = v;
 }
 
-__global__ __launch_bounds__(PD_BLOCK) void k_mx_alpha(MxArgs a, int ntiles) {
-  pd_mx_state* st = a.st + blockIdx.x;
-  if (st->phase != 1) return;
-  __shared__ double lds4[4];
-  int t0, nt;
-  spec_range(a.batch, ntiles, t0, nt);
-  const double yp = fold_partials(a.part + t0, nt, lds4);
-  if (threadIdx.x == 0) st->alpha = st->rz / yp;
-}
-
-// d += a p; r -= a y (free rows); z = M32 r; partials <r,z>, <r,r>
+// alpha = rz / <y,p>, formed by every workgroup from its specimen's <y,p>
+// partials in the one fold order; then d += a p; r -= a y (free rows);
+// z = M32 r; partials <r,z>, <r,r> into the third and fourth partial rows
+// (the first row is still being folded by other workgroups)
 __global__ __launch_bounds__(PD_BLOCK) void k_mx_update(MxArgs a) {
   __shared__ double lds4[4];
   const int ntiles = gridDim.x;
   const int tile = xcd_tile(blockIdx.x, ntiles);
-  const pd_mx_state* st = a.st + spec_of(a.batch, tile);
+  const int s = spec_of(a.batch, tile);
+  pd_mx_state* st = a.st + s;
   if (st->phase != 1) return;
   const int p = tile * PD_BLOCK + threadIdx.x;
-  const float af = (float)st->alpha;
+  const bool row = p < a.n;
+  // the row's loads go out ahead of the fold
+  float4 pp = make_float4(0.0f, 0.0f, 0.0f, 0.0f), dd = pp, rr = pp, yy = pp;
+  bool fr = false;
+  if (row) {
+    pp = a.p[p];
+    dd = a.d[p];
+    rr = a.r[p];
+    fr = a.bc[p] == 0;
+    if (fr) yy = a.y[p];
+  }
+  int t0, nt;
+  spec_range_of(a.batch, s, ntiles, t0, nt);
+  const double yp = fold_partials(a.part + t0, nt, lds4);
+  const double alpha = st->rz / yp;
+  if (tile == t0 && threadIdx.x == 0) st->alpha = alpha;
+  const float af = (float)alpha;
   double t_rz = 0.0, t_rr = 0.0;
-  if (p < a.n) {
-    const float4 pp = a.p[p];
-    float4 dd = a.d[p];
+  if (row) {
     dd.x = af * pp.x + dd.x;
     dd.y = af * pp.y + dd.y;
     dd.z = af * pp.z + dd.z;
     a.d[p] = dd;
-    float4 rr = a.r[p];
-    if (a.bc[p] == 0) {
-      const float4 yy = a.y[p];
+    if (fr) {
       const float na = -af;
       rr.x = na * yy.x + rr.x;
       rr.y = na * yy.y + rr.y;
@@ -361,8 +367,8 @@ __global__ __launch_bounds__(PD_BLOCK) void k_mx_update(MxArgs a) {
   const double v0 = tile_sum(t_rz, lds4);
   const double v1 = tile_sum(t_rr, lds4);
   if (threadIdx.x == 0) {
-    a.part[tile] = v0;
-    a.part[ntiles + tile] = v1;
+    a.part[2 * (int64_t)ntiles + tile] = v0;
+    a.part[3 * (int64_t)ntiles + tile] = v1;
   }
 }
 
@@ -372,8 +378,8 @@ __global__ __launch_bounds__(PD_BLOCK) void k_mx_beta(MxArgs a, int ntiles) {
   __shared__ double lds4[4];
   int t0, nt;
   spec_range(a.batch, ntiles, t0, nt);
-  const double rz = fold_partials(a.part + t0, nt, lds4);
-  const double rr32 = fold_partials(a.part + ntiles + t0, nt, lds4);
+  const double rz = fold_partials(a.part + 2 * (int64_t)ntiles + t0, nt, lds4);
+  const double rr32 = fold_partials(a.part + 3 * (int64_t)ntiles + t0, nt, lds4);
   if (threadIdx.x == 0) {
     pd_mx_state* st = a.st + blockIdx.x;
     st->beta = rz / st->rz;
@@ -469,7 +475,6 @@ void launch_mx_spmv(const MxArgs& a, hipStream_t s) {
 void launch_mx_update(const MxArgs& a, hipStream_t s) {
   if (!a.n) return;
   const int g = tiles_of(a.n);
-  hipLaunchKernelGGL(k_mx_alpha, dim3(a.batch.nspec), dim3(PD_BLOCK), 0, s, a, g);
   hipLaunchKernelGGL(k_mx_update, dim3(g), dim3(PD_BLOCK), 0, s, a);
   hipLaunchKernelGGL(k_mx_beta, dim3(a.batch.nspec), dim3(PD_BLOCK), 0, s, a, g);
 }

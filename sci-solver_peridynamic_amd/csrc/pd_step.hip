@@ -648,11 +648,12 @@ void launch_spmv_plain(const SpmvView& A, const double4* x, double4* y, hipStrea
 // ---------------------------------------------------------------------------
 // CG (CUSP krylov::cg structure, core/simulator.cu:763-770) with scalars
 // kept on the device.  Every streaming kernel writes one partial sum per
-// 256-row tile; a fold kernel with one workgroup per specimen then folds
-// that specimen's partials in the fixed order of the oracle's orc_dot and
-// publishes alpha, or beta plus the monitor test.  Kernel boundaries order
-// the writes, so no fences are needed.  The kernels of a tile or specimen
-// return at once when its solve is done.
+// 256-row tile; the partials of a specimen are folded in the fixed order of
+// the oracle's orc_dot, by a fold kernel with one workgroup per specimen
+// (the start, beta plus the monitor test) or by every workgroup of the
+// update for itself (alpha).  Kernel boundaries order the writes, so no
+// fences are needed.  The kernels of a tile or specimen return at once when
+// its solve is done.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void precond(const double* minv, int pc, int p, double r0,
                                         double r1, double r2, double& z0, double& z1,
@@ -855,40 +856,45 @@ __global__ __launch_bounds__(PD_BLOCK) void k_write_blocks(SpmvView A, double* v
   }
 }
 
-// alpha = rz / <y,p>
-__global__ __launch_bounds__(PD_BLOCK) void k_cg_alpha(CgArgs a, int ntiles) {
-  pd_cg_state* st = a.st + blockIdx.x;
-  if (st->done) return;
-  __shared__ double lds4[4];
-  int t0, nt;
-  spec_range(a.batch, ntiles, t0, nt);
-  const double yp = fold_partials(a.part + t0, nt, lds4);
-  if (threadIdx.x == 0) {
-    st->yp = yp;
-    st->alpha = st->rz / yp;
-  }
-}
-
+// alpha = rz / <y,p>, formed by every workgroup from its specimen's <y,p>
+// partials in the one fold order (so all hold the same value); then
 // x += alpha p; r -= alpha y (free rows); z = M r; partials <r,z>, <r,r>
+// into the third and fourth partial rows (the first row is still being
+// folded by other workgroups).
 __global__ __launch_bounds__(PD_BLOCK) void k_cg_update(CgArgs a) {
   __shared__ double lds4[4];
   const int ntiles = gridDim.x;
   const int tile = xcd_tile(blockIdx.x, ntiles);
-  const pd_cg_state* st = a.st + spec_of(a.batch, tile);
+  const int s = spec_of(a.batch, tile);
+  pd_cg_state* st = a.st + s;
   if (st->done) return;
   const int p = tile * PD_BLOCK + threadIdx.x;
-  const double alpha = st->alpha;
+  const bool row = p < a.A.n;
+  // the row's loads go out ahead of the fold
+  double4 pp = make_double4(0.0, 0.0, 0.0, 0.0), xx = pp, rr = pp, yy = pp;
+  bool fr = false;
+  if (row) {
+    pp = a.p[p];
+    xx = a.x[p];
+    rr = a.r[p];
+    fr = a.bc[p] == 0;
+    if (fr) yy = a.y[p];
+  }
+  int t0, nt;
+  spec_range_of(a.batch, s, ntiles, t0, nt);
+  const double yp = fold_partials(a.part + t0, nt, lds4);
+  const double alpha = st->rz / yp;
+  if (tile == t0 && threadIdx.x == 0) {
+    st->yp = yp;
+    st->alpha = alpha;
+  }
   double t_rz = 0.0, t_rr = 0.0;
-  if (p < a.A.n) {
-    const double4 pp = a.p[p];
-    double4 xx = a.x[p];
+  if (row) {
     xx.x = alpha * pp.x + xx.x;
     xx.y = alpha * pp.y + xx.y;
     xx.z = alpha * pp.z + xx.z;
     a.x[p] = xx;
-    double4 rr = a.r[p];
-    if (a.bc[p] == 0) {
-      const double4 yy = a.y[p];
+    if (fr) {
       const double na = -alpha;
       rr.x = na * yy.x + rr.x;
       rr.y = na * yy.y + rr.y;
@@ -904,8 +910,8 @@ __global__ __launch_bounds__(PD_BLOCK) void k_cg_update(CgArgs a) {
   const double v0 = tile_sum(t_rz, lds4);
   const double v1 = tile_sum(t_rr, lds4);
   if (threadIdx.x == 0) {
-    a.part[tile] = v0;
-    a.part[ntiles + tile] = v1;
+    a.part[2 * (int64_t)ntiles + tile] = v0;
+    a.part[3 * (int64_t)ntiles + tile] = v1;
   }
 }
 
@@ -916,8 +922,8 @@ __global__ __launch_bounds__(PD_BLOCK) void k_cg_beta(CgArgs a, int ntiles) {
   __shared__ double lds4[4];
   int t0, nt;
   spec_range(a.batch, ntiles, t0, nt);
-  const double rz = fold_partials(a.part + t0, nt, lds4);
-  const double rr = fold_partials(a.part + ntiles + t0, nt, lds4);
+  const double rz = fold_partials(a.part + 2 * (int64_t)ntiles + t0, nt, lds4);
+  const double rr = fold_partials(a.part + 3 * (int64_t)ntiles + t0, nt, lds4);
   if (threadIdx.x == 0) {
     pd_cg_state* st = a.st + blockIdx.x;
     const double rz_old = st->rz;
@@ -1004,7 +1010,6 @@ void launch_write_blocks(const SpmvView& A, double* vals, hipStream_t s) {
 void launch_cg_update(const CgArgs& a, hipStream_t s) {
   if (!a.A.n) return;
   const int g = tiles_of(a.A.n);
-  hipLaunchKernelGGL(k_cg_alpha, dim3(a.batch.nspec), dim3(PD_BLOCK), 0, s, a, g);
   hipLaunchKernelGGL(k_cg_update, dim3(g), dim3(PD_BLOCK), 0, s, a);
   hipLaunchKernelGGL(k_cg_beta, dim3(a.batch.nspec), dim3(PD_BLOCK), 0, s, a, g);
 }
