@@ -63,7 +63,7 @@ int pdh_describe(const char* params_path, char* buf, int32_t len) {
       "\"stretch_seed\": %u, \"final_step\": %d, \"adaptive_integration\": %d, "
       "\"scale_factor\": %.17g, \"projectile_spacing\": %.17g, \"precision\": %d, "
       "\"inner_rtol\": %.17g, \"max_refine\": %d, \"contact_mode\": %d, "
-      "\"matrix_free\": %d}",
+      "\"matrix_free\": %d, \"cell_size\": %.17g, \"c_times_v\": %.17g}",
       s.num_pd_particle, scene_ok ? 1 : 0, s.num_pd_bodies, s.pd_horizon, s.pd_particle_radius,
       s.pd_c_bond, s.pd_particle_mass, s.projectile_horizon, s.projectile_particle_radius,
       s.projectile_c_bond, s.pd_particle_mass, dt, s.pd_gravity, s.pd_phase_in,
@@ -74,7 +74,7 @@ int pdh_describe(const char* params_path, char* buf, int32_t len) {
       r.stretch_seed, r.final_step, r.adaptive_integration, s.scaleFactor, s.projectile_spacing,
       r.cg_precision, r.cg_inner_rtol > 0.0 ? r.cg_inner_rtol : 1e-5,
       r.cg_max_refine > 0 ? r.cg_max_refine : 20, r.contact_mode,
-      r.cg_precision == 0 ? r.cg_matrix_free : 0);
+      r.cg_precision == 0 ? r.cg_matrix_free : 0, s.cell_size, s.pd_C_times_V);
   if (w < 0 || w >= len || !buf) return -1;
   memcpy(buf, tmp, (size_t)w + 1);
   return w;
