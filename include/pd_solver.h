@@ -335,6 +335,24 @@ int pd_trainer_add(pd_trainer* t, pd_ctx* ctx, uint64_t seed, int32_t stride, in
 int pd_trainer_fit(pd_trainer* t, int32_t steps, int32_t batch, float lr, uint64_t seed,
                    float* losses);
 int pd_trainer_params(pd_trainer* t, float* params, int32_t count);
+
+/* pd_trainer_fit without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory while they
+ * run (code under HIP graph capture, tight serving or training loops,
+ * harnesses that forbid hipMalloc).  pd_trainer_fit allocates its scratch
+ * (per-wave gradient partials, batch / 32 rows of PD_LEARNED_PARAMS - 32
+ * floats, and one loss sum per step) when a call needs more than an earlier
+ * one; this variant keeps both in one library-owned device array per GPU
+ * instead, sized for batch <= PD_FIT_NOALLOC_MAX_BATCH and
+ * steps <= PD_FIT_NOALLOC_MAX_STEPS (larger requests fail with
+ * PD_ERR_ARG before any work), and otherwise runs the same kernels with the
+ * same results.  Every call on one GPU shares that array: calls that overlap
+ * on one GPU (two trainers on different streams or threads) race and must
+ * be serialised by the caller. */
+#define PD_FIT_NOALLOC_MAX_BATCH 4096
+#define PD_FIT_NOALLOC_MAX_STEPS 16384
+int pd_trainer_fit_noalloc(pd_trainer* t, int32_t steps, int32_t batch, float lr, uint64_t seed,
+                           float* losses);
 /* samples [first, first + count), 28 floats each: features[16], D (6),
  * (A e)_p (3), e_p (3) */
 int pd_trainer_samples(pd_trainer* t, int64_t first, int64_t count, float* out);

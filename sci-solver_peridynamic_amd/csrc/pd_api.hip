@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <functional>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -2067,6 +2068,44 @@ int pd_get_precond(pd_ctx* c, double* minv9) {
   return PD_OK;
 }
 
+// Device scratch of pd_trainer_fit_noalloc, one copy per GPU for the life of
+// the process: rows of per-wave gradient partials (PC_NWEIGHT + 1 floats
+// each, for batches up to PD_FIT_NOALLOC_MAX_BATCH samples), then one loss
+// sum per step (up to PD_FIT_NOALLOC_MAX_STEPS).  k_pc_train writes every
+// entry of a wave's row and k_pc_adam every loss it is given, so nothing
+// needs clearing between calls.  Host code never passes `scratch` itself to
+// a kernel (in host code the name is not a device address): it passes the
+// address hipGetSymbolAddress returned for the current device.
+#define PD_FIT_PARTIAL_FLOATS ((PD_FIT_NOALLOC_MAX_BATCH / 32) * (PC_NWEIGHT + 1))
+#define PD_FIT_SCRATCH_FLOATS (PD_FIT_PARTIAL_FLOATS + PD_FIT_NOALLOC_MAX_STEPS)
+static __device__ __align__(256) float scratch[PD_FIT_SCRATCH_FLOATS];
+
+#define PD_SCRATCH_DEVICES 64
+static std::mutex g_scratch_mu;
+static float* g_scratch_ptr[PD_SCRATCH_DEVICES];
+
+// The device address of `scratch` on `device`, looked up once per device
+// (the device must be current).  Called by pd_trainer_create, so a trainer
+// has its GPU's address before any fit.
+static int scratch_address(int device, float** out, std::string& err) {
+  if (device < 0 || device >= PD_SCRATCH_DEVICES) {
+    err = "device ordinal outside the scratch table";
+    return PD_ERR_ARG;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+      return PD_ERR_HIP;
+    }
+    g_scratch_ptr[device] = (float*)ptr;
+  }
+  *out = g_scratch_ptr[device];
+  return PD_OK;
+}
+
 static int tfail(pd_trainer* t, int code, const std::string& msg) {
   if (t) t->err = msg;
   g_last_error = msg;
@@ -2095,6 +2134,11 @@ int pd_trainer_create(int32_t device, int64_t capacity, uint64_t seed, pd_traine
   if (hipSetDevice(device) != hipSuccess) return bail(tfail(t, PD_ERR_ARG, "bad device"));
   if (hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking) != hipSuccess)
     return bail(tfail(t, PD_ERR_HIP, "hipStreamCreate failed"));
+  {  // the no-allocation fit's scratch on this device (allocates nothing)
+    float* sp = nullptr;
+    std::string msg;
+    if (int e = scratch_address(device, &sp, msg)) return bail(tfail(t, e, msg));
+  }
   if (hipMalloc((void**)&t->samples, sizeof(float) * PC_SAMPLE * capacity) != hipSuccess ||
       hipMalloc((void**)&t->p, sizeof(float) * PC_NPARAM) != hipSuccess ||
       hipMalloc((void**)&t->m, sizeof(float) * PC_NWEIGHT) != hipSuccess ||
@@ -2206,12 +2250,58 @@ int pd_trainer_add(pd_trainer* t, pd_ctx* c, uint64_t seed, int32_t stride, int6
   return PD_OK;
 }
 
-int pd_trainer_fit(pd_trainer* t, int32_t steps, int32_t batch, float lr, uint64_t seed,
-                   float* losses) {
+}  // extern "C"
+
+namespace {
+// The fit loop of pd_trainer_fit and pd_trainer_fit_noalloc: `steps` Adam
+// steps on batches drawn from the trainer's samples, per-wave gradient
+// partials in `partial` (batch / 32 rows), loss sums in `loss` (steps).
+int run_fit(pd_trainer* t, int32_t steps, int32_t batch, float lr, uint64_t seed, float* partial,
+            float* loss, float* losses) {
+  const int32_t waves = batch / 32;
+  if (!t->normed) {
+    pd::launch_pc_norm(t->samples, t->count, t->p, t->stream);
+    t->normed = true;
+  }
+  for (int32_t k = 0; k < steps; ++k) {
+    pd::PcTrainArgs a;
+    a.samples = t->samples;
+    a.nsamples = t->count;
+    a.batch = batch;
+    a.seed = seed;
+    a.step = t->steps_done;
+    a.w = t->w;
+    a.p = t->p;
+    a.partial = partial;
+    pd::launch_pc_train(a, t->stream);
+    pd::launch_pc_adam(partial, waves, t->p, t->m, t->v, t->w, t->steps_done, lr, loss + k,
+                       t->stream);
+    ++t->steps_done;
+  }
+  THIPCHK(t, hipGetLastError());
+  if (losses)
+    THIPCHK(t, hipMemcpyAsync(losses, loss, sizeof(float) * steps, hipMemcpyDeviceToHost,
+                              t->stream));
+  THIPCHK(t, hipStreamSynchronize(t->stream));
+  if (losses)
+    for (int32_t k = 0; k < steps; ++k) losses[k] /= (float)batch;  // mean |M v - e|^2
+  return PD_OK;
+}
+
+int check_fit(pd_trainer* t, int32_t steps, int32_t batch) {
   if (!t) return tfail(nullptr, PD_ERR_ARG, "null trainer");
   if (t->count < 1) return tfail(t, PD_ERR_STATE, "no samples");
   if (steps < 1 || batch < 128 || batch % 128 != 0)
     return tfail(t, PD_ERR_ARG, "steps >= 1 and batch a positive multiple of 128");
+  return PD_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int pd_trainer_fit(pd_trainer* t, int32_t steps, int32_t batch, float lr, uint64_t seed,
+                   float* losses) {
+  if (int e = check_fit(t, steps, batch)) return e;
   THIPCHK(t, hipSetDevice(t->device));
   const int32_t waves = batch / 32;
   if (waves > t->partial_waves) {
@@ -2226,33 +2316,25 @@ int pd_trainer_fit(pd_trainer* t, int32_t steps, int32_t batch, float lr, uint64
     THIPCHK(t, hipMalloc((void**)&t->loss, sizeof(float) * steps));
     t->loss_cap = steps;
   }
-  if (!t->normed) {
-    pd::launch_pc_norm(t->samples, t->count, t->p, t->stream);
-    t->normed = true;
-  }
-  for (int32_t k = 0; k < steps; ++k) {
-    pd::PcTrainArgs a;
-    a.samples = t->samples;
-    a.nsamples = t->count;
-    a.batch = batch;
-    a.seed = seed;
-    a.step = t->steps_done;
-    a.w = t->w;
-    a.p = t->p;
-    a.partial = t->partial;
-    pd::launch_pc_train(a, t->stream);
-    pd::launch_pc_adam(t->partial, waves, t->p, t->m, t->v, t->w, t->steps_done, lr, t->loss + k,
-                       t->stream);
-    ++t->steps_done;
-  }
-  THIPCHK(t, hipGetLastError());
-  if (losses)
-    THIPCHK(t, hipMemcpyAsync(losses, t->loss, sizeof(float) * steps, hipMemcpyDeviceToHost,
-                              t->stream));
-  THIPCHK(t, hipStreamSynchronize(t->stream));
-  if (losses)
-    for (int32_t k = 0; k < steps; ++k) losses[k] /= (float)batch;  // mean |M v - e|^2
-  return PD_OK;
+  return run_fit(t, steps, batch, lr, seed, t->partial, t->loss, losses);
+}
+
+int pd_trainer_fit_noalloc(pd_trainer* t, int32_t steps, int32_t batch, float lr, uint64_t seed,
+                           float* losses) {
+  if (int e = check_fit(t, steps, batch)) return e;
+  if (batch > PD_FIT_NOALLOC_MAX_BATCH)
+    return tfail(t, PD_ERR_ARG,
+                 "batch " + std::to_string(batch) + " exceeds the no-allocation fit's maximum " +
+                     std::to_string(PD_FIT_NOALLOC_MAX_BATCH));
+  if (steps > PD_FIT_NOALLOC_MAX_STEPS)
+    return tfail(t, PD_ERR_ARG,
+                 "steps " + std::to_string(steps) + " exceeds the no-allocation fit's maximum " +
+                     std::to_string(PD_FIT_NOALLOC_MAX_STEPS));
+  THIPCHK(t, hipSetDevice(t->device));
+  float* sp = nullptr;
+  std::string msg;
+  if (int e = scratch_address(t->device, &sp, msg)) return tfail(t, e, msg);
+  return run_fit(t, steps, batch, lr, seed, sp, sp + PD_FIT_PARTIAL_FLOATS, losses);
 }
 
 int pd_trainer_params(pd_trainer* t, float* params, int32_t count) {

@@ -164,6 +164,7 @@ def lib():
         L.pd_trainer_last_error.restype = C.c_char_p
         L.pd_trainer_add.argtypes = [vp, vp, C.c_uint64, C.c_int32, C.POINTER(C.c_int64)]
         L.pd_trainer_fit.argtypes = [vp, C.c_int32, C.c_int32, C.c_float, C.c_uint64, vp]
+        L.pd_trainer_fit_noalloc.argtypes = [vp, C.c_int32, C.c_int32, C.c_float, C.c_uint64, vp]
         L.pd_trainer_params.argtypes = [vp, vp, C.c_int32]
         L.pd_trainer_samples.argtypes = [vp, C.c_int64, C.c_int64, vp]
         L.pd_get_specimen_stats.argtypes = [vp, C.c_int32, C.POINTER(SpecimenStats)]
@@ -174,7 +175,7 @@ def lib():
                      "pd_spmv_bytes", "pd_check_division", "pd_set_specimens", "pd_get_specimen_stats",
                      "pd_tune_spmv", "pd_learned_set", "pd_get_precond", "pd_trainer_create",
                      "pd_trainer_destroy", "pd_trainer_add", "pd_trainer_fit",
-                     "pd_trainer_params", "pd_trainer_samples"):
+                     "pd_trainer_fit_noalloc", "pd_trainer_params", "pd_trainer_samples"):
             getattr(L, name).restype = C.c_int
         _lib = L
     return _lib
@@ -361,6 +362,8 @@ _hlib = None
 
 
 LEARNED_PARAMS = 5801
+FIT_NOALLOC_MAX_BATCH = 4096  # PD_FIT_NOALLOC_MAX_BATCH
+FIT_NOALLOC_MAX_STEPS = 16384  # PD_FIT_NOALLOC_MAX_STEPS
 LEARNED_LAYOUT = dict(W1=(0, (64, 16)), b1=(1024, (64,)), W2=(1088, (64, 64)), b2=(5184, (64,)),
                       W3=(5248, (8, 64)), b3=(5760, (8,)), mean=(5768, (16,)),
                       istd=(5784, (16,)), eps=(5800, ()))
@@ -384,9 +387,12 @@ class Trainer:
         self._check(lib().pd_trainer_add(self._h, ctx._h, seed, stride, C.byref(n)))
         return n.value
 
-    def fit(self, steps, batch=8192, lr=1e-3, seed=7):
+    def fit(self, steps, batch=8192, lr=1e-3, seed=7, noalloc=False):
+        """Adam steps on the samples; noalloc: pd_trainer_fit_noalloc (no
+        device allocation, batch and steps bounded by FIT_NOALLOC_MAX_*)."""
         losses = np.zeros(steps, np.float32)
-        self._check(lib().pd_trainer_fit(self._h, steps, batch, lr, seed, _ptr(losses)))
+        fn = lib().pd_trainer_fit_noalloc if noalloc else lib().pd_trainer_fit
+        self._check(fn(self._h, steps, batch, lr, seed, _ptr(losses)))
         return losses
 
     def params(self):
