@@ -2,23 +2,28 @@
 
 Workload (sci-solver_peridynamic_amd/configs/impact_100.txt): 100^3 jittered
 lattice + 5,000-particle spherical projectile at 20 m/s, implicit Euler
-dt = 5e-5 s, block-Jacobi CG to rtol 1e-6, FP64, 200 steps in all
-(warmup + timed steps; the defaults cover the 200).
+dt = 5e-5 s, block-Jacobi CG to rtol 1e-6, FP64, 200 steps in all.
+
+The projectile reaches the target at step 37 (first contact force) and the
+first bonds break at step 41.  The steps before the impact run untimed (at
+least --warmup of them), so the timed window of --steps steps starts at the
+impact: contact and bond breaking are always inside it.  A separate
+"full_run" object times all 200 steps from the first.
 
 Prints one JSON line with the metric, the roofline of the dominant kernel
 (the CG SpMV, timed with HIP events on the engine's stream) and a CPU
-baseline (the CPU oracle on a bounded sample of the same workload).
+baseline (the CPU oracle on the same timed window of the same workload).
 
-    python bench.py [--gpus 1] [--steps 195] [--warmup 5] [--no-cpu-baseline]
+    python bench.py [--gpus 1] [--steps 164] [--warmup 5] [--no-cpu-baseline]
                     [--precision fp64|mixed] [--no-mixed]
 
 The FP64 line also carries a "mixed_precision" object (the same run with the
 float inner CG and FP64 refinement, its SpMV roofline, and its per-step
-position error against the FP64 path), a "matrix_free" object (the same FP64
-run with the blocks recomputed in every SpMV: identical results), the learned
-preconditioner rows and a "batched_specimens" object (64 independent tension
-specimens in one context, aggregate particle-steps/s: FP64 with the
-matrix-free operator, and the mixed solver captured and eager).
+position error against the FP64 path over all 200 steps), a "matrix_free"
+object (the same FP64 run with the blocks recomputed in every SpMV: identical
+results), the learned preconditioner rows and a "batched_specimens" object
+(64 independent tension specimens in one context, aggregate particle-steps/s:
+FP64 with the matrix-free operator, and the mixed solver captured and eager).
 """
 from __future__ import annotations
 
@@ -38,6 +43,11 @@ import numpy as np  # noqa: E402
 import pdsolver as pds  # noqa: E402
 
 CONFIG = os.path.join(PKG, "configs", "impact_100.txt")
+# impact_100: the first step with a contact force (the CPU oracle's run:
+# contact from step 37, first broken bonds at step 41); the timed window
+# starts here unless --warmup asks for more untimed steps
+IMPACT_STEP = 37
+RUN_STEPS = 200
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILES = {0: os.path.join(ROOT, "profiles", "cg_spmv_impact_100_traffic.txt"),
                  1: os.path.join(ROOT, "profiles", "mx_spmv_impact_100_traffic.txt")}
@@ -75,8 +85,10 @@ def step_params(desc, step, precision, **kw):
 NETWORK = os.path.join(PKG, "networks", "impact_precond.txt")
 
 
-def gpu_run(steps, warmup, precision=0, copy_calibration=0, precond=None, network=None,
-            matrix_free=0):
+def gpu_run(steps, untimed, precision=0, copy_calibration=0, precond=None, network=None,
+            matrix_free=0, tune=True):
+    """`untimed` steps, then `steps` timed steps (steps untimed + 1 ..
+    untimed + steps of the run)."""
     desc = pds.describe(CONFIG)
     sc = pds.scene(CONFIG)
     n = desc["num_pd_particle"]
@@ -91,18 +103,17 @@ def gpu_run(steps, warmup, precision=0, copy_calibration=0, precond=None, networ
     log(f"[{'mixed' if precision else 'fp64'}{' matrix-free' if matrix_free else ''} "
         f"precond {extra.get('precond', desc['precond'])}] "
         f"particles {n}, directed bonds {bst.total_bonds} "
-        f"({bst.min_bonds}..{bst.max_bonds} per particle), bond build {bst.ms_build:.1f} ms")
+        f"({bst.min_bonds}..{bst.max_bonds} per particle), bond build {bst.ms_build:.1f} ms, "
+        f"timed steps {untimed + 1}..{untimed + steps}")
     step = 0
-    for _ in range(warmup):
+    for _ in range(untimed):
         step += 1
-        # block Jacobi first: the learned solve's fallback budget comes from it
-        kw = dict(extra, precond=1) if step == 1 else extra
-        ctx.step(step_params(desc, step, precision, **kw))
+        ctx.step(step_params(desc, step, precision, **extra))
     ctx.synchronize()
     # the autotuner picks the SpMV variant on the resident matrix (outside
     # the timed region); spmv_unroll = 0 then uses it
     # (the matrix-free operator has no stored-block variants to tune)
-    tune_ms, unroll = (ctx.tune_spmv(10) if warmup > 0 and not matrix_free
+    tune_ms, unroll = (ctx.tune_spmv(10) if tune and untimed > 0 and not matrix_free
                        else (np.zeros(3), 0))
     spmv_ms = 0.0
     spmv_n = 0
@@ -112,13 +123,16 @@ def gpu_run(steps, warmup, precision=0, copy_calibration=0, precond=None, networ
     iters = []
     refinements = []
     fallbacks = 0
+    probes = 0
     broken = 0
     phase = dict(contact=0.0, assemble=0.0, solve=0.0, update=0.0)
+    first_timed = step + 1
     t0 = time.perf_counter()
     for _ in range(steps):
         step += 1
         st = ctx.step(step_params(desc, step, precision, timing=1, **extra))
         fallbacks += st.precond_fallbacks
+        probes += st.precond_probe
         spmv_ms += st.ms_spmv_total
         spmv_n += st.spmv_launches
         store_ms += st.ms_spmv_store
@@ -152,7 +166,8 @@ def gpu_run(steps, warmup, precision=0, copy_calibration=0, precond=None, networ
                 spmv_bytes=spmv_bytes, phase=phase, build_ms=bst.ms_build,
                 plain_ms=plain_ms, plain_bytes=plain_bytes, resid_ms=resid_ms, resid_n=resid_n,
                 refinements=refinements, unroll=unroll, tune_ms=[float(t) for t in tune_ms],
-                fallbacks=fallbacks, digest=digest, store_ms=store_ms)
+                fallbacks=fallbacks, probes=probes, digest=digest, store_ms=store_ms,
+                timed_steps=[first_timed, step])
 
 
 BATCH_CONFIG = os.path.join(PKG, "configs", "batch_64.txt")
@@ -304,9 +319,24 @@ def mf_roofline(r):
     }
 
 
-def cpu_baseline(sample_steps=2):
-    """The CPU oracle on the same workload: kd-tree bonds per body (not
-    timed), then `sample_steps` steps timed with a wall clock."""
+def cpu_model():
+    """CPU model name and logical CPU count of this host."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, os.cpu_count()
+
+
+def cpu_baseline(first, count):
+    """The CPU oracle on the same workload and the GPU line's timed window:
+    kd-tree bonds per body and steps 1 .. first - 1 untimed, then steps
+    first .. first + count - 1 timed with a wall clock."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc
     desc = pds.describe(CONFIG)
@@ -315,37 +345,43 @@ def cpu_baseline(sample_steps=2):
     mats = pds.materials_of(desc)
     body = sc["body"]
     counts = np.zeros(n, np.int32)
-    lists = np.zeros((n, 528), np.int32)
+    lists = np.zeros((n, 128), np.int32)
     for b in range(len(mats)):
         idx = np.nonzero(body == b)[0]
-        c, l, dropped = orc.bonds_kdtree(sc["x4"][idx], mats[b][0])
+        c, l, dropped = orc.bonds_kdtree(sc["x4"][idx], mats[b][0], max_bonds=128)
+        assert dropped == 0
         counts[idx] = c
-        lists[idx] = np.where(np.arange(528)[None, :] < c[:, None], idx[l], 0)
+        lists[idx] = np.where(np.arange(128)[None, :] < c[:, None], idx[l], 0)
         del l
     rs, col = orc.lists_to_csr(counts, lists)
     del lists
     s00 = orc.stretch_init(n, desc["s0"], desc["stretch_seed"])
     S = orc.OracleSystem(sc["x4"], sc["v4"], s00, rs, col, mats, body=body, bc=sc["bc"],
                          vbc=sc["vbc"])
-    # at least `sample_steps` steps, continuing up to ~10 s of CPU work
-    t0 = time.perf_counter()
-    k = 0
-    while k < sample_steps or (time.perf_counter() - t0 < 10.0 and k < 20):
-        k += 1
+    t_pre = time.perf_counter()
+    for k in range(1, first):
         S.step(orc.make_params(**pds.step_kwargs(desc, k)))
+    t_pre = time.perf_counter() - t_pre
+    broken = 0
+    t0 = time.perf_counter()
+    for k in range(first, first + count):
+        broken += S.step(orc.make_params(**pds.step_kwargs(desc, k))).broken_bonds
     dt = time.perf_counter() - t0
-    sample_steps = k
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return dict(value=sample_steps / dt, unit="steps/s", cores=threads, kind="port",
-                sample=f"steps 1-{sample_steps} of the 200-step impact_100 run "
-                       f"({n} particles), CPU oracle (C/OpenMP restatement) after its "
-                       f"own kd-tree bond build, wall clock {dt:.2f} s")
+    model, logical = cpu_model()
+    return dict(value=count / dt, unit="steps/s", cores=threads, kind="port",
+                cpu_model=model, logical_cpus=logical,
+                sample=f"steps {first}-{first + count - 1} of the 200-step impact_100 run "
+                       f"({n} particles; contact from step {IMPACT_STEP}, {broken} bonds broken "
+                       f"in the sample), CPU oracle (C/OpenMP restatement, {threads} threads) "
+                       f"after its own kd-tree bond build and steps 1-{first - 1} untimed "
+                       f"({t_pre:.1f} s); wall clock {dt:.2f} s")
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=195)
+    ap.add_argument("--steps", type=int, default=RUN_STEPS - IMPACT_STEP + 1)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64",
                     help="solver of the timed line (mixed: float inner CG, FP64 refinement)")
@@ -361,7 +397,10 @@ def main():
     ap.add_argument("--operator", choices=["stored", "mf"], default="stored",
                     help="FP64 operator of the timed line (mf: bond blocks recomputed in "
                          "every SpMV, results identical)")
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=20,
+                    help="steps of the timed window the CPU oracle times (at most --steps)")
+    ap.add_argument("--no-full-run", action="store_true",
+                    help="skip the separately timed run of all 200 steps")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="after the run, N rounds of device copies of known size")
     a = ap.parse_args()
@@ -372,7 +411,9 @@ def main():
         sys.exit(2)
     prec = 1 if a.precision == "mixed" else 0
     mf = a.operator == "mf" and not prec
-    r = gpu_run(a.steps, a.warmup, prec, a.copy_calibration, matrix_free=1 if mf else 0)
+    # the pre-impact steps run untimed: the timed window starts at the impact
+    untimed = max(a.warmup, IMPACT_STEP - 1)
+    r = gpu_run(a.steps, untimed, prec, a.copy_calibration, matrix_free=1 if mf else 0)
     ms_step = 1000.0 * r["elapsed"] / r["steps"]
     kernel = ("k_mx_spmv (float inner CG SpMV y = A32 p + <y,p> partial)" if prec else
               "k_cg_spmv (CG SpMV y = A p + <y,p> partial)")
@@ -394,6 +435,8 @@ def main():
                         "20 m/s, implicit Euler dt 5e-5 s, block-Jacobi CG rtol 1e-6, " +
                         ("float inner CG with FP64 residual / refinement" if prec else
                          "FP64, matrix-free operator" if mf else "FP64"),
+            "untimed_steps": untimed,
+            "timed_steps": r["timed_steps"],
             "particles": r["n"],
             "directed_bonds": r["bonds"],
             "live_bonds_end": r["live"],
@@ -416,9 +459,26 @@ def main():
     if prec:
         out["config"]["refinements_mean"] = float(np.mean(r["refinements"]))
         out["config"]["resid_spmv_ms_mean"] = r["resid_ms"] / max(r["resid_n"], 1)
+    if not a.no_full_run:
+        # all 200 steps of the workload timed from the first (no untimed
+        # steps, no tuning: the SpMV runs its default 4 slots per trip)
+        fr = gpu_run(RUN_STEPS, 0, prec, matrix_free=1 if mf else 0, tune=False)
+        out["full_run"] = {
+            "steps": fr["steps"],
+            "value": fr["steps"] / fr["elapsed"],
+            "unit": "steps/s",
+            "ms_per_step": 1000.0 * fr["elapsed"] / fr["steps"],
+            "cg_iterations_mean": float(np.mean(fr["iters"])),
+            "cg_iterations_max": int(np.max(fr["iters"])),
+            "broken_bonds": fr["broken"],
+            "live_bonds_end": fr["live"],
+            "phase_ms_per_step": {k: v / fr["steps"] for k, v in fr["phase"].items()},
+            "identical_to_windowed_run": fr["digest"] == r["digest"]
+            if r["timed_steps"][1] == RUN_STEPS else None,
+        }
     m_run = None
     if not prec and not a.no_mixed:
-        m = gpu_run(a.steps, a.warmup, 1)
+        m = gpu_run(a.steps, untimed, 1)
         m_run = m
         m_rate = m["steps"] / m["elapsed"]
         out["mixed_precision"] = {
@@ -434,10 +494,10 @@ def main():
             "plain_spmv_gbs": m["plain_bytes"] / (m["plain_ms"] * 1e-3) / 1e9,
             "spmv_unroll": m["unroll"],
             "spmv_tune_ms_2_4_8": m["tune_ms"],
-            "position_error_vs_fp64": mixed_error(a.warmup + a.steps),
+            "position_error_vs_fp64": mixed_error(RUN_STEPS),
         }
     if not prec and not mf and not a.no_mf:
-        f = gpu_run(a.steps, a.warmup, 0, matrix_free=1)
+        f = gpu_run(a.steps, untimed, 0, matrix_free=1)
         f_rate = f["steps"] / f["elapsed"]
         out["matrix_free"] = {
             "value": f_rate,
@@ -456,7 +516,7 @@ def main():
         for prec, mfree, name in ((0, 0, "fp64"), (1, 0, "mixed"), (0, 1, "fp64_matrix_free")):
             if mfree and (mf or a.no_mf):
                 continue
-            L = gpu_run(a.steps, a.warmup, prec, precond=2, network=net, matrix_free=mfree)
+            L = gpu_run(a.steps, untimed, prec, precond=2, network=net, matrix_free=mfree)
             base = r if prec == 0 or m_run is None else m_run
             rows[name] = {
                 "value": L["steps"] / L["elapsed"],
@@ -465,13 +525,16 @@ def main():
                 "cg_iterations_mean": float(np.mean(L["iters"])),
                 "block_jacobi_iterations_mean": float(np.mean(base["iters"])),
                 "fallbacks": L["fallbacks"],
+                "block_jacobi_probe_steps": L["probes"],
                 "phase_ms_per_step": {k: v / L["steps"] for k, v in L["phase"].items()},
             }
         out["learned_preconditioner"] = {
             "network": "16-64-64-8 MLP, BF16 weights on MFMA, FP32 accumulation; trained by "
                        "train_precond.py on samples of 20 impact_100 runs with scene seeds "
                        "101-120; evaluated here on the shipped seeds (1973 / 2024)",
-            "fallback": "block Jacobi after 1.5x the last block-Jacobi iteration count",
+            "fallback": "block Jacobi after 1.5x the iteration count of the context's latest "
+                        "block-Jacobi solve; every 16th precond-2 step (and the first) is a "
+                        "block-Jacobi probe that refreshes that count",
             **rows,
         }
     if not a.no_batched:
@@ -494,7 +557,7 @@ def main():
         }
     if not a.no_cpu_baseline:
         log("cpu baseline: CPU oracle on the same workload ...")
-        out["cpu_baseline"] = cpu_baseline(a.cpu_steps)
+        out["cpu_baseline"] = cpu_baseline(r["timed_steps"][0], max(1, min(a.cpu_steps, a.steps)))
     print(json.dumps(out), flush=True)
 
 

@@ -136,6 +136,15 @@ typedef struct pd_step_params {
    * pd_spmv, the learned preconditioner's trainer and every solve apply
    * the recomputed operator. */
   int32_t matrix_free;
+  /* precond 2, pd_step only: the fallback's baseline is the iteration
+   * count of this context's latest block-Jacobi solve.  A precond-2 step is
+   * a probe when the context has no such count yet, when the previous
+   * precond-2 step fell back to block Jacobi, or when learned_probe
+   * precond-2 steps have run since the count was refreshed (0: 16;
+   * negative: no periodic probes): its solve uses the block-Jacobi inverse
+   * instead of the learned blocks and refreshes the count.  pd_solve never
+   * probes. */
+  int32_t learned_probe;
 } pd_step_params;
 
 typedef struct pd_step_stats {
@@ -164,6 +173,8 @@ typedef struct pd_step_stats {
   /* stored-block FP64 steps: the solve's first product, which also writes
    * the bond blocks the assembly left to it (not in ms_spmv_total) */
   double ms_spmv_store;
+  int32_t precond_probe; /* precond 2: 1 if this step's solve was a
+                            block-Jacobi probe (learned_probe) */
 } pd_step_stats;
 
 /* Solver result of one specimen (pd_set_specimens). */
@@ -318,7 +329,10 @@ int pd_tune_spmv(pd_ctx* ctx, int32_t iters, double* ms3, int32_t* best);
  * W3[8][64] b3[8] feature_mean[16] feature_inv_std[16] eps. */
 #define PD_LEARNED_PARAMS 5801
 int pd_learned_set(pd_ctx* ctx, const float* params, int32_t count);
-/* The preconditioner blocks of the last solve, 9 doubles per particle. */
+/* The preconditioner blocks of the last solve, 9 doubles per particle: the
+ * learned blocks M_p when it was a precond-2 solve (also one that fell back
+ * to block Jacobi), else the block-Jacobi inverse (after a mixed-precision
+ * assembly its float copy, widened). */
 int pd_get_precond(pd_ctx* ctx, double* minv9);
 
 /* Training of the network on systems the simulator assembled: samples are

@@ -127,7 +127,15 @@ struct pd_ctx {
   float* pc_p = nullptr;         // [PC_NPARAM]
   __bf16* pc_w = nullptr;        // [PC_NWEIGHT]
   bool pc_loaded = false;
+  // the learned blocks M_p (FP64 planes, and their float copy for the mixed
+  // solve), kept apart from the assembly's block-Jacobi inverse so that a
+  // fallback continues with block Jacobi at once (allocated on first use)
+  double* pc_minv = nullptr;
+  float* pc_minv32 = nullptr;
+  bool last_learned = false;  // the last solve applied the learned blocks
   int bj_iters = 0;  // iterations of the last block-Jacobi solve
+  int pc_since_probe = 0;  // precond-2 steps since bj_iters was refreshed
+  bool pc_fell_back = false;  // the last learned solve fell back to block Jacobi
   hipGraphExec_t g_solve = nullptr, g_update = nullptr;
   std::string graph_key;
   int graph_k = 0;
@@ -486,7 +494,7 @@ pd::MxArgs mx_args(pd_ctx* c, int precond) {
   a.val32 = c->val32;
   a.diag = c->diag;
   a.diag32 = c->diag32;
-  a.minv32 = c->minv32;
+  a.minv32 = precond == 2 ? c->pc_minv32 : c->minv32;
   a.bc = c->bc;
   a.b = c->b;
   a.x = c->sol;
@@ -507,7 +515,7 @@ pd::CgArgs cg_args(pd_ctx* c, int precond) {
   pd::CgArgs a;
   a.A = spmv_view(c);
   a.batch = batch_view(c);
-  a.minv = c->minv;
+  a.minv = precond == 2 ? c->pc_minv : c->minv;
   a.bc = c->bc;
   a.b = c->b;
   a.x = c->sol;
@@ -644,13 +652,12 @@ int ensure_blocks(pd_ctx* c) {
   return PD_OK;
 }
 
-// The solve can take its first residual and start from the assembly pass
-// (the blocks and the block-Jacobi inverse are in registers there); the
-// learned preconditioner's blocks are only formed after the assembly.
-bool fused_resid(const pd_step_params* prm) { return prm->precond != 2; }
-
-// fuse: also produce the solve's first residual and start (fused_resid(prm)
-// must hold; the next solve is run with fused = true)
+// fuse: also produce the solve's first residual and start, from the blocks
+// and the block-Jacobi inverse the pass has in registers (the next solve is
+// run with fused = true).  With the learned preconditioner (precond 2) the
+// start is the block-Jacobi one; run_solve_pc replaces z = M r and its
+// partial once the learned blocks exist (launch_pc_start), or keeps it for a
+// block-Jacobi probe.
 int enqueue_assemble(pd_ctx* c, const pd_step_params* prm,
                      const pd_step_consts* consts = nullptr, bool fuse = false) {
   if (prm->precision == 1)
@@ -678,7 +685,7 @@ int enqueue_assemble(pd_ctx* c, const pd_step_params* prm,
   c->asm_dt2 = dt2;
   c->asm_consts = consts;
   a.fuse = fuse ? 1 : 0;
-  a.pc = prm->precond;
+  a.pc = prm->precond != 0 ? 1 : 0;  // precond 2: the block-Jacobi start first
   a.xs = c->sol;
   a.r64 = c->r;
   a.p64 = c->p;
@@ -1037,11 +1044,17 @@ int enqueue_update(pd_ctx* c, const pd_step_params* prm,
   return PD_OK;
 }
 
-// The learned blocks M_p replace the block-Jacobi inverse written by the
-// assembly (minv, and minv32 for the mixed solve).
+// The learned blocks M_p of the assembled system into pc_minv (and pc_minv32
+// for the mixed solve); the solve with precond 2 applies them in place of the
+// assembly's block-Jacobi inverse (minv / minv32), which stays intact.
 int apply_learned(pd_ctx* c, const pd_step_params* prm) {
+  const size_t npad = (size_t)c->nslices * 64;
   if (!c->pc_feat)
     if (int e = dalloc(c, &c->pc_feat, (size_t)std::max(c->n, 1) * PC_NIN)) return e;
+  if (!c->pc_minv)
+    if (int e = dalloc(c, &c->pc_minv, npad * 6)) return e;
+  if (prm->precision == 1 && !c->pc_minv32)
+    if (int e = dalloc(c, &c->pc_minv32, npad * 6)) return e;
   pd::PcFeatArgs fa;
   fa.n = c->n;
   fa.slice_off = c->slice_off;
@@ -1059,9 +1072,30 @@ int apply_learned(pd_ctx* c, const pd_step_params* prm) {
   pd::PcNet net;
   net.w = c->pc_w;
   net.p = c->pc_p;
-  pd::launch_pc_infer(c->n, c->pc_feat, net, c->diag, c->act, c->minv,
-                      prm->precision == 1 ? c->minv32 : nullptr, c->stream);
+  pd::launch_pc_infer(c->n, c->pc_feat, net, c->diag, c->act, c->pc_minv,
+                      prm->precision == 1 ? c->pc_minv32 : nullptr, c->stream);
   CHECK_LAUNCH(c, "learned preconditioner");
+  return PD_OK;
+}
+
+// The learned preconditioner's start after a fused assembly: z = M r with
+// the learned blocks just written, p = z, and the <r, z> tile partials in
+// place of the assembly's block-Jacobi ones.
+int learned_start(pd_ctx* c, const pd_step_params* prm) {
+  pd::PcStartArgs a;
+  a.n = c->n;
+  a.mixed = prm->precision == 1 ? 1 : 0;
+  a.act = c->act;
+  a.minv = c->pc_minv;
+  a.minv32 = c->pc_minv32;
+  a.r64 = c->r;
+  a.p64 = c->p;
+  a.rec = (c->mf || c->blocks_deferred) ? c->rec : nullptr;
+  a.r32 = c->r32;
+  a.p32 = c->p32;
+  a.part = c->part;
+  pd::launch_pc_start(a, c->stream);
+  CHECK_LAUNCH(c, "learned start");
   return PD_OK;
 }
 
@@ -1069,27 +1103,52 @@ int apply_learned(pd_ctx* c, const pd_step_params* prm) {
 // iteration count; the learned preconditioner gets at most
 // learned_fallback x that many iterations (when known), after which the
 // block-Jacobi inverse is restored and the solve goes on from its iterate.
+// may_probe (pd_step): a precond-2 solve without a current block-Jacobi
+// count (none yet, learned_probe steps since the last, or a fallback in the
+// previous step) runs block Jacobi instead (the assembly formed its inverse)
+// and refreshes the count.  fused: the assembly formed the block-Jacobi start.
 int run_solve_pc(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st,
-                 bool fused = false) {
+                 bool fused = false, bool may_probe = false) {
+  c->last_learned = false;
   if (prm->precond != 2) {
     if (int e = run_solve(c, prm, st, 0, fused)) return e;
-    if (prm->precond == 1) c->bj_iters = prm->precision == 1 ? c->last_mx_iters : c->last_iters;
+    if (prm->precond == 1) {
+      c->bj_iters = prm->precision == 1 ? c->last_mx_iters : c->last_iters;
+      c->pc_since_probe = 0;
+    }
     return PD_OK;
   }
   if (c->mat_prec != prm->precision)
     return fail(c, PD_ERR_STATE, "the matrix was not assembled with this precision");
+  if (may_probe) {
+    const int period = prm->learned_probe == 0 ? 16 : prm->learned_probe;
+    if (c->bj_iters == 0 || c->pc_fell_back || (period > 0 && c->pc_since_probe >= period)) {
+      pd_step_params p1 = *prm;
+      p1.precond = 1;
+      if (int e = run_solve(c, &p1, st, 0, fused)) return e;
+      c->bj_iters = prm->precision == 1 ? c->last_mx_iters : c->last_iters;
+      c->pc_since_probe = 0;
+      c->pc_fell_back = false;
+      if (st) st->precond_probe = 1;
+      return PD_OK;
+    }
+    ++c->pc_since_probe;
+  }
   if (int e = apply_learned(c, prm)) return e;
+  c->last_learned = true;
+  if (fused)
+    if (int e = learned_start(c, prm)) return e;
   pd_step_params p2 = *prm;
   const double f = prm->learned_fallback > 0.0 ? prm->learned_fallback : 1.5;
   if (c->bj_iters > 0)
     p2.max_iter = std::min(prm->max_iter, (int32_t)ceil(f * (double)c->bj_iters));
-  if (int e = run_solve(c, &p2, st)) return e;
+  if (int e = run_solve(c, &p2, st, 0, fused)) return e;
   if (c->last_converged || p2.max_iter >= prm->max_iter) return PD_OK;
-  // fall back to block Jacobi from the current iterate
+  // fall back to block Jacobi (the assembly's inverse, still in minv /
+  // minv32) from the current iterate; the next step probes (its count is
+  // then the baseline)
+  c->pc_fell_back = true;
   const int used = prm->precision == 1 ? c->last_mx_iters : c->last_iters;
-  pd::launch_block_jacobi(c->n, c->diag, c->act, c->minv,
-                          prm->precision == 1 ? c->minv32 : nullptr, c->stream);
-  CHECK_LAUNCH(c, "block Jacobi");
   pd_step_params p3 = *prm;
   p3.precond = 1;
   p3.max_iter = std::max(0, prm->max_iter - used);
@@ -1173,7 +1232,7 @@ int capture_step(pd_ctx* c, const pd_step_params* prm, int K) {
   if (prm->matrix_free || defer_blocks(prm))  // no allocation inside the capture
     if (int e = ensure_rec(c)) return e;
   int e = capture(c, &c->g_solve, [&]() -> int {
-    const bool fuse = fused_resid(prm);
+    const bool fuse = true;  // precond 0 or 1 (check_params)
     if (int e2 = enqueue_assemble(c, prm, c->consts, fuse)) return e2;
     pd::launch_copy(c->s, c->s_new, c->n, c->stream);  // core/simulator.cuh:2612
     if (prm->precision == 1) {
@@ -1291,6 +1350,7 @@ void zero_timing(pd_step_stats* st) {
   st->ms_resid_total = 0.0;
   st->resid_launches = 0;
   st->precond_fallbacks = 0;
+  st->precond_probe = 0;
 }
 
 }  // namespace
@@ -1441,6 +1501,7 @@ int pd_ctx_destroy(pd_ctx* c) {
   dfree(c->consts);
   if (c->h_consts) (void)hipHostFree(c->h_consts);
   dfree(c->bond_stats); dfree(c->pc_feat); dfree(c->pc_p); dfree(c->pc_w);
+  dfree(c->pc_minv); dfree(c->pc_minv32);
   if (c->h_mx) (void)hipHostFree(c->h_mx);
   if (c->h_st) (void)hipHostFree(c->h_st);
   if (c->h_cnt) (void)hipHostFree(c->h_cnt);
@@ -1753,11 +1814,10 @@ int pd_step(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st) {
   if (prm->contact_radius > 0.0)
     if (int e = enqueue_contact(c, prm)) return e;
   if (timing) HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
-  const bool fuse = fused_resid(prm);
-  if (int e = enqueue_assemble(c, prm, nullptr, fuse)) return e;
+  if (int e = enqueue_assemble(c, prm, nullptr, true)) return e;
   if (timing) HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
   std::swap(c->s, c->s_new);  // core/simulator.cuh:2612
-  if (int e = run_solve_pc(c, prm, st, fuse)) return e;
+  if (int e = run_solve_pc(c, prm, st, true, true)) return e;
   read_assemble_stats(c, prm, st);  // the solve's waits covered the copies
   if (timing) HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
   if (int e = enqueue_update(c, prm)) return e;
@@ -2055,9 +2115,21 @@ int pd_get_precond(pd_ctx* c, double* minv9) {
   if (!minv9) return fail(c, PD_ERR_ARG, "null output");
   HIPCHK(c, hipSetDevice(c->device));
   std::vector<double> mv((size_t)c->nslices * 64 * 6);
-  if (!mv.empty())
-    HIPCHK(c, hipMemcpyAsync(mv.data(), c->minv, mv.size() * sizeof(double),
-                             hipMemcpyDeviceToHost, c->stream));
+  if (!mv.empty()) {
+    if (c->last_learned) {  // the learned blocks (also after a fallback)
+      HIPCHK(c, hipMemcpyAsync(mv.data(), c->pc_minv, mv.size() * sizeof(double),
+                               hipMemcpyDeviceToHost, c->stream));
+    } else if (c->mat_prec == 1) {  // the mixed assembly's float inverse
+      std::vector<float> mf(mv.size());
+      HIPCHK(c, hipMemcpyAsync(mf.data(), c->minv32, mf.size() * sizeof(float),
+                               hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      for (size_t i = 0; i < mv.size(); ++i) mv[i] = (double)mf[i];
+    } else {
+      HIPCHK(c, hipMemcpyAsync(mv.data(), c->minv, mv.size() * sizeof(double),
+                               hipMemcpyDeviceToHost, c->stream));
+    }
+  }
   HIPCHK(c, hipStreamSynchronize(c->stream));
   for (int32_t p = 0; p < c->n; ++p) {
     const double* six = mv.data() + (size_t)(p >> 6) * 384 + (p & 63);

@@ -230,9 +230,22 @@ void launch_pc_features(const PcFeatArgs& a, hipStream_t s);
 // minv32 is non-null, its float copy
 void launch_pc_infer(int32_t n, const float* feat, const PcNet& net, const double* diag,
                      const int32_t* act, double* minv, float* minv32, hipStream_t s);
-// block-Jacobi inverse of every visible row's diagonal block (the fallback)
-void launch_block_jacobi(int32_t n, const double* diag, const int32_t* act, double* minv,
-                         float* minv32, hipStream_t s);
+// The solve's start with the learned blocks after an assembly that formed
+// the first residual (fuse): z = M r, p = z, the <r, z> tile partials.
+struct PcStartArgs {
+  int32_t n;
+  int32_t mixed;
+  const int32_t* act;
+  const double* minv;    // FP64 planes (mixed 0)
+  const float* minv32;   // float planes (mixed 1)
+  const double4* r64;
+  double4* p64;
+  double* rec;           // FP64: the gather record's p part, when in use
+  const float4* r32;
+  float4* p32;
+  double* part;          // the solve's tile partials
+};
+void launch_pc_start(const PcStartArgs& a, hipStream_t s);
 
 // training samples: feat[NIN], D (6, float), v = (A e)_p (3), e_p (3)
 #define PC_SAMPLE 28

@@ -135,6 +135,26 @@ __device__ __forceinline__ void learned_block(const T Lc[6], T eps, const T ci[6
   out[5] = M[2][2];
 }
 
+// The block-Jacobi inverse of (d00 d01 d02 d11 d12 d22) by cofactors, the
+// assembly's arithmetic (six distinct entries)
+__device__ __forceinline__ void bj_inverse(const double d[6], double m[6]) {
+  const double d00 = d[0], d01 = d[1], d02 = d[2], d11 = d[3], d12 = d[4], d22 = d[5];
+  const double c00 = d11 * d22 - d12 * d12;
+  const double c01 = d02 * d12 - d01 * d22;
+  const double c02 = d01 * d12 - d02 * d11;
+  const double c11 = d00 * d22 - d02 * d02;
+  const double c12 = d01 * d02 - d00 * d12;
+  const double c22 = d00 * d11 - d01 * d01;
+  const double det = (d00 * c00 + d01 * c01) + d02 * c02;
+  const double id = 1.0 / det;
+  m[0] = c00 * id;
+  m[1] = c01 * id;
+  m[2] = c02 * id;
+  m[3] = c11 * id;
+  m[4] = c12 * id;
+  m[5] = c22 * id;
+}
+
 __global__ __launch_bounds__(PD_BLOCK) void k_pc_features(PcFeatArgs a) {
   const int p = blockIdx.x * PD_BLOCK + threadIdx.x;
   if (p >= a.n) return;
@@ -272,9 +292,12 @@ __global__ __launch_bounds__(PD_BLOCK) void k_pc_infer(int n, const float* feat,
   const double d[6] = {dg[0], dg[64], dg[128], dg[192], dg[256], dg[320]};
   double ci[6], L[6], m[6];
   bool cl[3];
-  if (!chol_inv(d, ci)) return;  // keeps the block-Jacobi inverse
-  make_l(outs[w][l], L, cl);
-  learned_block(L, (double)P[PC_OFF_EPS], ci, m);
+  if (chol_inv(d, ci)) {
+    make_l(outs[w][l], L, cl);
+    learned_block(L, (double)P[PC_OFF_EPS], ci, m);
+  } else {  // a block without a Cholesky factor keeps block Jacobi
+    bj_inverse(d, m);
+  }
   double* mo = minv + (int64_t)s * 6 * 64 + lane;
 #pragma unroll
   for (int c = 0; c < 6; ++c) mo[c * 64] = m[c];
@@ -285,31 +308,51 @@ __global__ __launch_bounds__(PD_BLOCK) void k_pc_infer(int n, const float* feat,
   }
 }
 
-// the block-Jacobi inverse, as the assembly writes it
-__global__ __launch_bounds__(PD_BLOCK) void k_block_jacobi(int n, const double* diag,
-                                                          const int32_t* act, double* minv,
-                                                          float* minv32) {
-  const int p = blockIdx.x * PD_BLOCK + threadIdx.x;
-  if (p >= n || act[p] == PD_INVISIBLE) return;
-  const int s = p >> 6, lane = p & 63;
-  const double* dg = diag + (int64_t)s * 6 * 64 + lane;
-  const double d00 = dg[0], d01 = dg[64], d02 = dg[128], d11 = dg[192], d12 = dg[256],
-               d22 = dg[320];
-  const double c00 = d11 * d22 - d12 * d12;
-  const double c01 = d02 * d12 - d01 * d22;
-  const double c02 = d01 * d12 - d02 * d11;
-  const double c11 = d00 * d22 - d02 * d02;
-  const double c12 = d01 * d02 - d00 * d12;
-  const double c22 = d00 * d11 - d01 * d01;
-  const double det = (d00 * c00 + d01 * c01) + d02 * c02;
-  const double id = 1.0 / det;
-  const double m[6] = {c00 * id, c01 * id, c02 * id, c11 * id, c12 * id, c22 * id};
-  double* mo = minv + (int64_t)s * 6 * 64 + lane;
-  for (int c = 0; c < 6; ++c) mo[c * 64] = m[c];
-  if (minv32) {
-    float* mf = minv32 + (int64_t)s * 6 * 64 + lane;
-    for (int c = 0; c < 6; ++c) mf[c * 64] = (float)m[c];
+
+// The solver's start after a fused assembly when the learned blocks replace
+// block Jacobi: z = M r of the residual the assembly formed, p = z (FP64: and
+// the p part of the gather record), and the <r, z> tile partial in the slot
+// the assembly keeps for it (FP64: part[tile]; mixed, on the float vectors:
+// part[ntiles + tile]).  The arithmetic is the assembly's block-Jacobi start
+// with M_p in place of the block-Jacobi inverse.
+template <bool MIXED>
+__global__ __launch_bounds__(PD_BLOCK) void k_pc_start(PcStartArgs a) {
+  __shared__ double lds4[4];
+  const int tile = blockIdx.x;
+  const int p = tile * PD_BLOCK + threadIdx.x;
+  double t = 0.0;
+  if (p < a.n && a.act[p] != PD_INVISIBLE) {
+    const int s = p >> 6, lane = p & 63;
+    if constexpr (MIXED) {
+      const float* mo = a.minv32 + (int64_t)s * 6 * 64 + lane;
+      const float m0 = mo[0], m1 = mo[64], m2 = mo[128], m3 = mo[192], m4 = mo[256],
+                  m5 = mo[320];
+      const float4 h = a.r32[p];
+      const float z0 = m0 * h.x + m1 * h.y + m2 * h.z;
+      const float z1 = m1 * h.x + m3 * h.y + m4 * h.z;
+      const float z2 = m2 * h.x + m4 * h.y + m5 * h.z;
+      a.p32[p] = make_float4(z0, z1, z2, 0.0f);
+      t = ((double)h.x * (double)z0 + (double)h.y * (double)z1) + (double)h.z * (double)z2;
+    } else {
+      const double* mo = a.minv + (int64_t)s * 6 * 64 + lane;
+      const double m0 = mo[0], m1 = mo[64], m2 = mo[128], m3 = mo[192], m4 = mo[256],
+                   m5 = mo[320];
+      const double4 r = a.r64[p];
+      const double z0 = m0 * r.x + m1 * r.y + m2 * r.z;
+      const double z1 = m1 * r.x + m3 * r.y + m4 * r.z;
+      const double z2 = m2 * r.x + m4 * r.y + m5 * r.z;
+      a.p64[p] = make_double4(z0, z1, z2, 0.0);
+      if (a.rec) {
+        double* rp = a.rec + 6 * (int64_t)p;
+        rp[3] = z0;
+        rp[4] = z1;
+        rp[5] = z2;
+      }
+      t = (r.x * z0 + r.y * z1) + r.z * z2;
+    }
   }
+  const double v = tile_sum(t, lds4);
+  if (threadIdx.x == 0) a.part[(MIXED ? (int64_t)gridDim.x : 0) + tile] = v;
 }
 
 // random-sign probe vectors
@@ -646,10 +689,12 @@ void launch_pc_infer(int32_t n, const float* feat, const PcNet& net, const doubl
   if (n) hipLaunchKernelGGL(k_pc_infer, dim3((n + 127) / 128), dim3(PD_BLOCK), 0, s, n, feat, net,
                             diag, act, minv, minv32);
 }
-void launch_block_jacobi(int32_t n, const double* diag, const int32_t* act, double* minv,
-                         float* minv32, hipStream_t s) {
-  if (n) hipLaunchKernelGGL(k_block_jacobi, dim3(blocks_of(n)), dim3(PD_BLOCK), 0, s, n, diag, act,
-                            minv, minv32);
+void launch_pc_start(const PcStartArgs& a, hipStream_t s) {
+  if (!a.n) return;
+  if (a.mixed)
+    hipLaunchKernelGGL(k_pc_start<true>, dim3(blocks_of(a.n)), dim3(PD_BLOCK), 0, s, a);
+  else
+    hipLaunchKernelGGL(k_pc_start<false>, dim3(blocks_of(a.n)), dim3(PD_BLOCK), 0, s, a);
 }
 void launch_pc_probe(int32_t n, uint64_t seed, double4* e, hipStream_t s) {
   if (n) hipLaunchKernelGGL(k_pc_probe, dim3(blocks_of(n)), dim3(PD_BLOCK), 0, s, n, seed, e);

@@ -62,6 +62,7 @@ class StepParams(C.Structure):
         ("inner_rtol", C.c_double), ("inner_max_iter", C.c_int32),
         ("contact_mode", C.c_int32), ("spmv_unroll", C.c_int32), ("graph", C.c_int32),
         ("learned_fallback", C.c_double), ("matrix_free", C.c_int32),
+        ("learned_probe", C.c_int32),
     ]
 
 
@@ -76,7 +77,7 @@ class StepStats(C.Structure):
         ("spmv_launches", C.c_int32), ("host_syncs", C.c_int32),
         ("refinements", C.c_int32), ("ms_resid_total", C.c_double),
         ("resid_launches", C.c_int32), ("precond_fallbacks", C.c_int32),
-        ("ms_spmv_store", C.c_double),
+        ("ms_spmv_store", C.c_double), ("precond_probe", C.c_int32),
     ]
 
     def as_dict(self):
@@ -100,7 +101,7 @@ DEFAULT_PARAMS = dict(dt=1e-4, clock_scale=1.0, gravity=9.8, integrator=0, preco
                       contact_radius=0.0, contact_stiffness=0.0, contact_volume=1.0,
                       contact_cell=0.0, track_gap=0, timing=0, precision=0, max_refine=20,
                       inner_rtol=1e-5, contact_mode=0, spmv_unroll=0, graph=0,
-                      learned_fallback=0.0, matrix_free=0)
+                      learned_fallback=0.0, matrix_free=0, learned_probe=0)
 
 
 def make_params(**kw) -> StepParams:

@@ -190,3 +190,75 @@ def test_full_size_bonds_and_operator():
     assert np.linalg.norm(r[free]) <= 1.0001 * desc["rtol"] * np.linalg.norm(b[free, :3])
     say("solve", st.cg_iterations, st.residual_norm / st.b_norm)
     ctx.close()
+
+
+def test_full_size_impact_steps_match_oracle():
+    """The 1,005,000-particle impact workload stepped on the GPU and in the
+    CPU oracle side by side through the projectile's first contact (step 37)
+    and the first broken bonds (step 41) to step 60: per step the CG
+    iteration counts and broken-bond counts are equal and the positions
+    agree within the parity bar (reported: how many steps are bit-equal).
+    The reference step is core/simulator.cu:297-467."""
+    say("case full-size impact_100 stepped")
+    cfg = os.path.join(CFG, "impact_100.txt")
+    desc = pds.describe(cfg)
+    sc = pds.scene(cfg)
+    n = desc["num_pd_particle"]
+    mats = pds.materials_of(desc)
+    counts = np.zeros(n, np.int32)
+    lists = np.zeros((n, 128), np.int32)
+    for b in range(len(mats)):
+        idx = np.nonzero(sc["body"] == b)[0]
+        c, l, dropped = orc.bonds_kdtree(sc["x4"][idx], mats[b][0], max_bonds=128)
+        assert dropped == 0
+        counts[idx] = c
+        lists[idx] = np.where(np.arange(128)[None, :] < c[:, None], idx[l], 0)
+        del l
+    rs, col = orc.lists_to_csr(counts, lists)
+    del lists
+    s00 = orc.stretch_init(n, desc["s0"], desc["stretch_seed"])
+    S = orc.OracleSystem(sc["x4"], sc["v4"], s00, rs, col, mats, body=sc["body"], bc=sc["bc"],
+                         vbc=sc["vbc"])
+    ctx = pds.Context(n, mats)
+    pds.load_state(ctx, sc["x4"], sc["v4"], pds.stretch_init(n, desc["s0"], desc["stretch_seed"]),
+                   body=sc["body"], bc=sc["bc"], vbc=sc["vbc"])
+    bst = ctx.build_bonds()
+    assert bst.total_bonds == int(counts.sum())
+    say("bonds", bst.total_bonds)
+    steps = 60
+    first_contact = first_break = first_near = None
+    exact_steps = 0
+    worst_rel = 0.0
+    broken_total = 0
+    for k in range(1, steps + 1):
+        kw = pds.step_kwargs(desc, k)
+        so = S.step(orc.make_params(**kw))
+        sg = ctx.step(pds.make_params(**dict(kw, track_gap=1)))
+        assert so.converged and sg.converged
+        if first_near is None and so.min_gap < 1e-6:
+            first_near = (k, so.min_gap)
+        if first_near is None:
+            assert sg.broken_bonds == so.broken_bonds, (k, sg.broken_bonds, so.broken_bonds)
+            assert sg.cg_iterations == so.cg_iterations, (k, sg.cg_iterations, so.cg_iterations)
+        if first_contact is None and np.abs(S.fc[:, :3]).max() > 0:
+            first_contact = k
+            fg = ctx.download("contact_force")
+            assert np.array_equal(fg[:, :3], S.fc[:, :3]), k
+        if first_break is None and so.broken_bonds > 0:
+            first_break = k
+        broken_total += so.broken_bonds
+        xg = ctx.download("position")
+        d = np.abs(xg[:, :3] - S.x[:, :3]).max()
+        rel = d / np.abs(S.x[:, :3]).max()
+        worst_rel = max(worst_rel, rel)
+        exact_steps += int(d == 0.0)
+        assert rel <= REL_TOL, (k, rel)
+        if k % 10 == 0 or k in (first_contact, first_break):
+            say(dict(step=k, it=(so.cg_iterations, sg.cg_iterations),
+                     broken=(so.broken_bonds, sg.broken_bonds), live=sg.live_bonds, rel=rel))
+    assert first_contact is not None and first_contact < steps
+    assert first_break is not None and broken_total > 1000
+    say(dict(steps=steps, first_contact=first_contact, first_break=first_break,
+             broken_total=broken_total, bitwise_equal_steps=exact_steps, worst_rel=worst_rel,
+             first_near=first_near))
+    ctx.close()

@@ -3,6 +3,7 @@ network against a numpy float32 restatement (BF16 rounding at the same
 points), the preconditioned solve against the oracle given the same blocks
 (bit for bit), the block-Jacobi fallback, and the trainer's samples and
 loss."""
+import os
 import sys
 
 import numpy as np
@@ -52,13 +53,14 @@ def get(p, name):
     return p[off:off + int(np.prod(shape))].reshape(shape)
 
 
-def features(S, mats):
-    """restated features of the oracle system after assembly"""
+def features(S, mats, rows=None, body=None):
+    """restated features of the oracle system after assembly (rows: the
+    rows to fill, default all; body: per-row body index into mats)"""
     n = S.n
     f = np.zeros((n, 16), np.float32)
-    h = mats[0][0]
-    m = mats[0][3]
-    for p in range(n):
+    for p in (range(n) if rows is None else rows):
+        h = mats[0 if body is None else body[p]][0]
+        m = mats[0 if body is None else body[p]][3]
         rs = S.row_start[p]
         top = S.top[p]
         cap = S.row_start[p + 1] - rs
@@ -216,4 +218,101 @@ def test_trainer_samples_and_fit():
     st = ctx.solve(pds.make_params(dt=1e-4, precond=2, rtol=1e-8))
     assert st.converged
     tr.close()
+    ctx.close()
+
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+NETWORK = os.path.join(ROOT, "sci-solver_peridynamic_amd", "networks", "impact_precond.txt")
+IMPACT_30 = os.path.join(ROOT, "sci-solver_peridynamic_amd", "configs", "impact_30.txt")
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_shipped_network_on_impact_matches_oracle(precision):
+    """The learned preconditioner on its own workload: the shipped network
+    (networks/impact_precond.txt) steps the impact_30 scene with precond 2
+    for 60 steps, through the projectile's first contact and the first
+    broken bonds.  Each step the oracle takes the GPU's preconditioner
+    blocks (block Jacobi on the probe steps, which refresh the fallback's
+    baseline) and must give the same broken bonds, CG iterations, residual
+    and positions bit for bit.  On two steps, one before the contact and one
+    after bonds broke, the network's blocks are checked against the numpy
+    restatement of the features (broken-bond distance included) and the
+    MLP.  The reference solve the blocks precondition is the identity-
+    preconditioned CG at core/simulator.cu:763-770."""
+    from test_gpu_configs import setup_pair
+    say("case shipped network impact_30 precision", precision)
+    net = np.loadtxt(NETWORK, dtype=np.float32)
+    assert net.shape == (pds.LEARNED_PARAMS,)
+    desc, sc, S, ctx = setup_pair(IMPACT_30)
+    mats = pds.materials_of(desc)
+    ctx.learned_set(net)
+    steps = 60
+    probes = learned_steps = fallbacks = 0
+    bj = 0  # iterations of the latest block-Jacobi probe: the fallback's baseline
+    its = {"learned": [], "probe": []}
+    checked = []
+    first_contact = first_break = None
+    broken_total = 0
+    for k in range(1, steps + 1):
+        kw = dict(pds.step_kwargs(desc, k), precision=precision, precond=2)
+        sg = ctx.step(pds.make_params(**kw))
+        prm = orc.make_params(**kw)
+        S.contact(prm)
+        sa = S.assemble(prm)
+        S.swap_stretch()
+        if sg.precond_probe:
+            so = S.solve(orc.make_params(**dict(kw, precond=1)))
+            probes += 1
+            bj = sg.cg_iterations
+            its["probe"].append(sg.cg_iterations)
+        else:
+            assert bj > 0
+            M = ctx.get_precond()  # the learned blocks, also after a fallback
+            budget = int(np.ceil(1.5 * bj))
+            S.minv[:] = M
+            so = S.solve(orc.make_params(**dict(kw, max_iter=budget)))
+            if sg.precond_fallbacks:
+                # the learned solve used its budget; block Jacobi goes on from
+                # its iterate with the rest of the iteration limit
+                assert not so.converged and so.cg_iterations == budget, k
+                s3 = S.solve(orc.make_params(**dict(kw, precond=1,
+                                                    max_iter=kw["max_iter"] - budget)))
+                so.cg_iterations = budget + s3.cg_iterations
+                so.converged, so.residual_norm = s3.converged, s3.residual_norm
+                fallbacks += 1
+            learned_steps += 1
+            its["learned"].append(sg.cg_iterations)
+            want = (k == 25) or (first_break is not None and len(checked) < 2 and k >= 50)
+            if want:
+                rows = np.arange(0, S.n, 7)
+                brk = np.nonzero(S.top < np.diff(S.row_start) - 1)[0]
+                rows = np.union1d(rows, brk)
+                f = features(S, mats, rows=rows, body=sc["body"])
+                ref = blocks(net, mlp(net, f[rows]), S.diag[rows])
+                scale = np.abs(ref).max(axis=1, keepdims=True)
+                err = np.abs(M[rows] - ref) / scale
+                say(dict(step=k, rows=len(rows), rows_with_broken=len(brk),
+                         max_rel=float(err.max()), median_rel=float(np.median(err)),
+                         feature9_min=float(f[rows, 9].min())))
+                assert np.median(err) < 2e-3 and err.max() < 5e-2
+                if k >= 50:
+                    assert len(brk) > 0 and f[brk, 9].min() < 2.0
+                checked.append(k)
+        S.update(prm)
+        assert sa.broken_bonds == sg.broken_bonds, (k, sa.broken_bonds, sg.broken_bonds)
+        assert so.converged and sg.converged
+        assert (so.cg_iterations, so.residual_norm) == (sg.cg_iterations, sg.residual_norm), k
+        assert np.array_equal(ctx.download("solution")[:, :3], S.sol[:, :3]), k
+        assert np.array_equal(ctx.download("position")[:, :3], S.x[:, :3]), k
+        if first_contact is None and np.abs(S.fc[:, :3]).max() > 0:
+            first_contact = k
+        if first_break is None and sa.broken_bonds > 0:
+            first_break = k
+        broken_total += sa.broken_bonds
+    assert probes >= 2 and learned_steps >= 50
+    assert first_contact is not None and first_break is not None and len(checked) == 2
+    say(dict(precision=precision, probes=probes, learned_steps=learned_steps, fallbacks=fallbacks,
+             first_contact=first_contact, first_break=first_break, broken=broken_total,
+             learned_iterations_mean=float(np.mean(its["learned"])),
+             probe_iterations=its["probe"], checked_steps=checked))
     ctx.close()
