@@ -83,6 +83,14 @@ def step_params(desc, step, precision, **kw):
 
 
 NETWORK = os.path.join(PKG, "networks", "impact_precond.txt")
+# why a block-diagonal correction cannot cut iterations here (the derivation
+# is in the package's design notes): for block-diagonal M the probe loss is
+# minimised by M*_p = D_p (D_p^2 + sum_q A_pq A_qp)^-1; at impact_100,
+# dt^2 sum_q K_pq ~ 0.8 m and each off-diagonal block ~ dt^2 k ~ 2.5e-5, so
+# sum_q A_pq^2 / D_p^2 ~ 0.02 and M* lies within ~2 % of D_p^-1 (block Jacobi)
+LEARNED_BOUND = ("block-diagonal M: the probe-loss optimum M*_p = D_p (D_p^2 + sum_q A_pq A_qp)^-1 "
+                 "is within ~2% of block Jacobi D_p^-1 at impact_100 (sum_q A_pq^2 / D_p^2 ~ 0.02), "
+                 "so the network cannot cut the 4-6 block-Jacobi iterations per step")
 
 
 def gpu_run(steps, untimed, precision=0, copy_calibration=0, precond=None, network=None,
@@ -97,10 +105,11 @@ def gpu_run(steps, untimed, precision=0, copy_calibration=0, precond=None, netwo
     if precond is not None:
         extra["precond"] = precond
     if matrix_free:
-        extra["matrix_free"] = 1
+        extra["matrix_free"] = matrix_free
     if network is not None:
         ctx.learned_set(network)
-    log(f"[{'mixed' if precision else 'fp64'}{' matrix-free' if matrix_free else ''} "
+    opname = {0: "", 1: " matrix-free", 2: " operator per step"}[matrix_free]
+    log(f"[{'mixed' if precision else 'fp64'}{opname} "
         f"precond {extra.get('precond', desc['precond'])}] "
         f"particles {n}, directed bonds {bst.total_bonds} "
         f"({bst.min_bonds}..{bst.max_bonds} per particle), bond build {bst.ms_build:.1f} ms, "
@@ -113,7 +122,7 @@ def gpu_run(steps, untimed, precision=0, copy_calibration=0, precond=None, netwo
     # the autotuner picks the SpMV variant on the resident matrix (outside
     # the timed region); spmv_unroll = 0 then uses it
     # (the matrix-free operator has no stored-block variants to tune)
-    tune_ms, unroll = (ctx.tune_spmv(10) if tune and untimed > 0 and not matrix_free
+    tune_ms, unroll = (ctx.tune_spmv(10) if tune and untimed > 0 and matrix_free == 0
                        else (np.zeros(3), 0))
     spmv_ms = 0.0
     spmv_n = 0
@@ -124,6 +133,7 @@ def gpu_run(steps, untimed, precision=0, copy_calibration=0, precond=None, netwo
     refinements = []
     fallbacks = 0
     probes = 0
+    mf_steps = 0
     broken = 0
     phase = dict(contact=0.0, assemble=0.0, solve=0.0, update=0.0)
     first_timed = step + 1
@@ -133,6 +143,7 @@ def gpu_run(steps, untimed, precision=0, copy_calibration=0, precond=None, netwo
         st = ctx.step(step_params(desc, step, precision, timing=1, **extra))
         fallbacks += st.precond_fallbacks
         probes += st.precond_probe
+        mf_steps += st.operator_mf
         spmv_ms += st.ms_spmv_total
         spmv_n += st.spmv_launches
         store_ms += st.ms_spmv_store
@@ -167,6 +178,7 @@ def gpu_run(steps, untimed, precision=0, copy_calibration=0, precond=None, netwo
                 plain_ms=plain_ms, plain_bytes=plain_bytes, resid_ms=resid_ms, resid_n=resid_n,
                 refinements=refinements, unroll=unroll, tune_ms=[float(t) for t in tune_ms],
                 fallbacks=fallbacks, probes=probes, digest=digest, store_ms=store_ms,
+                mf_steps=mf_steps,
                 timed_steps=[first_timed, step])
 
 
@@ -378,6 +390,25 @@ def cpu_baseline(first, count):
                        f"({t_pre:.1f} s); wall clock {dt:.2f} s")
 
 
+OPERATORS = {"stored": 0, "mf": 1, "auto": 2}
+
+
+def operator_row(x, ref_value, ref_digest):
+    """A comparison row: the same FP64 run with another operator."""
+    rate = x["steps"] / x["elapsed"]
+    row = {
+        "value": rate,
+        "unit": "steps/s",
+        "speedup_vs_timed_line": rate / ref_value,
+        "ms_per_step": 1000.0 * x["elapsed"] / x["steps"],
+        "identical_to_timed_line": x["digest"] == ref_digest,
+        "final_state_digest": x["digest"],
+        "cg_iterations_mean": float(np.mean(x["iters"])),
+        "phase_ms_per_step": {k: v / x["steps"] for k, v in x["phase"].items()},
+    }
+    return row
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -392,11 +423,12 @@ def main():
                     help="skip the batched-specimen report")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the learned-preconditioner report")
-    ap.add_argument("--no-mf", action="store_true",
-                    help="skip the matrix-free FP64 operator report")
-    ap.add_argument("--operator", choices=["stored", "mf"], default="stored",
-                    help="FP64 operator of the timed line (mf: bond blocks recomputed in "
-                         "every SpMV, results identical)")
+    ap.add_argument("--no-stored", action="store_true",
+                    help="skip the stored-block FP64 run (the HBM SpMV half of the metric)")
+    ap.add_argument("--operator", choices=list(OPERATORS), default="auto",
+                    help="FP64 operator of the timed line: auto (per step the cheaper of the "
+                         "two, measured), stored (3x3 BSR blocks streamed by every SpMV) or mf "
+                         "(blocks recomputed in every SpMV); the results are identical")
     ap.add_argument("--cpu-steps", type=int, default=20,
                     help="steps of the timed window the CPU oracle times (at most --steps)")
     ap.add_argument("--no-full-run", action="store_true",
@@ -410,13 +442,18 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     prec = 1 if a.precision == "mixed" else 0
-    mf = a.operator == "mf" and not prec
+    op = OPERATORS[a.operator] if not prec else 0
     # the pre-impact steps run untimed: the timed window starts at the impact
     untimed = max(a.warmup, IMPACT_STEP - 1)
-    r = gpu_run(a.steps, untimed, prec, a.copy_calibration, matrix_free=1 if mf else 0)
+    r = gpu_run(a.steps, untimed, prec, a.copy_calibration, matrix_free=op)
+    # the dominant kernel of the timed steps: the recomputing CG SpMV when
+    # most steps used it, else the streaming one
+    mf = r["mf_steps"] * 2 > r["steps"]
     ms_step = 1000.0 * r["elapsed"] / r["steps"]
     kernel = ("k_mx_spmv (float inner CG SpMV y = A32 p + <y,p> partial)" if prec else
               "k_cg_spmv (CG SpMV y = A p + <y,p> partial)")
+    opdesc = {0: "stored 3x3 BSR blocks", 1: "blocks recomputed in every SpMV",
+              2: "per step the cheaper of stored and recomputed blocks (measured)"}[op]
     out = {
         "metric": "implicit steps/s @1M particles, CG rtol 1e-6; SpMV HBM GB/s vs ~8 TB/s peak",
         "value": r["steps"] / r["elapsed"],
@@ -434,7 +471,7 @@ def main():
             "workload": "impact_100: 100^3 jittered lattice + 5000-particle projectile at "
                         "20 m/s, implicit Euler dt 5e-5 s, block-Jacobi CG rtol 1e-6, " +
                         ("float inner CG with FP64 residual / refinement" if prec else
-                         "FP64, matrix-free operator" if mf else "FP64"),
+                         "FP64, operator: " + opdesc),
             "untimed_steps": untimed,
             "timed_steps": r["timed_steps"],
             "particles": r["n"],
@@ -443,26 +480,47 @@ def main():
             "cg_iterations_mean": float(np.mean(r["iters"])),
             "cg_iterations_max": int(np.max(r["iters"])),
             "broken_bonds": r["broken"],
+            "steps_recomputing_blocks": r["mf_steps"],
             "phase_ms_per_step": {k: v / r["steps"] for k, v in r["phase"].items()},
             "bond_build_ms": r["build_ms"],
-            "plain_spmv_gbs": r["plain_bytes"] / (r["plain_ms"] * 1e-3) / 1e9,
-            "spmv_unroll": r["unroll"],
-            "spmv_tune_ms_2_4_8": r["tune_ms"],
         },
-        "roofline": mf_roofline(r) if mf else spmv_roofline(r, kernel, read_traffic(prec)),
+        "roofline": (mf_roofline(r) if mf else
+                     spmv_roofline(r, kernel, read_traffic(prec))),
     }
     if not prec and not mf:
         out["step_roofline"] = step_roofline(r)
-        # the solve's first product also writes the bond blocks (the assembly
-        # leaves them to it); it is not among the streaming launches above
         out["config"]["first_product_with_block_stores_ms"] = r["store_ms"] / r["steps"]
+        out["config"]["spmv_unroll"] = r["unroll"]
+        out["config"]["spmv_tune_ms_2_4_8"] = r["tune_ms"]
     if prec:
         out["config"]["refinements_mean"] = float(np.mean(r["refinements"]))
         out["config"]["resid_spmv_ms_mean"] = r["resid_ms"] / max(r["resid_n"], 1)
+        out["config"]["plain_spmv_gbs"] = r["plain_bytes"] / (r["plain_ms"] * 1e-3) / 1e9
+    if not prec and op != 0 and not a.no_stored:
+        # the same run on stored blocks: every product streams the 3x3 BSR
+        # matrix from HBM -- the SpMV GB/s half of the metric
+        sb = gpu_run(a.steps, untimed, 0, matrix_free=0)
+        row = operator_row(sb, out["value"], r["digest"])
+        row["roofline"] = spmv_roofline(sb, "k_cg_spmv (CG SpMV y = A p + <y,p> partial, "
+                                            "streamed FP64 blocks)", read_traffic(0))
+        row["step_roofline"] = step_roofline(sb)
+        row["first_product_with_block_stores_ms"] = sb["store_ms"] / sb["steps"]
+        row["plain_spmv_gbs"] = sb["plain_bytes"] / (sb["plain_ms"] * 1e-3) / 1e9
+        row["spmv_unroll"] = sb["unroll"]
+        row["spmv_tune_ms_2_4_8"] = sb["tune_ms"]
+        out["stored_blocks"] = row
+        out["spmv_hbm_gbs"] = row["roofline"]["achieved"]
+    elif not prec and not mf:
+        out["spmv_hbm_gbs"] = out["roofline"]["achieved"]
+    if not prec and op == 0 and not a.no_stored:
+        x = gpu_run(a.steps, untimed, 0, matrix_free=2)
+        out["operator_per_step"] = dict(operator_row(x, out["value"], r["digest"]),
+                                        steps_recomputing_blocks=x["mf_steps"],
+                                        roofline=mf_roofline(x))
     if not a.no_full_run:
         # all 200 steps of the workload timed from the first (no untimed
         # steps, no tuning: the SpMV runs its default 4 slots per trip)
-        fr = gpu_run(RUN_STEPS, 0, prec, matrix_free=1 if mf else 0, tune=False)
+        fr = gpu_run(RUN_STEPS, 0, prec, matrix_free=op, tune=False)
         out["full_run"] = {
             "steps": fr["steps"],
             "value": fr["steps"] / fr["elapsed"],
@@ -472,9 +530,8 @@ def main():
             "cg_iterations_max": int(np.max(fr["iters"])),
             "broken_bonds": fr["broken"],
             "live_bonds_end": fr["live"],
+            "steps_recomputing_blocks": fr["mf_steps"],
             "phase_ms_per_step": {k: v / fr["steps"] for k, v in fr["phase"].items()},
-            "identical_to_windowed_run": fr["digest"] == r["digest"]
-            if r["timed_steps"][1] == RUN_STEPS else None,
         }
     m_run = None
     if not prec and not a.no_mixed:
@@ -496,32 +553,22 @@ def main():
             "spmv_tune_ms_2_4_8": m["tune_ms"],
             "position_error_vs_fp64": mixed_error(RUN_STEPS),
         }
-    if not prec and not mf and not a.no_mf:
-        f = gpu_run(a.steps, untimed, 0, matrix_free=1)
-        f_rate = f["steps"] / f["elapsed"]
-        out["matrix_free"] = {
-            "value": f_rate,
-            "unit": "steps/s",
-            "speedup_vs_stored": f_rate / out["value"],
-            "ms_per_step": 1000.0 * f["elapsed"] / f["steps"],
-            "identical_to_stored": f["digest"] == r["digest"],
-            "final_state_digest": f["digest"],
-            "cg_iterations_mean": float(np.mean(f["iters"])),
-            "phase_ms_per_step": {k: v / f["steps"] for k, v in f["phase"].items()},
-            "roofline": mf_roofline(f),
-        }
     if not a.no_learned and os.path.exists(NETWORK):
         net = np.loadtxt(NETWORK, dtype=np.float32)
         rows = {}
-        for prec, mfree, name in ((0, 0, "fp64"), (1, 0, "mixed"), (0, 1, "fp64_matrix_free")):
-            if mfree and (mf or a.no_mf):
+        for lprec, name in ((0, "fp64"), (1, "mixed")):
+            if lprec == 1 and m_run is None and not prec:
                 continue
-            L = gpu_run(a.steps, untimed, prec, precond=2, network=net, matrix_free=mfree)
-            base = r if prec == 0 or m_run is None else m_run
+            L = gpu_run(a.steps, untimed, lprec, precond=2, network=net,
+                        matrix_free=op if lprec == 0 else 0)
+            base = r if lprec == prec else m_run
             rows[name] = {
                 "value": L["steps"] / L["elapsed"],
                 "unit": "steps/s",
-                "speedup_vs_fp64_block_jacobi": (L["steps"] / L["elapsed"]) / out["value"],
+                "speedup_vs_block_jacobi_same_precision": (L["steps"] / L["elapsed"]) /
+                                                          (base["steps"] / base["elapsed"]),
+                "speedup_vs_fp64_block_jacobi": (L["steps"] / L["elapsed"]) /
+                                                (r["steps"] / r["elapsed"]) if not prec else None,
                 "cg_iterations_mean": float(np.mean(L["iters"])),
                 "block_jacobi_iterations_mean": float(np.mean(base["iters"])),
                 "fallbacks": L["fallbacks"],
@@ -533,24 +580,25 @@ def main():
                        "train_precond.py on samples of 20 impact_100 runs with scene seeds "
                        "101-120; evaluated here on the shipped seeds (1973 / 2024)",
             "fallback": "block Jacobi after 1.5x the iteration count of the context's latest "
-                        "block-Jacobi solve; every 16th precond-2 step (and the first) is a "
-                        "block-Jacobi probe that refreshes that count",
+                        "block-Jacobi solve; every 16th precond-2 step (and the first, and the "
+                        "one after a fallback) is a block-Jacobi probe that refreshes that count",
+            "bound_on_gain": LEARNED_BOUND,
             **rows,
         }
     if not a.no_batched:
         bw = max(1, min(a.warmup, 5))
-        mf = batched_run(95, bw, 1, precision=0, matrix_free=1)
+        bmf = batched_run(95, bw, 1, precision=0, matrix_free=1)
         graph = batched_run(95, bw, 1)
         eager = batched_run(95, bw, 0)
         out["batched_specimens"] = {
             "metric": "aggregate particle-steps/s",
-            "value": mf["value"],
+            "value": bmf["value"],
             "unit": "particle-steps/s",
             "workload": "batch_64: 64 independent tension specimens in one context, counts "
                         "log-uniform in [5000, 60000], aspect 1..4, s0 U[0.01, 0.03], dt 1e-4 s, "
                         "block-Jacobi CG rtol 1e-6, one launch per phase for all specimens, "
                         "hipGraph-captured steps; value: FP64 with the matrix-free operator",
-            "fp64_matrix_free_graph": mf,
+            "fp64_matrix_free_graph": bmf,
             "mixed_graph": dict(graph, solver="float inner CG with FP64 refinement, "
                                               "autotuned SpMV slots per trip"),
             "mixed_eager": eager,

@@ -134,7 +134,11 @@ typedef struct pd_step_params {
    * so every product, and every result, is identical to 0.  Needs
    * precision 0; pd_get_matrix then fails (no stored bond blocks), and
    * pd_spmv, the learned preconditioner's trainer and every solve apply
-   * the recomputed operator. */
+   * the recomputed operator.  2 (pd_step; pd_assemble takes the current
+   * choice): per step, whichever of 0 and 1 is cheaper, from this
+   * context's own timings of both products and the previous step's
+   * iteration count (the first steps time each operator in turn, with the
+   * timing of their products on); precision 1 streams its float blocks. */
   int32_t matrix_free;
   /* precond 2, pd_step only: the fallback's baseline is the iteration
    * count of this context's latest block-Jacobi solve.  A precond-2 step is
@@ -175,6 +179,12 @@ typedef struct pd_step_stats {
   double ms_spmv_store;
   int32_t precond_probe; /* precond 2: 1 if this step's solve was a
                             block-Jacobi probe (learned_probe) */
+  int32_t operator_mf;   /* 1 if this step's products recomputed the bond
+                            blocks (matrix_free 1, or 2's choice) */
+  int32_t contact_grid;  /* the contact grid build of this step: 0 none
+                            (contact off or no candidates), 1 / 2 the
+                            one-workgroup build with one / two radix passes,
+                            3 the multi-launch build */
 } pd_step_stats;
 
 /* Solver result of one specimen (pd_set_specimens). */

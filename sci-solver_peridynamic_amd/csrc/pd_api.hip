@@ -92,6 +92,9 @@ struct pd_ctx {
   int64_t cell_cap = 0;
   int32_t* counts = nullptr;
   int32_t* widths = nullptr;
+  // the last contact grid build: 0 none, 1 / 2 the one-workgroup build with
+  // one / two radix passes, 3 the multi-launch build
+  int last_grid = 0;
   uint32_t* zone_cnt = nullptr;  // contact candidate tile counts + total
   int32_t* cand = nullptr;       // contact candidates
 
@@ -158,6 +161,11 @@ struct pd_ctx {
   // product (launch_cg_spmv_store); cleared when that product or
   // ensure_blocks has written them
   bool blocks_deferred = false;
+  // matrix_free = 2 (pd_step): measured times of the two operators, ms per
+  // CG product (0: not measured yet); reset when the bonds change
+  double op_t_mf = 0.0;     // recomputing product
+  double op_t_store = 0.0;  // the storing first product of a stored-block step
+  double op_t_st = 0.0;     // a streaming product of stored blocks
 };
 
 struct pd_trainer {
@@ -286,9 +294,10 @@ int check_params(pd_ctx* c, const pd_step_params* prm) {
     c->unroll[k] = prm->spmv_unroll ? prm->spmv_unroll : (c->tuned[k] ? c->tuned[k] : 4);
   if (prm->precision != 0 && prm->precision != 1)
     return fail(c, PD_ERR_ARG, "precision must be 0 (FP64) or 1 (float inner solve + FP64 refinement)");
-  if (prm->matrix_free != 0 && prm->matrix_free != 1)
-    return fail(c, PD_ERR_ARG, "matrix_free must be 0 (stored blocks) or 1 (recomputed blocks)");
-  if (prm->matrix_free && prm->precision != 0)
+  if (prm->matrix_free < 0 || prm->matrix_free > 2)
+    return fail(c, PD_ERR_ARG,
+                "matrix_free must be 0 (stored blocks), 1 (recomputed blocks) or 2 (per step)");
+  if (prm->matrix_free == 1 && prm->precision != 0)
     return fail(c, PD_ERR_ARG, "matrix_free needs precision 0 (the float solve streams float blocks)");
   if (prm->precision == 1 && (prm->inner_rtol < 0.0 || prm->max_refine < 0 || prm->inner_max_iter < 0))
     return fail(c, PD_ERR_ARG, "inner_rtol, max_refine and inner_max_iter must be >= 0");
@@ -384,6 +393,7 @@ int grid_over(pd_ctx* c, double cell, const double* mn, const int32_t* c0, const
   // a candidate subset of contact size: the whole build in one launch
   const bool small = subset && m > 0 && m <= PD_SMALL_GRID_MAX_M &&
                      ncell <= PD_SMALL_GRID_MAX_CELLS;
+  c->last_grid = small ? (bits <= 8 ? 1 : 2) : 3;
   int where;
   if (small) {
     where = pd::launch_small_grid(c->x, subset, m, mn, cell, c0, dims, (int32_t)ncell, bits,
@@ -537,6 +547,7 @@ pd::CgArgs cg_args(pd_ctx* c, int precond) {
 // and cell edges of the full grid, and candidates stay in ascending order,
 // so every particle sums its contacts in the same order as on the full grid.
 int enqueue_contact(pd_ctx* c, const pd_step_params* prm) {
+  c->last_grid = 0;
   const int nb = c->nbody;
   double bb[PD_MAX_BODIES][6];
   double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
@@ -1339,6 +1350,24 @@ int run_step_graph(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st) {
   return PD_OK;
 }
 
+// matrix_free = 2: the operator of this step, 1 (recompute the blocks in
+// every product) or 0 (the first product stores them, the others stream
+// them).  Both give identical results, so the choice is by cost alone: with
+// n the previous solve's iteration count, recompute when n t_mf <= t_store
+// + (n - 1) t_st.  Until both have been measured on this context the steps
+// measure them in turn (recomputing first); captured steps, which cannot
+// time their products, recompute until then.  Mixed precision streams its
+// float blocks (0).
+int auto_operator(pd_ctx* c, const pd_step_params* prm) {
+  if (prm->precision != 0) return 0;
+  if (c->op_t_mf <= 0.0) return 1;
+  if (c->op_t_store <= 0.0) return prm->graph ? 1 : 0;
+  const double n = (double)std::max(1, c->last_iters);
+  return n * c->op_t_mf <= c->op_t_store + (n - 1.0) * c->op_t_st ? 1 : 0;
+}
+
+void reset_operator_times(pd_ctx* c) { c->op_t_mf = c->op_t_store = c->op_t_st = 0.0; }
+
 void zero_timing(pd_step_stats* st) {
   if (!st) return;
   st->ms_contact = st->ms_assemble = st->ms_solve = st->ms_update = 0.0;
@@ -1351,7 +1380,48 @@ void zero_timing(pd_step_stats* st) {
   st->resid_launches = 0;
   st->precond_fallbacks = 0;
   st->precond_probe = 0;
+  st->operator_mf = 0;
+  st->contact_grid = 0;
 }
+
+// One step (contact, assembly, stretch commit, solve, update) with the
+// operator prm->matrix_free (0 or 1) names.
+int run_step(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st) {
+  if (prm->graph) {
+    const int e = run_step_graph(c, prm, st);
+    if (st) st->operator_mf = c->mf;
+    return e;
+  }
+  const bool timing = prm->timing != 0;
+  if (timing)
+    if (int e = ensure_events(c, 8)) return e;
+  if (timing) HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+  if (prm->contact_radius > 0.0) {
+    if (int e = enqueue_contact(c, prm)) return e;
+    if (st) st->contact_grid = c->last_grid;
+  }
+  if (timing) HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+  if (int e = enqueue_assemble(c, prm, nullptr, true)) return e;
+  if (timing) HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+  std::swap(c->s, c->s_new);  // core/simulator.cuh:2612
+  if (int e = run_solve_pc(c, prm, st, true, true)) return e;
+  read_assemble_stats(c, prm, st);  // the solve's waits covered the copies
+  if (timing) HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
+  if (int e = enqueue_update(c, prm)) return e;
+  if (timing) HIPCHK(c, hipEventRecord(c->ev[4], c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (st && timing) {
+    st->ms_contact = elapsed(c->ev[0], c->ev[1]);
+    st->ms_assemble = elapsed(c->ev[1], c->ev[2]);
+    st->ms_solve = elapsed(c->ev[2], c->ev[3]);
+    st->ms_update = elapsed(c->ev[3], c->ev[4]);
+  }
+  if (st) st->operator_mf = c->mf;
+  if (!c->last_converged)
+    return fail(c, PD_ERR_NOT_CONVERGED, "CG reached the iteration limit before rtol");
+  return PD_OK;
+}
+
 
 }  // namespace
 
@@ -1626,6 +1696,7 @@ int pd_build_bonds(pd_ctx* c, pd_bond_stats* stats) {
   c->d0_valid = false;
   c->blocks_deferred = false;  // new bond lists: no matrix to keep
   c->asm_fresh = false;
+  reset_operator_times(c);
   if (stats) {
     stats->total_bonds = total;
     stats->min_bonds = mn;
@@ -1682,6 +1753,7 @@ int pd_set_bonds(pd_ctx* c, const int32_t* counts, const int32_t* lists, int32_t
   c->d0_valid = false;
   c->blocks_deferred = false;  // new bond lists: no matrix to keep
   c->asm_fresh = false;
+  reset_operator_times(c);
   return PD_OK;
 }
 
@@ -1763,13 +1835,18 @@ int pd_get_matrix(pd_ctx* c, double* diag9, double* vals9, int32_t stride) {
   return PD_OK;
 }
 
-int pd_assemble(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st) {
+int pd_assemble(pd_ctx* c, const pd_step_params* prm_in, pd_step_stats* st) {
   if (!c) return fail(nullptr, PD_ERR_ARG, "null context");
-  if (int e = check_params(c, prm)) return e;
+  if (int e = check_params(c, prm_in)) return e;
   HIPCHK(c, hipSetDevice(c->device));
   zero_timing(st);
-  if (prm->contact_radius > 0.0)
+  pd_step_params q = *prm_in;
+  if (q.matrix_free == 2) q.matrix_free = auto_operator(c, &q);
+  const pd_step_params* prm = &q;
+  if (prm->contact_radius > 0.0) {
     if (int e = enqueue_contact(c, prm)) return e;
+    if (st) st->contact_grid = c->last_grid;
+  }
   if (int e = enqueue_assemble(c, prm)) return e;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   read_assemble_stats(c, prm, st);
@@ -1806,32 +1883,27 @@ int pd_step(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st) {
   if (int e = check_params(c, prm)) return e;
   HIPCHK(c, hipSetDevice(c->device));
   zero_timing(st);
-  if (prm->graph) return run_step_graph(c, prm, st);
-  const bool timing = prm->timing != 0;
-  if (timing)
-    if (int e = ensure_events(c, 8)) return e;
-  if (timing) HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
-  if (prm->contact_radius > 0.0)
-    if (int e = enqueue_contact(c, prm)) return e;
-  if (timing) HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
-  if (int e = enqueue_assemble(c, prm, nullptr, true)) return e;
-  if (timing) HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
-  std::swap(c->s, c->s_new);  // core/simulator.cuh:2612
-  if (int e = run_solve_pc(c, prm, st, true, true)) return e;
-  read_assemble_stats(c, prm, st);  // the solve's waits covered the copies
-  if (timing) HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
-  if (int e = enqueue_update(c, prm)) return e;
-  if (timing) HIPCHK(c, hipEventRecord(c->ev[4], c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
-  if (st && timing) {
-    st->ms_contact = elapsed(c->ev[0], c->ev[1]);
-    st->ms_assemble = elapsed(c->ev[1], c->ev[2]);
-    st->ms_solve = elapsed(c->ev[2], c->ev[3]);
-    st->ms_update = elapsed(c->ev[3], c->ev[4]);
+  if (prm->matrix_free != 2) return run_step(c, prm, st);
+  // the per-step operator choice; a step that measures an operator times
+  // its products (timing = 1)
+  pd_step_params q = *prm;
+  pd_step_stats local;
+  pd_step_stats* s = st ? st : &local;
+  zero_timing(s);
+  q.matrix_free = auto_operator(c, prm);
+  const bool measure = q.precision == 0 && !q.graph &&
+                       (q.matrix_free ? c->op_t_mf <= 0.0 : c->op_t_store <= 0.0);
+  if (measure) q.timing = 1;
+  const int e = run_step(c, &q, s);
+  if (measure && (e == PD_OK || e == PD_ERR_NOT_CONVERGED)) {
+    if (q.matrix_free && s->spmv_launches > 0) {
+      c->op_t_mf = s->ms_spmv_total / s->spmv_launches;
+    } else if (!q.matrix_free && s->ms_spmv_store > 0.0) {
+      c->op_t_store = s->ms_spmv_store;
+      c->op_t_st = s->spmv_launches > 0 ? s->ms_spmv_total / s->spmv_launches : s->ms_spmv_store;
+    }
   }
-  if (!c->last_converged)
-    return fail(c, PD_ERR_NOT_CONVERGED, "CG reached the iteration limit before rtol");
-  return PD_OK;
+  return e;
 }
 
 int pd_spmv(pd_ctx* c, const double* x4, double* y4) {
@@ -1949,6 +2021,7 @@ int pd_set_specimens(pd_ctx* c, int32_t count, const int32_t* first, const int32
   if (!c) return fail(nullptr, PD_ERR_ARG, "null context");
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  reset_operator_times(c);
   const int32_t n = c->n;
   if (count == 0) {  // back to one specimen over all rows
     dfree(c->tile_spec);

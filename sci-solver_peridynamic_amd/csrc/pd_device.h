@@ -96,6 +96,13 @@ struct pd_step_counters {
   unsigned int pad;
 };
 
+// The CG updates form alpha = rz / <y, p> in every workgroup from the
+// specimen's <y, p> tile partials (one read of the partial row per
+// workgroup, hidden behind the row loads); a specimen of more tiles than
+// this has alpha folded once, by one workgroup, ahead of the update, since
+// the per-workgroup reads grow with the square of its tile count.
+#define PD_ALPHA_FOLD_TILES 8192
+
 __device__ __forceinline__ double4 ld4(const double4* p) { return *p; }
 
 // Blocks of the workgroup grid are handed to the 8 XCDs round-robin; this

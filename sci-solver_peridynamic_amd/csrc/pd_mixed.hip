@@ -341,9 +341,14 @@ __global__ __launch_bounds__(PD_BLOCK) void k_mx_update(MxArgs a) {
   }
   int t0, nt;
   spec_range_of(a.batch, s, ntiles, t0, nt);
-  const double yp = fold_partials(a.part + t0, nt, lds4);
-  const double alpha = st->rz / yp;
-  if (tile == t0 && threadIdx.x == 0) st->alpha = alpha;
+  double alpha;
+  if (nt > PD_ALPHA_FOLD_TILES) {
+    alpha = st->alpha;  // k_mx_alpha folded it ahead of this launch
+  } else {
+    const double yp = fold_partials(a.part + t0, nt, lds4);
+    alpha = st->rz / yp;
+    if (tile == t0 && threadIdx.x == 0) st->alpha = alpha;
+  }
   const float af = (float)alpha;
   double t_rz = 0.0, t_rr = 0.0;
   if (row) {
@@ -370,6 +375,19 @@ __global__ __launch_bounds__(PD_BLOCK) void k_mx_update(MxArgs a) {
     a.part[2 * (int64_t)ntiles + tile] = v0;
     a.part[3 * (int64_t)ntiles + tile] = v1;
   }
+}
+
+// alpha of the specimens with more than PD_ALPHA_FOLD_TILES tiles, one
+// workgroup each (the same fold and quotient as k_mx_update's)
+__global__ __launch_bounds__(PD_BLOCK) void k_mx_alpha(MxArgs a, int ntiles) {
+  pd_mx_state* st = a.st + blockIdx.x;
+  if (st->phase != 1) return;
+  __shared__ double lds4[4];
+  int t0, nt;
+  spec_range(a.batch, ntiles, t0, nt);
+  if (nt <= PD_ALPHA_FOLD_TILES) return;
+  const double yp = fold_partials(a.part + t0, nt, lds4);
+  if (threadIdx.x == 0) st->alpha = st->rz / yp;
 }
 
 // beta and the inner test ahead of the next inner iteration
@@ -475,6 +493,8 @@ void launch_mx_spmv(const MxArgs& a, hipStream_t s) {
 void launch_mx_update(const MxArgs& a, hipStream_t s) {
   if (!a.n) return;
   const int g = tiles_of(a.n);
+  if (g > PD_ALPHA_FOLD_TILES)  // some specimen may exceed the threshold
+    hipLaunchKernelGGL(k_mx_alpha, dim3(a.batch.nspec), dim3(PD_BLOCK), 0, s, a, g);
   hipLaunchKernelGGL(k_mx_update, dim3(g), dim3(PD_BLOCK), 0, s, a);
   hipLaunchKernelGGL(k_mx_beta, dim3(a.batch.nspec), dim3(PD_BLOCK), 0, s, a, g);
 }

@@ -882,11 +882,16 @@ __global__ __launch_bounds__(PD_BLOCK) void k_cg_update(CgArgs a) {
   }
   int t0, nt;
   spec_range_of(a.batch, s, ntiles, t0, nt);
-  const double yp = fold_partials(a.part + t0, nt, lds4);
-  const double alpha = st->rz / yp;
-  if (tile == t0 && threadIdx.x == 0) {
-    st->yp = yp;
-    st->alpha = alpha;
+  double alpha;
+  if (nt > PD_ALPHA_FOLD_TILES) {
+    alpha = st->alpha;  // k_cg_alpha folded it ahead of this launch
+  } else {
+    const double yp = fold_partials(a.part + t0, nt, lds4);
+    alpha = st->rz / yp;
+    if (tile == t0 && threadIdx.x == 0) {
+      st->yp = yp;
+      st->alpha = alpha;
+    }
   }
   double t_rz = 0.0, t_rr = 0.0;
   if (row) {
@@ -912,6 +917,22 @@ __global__ __launch_bounds__(PD_BLOCK) void k_cg_update(CgArgs a) {
   if (threadIdx.x == 0) {
     a.part[2 * (int64_t)ntiles + tile] = v0;
     a.part[3 * (int64_t)ntiles + tile] = v1;
+  }
+}
+
+// alpha of the specimens with more than PD_ALPHA_FOLD_TILES tiles, one
+// workgroup each (the same fold and quotient as k_cg_update's)
+__global__ __launch_bounds__(PD_BLOCK) void k_cg_alpha(CgArgs a, int ntiles) {
+  pd_cg_state* st = a.st + blockIdx.x;
+  if (st->done) return;
+  __shared__ double lds4[4];
+  int t0, nt;
+  spec_range(a.batch, ntiles, t0, nt);
+  if (nt <= PD_ALPHA_FOLD_TILES) return;
+  const double yp = fold_partials(a.part + t0, nt, lds4);
+  if (threadIdx.x == 0) {
+    st->yp = yp;
+    st->alpha = st->rz / yp;
   }
 }
 
@@ -1010,6 +1031,8 @@ void launch_write_blocks(const SpmvView& A, double* vals, hipStream_t s) {
 void launch_cg_update(const CgArgs& a, hipStream_t s) {
   if (!a.A.n) return;
   const int g = tiles_of(a.A.n);
+  if (g > PD_ALPHA_FOLD_TILES)  // some specimen may exceed the threshold
+    hipLaunchKernelGGL(k_cg_alpha, dim3(a.batch.nspec), dim3(PD_BLOCK), 0, s, a, g);
   hipLaunchKernelGGL(k_cg_update, dim3(g), dim3(PD_BLOCK), 0, s, a);
   hipLaunchKernelGGL(k_cg_beta, dim3(a.batch.nspec), dim3(PD_BLOCK), 0, s, a, g);
 }

@@ -197,7 +197,10 @@ void SystemParameter::loadParametersFromFile(const char* _paramFile) {
     if (name == "cg_max_iterations") r.cg_max_iterations = atoi(v);
     if (name == "cg_precision") r.cg_precision = (value == "mixed" || value == "1") ? 1 : 0;
     if (name == "cg_inner_rtol") r.cg_inner_rtol = atof(v);
-    if (name == "cg_operator") r.cg_matrix_free = (value == "matrix_free" || value == "1") ? 1 : 0;
+    if (name == "cg_operator")
+      r.cg_matrix_free = (value == "matrix_free" || value == "1") ? 1
+                         : (value == "auto" || value == "2")      ? 2
+                                                                  : 0;
     if (name == "cg_max_refine") r.cg_max_refine = atoi(v);
     if (name == "scene") r.scene_lattice = (value == "lattice") ? 1 : (value == "batch" ? 2 : 0);
     if (name == "batch_count") r.batch_count = atoi(v);

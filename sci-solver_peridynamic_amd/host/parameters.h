@@ -159,7 +159,8 @@ struct RunningParameters {
   int batch_min_particles, batch_max_particles;  // log-uniform
   double batch_max_aspect;                       // long axis z : x = y, uniform in [1, max]
   double batch_s0_min, batch_s0_max;             // s0 per specimen, uniform
-  int cg_matrix_free;           // 1: FP64 SpMV recomputes the bond blocks (cg_operator)
+  int cg_matrix_free;           // cg_operator: 1 the FP64 SpMV recomputes the bond blocks,
+                                // 2 (auto) the cheaper of stored / recomputed per step
 };
 
 class SystemParameter {

@@ -78,6 +78,7 @@ class StepStats(C.Structure):
         ("refinements", C.c_int32), ("ms_resid_total", C.c_double),
         ("resid_launches", C.c_int32), ("precond_fallbacks", C.c_int32),
         ("ms_spmv_store", C.c_double), ("precond_probe", C.c_int32),
+        ("operator_mf", C.c_int32), ("contact_grid", C.c_int32),
     ]
 
     def as_dict(self):

@@ -166,7 +166,7 @@ def test_mf_batch_graph_equals_stored_eager(mode):
 def test_mf_argument_and_state_errors():
     say("case mf errors")
     sys_, ctx = deformed_pair()
-    for bad in (dict(matrix_free=1, precision=1), dict(matrix_free=2), dict(matrix_free=-1)):
+    for bad in (dict(matrix_free=1, precision=1), dict(matrix_free=3), dict(matrix_free=-1)):
         with pytest.raises(pds.PDError) as e:
             ctx.assemble(pds.make_params(**bad))
         assert e.value.code == pds.PD_ERR_ARG
@@ -185,3 +185,41 @@ def test_mf_argument_and_state_errors():
     assert e.value.code == pds.PD_ERR_ARG
     assert ctx.check_division(0) == 0
     ctx.close()
+
+
+def test_operator_per_step_equals_both():
+    """matrix_free = 2 (per step the cheaper of stored and recomputed
+    blocks, timed on the context itself): the first step times the
+    recomputing operator, the second the storing one, and every step after
+    that picks by cost; the trajectory equals the stored and the recomputing
+    runs bit for bit, and the stored matrix after a storing step equals the
+    stored run's."""
+    from test_gpu_configs import setup_pair
+    say("case operator per step")
+    cfg = os.path.join(ROOT, "sci-solver_peridynamic_amd", "configs", "impact_30.txt")
+    desc, sc, S, a = setup_pair(cfg)
+    ctxs = {0: a}
+    for mode in (1, 2):
+        ctxs[mode] = pds.Context(desc["num_pd_particle"], pds.materials_of(desc))
+        pds.load_state(ctxs[mode], sc["x4"], sc["v4"],
+                       pds.stretch_init(desc["num_pd_particle"], desc["s0"], desc["stretch_seed"]),
+                       body=sc["body"], bc=sc["bc"], vbc=sc["vbc"])
+        ctxs[mode].build_bonds()
+    used = []
+    for k in range(1, 46):
+        kw = pds.step_kwargs(desc, k)
+        st = {m: c.step(pds.make_params(**dict(kw, matrix_free=m))) for m, c in ctxs.items()}
+        used.append(st[2].operator_mf)
+        assert st[0].operator_mf == 0 and st[1].operator_mf == 1
+        assert st[2].cg_iterations == st[0].cg_iterations == st[1].cg_iterations, k
+        assert st[2].broken_bonds == st[0].broken_bonds, k
+        xs = {m: c.download("position") for m, c in ctxs.items()}
+        assert np.array_equal(xs[2], xs[0]) and np.array_equal(xs[1], xs[0]), k
+        if k == 2:  # the storing step: its matrix is the stored run's
+            d0, v0 = ctxs[0].get_matrix()
+            d2, v2 = ctxs[2].get_matrix()
+            assert np.array_equal(d0, d2) and np.array_equal(v0, v2)
+    assert used[0] == 1 and used[1] == 0
+    say(dict(steps=len(used), recomputing=int(sum(used)), choices=used))
+    for c in ctxs.values():
+        c.close()

@@ -183,8 +183,10 @@ def test_tension_steps_small():
 def contact_scene(case):
     """(x4, body, materials, contact kwargs) of a contact case.  two_bodies and
     three_bodies build their per-step grid in one workgroup (few candidates,
-    few cells); split_far's grid spans more than 65536 cells and interleaved
-    selects more than 16384 candidates, so both take the multi-launch build."""
+    few cells; two_bodies' grid has at most 256 cells, one radix pass,
+    three_bodies' more, two passes); split_far's grid spans more than 65536
+    cells and interleaved selects more than 16384 candidates, so both take
+    the multi-launch build."""
     if case == "two_bodies":
         a = lattice(8, 8, 4, 0.01, 0.05, 1)
         b = lattice(4, 4, 4, 0.0047, 0.05, 2, origin=(0.02, 0.02, 0.0335))
@@ -217,10 +219,15 @@ def test_contact_bit_exact(case):
     x4, body, mats, kw = contact_scene(case)
     sys_, ctx = make_pair(x4, mats, body=body)
     sys_.contact(orc.make_params(**kw))
-    ctx.assemble(pds.make_params(**kw))
+    st = ctx.assemble(pds.make_params(**kw))
     fc = ctx.download("contact_force")
     assert np.abs(sys_.fc).max() > 0, "the case must have contacts"
     assert np.array_equal(fc[:, :3], sys_.fc[:, :3])
+    # the grid build each case is meant to cover: one workgroup with one
+    # radix pass (<= 256 cells), with two, and the multi-launch build
+    say("contact grid build", st.contact_grid)
+    want = {"two_bodies": 1, "three_bodies": 2, "split_far": 3, "interleaved": 3}[case]
+    assert st.contact_grid == want, (case, st.contact_grid)
     ctx.close()
 
 
@@ -255,4 +262,47 @@ def test_reference_units_identity_cg():
     for rr in rec:
         say(rr)
         assert rr["it_g"] == rr["it_o"] and rr["rel"] <= 1e-9
+    ctx.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_solve_above_alpha_fold_threshold(precision):
+    """More than 8192 tiles of 256 rows in one specimen: the CG updates take
+    alpha from the one-workgroup fold ahead of them (the per-workgroup fold
+    reads grow with the square of the tile count above that size) -- FP64
+    and the float inner solve; iterations, residual and solution equal the
+    oracle's bit for bit."""
+    say("case alpha fold above threshold, precision", precision)
+    h = 0.01
+    x4 = lattice(130, 130, 125, h, 0.05, 21)  # 2,112,500 rows: 8,252 tiles
+    n = x4.shape[0]
+    assert (n + 255) // 256 > 8192
+    mats = [material(h, horizon_factor=1.5)]
+    counts, lists, dropped = orc.bonds_kdtree(x4, mats[0][0], max_bonds=32)
+    assert dropped == 0
+    rs, col = orc.lists_to_csr(counts, lists)
+    del lists
+    rng = np.random.default_rng(22)
+    v4 = np.zeros_like(x4)
+    v4[:, :3] = rng.uniform(-1, 1, (n, 3))
+    s00 = orc.stretch_init(n, 0.02, 1973)
+    S = orc.OracleSystem(x4, v4, s00, rs, col, mats)
+    S.x[:, :3] += rng.uniform(-1, 1, (n, 3)) * 0.03 * h
+    S.s[:] = 10.0
+    ctx = pds.Context(n, mats)
+    pds.load_state(ctx, x4, v4, s00)
+    bst = ctx.build_bonds()
+    assert bst.total_bonds == int(counts.sum())
+    ctx.upload("position", S.x)
+    ctx.upload("stretch", S.s)
+    kw = dict(dt=2e-4, precond=1, rtol=1e-10, precision=precision)
+    S.assemble(orc.make_params(**kw))
+    ctx.assemble(pds.make_params(**kw))
+    so = S.solve(orc.make_params(**kw))
+    sg = ctx.solve(pds.make_params(**kw))
+    say(dict(n=n, tiles=(n + 255) // 256, it=(so.cg_iterations, sg.cg_iterations),
+             rel=sg.residual_norm / sg.b_norm))
+    assert so.converged and sg.converged and sg.cg_iterations >= 3
+    assert (sg.cg_iterations, sg.residual_norm) == (so.cg_iterations, so.residual_norm)
+    assert np.array_equal(ctx.download("solution")[:, :3], S.sol[:, :3])
     ctx.close()
