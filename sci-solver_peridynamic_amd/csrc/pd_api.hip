@@ -149,6 +149,9 @@ struct pd_ctx {
   // the swap-removes; recomputed when bonds or reference positions change
   double* d0 = nullptr;
   bool d0_valid = false;
+  unsigned int* d0_bad = nullptr;  // some d0 outside the shared quotients' range
+  // the assembly record, per particle (x, y, z, s, xs.x, xs.y, xs.z, 0)
+  double* arec = nullptr;
   double asm_dt2 = 0.0;                      // its dt^2 scale
   const pd_step_consts* asm_consts = nullptr;  // captured step: dt2 read here
   // per particle the position and the CG direction side by side (6 doubles,
@@ -538,6 +541,7 @@ pd::CgArgs cg_args(pd_ctx* c, int precond) {
   a.precond = precond;
   a.vals = c->val;
   a.rec = c->rec;
+  a.mf_gate = 0;
   return a;
 }
 
@@ -675,21 +679,40 @@ int enqueue_assemble(pd_ctx* c, const pd_step_params* prm,
     if (int e = ensure_mixed(c)) return e;
   pd::launch_reset_counters(c->counters, c->stream);
   const double dt2 = prm->integrator == 1 ? (prm->dt * prm->dt) / 4.0 : prm->dt * prm->dt;
-  if (fuse) {
-    pd::launch_set_dirichlet(c->n, c->bc, c->act, c->vbc, prm->integrator, prm->dt, consts,
-                             c->sol, c->stream);
-    CHECK_LAUNCH(c, "dirichlet values");
-  }
   if (!c->d0_valid) {  // the assembly and the matrix-free SpMV read them
-    pd::launch_bond_d0(c->n, c->slice_off, c->cap, c->col, c->x0, c->d0, c->stream);
+    pd::launch_bond_d0(c->n, c->slice_off, c->cap, c->col, c->x0, c->d0, c->d0_bad, c->stream);
     CHECK_LAUNCH(c, "reference bond lengths");
     c->d0_valid = true;
   }
   const bool defer = defer_blocks(prm);
   if (prm->matrix_free || defer)
     if (int e = ensure_rec(c)) return e;
+  {  // the assembly record, the Dirichlet start values (fuse), the range flag
+    pd::AsmPrepArgs pa;
+    pa.n = c->n;
+    pa.fuse = fuse ? 1 : 0;
+    pa.integrator = prm->integrator;
+    pa.c_bad = 0;
+    for (int b = 0; b < c->nbody; ++b)
+      if (!(c->mat[b].c_bond >= 0x1p-300 && c->mat[b].c_bond <= 0x1p300)) pa.c_bad = 1;
+    pa.dt = prm->dt;
+    pa.consts = consts;
+    pa.x = c->x;
+    pa.s = c->s;
+    pa.sol = c->sol;
+    pa.bc = c->bc;
+    pa.act = c->act;
+    pa.vbc = c->vbc;
+    pa.d0_bad = c->d0_bad;
+    pa.arec = c->arec;
+    pa.rec = prm->matrix_free || defer ? c->rec : nullptr;
+    pa.counters = c->counters;
+    pd::launch_asm_prep(pa, c->stream);
+    CHECK_LAUNCH(c, "assembly record");
+  }
   pd::AsmArgs a;
   a.d0 = c->d0;
+  a.arec = c->arec;
   a.rec = prm->matrix_free || defer ? c->rec : nullptr;
   c->mf = prm->matrix_free ? 1 : 0;
   c->blocks_deferred = defer;
@@ -1236,7 +1259,7 @@ int capture_step(pd_ctx* c, const pd_step_params* prm, int K) {
   destroy_graphs(c);
   const int ns = c->nspec;
   if (!c->d0_valid) {  // once, outside the captured step
-    pd::launch_bond_d0(c->n, c->slice_off, c->cap, c->col, c->x0, c->d0, c->stream);
+    pd::launch_bond_d0(c->n, c->slice_off, c->cap, c->col, c->x0, c->d0, c->d0_bad, c->stream);
     CHECK_LAUNCH(c, "reference bond lengths");
     c->d0_valid = true;
   }
@@ -1500,6 +1523,8 @@ int pd_ctx_create(const pd_config* cfg, pd_ctx** out) {
   ALLOC(c->st, 1);
   c->state_cap = 1;
   ALLOC(c->counters, 1);
+  ALLOC(c->d0_bad, 1);
+  ALLOC(c->arec, n * 8);
   ALLOC(c->dmat, PD_MAX_BODIES);
   ALLOC(c->dhorizon, PD_MAX_BODIES);
   ALLOC(c->gap_val, n);
@@ -1558,7 +1583,7 @@ int pd_ctx_destroy(pd_ctx* c) {
   dfree(c->act); dfree(c->body); dfree(c->bc); dfree(c->top); dfree(c->cap);
   dfree(c->slice_off); dfree(c->col); dfree(c->val); dfree(c->d0); dfree(c->diag); dfree(c->minv);
   dfree(c->r); dfree(c->z); dfree(c->p); dfree(c->y); dfree(c->part); dfree(c->st);
-  dfree(c->counters); dfree(c->dmat); dfree(c->dhorizon); dfree(c->rec);
+  dfree(c->counters); dfree(c->d0_bad); dfree(c->arec); dfree(c->dmat); dfree(c->dhorizon); dfree(c->rec);
   dfree(c->gap_val); dfree(c->gap_q); dfree(c->gpart_v); dfree(c->gpart_p); dfree(c->gpart_q);
   dfree(c->gap_out); dfree(c->gap_pq);
   dfree(c->keys_a); dfree(c->keys_b); dfree(c->idx_a); dfree(c->idx_b); dfree(c->hist);

@@ -53,7 +53,10 @@ __device__ __forceinline__ bool range_ok_coord(double v) {
   const double m = fabs(v);
   return m == 0.0 || (m >= 0x1p-500 && m <= 0x1p100);
 }
-// c_bond / d0 with c_bond in [2^-300, 2^300] and d0 in [2^-400, 2^100].
+// c_bond / d0 with c_bond in [2^-300, 2^300] and d0 in [2^-400, 2^100]; and
+// the current length over d0: the length of a difference of in-range
+// coordinates is 0 or in [2^-537, 2^102] (the smallest nonzero square is the
+// smallest subnormal, 2^-1074), so dij / d0 lies in [2^-637, 2^502].
 __device__ __forceinline__ bool range_ok_len(double d0) {
   return d0 >= 0x1p-400 && d0 <= 0x1p100;
 }
@@ -94,25 +97,34 @@ __device__ __forceinline__ double ref_len(const double4 opos_p, const double4 oq
 }
 
 // The bond's 3x3 Jacobian F (six distinct entries), cg_solver.cuh:292-299:
-// (c / d0) [(1 - d0/dij) I + (d0/dij) e e^T].  SHARED: c / d0 through
-// rcp_nr (identical results inside the ranges above).
-template <bool SHARED = false>
-__device__ __forceinline__ void bond_jac(double e0, double e1, double e2, double dij, double d0,
-                                         double c_bond, double& F00, double& F01, double& F02,
-                                         double& F11, double& F12, double& F22) {
+// (c / d0) [(1 - d0/dij) I + (d0/dij) e e^T], with the factor sc = c / d0
+// given.
+__device__ __forceinline__ void bond_jac_sc(double e0, double e1, double e2, double dij,
+                                            double d0, double sc, double& F00, double& F01,
+                                            double& F02, double& F11, double& F12,
+                                            double& F22) {
   const double ratio = d0 / dij;
   const double dg = 1.0 - ratio;
-  double sc;
-  if constexpr (SHARED)
-    sc = div_rcp(c_bond, d0, rcp_nr(d0));
-  else
-    sc = c_bond / d0;
   F00 = (dg + (e0 * e0) * ratio) * sc;
   F01 = (0.0 + (e0 * e1) * ratio) * sc;
   F02 = (0.0 + (e0 * e2) * ratio) * sc;
   F11 = (dg + (e1 * e1) * ratio) * sc;
   F12 = (0.0 + (e1 * e2) * ratio) * sc;
   F22 = (dg + (e2 * e2) * ratio) * sc;
+}
+
+// The same from c: SHARED forms c / d0 through rcp_nr (identical results
+// inside the ranges above).
+template <bool SHARED = false>
+__device__ __forceinline__ void bond_jac(double e0, double e1, double e2, double dij, double d0,
+                                         double c_bond, double& F00, double& F01, double& F02,
+                                         double& F11, double& F12, double& F22) {
+  double sc;
+  if constexpr (SHARED)
+    sc = div_rcp(c_bond, d0, rcp_nr(d0));
+  else
+    sc = c_bond / d0;
+  bond_jac_sc(e0, e1, e2, dij, d0, sc, F00, F01, F02, F11, F12, F22);
 }
 
 __device__ __forceinline__ void blk6(double a00, double a01, double a02, double a11,

@@ -56,7 +56,7 @@ struct AsmArgs {
   // fuse = 1: the solve's first residual r = S(b - A xs), each row's bond
   // terms summed first and the diagonal term last, with the solver's start
   // and its tile partials in part (Dirichlet rows of xs must hold their
-  // prescribed values already: launch_set_dirichlet).  z = r (pc = 0) or
+  // prescribed values already: k_asm_prep).  z = r (pc = 0) or
   // the block-Jacobi inverse just formed times r (pc = 1).
   //   FP64:  r64 = r, p64 = z; partials <r,z>, <r,r>, <Sb,Sb>
   //   mixed: r32 = float(r), p32 = z, d32 = 0 (A32 with float bond blocks);
@@ -72,8 +72,31 @@ struct AsmArgs {
   double* part;
   double* gap_val;   // [n] when track_gap
   int32_t* gap_q;    // [n] when track_gap
+  pd_step_counters* counters;  // range_bad (set by k_asm_prep) selects SHARED
+  double* rec;  // non-null: write the p part of the matrix-free gather record
+  // per particle (x, y, z, s, xs.x, xs.y, xs.z, 0), written by k_asm_prep
+  const double* arec;
+};
+
+// The assembly record and the quotients' range flag ahead of the assembly
+// (k_asm_prep); with fuse, the Dirichlet rows' start values first.
+struct AsmPrepArgs {
+  int32_t n;
+  int32_t fuse;
+  int32_t integrator;
+  int32_t c_bad;  // a body's bond stiffness outside range_ok_stiff
+  double dt;
+  const pd_step_consts* consts;  // non-null: dt from here
+  const double4* x;
+  const double* s;
+  double4* sol;
+  const int32_t* bc;
+  const int32_t* act;
+  const double4* vbc;
+  const unsigned int* d0_bad;  // k_bond_d0's range flag
+  double* arec;                // [n][8]
+  double* rec;                 // non-null: the matrix-free record's position part
   pd_step_counters* counters;
-  double* rec;  // non-null: write the position and p parts of the gather record
 };
 
 struct SpmvView {
@@ -154,6 +177,9 @@ struct CgArgs {
   // to vals, which the later products stream
   double* vals;
   const double* rec;
+  // matrix-free products: 1 when k_cg_spmv_mf, launched ahead, did the
+  // product (range flag clear), so the general kernel returns at once
+  int32_t mf_gate;
 };
 
 // Mixed-precision solve: float matrix copy, float inner vectors, FP64
@@ -297,6 +323,7 @@ struct GridView {
 };
 
 void launch_assemble(const AsmArgs& a, hipStream_t s);
+void launch_asm_prep(const AsmPrepArgs& a, hipStream_t s);
 void launch_gap_reduce(const double* gap_val, const int32_t* gap_q, int32_t n,
                        double* part_val, int32_t* part_p, int32_t* part_q,
                        double* out_val, int32_t* out_pq, hipStream_t s);
@@ -343,14 +370,10 @@ void launch_copy(double* dst, const double* src, int32_t n, hipStream_t s);
 // random operand pairs; *bad counts the differing ones.
 void launch_check_division(int64_t samples, uint64_t seed, unsigned long long* bad,
                            hipStream_t s);
-// d0[slot] = |x0_p - x0_col[slot]| for every slot below the row's capacity
+// d0[slot] = |x0_p - x0_col[slot]| for every slot below the row's capacity;
+// *bad = 1 if one lies outside the shared quotients' range, else 0
 void launch_bond_d0(int32_t n, const int64_t* slice_off, const int32_t* cap, const int32_t* col,
-                    const double4* x0, double* d0, hipStream_t s);
-// sol of the Dirichlet rows: vbc (implicit Euler) or dt vbc (Newmark-beta),
-// the values the assembly writes; dt from consts when non-null
-void launch_set_dirichlet(int32_t n, const int32_t* bc, const int32_t* act, const double4* vbc,
-                          int32_t integrator, double dt, const pd_step_consts* consts,
-                          double4* sol, hipStream_t s);
+                    const double4* x0, double* d0, unsigned int* bad, hipStream_t s);
 // contact forces of the m candidates cand[0..m) (particles 0..m when cand
 // is null); fc of every other particle is set to zero
 void launch_contact(int32_t n, const double4* x, const int32_t* body,

@@ -22,11 +22,32 @@ namespace pd {
 // that recomputation, the stores adding to the arithmetic rather than
 // overlapping it), or every matrix-free product recomputes them.
 // ---------------------------------------------------------------------------
+// Neighbour q's position, stretch limit and (fuse) start value from the
+// assembly record.
+__device__ __forceinline__ void gather_asm(const AsmArgs& a, int q, double4& xq, double& sq,
+                                           double4& us) {
+  const dbl2v* r = (const dbl2v*)(a.arec + 8 * (int64_t)q);
+  const dbl2v r0 = r[0], r1 = r[1];
+  xq = make_double4(r0.x, r0.y, r1.x, 0.0);
+  sq = r1.y;
+  if (a.fuse) {
+    const dbl2v r2 = r[2], r3 = r[3];
+    us = make_double4(r2.x, r2.y, r3.x, 0.0);
+  }
+}
+
 // MIXED: the bond blocks and the block-Jacobi inverse are written rounded
 // to float (non-temporal stores: they are read back once per inner
 // iteration, never soon enough to gain from L2), the diagonal in both.
-template <bool MIXED>
+// Neighbour data come from the step's assembly record (position, stretch
+// limit and the solver's start value side by side, k_asm_prep).  SHARED:
+// the quotients by the float-rounded length (bond_dir) and by the reference
+// length (dij / d0, c / d0) share reciprocals; both variants are launched
+// and the one that does not match the range flag k_asm_prep set returns at
+// once.
+template <bool MIXED, bool SHARED>
 __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
+  if ((a.counters->range_bad == 0) != SHARED) return;
   if (a.consts) {
     a.dt = a.consts->dt;
     a.dt2 = a.consts->dt2;
@@ -36,7 +57,6 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int p = tile * PD_BLOCK + threadIdx.x;
   unsigned long long broken = 0, live = 0;
-  bool range_bad = false;
   double t_rr = 0.0, t_rz = 0.0, t_r32 = 0.0, t_bb = 0.0;  // fused residual partials
   if (p < a.n && a.act[p] != PD_INVISIBLE) {
     const pd_mat_dev mt = a.mat[a.body[p]];
@@ -44,15 +64,6 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
     const int lane = p & 63;
     const int64_t base = a.slice_off[s];
     const double4 pos_p = a.x[p];
-    // operands of the matrix-free operator's shared-reciprocal quotients
-    range_bad = !(range_ok_coord(pos_p.x) && range_ok_coord(pos_p.y) &&
-                  range_ok_coord(pos_p.z) && range_ok_stiff(mt.c_bond));
-    if (a.rec) {  // the matrix-free operator's gather record: position part
-      double* rp = a.rec + 6 * (int64_t)p;
-      rp[0] = pos_p.x;
-      rp[1] = pos_p.y;
-      rp[2] = pos_p.z;
-    }
     double f0 = 0.0, f1 = 0.0, f2 = 0.0;
     int top = a.top[p];
     const int otop = top;
@@ -76,10 +87,8 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
     double l_n = 0.0;  // the slot's reference length
     if (top >= 0) {
       q_n = a.col[base * 64 + lane];
-      xq_n = a.x[q_n];
       l_n = a.d0[base * 64 + lane];
-      sq_n = a.s[q_n];
-      if (a.fuse) us_n = a.xs[q_n];
+      gather_asm(a, q_n, xq_n, sq_n, us_n);
     }
     if (top >= 1) q_nn = a.col[(base + 1) * 64 + lane];
     while (bnd <= top) {
@@ -91,16 +100,15 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
       const double4 u = us_n;
       if (bnd + 1 <= top) {
         q_n = q_nn;
-        xq_n = a.x[q_n];
         l_n = a.d0[(slot + 1) * 64 + lane];
-        sq_n = a.s[q_n];
-        if (a.fuse) us_n = a.xs[q_n];
+        gather_asm(a, q_n, xq_n, sq_n, us_n);
       }
       if (bnd + 2 <= top) q_nn = a.col[(slot + 2) * 64 + lane];
-      range_bad |= !range_ok_len(d0);
       double e0, e1, e2, dij;
-      bond_dir(pos_p, xq, e0, e1, e2, dij);
-      const double drdivd = dij / d0 - 1.0;
+      bond_dir<SHARED>(pos_p, xq, e0, e1, e2, dij);
+      double y0 = 0.0;  // SHARED: 1 / d0 for dij / d0 and c / d0
+      if constexpr (SHARED) y0 = rcp_nr(d0);
+      const double drdivd = (SHARED ? div_rcp(dij, d0, y0) : dij / d0) - 1.0;
       // cg_solver.cuh:264: threshold (float)max(s_p, s_q)
       const float sq_f = (float)sqv;
       const double thr = (double)(stretch_f > sq_f ? stretch_f : sq_f);
@@ -125,10 +133,8 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
         --top;
         // slot bnd again, now holding `last`: restart the pipeline there
         q_n = last;
-        xq_n = a.x[last];
         l_n = a.d0[slot * 64 + lane];
-        sq_n = a.s[last];
-        if (a.fuse) us_n = a.xs[last];
+        gather_asm(a, last, xq_n, sq_n, us_n);
         if (bnd + 1 <= top) q_nn = a.col[(slot + 1) * 64 + lane];
         continue;
       }
@@ -142,7 +148,8 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
       f1 += kk * e1;
       f2 += kk * e2;
       double F00, F01, F02, F11, F12, F22;
-      bond_jac(e0, e1, e2, dij, d0, mt.c_bond, F00, F01, F02, F11, F12, F22);
+      const double sc = SHARED ? div_rcp(mt.c_bond, d0, y0) : mt.c_bond / d0;
+      bond_jac_sc(e0, e1, e2, dij, d0, sc, F00, F01, F02, F11, F12, F22);
       k00 += F00;
       k01 += F01;
       k02 += F02;
@@ -302,7 +309,7 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
         t_rz = (r0 * z0 + r1 * z1) + r2 * z2;
       }
     }
-    if (a.bc[p] && !a.fuse) {  // fused: launch_set_dirichlet wrote it
+    if (a.bc[p] && !a.fuse) {  // fused: k_asm_prep wrote it
       const double4 vb = a.vbc[p];
       if (a.integrator == 1)
         a.sol[p] = make_double4(a.dt * vb.x, a.dt * vb.y, a.dt * vb.z, 0.0);
@@ -374,16 +381,64 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
     if (broken) atomicAdd(&a.counters->broken, broken);
     if (live) atomicAdd(&a.counters->live, live);
   }
-  if (range_bad) atomicOr(&a.counters->range_bad, 1u);
 }
 
 void launch_assemble(const AsmArgs& a, hipStream_t s) {
   const int nb = (a.n + PD_BLOCK - 1) / PD_BLOCK;
   if (!nb) return;
-  if (a.mixed)
-    hipLaunchKernelGGL(k_assemble<true>, dim3(nb), dim3(PD_BLOCK), 0, s, a);
-  else
-    hipLaunchKernelGGL(k_assemble<false>, dim3(nb), dim3(PD_BLOCK), 0, s, a);
+  // both variants; the one the range flag does not select returns at once
+  if (a.mixed) {
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_assemble<true, true>), dim3(nb), dim3(PD_BLOCK), 0, s, a);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_assemble<true, false>), dim3(nb), dim3(PD_BLOCK), 0, s, a);
+  } else {
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_assemble<false, true>), dim3(nb), dim3(PD_BLOCK), 0, s, a);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_assemble<false, false>), dim3(nb), dim3(PD_BLOCK), 0, s, a);
+  }
+}
+
+// The step's assembly record, ahead of the assembly: per particle
+// (x, y, z, s, xs.x, xs.y, xs.z, 0) -- position, stretch limit and the
+// solver's start value, one 64-byte gather per bond in the assembly.  With
+// fuse, the Dirichlet rows' start values are set first (vbc, or dt vbc for
+// Newmark-beta).  The position part of the matrix-free record is written
+// here too, and the range flag of the shared-reciprocal quotients: set when
+// a visible particle's coordinate, a reference length (d0_bad, from
+// k_bond_d0) or a bond stiffness (c_bad) lies outside the ranges of
+// range_ok_*.
+__global__ __launch_bounds__(PD_BLOCK) void k_asm_prep(AsmPrepArgs a) {
+  const int p = blockIdx.x * PD_BLOCK + threadIdx.x;
+  bool bad = false;
+  if (p < a.n) {
+    const double4 xp = a.x[p];
+    double4 xs = a.sol[p];
+    const bool vis = a.act[p] != PD_INVISIBLE;
+    if (a.fuse && a.bc[p] && vis) {
+      const double dt = a.consts ? a.consts->dt : a.dt;
+      const double4 vb = a.vbc[p];
+      xs = a.integrator == 1 ? make_double4(dt * vb.x, dt * vb.y, dt * vb.z, 0.0)
+                             : make_double4(vb.x, vb.y, vb.z, 0.0);
+      a.sol[p] = xs;
+    }
+    dbl2v* r = (dbl2v*)(a.arec + 8 * (int64_t)p);
+    r[0] = dbl2v{xp.x, xp.y};
+    r[1] = dbl2v{xp.z, a.s[p]};
+    r[2] = dbl2v{xs.x, xs.y};
+    r[3] = dbl2v{xs.z, 0.0};
+    if (a.rec) {
+      double* rp = a.rec + 6 * (int64_t)p;
+      rp[0] = xp.x;
+      rp[1] = xp.y;
+      rp[2] = xp.z;
+    }
+    bad = vis && !(range_ok_coord(xp.x) && range_ok_coord(xp.y) && range_ok_coord(xp.z));
+  }
+  if (p == 0 && (a.c_bad || *a.d0_bad)) bad = true;
+  if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(&a.counters->range_bad, 1u);
+}
+
+void launch_asm_prep(const AsmPrepArgs& a, hipStream_t s) {
+  const int nb = (a.n + PD_BLOCK - 1) / PD_BLOCK;
+  if (nb) hipLaunchKernelGGL(k_asm_prep, dim3(nb), dim3(PD_BLOCK), 0, s, a);
 }
 
 __global__ void k_reset_counters(pd_step_counters* c) {
@@ -587,8 +642,9 @@ __global__ __launch_bounds__(PD_BLOCK) void k_spmv_plain(SpmvView A,
 
 // Self-check of the shared-reciprocal quotients against IEEE division over
 // random operands spanning the ranges the range flag admits (odd samples:
-// a coordinate difference over a float bond length; even samples: a bond
-// stiffness over a reference length); the exponent ends are drawn often.
+// a coordinate difference over a float bond length; samples 2 mod 4: a
+// current bond length over a reference length; 0 mod 4: a bond stiffness
+// over a reference length); the exponent ends are drawn often.
 __device__ __forceinline__ uint64_t mix64(uint64_t z) {
   z += 0x9e3779b97f4a7c15ull;
   z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
@@ -615,7 +671,10 @@ __global__ __launch_bounds__(PD_BLOCK) void k_check_division(int64_t samples, ui
     float bf = (h2 & 15) == 0 ? small_f : fabsf((float)draw_f64(h2, -34, 101));
     if (!(bf >= small_f)) bf = small_f;
     b = (double)bf;
-  } else {
+  } else if ((i & 3) == 2) {  // a current bond length over a reference length
+    a = (h1 & 15) == 0 ? 0.0 : fabs(draw_f64(h1, -537, 101));
+    b = fabs(draw_f64(h2, -400, 99));
+  } else {  // a bond stiffness over a reference length
     a = fabs(draw_f64(h1, -300, 299));
     b = fabs(draw_f64(h2, -400, 99));
   }
@@ -784,6 +843,7 @@ __global__ __launch_bounds__(PD_BLOCK) void k_cg_init_fold(CgArgs a, int ntiles)
 // y = A p; partial <y,p>
 template <int U>
 __global__ __launch_bounds__(PD_BLOCK) void k_cg_spmv(CgArgs a) {
+  if (a.mf_gate && *a.A.range_bad == 0) return;  // k_cg_spmv_mf did it
   __shared__ double lds4[4];
   const int ntiles = gridDim.x;
   const int tile = xcd_tile(blockIdx.x, ntiles);
@@ -796,6 +856,30 @@ __global__ __launch_bounds__(PD_BLOCK) void k_cg_spmv(CgArgs a) {
       row_of<U, false, true, true>(a.A, a.p, p, y0, y1, y2);
     else
       row_of<U, false, false, true>(a.A, a.p, p, y0, y1, y2);
+    a.y[p] = make_double4(y0, y1, y2, 0.0);
+    const double4 pp = a.p[p];
+    t = term3(y0, y1, y2, pp.x, pp.y, pp.z);
+  }
+  const double v = tile_sum(t, lds4);
+  if (threadIdx.x == 0) a.part[tile] = v;
+}
+
+// The matrix-free CG product in its common case alone: gathers from the
+// record, shared reciprocals (the range flag clear, else it returns at once
+// and k_cg_spmv takes the product), two bonds per trip.  Without the other
+// variants the kernel needs 82 VGPRs instead of 126 (five waves per SIMD
+// instead of four): 570 against 593 us per product at 1M particles.
+__global__ __launch_bounds__(PD_BLOCK) void k_cg_spmv_mf(CgArgs a) {
+  if (*a.A.range_bad != 0) return;
+  __shared__ double lds4[4];
+  const int ntiles = gridDim.x;
+  const int tile = xcd_tile(blockIdx.x, ntiles);
+  if (a.st[spec_of(a.batch, tile)].done) return;
+  const int p = tile * PD_BLOCK + threadIdx.x;
+  double t = 0.0;
+  if (p < a.A.n) {
+    double y0, y1, y2;
+    spmv_row_mf<2, false, true, true>(a.A, a.p, p, y0, y1, y2, a.A.rec);
     a.y[p] = make_double4(y0, y1, y2, 0.0);
     const double4 pp = a.p[p];
     t = term3(y0, y1, y2, pp.x, pp.y, pp.z);
@@ -1011,7 +1095,13 @@ void launch_cg_spmv(const CgArgs& a, hipStream_t s) {
   if (!a.A.n) return;
   const int g = tiles_of(a.A.n);
   if (a.A.mf) {
-    hipLaunchKernelGGL(k_cg_spmv<0>, dim3(g), dim3(PD_BLOCK), 0, s, a);
+    CgArgs b = a;
+    b.mf_gate = 0;
+    if (a.A.rec && a.A.range_bad) {  // the common case first, the general kernel if needed
+      hipLaunchKernelGGL(k_cg_spmv_mf, dim3(g), dim3(PD_BLOCK), 0, s, a);
+      b.mf_gate = 1;
+    }
+    hipLaunchKernelGGL(k_cg_spmv<0>, dim3(g), dim3(PD_BLOCK), 0, s, b);
     return;
   }
   switch (a.A.unroll) {
@@ -1127,50 +1217,39 @@ void launch_update(int32_t n, double4* x, double4* v, const double4* sol,
   if (nb) hipLaunchKernelGGL(k_update, dim3(nb), dim3(PD_BLOCK), 0, s, a);
 }
 
-__global__ __launch_bounds__(PD_BLOCK) void k_set_dirichlet(int n, const int32_t* bc,
-                                                            const int32_t* act,
-                                                            const double4* vbc, int integrator,
-                                                            double dt,
-                                                            const pd_step_consts* consts,
-                                                            double4* sol) {
-  const int p = blockIdx.x * PD_BLOCK + threadIdx.x;
-  if (p >= n || !bc[p] || act[p] == PD_INVISIBLE) return;
-  if (consts) dt = consts->dt;
-  const double4 vb = vbc[p];
-  if (integrator == 1)
-    sol[p] = make_double4(dt * vb.x, dt * vb.y, dt * vb.z, 0.0);
-  else
-    sol[p] = make_double4(vb.x, vb.y, vb.z, 0.0);
-}
-
-void launch_set_dirichlet(int32_t n, const int32_t* bc, const int32_t* act, const double4* vbc,
-                          int32_t integrator, double dt, const pd_step_consts* consts,
-                          double4* sol, hipStream_t s) {
-  const int nb = (n + PD_BLOCK - 1) / PD_BLOCK;
-  if (nb)
-    hipLaunchKernelGGL(k_set_dirichlet, dim3(nb), dim3(PD_BLOCK), 0, s, n, bc, act, vbc,
-                       integrator, dt, consts, sol);
-}
-
+// d0 of every slot below the row's capacity; *bad (cleared first by
+// k_clear_word) set if one lies outside range_ok_len
 __global__ __launch_bounds__(PD_BLOCK) void k_bond_d0(int n, const int64_t* slice_off,
                                                       const int32_t* cap, const int32_t* col,
-                                                      const double4* x0, double* d0) {
+                                                      const double4* x0, double* d0,
+                                                      unsigned int* bad) {
   const int p = blockIdx.x * PD_BLOCK + threadIdx.x;
-  if (p >= n) return;
-  const int64_t base = slice_off[p >> 6];
-  const int lane = p & 63;
-  const double4 op = x0[p];
-  for (int k = 0; k < cap[p]; ++k) {
-    const int64_t i = (base + k) * 64 + lane;
-    d0[i] = ref_len(op, x0[col[i]]);
+  bool out = false;
+  if (p < n) {
+    const int64_t base = slice_off[p >> 6];
+    const int lane = p & 63;
+    const double4 op = x0[p];
+    for (int k = 0; k < cap[p]; ++k) {
+      const int64_t i = (base + k) * 64 + lane;
+      const double l = ref_len(op, x0[col[i]]);
+      d0[i] = l;
+      out = out || !range_ok_len(l);
+    }
   }
+  if (__any(out) && (threadIdx.x & 63) == 0) atomicOr(bad, 1u);
+}
+
+__global__ void k_clear_word(unsigned int* w) {
+  if (threadIdx.x == 0) *w = 0u;
 }
 
 void launch_bond_d0(int32_t n, const int64_t* slice_off, const int32_t* cap, const int32_t* col,
-                    const double4* x0, double* d0, hipStream_t s) {
+                    const double4* x0, double* d0, unsigned int* bad, hipStream_t s) {
+  hipLaunchKernelGGL(k_clear_word, dim3(1), dim3(64), 0, s, bad);
   const int nb = (n + PD_BLOCK - 1) / PD_BLOCK;
   if (nb)
-    hipLaunchKernelGGL(k_bond_d0, dim3(nb), dim3(PD_BLOCK), 0, s, n, slice_off, cap, col, x0, d0);
+    hipLaunchKernelGGL(k_bond_d0, dim3(nb), dim3(PD_BLOCK), 0, s, n, slice_off, cap, col, x0, d0,
+                       bad);
 }
 
 __global__ __launch_bounds__(PD_BLOCK) void k_copy(double* dst, const double* src, int n) {
