@@ -24,13 +24,14 @@ namespace pd {
 // ---------------------------------------------------------------------------
 // Neighbour q's position, stretch limit and (fuse) start value from the
 // assembly record.
+template <bool FUSE>
 __device__ __forceinline__ void gather_asm(const AsmArgs& a, int q, double4& xq, double& sq,
                                            double4& us) {
   const dbl2v* r = (const dbl2v*)(a.arec + 8 * (int64_t)q);
   const dbl2v r0 = r[0], r1 = r[1];
   xq = make_double4(r0.x, r0.y, r1.x, 0.0);
   sq = r1.y;
-  if (a.fuse) {
+  if constexpr (FUSE) {
     const dbl2v r2 = r[2], r3 = r[3];
     us = make_double4(r2.x, r2.y, r3.x, 0.0);
   }
@@ -45,7 +46,8 @@ __device__ __forceinline__ void gather_asm(const AsmArgs& a, int q, double4& xq,
 // length (dij / d0, c / d0) share reciprocals; both variants are launched
 // and the one that does not match the range flag k_asm_prep set returns at
 // once.
-template <bool MIXED, bool SHARED>
+// FUSE and GAP: a.fuse and a.track_gap as compile-time choices.
+template <bool MIXED, bool SHARED, bool FUSE, bool GAP>
 __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
   if ((a.counters->range_bad == 0) != SHARED) return;
   if (a.consts) {
@@ -88,7 +90,7 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
     if (top >= 0) {
       q_n = a.col[base * 64 + lane];
       l_n = a.d0[base * 64 + lane];
-      gather_asm(a, q_n, xq_n, sq_n, us_n);
+      gather_asm<FUSE>(a, q_n, xq_n, sq_n, us_n);
     }
     if (top >= 1) q_nn = a.col[(base + 1) * 64 + lane];
     while (bnd <= top) {
@@ -101,7 +103,7 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
       if (bnd + 1 <= top) {
         q_n = q_nn;
         l_n = a.d0[(slot + 1) * 64 + lane];
-        gather_asm(a, q_n, xq_n, sq_n, us_n);
+        gather_asm<FUSE>(a, q_n, xq_n, sq_n, us_n);
       }
       if (bnd + 2 <= top) q_nn = a.col[(slot + 2) * 64 + lane];
       double e0, e1, e2, dij;
@@ -112,7 +114,7 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
       // cg_solver.cuh:264: threshold (float)max(s_p, s_q)
       const float sq_f = (float)sqv;
       const double thr = (double)(stretch_f > sq_f ? stretch_f : sq_f);
-      if (a.track_gap) {
+      if constexpr (GAP) {
         const double gap = fabs(drdivd - thr);
         if (gap < gmin) {
           gmin = gap;
@@ -134,7 +136,7 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
         // slot bnd again, now holding `last`: restart the pipeline there
         q_n = last;
         l_n = a.d0[slot * 64 + lane];
-        gather_asm(a, last, xq_n, sq_n, us_n);
+        gather_asm<FUSE>(a, last, xq_n, sq_n, us_n);
         if (bnd + 1 <= top) q_nn = a.col[(slot + 1) * 64 + lane];
         continue;
       }
@@ -171,7 +173,7 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
       // FP64 blocks are written by the solve's first product (k_cg_spmv_store
       // or k_cg_init's storing variant) or, matrix-free, recomputed by every
       // product; here they feed the first residual of the FP64 system
-      if (a.fuse) {
+      if constexpr (FUSE) {
         rb0 += b00 * u.x + b01 * u.y + b02 * u.z;
         rb1 += b01 * u.x + b11 * u.y + b12 * u.z;
         rb2 += b02 * u.x + b12 * u.y + b22 * u.z;
@@ -265,7 +267,7 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
     }
     bo.w = 0.0;
     a.b[p] = bo;
-    if (a.fuse) {
+    if constexpr (FUSE) {
       // y = D x_p + (bond terms); r = S(b - y); the solver's start
       const double4 xp = a.xs[p];
       const double y0 = (d00 * xp.x + d01 * xp.y + d02 * xp.z) + rb0;
@@ -309,7 +311,7 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
         t_rz = (r0 * z0 + r1 * z1) + r2 * z2;
       }
     }
-    if (a.bc[p] && !a.fuse) {  // fused: k_asm_prep wrote it
+    if (!FUSE && a.bc[p]) {  // fused: k_asm_prep wrote it
       const double4 vb = a.vbc[p];
       if (a.integrator == 1)
         a.sol[p] = make_double4(a.dt * vb.x, a.dt * vb.y, a.dt * vb.z, 0.0);
@@ -324,16 +326,16 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
     if (a.bond_stats)
       a.bond_stats[p] = make_float2(top >= 0 ? (float)(ssum / (top + 1)) : 0.0f,
                                     top >= 0 ? (float)smax : 0.0f);
-    if (a.track_gap) {
+    if constexpr (GAP) {
       a.gap_val[p] = gmin;
       a.gap_q[p] = gq;
     }
   } else if (p < a.n) {
-    if (a.track_gap) {
+    if constexpr (GAP) {
       a.gap_val[p] = INFINITY;
       a.gap_q[p] = -1;
     }
-    if (a.fuse) {
+    if constexpr (FUSE) {
       if constexpr (MIXED) {
         const float4 z4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         a.r32[p] = z4;
@@ -346,7 +348,7 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
       }
     }
   }
-  if (a.fuse) {  // the start kernel's tile partials, all threads
+  if constexpr (FUSE) {  // the start kernel's tile partials, all threads
     __shared__ double lds4[4];
     const int ntiles = gridDim.x;
     if constexpr (MIXED) {
@@ -383,16 +385,28 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
   }
 }
 
+template <bool MIXED, bool FUSE, bool GAP>
+void launch_assemble_t(const AsmArgs& a, int nb, hipStream_t s) {
+  // both variants; the one the range flag does not select returns at once
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_assemble<MIXED, true, FUSE, GAP>), dim3(nb), dim3(PD_BLOCK),
+                     0, s, a);
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_assemble<MIXED, false, FUSE, GAP>), dim3(nb), dim3(PD_BLOCK),
+                     0, s, a);
+}
+
 void launch_assemble(const AsmArgs& a, hipStream_t s) {
   const int nb = (a.n + PD_BLOCK - 1) / PD_BLOCK;
   if (!nb) return;
-  // both variants; the one the range flag does not select returns at once
-  if (a.mixed) {
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_assemble<true, true>), dim3(nb), dim3(PD_BLOCK), 0, s, a);
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_assemble<true, false>), dim3(nb), dim3(PD_BLOCK), 0, s, a);
-  } else {
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_assemble<false, true>), dim3(nb), dim3(PD_BLOCK), 0, s, a);
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_assemble<false, false>), dim3(nb), dim3(PD_BLOCK), 0, s, a);
+  const int k = (a.mixed ? 4 : 0) + (a.fuse ? 2 : 0) + (a.track_gap ? 1 : 0);
+  switch (k) {
+    case 0: launch_assemble_t<false, false, false>(a, nb, s); break;
+    case 1: launch_assemble_t<false, false, true>(a, nb, s); break;
+    case 2: launch_assemble_t<false, true, false>(a, nb, s); break;
+    case 3: launch_assemble_t<false, true, true>(a, nb, s); break;
+    case 4: launch_assemble_t<true, false, false>(a, nb, s); break;
+    case 5: launch_assemble_t<true, false, true>(a, nb, s); break;
+    case 6: launch_assemble_t<true, true, false>(a, nb, s); break;
+    default: launch_assemble_t<true, true, true>(a, nb, s); break;
   }
 }
 
