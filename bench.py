@@ -50,7 +50,8 @@ IMPACT_STEP = 37
 RUN_STEPS = 200
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILES = {0: os.path.join(ROOT, "profiles", "cg_spmv_impact_100_traffic.txt"),
-                 1: os.path.join(ROOT, "profiles", "mx_spmv_impact_100_traffic.txt")}
+                 1: os.path.join(ROOT, "profiles", "mx_spmv_impact_100_traffic.txt"),
+                 "mf": os.path.join(ROOT, "profiles", "mf_spmv_impact_100_traffic.txt")}
 
 
 def log(*a):
@@ -58,8 +59,9 @@ def log(*a):
 
 
 def read_traffic(precision=0):
-    """HBM bytes per CG SpMV launch (FP64 k_cg_spmv, or the float inner
-    k_mx_spmv for precision 1) from the committed rocprof counter runs."""
+    """HBM bytes per CG SpMV launch (FP64 k_cg_spmv, the float inner
+    k_mx_spmv for precision 1, the recomputing k_cg_spmv_mf for "mf") from
+    the committed rocprof counter runs."""
     try:
         with open(TRAFFIC_FILES[precision]) as f:
             for line in f:
@@ -314,15 +316,18 @@ def mf_roofline(r):
     flops = MF_FLOPS_BOND * r["live"] + MF_FLOPS_ROW * r["n"]
     achieved = flops / (avg_ms * 1e-3) / 1e12
     return {
-        "kernel": "k_cg_spmv<0> (matrix-free CG SpMV: blocks recomputed from positions)",
-        # FP64 arithmetic and the gathers' latency share the limit (counters:
-        # VALU about 70 % busy, texture units about 55 %); the FP64 peak is the
-        # arithmetic ceiling it is quoted against
+        "kernel": "k_cg_spmv_mf (matrix-free CG SpMV: blocks recomputed from positions)",
+        # the vector memory path (texture address / data units 84 / 91 %
+        # busy) and FP64 issue (VALU 69 % busy, 125 instructions per bond for
+        # its 69 operations) share the limit; the FP64 peak is the arithmetic
+        # ceiling it is quoted against
         "bound": "fp64-valu",
         "achieved": achieved,
         "peak": FP64_PEAK_TFLOPS,
         "unit": "TFLOP/s",
         "frac": achieved / FP64_PEAK_TFLOPS,
+        # HBM bytes per launch from the calibrated counter runs
+        "traffic": read_traffic("mf"),
         "flops_per_launch": flops,
         "hbm_bytes_per_launch": r["spmv_bytes"],
         "hbm_gbs": r["spmv_bytes"] / (avg_ms * 1e-3) / 1e9,
