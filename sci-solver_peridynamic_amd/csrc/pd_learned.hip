@@ -53,19 +53,20 @@ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
 }
 
 // L from the network outputs: exp of the clamped diagonal outputs
-__device__ __forceinline__ void make_l(const float* o, double L[6], bool clamp_flags[3]) {
+template <typename T>
+__device__ __forceinline__ void make_l(const float* o, T L[6], bool clamp_flags[3]) {
   const float d0 = fminf(fmaxf(o[0], -4.0f), 4.0f);
   const float d1 = fminf(fmaxf(o[2], -4.0f), 4.0f);
   const float d2 = fminf(fmaxf(o[5], -4.0f), 4.0f);
   clamp_flags[0] = d0 != o[0];
   clamp_flags[1] = d1 != o[2];
   clamp_flags[2] = d2 != o[5];
-  L[0] = exp((double)d0);  // L00
-  L[1] = o[1];             // L10
-  L[2] = exp((double)d1);  // L11
-  L[3] = o[3];             // L20
-  L[4] = o[4];             // L21
-  L[5] = exp((double)d2);  // L22
+  L[0] = exp((T)d0);  // L00
+  L[1] = o[1];        // L10
+  L[2] = exp((T)d1);  // L11
+  L[3] = o[3];        // L20
+  L[4] = o[4];        // L21
+  L[5] = exp((T)d2);  // L22
 }
 
 // lower Cholesky factor of the symmetric block (d00 d01 d02 d11 d12 d22) and
@@ -290,11 +291,18 @@ __global__ __launch_bounds__(PD_BLOCK) void k_pc_infer(int n, const float* feat,
   const int s = p >> 6, lane = p & 63;
   const double* dg = diag + (int64_t)s * 6 * 64 + lane;
   const double d[6] = {dg[0], dg[64], dg[128], dg[192], dg[256], dg[320]};
-  double ci[6], L[6], m[6];
+  // the block algebra in float, as in training (the network is a BF16
+  // approximation and any SPD block serves; the solve takes M as given)
+  const float df[6] = {(float)d[0], (float)d[1], (float)d[2], (float)d[3], (float)d[4],
+                       (float)d[5]};
+  float ci[6], L[6], mf6[6];
+  double m[6];
   bool cl[3];
-  if (chol_inv(d, ci)) {
+  if (chol_inv(df, ci)) {
     make_l(outs[w][l], L, cl);
-    learned_block(L, (double)P[PC_OFF_EPS], ci, m);
+    learned_block(L, P[PC_OFF_EPS], ci, mf6);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) m[c] = (double)mf6[c];
   } else {  // a block without a Cholesky factor keeps block Jacobi
     bj_inverse(d, m);
   }
@@ -401,7 +409,7 @@ __device__ void sample_grad(const float* o, const float* smp, float eps, float* 
   if (!chol_inv(d, ci)) return;
   double Ld[6];
   bool cl[3];
-  make_l(o, Ld, cl);
+  make_l<double>(o, Ld, cl);
   const float Lc[6] = {(float)Ld[0], (float)Ld[1], (float)Ld[2], (float)Ld[3], (float)Ld[4],
                        (float)Ld[5]};
   float m6[6];
