@@ -305,12 +305,15 @@ int pd_synchronize(pd_ctx* ctx);
 int pd_bench_spmv(pd_ctx* ctx, int32_t iters, double* ms_per_launch,
                   double* bytes_per_launch);
 
-/* Self-check (no counterpart in the reference) of the matrix-free
- * operator's shared-reciprocal quotients: on `samples` random operand pairs
- * spanning the ranges under which the operator uses them (a coordinate
- * difference over a float bond length; a bond stiffness over a reference
- * length), counts in *mismatches the quotients whose bits differ from IEEE
- * division on this device.  0 is the expected answer. */
+/* Self-check (no counterpart in the reference) of the shared-reciprocal
+ * quotients and the unscaled square root that the assembly and the
+ * matrix-free operator use when the step's operands lie in range: on
+ * `samples` random operands spanning those ranges (a coordinate difference
+ * over a float bond length; a current over a reference length and back; a
+ * bond stiffness over a reference length; half of a horizon less a
+ * reference length over a radius; the root of a squared bond length),
+ * counts in *mismatches the results whose bits differ from IEEE division
+ * and sqrt on this device.  0 is the expected answer. */
 int pd_check_division(pd_ctx* ctx, int64_t samples, uint64_t seed, int64_t* mismatches);
 
 /* Algorithmic HBM bytes of one CG SpMV launch on the current bonds:

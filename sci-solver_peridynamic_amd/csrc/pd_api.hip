@@ -541,7 +541,6 @@ pd::CgArgs cg_args(pd_ctx* c, int precond) {
   a.precond = precond;
   a.vals = c->val;
   a.rec = c->rec;
-  a.mf_gate = 0;
   return a;
 }
 
@@ -693,8 +692,11 @@ int enqueue_assemble(pd_ctx* c, const pd_step_params* prm,
     pa.fuse = fuse ? 1 : 0;
     pa.integrator = prm->integrator;
     pa.c_bad = 0;
+    auto in_range = [](double v) { return v >= 0x1p-300 && v <= 0x1p300; };
     for (int b = 0; b < c->nbody; ++b)
-      if (!(c->mat[b].c_bond >= 0x1p-300 && c->mat[b].c_bond <= 0x1p300)) pa.c_bad = 1;
+      if (!in_range(c->mat[b].c_bond) || !in_range(c->mat[b].horizon) ||
+          !in_range(c->mat[b].radius))
+        pa.c_bad = 1;
     pa.dt = prm->dt;
     pa.consts = consts;
     pa.x = c->x;

@@ -30,9 +30,13 @@ typedef double vpair __attribute__((ext_vector_type(2)));
 // exactly rcp_nr(b) then div_rcp(a, b, y) below, so quotients by one
 // denominator can share y.  No scaling happens when b and 1/b are normal,
 // a is 0 or |a| >= 2^-969, a / b is normal and exponent(a) - exponent(b) <
-// 768; a = 0 gives a zero of the quotient's sign (copysign restores it, as
-// div_fixup does).  range_ok_* state sufficient ranges; callers take this
-// path only when the assembly found every operand inside them.
+// 768.  Every denominator here is positive: then a = +0 gives +0 as IEEE
+// division does, but a = -0 would give +0 where it gives -0, so no caller
+// passes -0 (a coordinate difference is -0 only for a -0 coordinate less a
+// +0 one, and the range flag admits no -0 coordinate; the other numerators
+// are positive or a difference horizon - d0 with a positive horizon, which
+// is never -0).  range_ok_* state sufficient ranges; callers take this path
+// only when the assembly found every operand inside them.
 __device__ __forceinline__ double rcp_nr(double b) {
   double y = __builtin_amdgcn_rcp(b);
   double t = __builtin_fma(-b, y, 1.0);
@@ -44,19 +48,43 @@ __device__ __forceinline__ double rcp_nr(double b) {
 __device__ __forceinline__ double div_rcp(double a, double b, double y) {
   const double q = a * y;
   const double r = __builtin_fma(-b, q, a);
-  return __builtin_copysign(__builtin_fma(r, y, q), q);
+  return __builtin_fma(r, y, q);
 }
-// A coordinate 0 or of magnitude in [2^-500, 2^100]: a difference of two
-// such values is 0 or of magnitude in [2^-552, 2^101], so it divided by the
-// bond length fd = max(|d|, 1e-10) (at most 2^102, a float) needs no scaling.
+// Square root of x in [2^-767, 2^1023] as gfx950 forms IEEE sqrt() for
+// such x, without the input scaling sqrt() applies below 2^-767 and the
+// selects for zero and infinity: g = rsq(x); s = x g, h = g / 2 refined
+// together by one coupled Newton step, then two Markstein corrections
+// s += (x - s^2) h.  The same operations in the same order, so the same
+// bits.  x = 0 gives NaN: callers exclude it.
+__device__ __forceinline__ double sqrt_nr(double x) {
+  const double g = __builtin_amdgcn_rsq(x);
+  double s = x * g;
+  double h = g * 0.5;
+  const double r = __builtin_fma(-h, s, 0.5);
+  s = __builtin_fma(s, r, s);
+  h = __builtin_fma(h, r, h);
+  double d = __builtin_fma(-s, s, x);
+  s = __builtin_fma(d, h, s);
+  d = __builtin_fma(-s, s, x);
+  return __builtin_fma(d, h, s);
+}
+
+// A coordinate +0 or of magnitude in [2^-300, 2^100]: a difference of two
+// such values is +0 or of magnitude in [2^-352, 2^101], so
+//  * it divided by the bond length fd = max(|d|, 1e-10) (at most 2^102, a
+//    float) needs no scaling;
+//  * the squared length of a bond is 0 or in [2^-704, 2^204], inside
+//    sqrt_nr's range when it is not 0, and the length dij is 0 or in
+//    [2^-352, 2^102].
 __device__ __forceinline__ bool range_ok_coord(double v) {
   const double m = fabs(v);
-  return m == 0.0 || (m >= 0x1p-500 && m <= 0x1p100);
+  return (m == 0.0 && !__builtin_signbit(v)) || (m >= 0x1p-300 && m <= 0x1p100);
 }
-// c_bond / d0 with c_bond in [2^-300, 2^300] and d0 in [2^-400, 2^100]; and
-// the current length over d0: the length of a difference of in-range
-// coordinates is 0 or in [2^-537, 2^102] (the smallest nonzero square is the
-// smallest subnormal, 2^-1074), so dij / d0 lies in [2^-637, 2^502].
+// With d0 in [2^-400, 2^100] and c_bond, the horizon and the radius in
+// [2^-300, 2^300]: c_bond / d0; dij / d0 in [2^-502, 2^502]; d0 / dij for a
+// nonzero dij in [2^-502, 2^452]; and (horizon - d0) / 2 / radius, whose
+// numerator is 0 or at least ulp(d0) >= 2^-452 in magnitude, need no
+// scaling either.
 __device__ __forceinline__ bool range_ok_len(double d0) {
   return d0 >= 0x1p-400 && d0 <= 0x1p100;
 }
@@ -67,14 +95,22 @@ __device__ __forceinline__ bool range_ok_stiff(double c) {
 // Current geometry of bond (p, q), cg_solver.cuh:253-256: the unit vector e
 // (the difference divided by the float-rounded length max(|d|, SMALL)) and
 // the current length dij.  SHARED: the three quotients share one
-// reciprocal (identical results inside the ranges above).
-template <bool SHARED = false>
+// reciprocal, and the square root is sqrt_nr, behind a test for a zero
+// length unless NONZERO (the caller knows every bond length is nonzero):
+// identical results inside the ranges above.
+template <bool SHARED = false, bool NONZERO = false>
 __device__ __forceinline__ void bond_dir(const double4 pos_p, const double4 xq, double& e0,
                                          double& e1, double& e2, double& dij) {
   e0 = xq.x - pos_p.x;
   e1 = xq.y - pos_p.y;
   e2 = xq.z - pos_p.z;
-  dij = sqrt(e0 * e0 + e1 * e1 + e2 * e2);
+  const double ss = e0 * e0 + e1 * e1 + e2 * e2;
+  if constexpr (!SHARED)
+    dij = sqrt(ss);
+  else if constexpr (NONZERO)
+    dij = sqrt_nr(ss);
+  else
+    dij = ss == 0.0 ? 0.0 : sqrt_nr(ss);
   const float dij_f = (float)dij;
   const float small_f = (float)1.0e-10;
   const double fd = (double)(dij_f > small_f ? dij_f : small_f);
@@ -98,12 +134,14 @@ __device__ __forceinline__ double ref_len(const double4 opos_p, const double4 oq
 
 // The bond's 3x3 Jacobian F (six distinct entries), cg_solver.cuh:292-299:
 // (c / d0) [(1 - d0/dij) I + (d0/dij) e e^T], with the factor sc = c / d0
-// given.
+// given.  SHARED: d0 / dij through rcp_nr(dij) (identical inside the ranges
+// above for a nonzero dij; a zero dij makes every entry NaN either way).
+template <bool SHARED = false>
 __device__ __forceinline__ void bond_jac_sc(double e0, double e1, double e2, double dij,
                                             double d0, double sc, double& F00, double& F01,
                                             double& F02, double& F11, double& F12,
                                             double& F22) {
-  const double ratio = d0 / dij;
+  const double ratio = SHARED ? div_rcp(d0, dij, rcp_nr(dij)) : d0 / dij;
   const double dg = 1.0 - ratio;
   F00 = (dg + (e0 * e0) * ratio) * sc;
   F01 = (0.0 + (e0 * e1) * ratio) * sc;
@@ -124,7 +162,7 @@ __device__ __forceinline__ void bond_jac(double e0, double e1, double e2, double
     sc = div_rcp(c_bond, d0, rcp_nr(d0));
   else
     sc = c_bond / d0;
-  bond_jac_sc(e0, e1, e2, dij, d0, sc, F00, F01, F02, F11, F12, F22);
+  bond_jac_sc<SHARED>(e0, e1, e2, dij, d0, sc, F00, F01, F02, F11, F12, F22);
 }
 
 __device__ __forceinline__ void blk6(double a00, double a01, double a02, double a11,
@@ -143,8 +181,9 @@ __device__ __forceinline__ void blk6(double a00, double a01, double a02, double 
 // spmv_row).  Bound by the vector memory pipeline (gathers) and FP64
 // arithmetic, not by HBM.  REC: the neighbour's position and x come from
 // A.rec (x must be the CG direction p the record holds): three 16-byte
-// loads instead of four.  SHARED: quotients share reciprocals (the
-// assembly found every operand in range).
+// loads instead of four.  SHARED: quotients share reciprocals and square
+// roots take sqrt_nr with no zero test (the assembly found every operand in
+// range and no live bond of zero length).
 typedef double dbl2v __attribute__((ext_vector_type(2)));
 
 // One bond block into the pair planes of vals (the assembly's layout).
@@ -214,7 +253,7 @@ __device__ __forceinline__ void spmv_row_mf(const SpmvView& A, const double4* __
 #pragma unroll
       for (int u = 0; u < V; ++u) {
         double e0, e1, e2, dij;
-        bond_dir<SHARED>(pos_p, xq[u], e0, e1, e2, dij);
+        bond_dir<SHARED, SHARED>(pos_p, xq[u], e0, e1, e2, dij);
         double F00, F01, F02, F11, F12, F22;
         bond_jac<SHARED>(e0, e1, e2, dij, l0[u], cb, F00, F01, F02, F11, F12, F22);
         const double b00 = (F00 * -1.0) * dt2, b01 = (F01 * -1.0) * dt2,
@@ -235,7 +274,7 @@ __device__ __forceinline__ void spmv_row_mf(const SpmvView& A, const double4* __
       double4 xq, vq;
       gather_mf<REC>(A, rec, x, q0, xq, vq);
       double e0, e1, e2, dij;
-      bond_dir<SHARED>(pos_p, xq, e0, e1, e2, dij);
+      bond_dir<SHARED, SHARED>(pos_p, xq, e0, e1, e2, dij);
       double F00, F01, F02, F11, F12, F22;
       bond_jac<SHARED>(e0, e1, e2, dij, ntl(lp + (int64_t)bnd * 64), cb, F00, F01, F02, F11,
                        F12, F22);

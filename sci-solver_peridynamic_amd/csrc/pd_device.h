@@ -89,9 +89,11 @@ struct pd_step_consts {
 struct pd_step_counters {
   unsigned long long broken;
   unsigned long long live;
-  // nonzero: some position, reference length or bond stiffness of the
-  // assembled bonds lies outside the ranges of the shared-reciprocal
-  // quotients, so the matrix-free operator divides one quotient at a time
+  // bit 0: some position, reference length, bond stiffness, horizon or
+  // radius lies outside the ranges of the shared-reciprocal quotients and
+  // of sqrt_nr (set ahead of the assembly); bit 1: a live bond has zero
+  // length (set by the assembly).  Nonzero: the matrix-free operator
+  // divides one quotient at a time and takes IEEE square roots.
   unsigned int range_bad;
   unsigned int pad;
 };

@@ -84,7 +84,7 @@ struct AsmPrepArgs {
   int32_t n;
   int32_t fuse;
   int32_t integrator;
-  int32_t c_bad;  // a body's bond stiffness outside range_ok_stiff
+  int32_t c_bad;  // a body's bond stiffness, horizon or radius outside range_ok_stiff
   double dt;
   const pd_step_consts* consts;  // non-null: dt from here
   const double4* x;
@@ -177,9 +177,6 @@ struct CgArgs {
   // to vals, which the later products stream
   double* vals;
   const double* rec;
-  // matrix-free products: 1 when k_cg_spmv_mf, launched ahead, did the
-  // product (range flag clear), so the general kernel returns at once
-  int32_t mf_gate;
 };
 
 // Mixed-precision solve: float matrix copy, float inner vectors, FP64

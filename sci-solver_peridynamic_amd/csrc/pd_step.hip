@@ -42,14 +42,17 @@ __device__ __forceinline__ void gather_asm(const AsmArgs& a, int q, double4& xq,
 // iteration, never soon enough to gain from L2), the diagonal in both.
 // Neighbour data come from the step's assembly record (position, stretch
 // limit and the solver's start value side by side, k_asm_prep).  SHARED:
-// the quotients by the float-rounded length (bond_dir) and by the reference
-// length (dij / d0, c / d0) share reciprocals; both variants are launched
-// and the one that does not match the range flag k_asm_prep set returns at
-// once.
+// the quotients by the float-rounded length (bond_dir), by the reference
+// length (dij / d0, c / d0), by the current length (d0 / dij) and by the
+// radius (the volume scale) share reciprocals and the square root is
+// sqrt_nr; both variants are launched and the one that does not match bit 0
+// of the range flag (k_asm_prep) returns at once.  The SHARED variant sets
+// bit 1 when a live bond has zero length (the matrix-free products then
+// leave the shared forms).
 // FUSE and GAP: a.fuse and a.track_gap as compile-time choices.
 template <bool MIXED, bool SHARED, bool FUSE, bool GAP>
 __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
-  if ((a.counters->range_bad == 0) != SHARED) return;
+  if (((a.counters->range_bad & 1u) == 0) != SHARED) return;
   if (a.consts) {
     a.dt = a.consts->dt;
     a.dt2 = a.consts->dt2;
@@ -59,6 +62,7 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int p = tile * PD_BLOCK + threadIdx.x;
   unsigned long long broken = 0, live = 0;
+  bool zero_len = false;  // SHARED: a bond of zero current length
   double t_rr = 0.0, t_rz = 0.0, t_r32 = 0.0, t_bb = 0.0;  // fused residual partials
   if (p < a.n && a.act[p] != PD_INVISIBLE) {
     const pd_mat_dev mt = a.mat[a.body[p]];
@@ -78,6 +82,7 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
     double gmin = INFINITY;
     int gq = -1;
     const double hr = mt.horizon - mt.radius;
+    const double yr = SHARED ? rcp_nr(mt.radius) : 0.0;  // 1 / radius for vscale
     int bnd = 0;
     // software pipeline: the neighbour data of slot bnd + 1 and the column
     // of slot bnd + 2 are in flight while slot bnd is computed, so neither
@@ -108,6 +113,7 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
       if (bnd + 2 <= top) q_nn = a.col[(slot + 2) * 64 + lane];
       double e0, e1, e2, dij;
       bond_dir<SHARED>(pos_p, xq, e0, e1, e2, dij);
+      if constexpr (SHARED) zero_len = zero_len || dij == 0.0;
       double y0 = 0.0;  // SHARED: 1 / d0 for dij / d0 and c / d0
       if constexpr (SHARED) y0 = rcp_nr(d0);
       const double drdivd = (SHARED ? div_rcp(dij, d0, y0) : dij / d0) - 1.0;
@@ -141,7 +147,9 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
         continue;
       }
       double vscale = 1.0;
-      if (d0 > hr) vscale = 0.5 + (mt.horizon - d0) / 2.0 / mt.radius;
+      if (d0 > hr)
+        vscale = 0.5 + (SHARED ? div_rcp((mt.horizon - d0) / 2.0, mt.radius, yr)
+                               : (mt.horizon - d0) / 2.0 / mt.radius);
       if (drdivd < smin) smin = drdivd;
       ssum += drdivd;
       smax = drdivd > smax ? drdivd : smax;
@@ -151,7 +159,7 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
       f2 += kk * e2;
       double F00, F01, F02, F11, F12, F22;
       const double sc = SHARED ? div_rcp(mt.c_bond, d0, y0) : mt.c_bond / d0;
-      bond_jac_sc(e0, e1, e2, dij, d0, sc, F00, F01, F02, F11, F12, F22);
+      bond_jac_sc<SHARED>(e0, e1, e2, dij, d0, sc, F00, F01, F02, F11, F12, F22);
       k00 += F00;
       k01 += F01;
       k02 += F02;
@@ -383,6 +391,8 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
     if (broken) atomicAdd(&a.counters->broken, broken);
     if (live) atomicAdd(&a.counters->live, live);
   }
+  if constexpr (SHARED)
+    if (__any(zero_len) && (threadIdx.x & 63) == 0) atomicOr(&a.counters->range_bad, 2u);
 }
 
 template <bool MIXED, bool FUSE, bool GAP>
@@ -415,10 +425,10 @@ void launch_assemble(const AsmArgs& a, hipStream_t s) {
 // solver's start value, one 64-byte gather per bond in the assembly.  With
 // fuse, the Dirichlet rows' start values are set first (vbc, or dt vbc for
 // Newmark-beta).  The position part of the matrix-free record is written
-// here too, and the range flag of the shared-reciprocal quotients: set when
-// a visible particle's coordinate, a reference length (d0_bad, from
-// k_bond_d0) or a bond stiffness (c_bad) lies outside the ranges of
-// range_ok_*.
+// here too, and bit 0 of the range flag of the shared-reciprocal quotients:
+// set when a particle's coordinate, a reference length (d0_bad, from
+// k_bond_d0) or a body's bond stiffness, horizon or radius (c_bad) lies
+// outside the ranges of range_ok_*.
 __global__ __launch_bounds__(PD_BLOCK) void k_asm_prep(AsmPrepArgs a) {
   const int p = blockIdx.x * PD_BLOCK + threadIdx.x;
   bool bad = false;
@@ -444,7 +454,9 @@ __global__ __launch_bounds__(PD_BLOCK) void k_asm_prep(AsmPrepArgs a) {
       rp[1] = xp.y;
       rp[2] = xp.z;
     }
-    bad = vis && !(range_ok_coord(xp.x) && range_ok_coord(xp.y) && range_ok_coord(xp.z));
+    // every particle: an invisible one may still be the neighbour of a
+    // visible one whose bonds were built before its activity changed
+    bad = !(range_ok_coord(xp.x) && range_ok_coord(xp.y) && range_ok_coord(xp.z));
   }
   if (p == 0 && (a.c_bad || *a.d0_bad)) bad = true;
   if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(&a.counters->range_bad, 1u);
@@ -654,11 +666,14 @@ __global__ __launch_bounds__(PD_BLOCK) void k_spmv_plain(SpmvView A,
   y[p] = make_double4(y0, y1, y2, 0.0);
 }
 
-// Self-check of the shared-reciprocal quotients against IEEE division over
-// random operands spanning the ranges the range flag admits (odd samples:
-// a coordinate difference over a float bond length; samples 2 mod 4: a
-// current bond length over a reference length; 0 mod 4: a bond stiffness
-// over a reference length); the exponent ends are drawn often.
+// Self-check of the shared-reciprocal quotients against IEEE division, and
+// of sqrt_nr against IEEE sqrt, over random operands spanning the ranges
+// the range flag admits (odd samples: a coordinate difference over a float
+// bond length; 2 mod 4: a current bond length over a reference length;
+// 0 mod 16: a bond stiffness over a reference length; 8 mod 16: half of a
+// horizon less a reference length over a radius; 4 mod 16: a reference
+// length over a current length; 12 mod 16: the square root of a squared
+// bond length); the exponent ends are drawn often.
 __device__ __forceinline__ uint64_t mix64(uint64_t z) {
   z += 0x9e3779b97f4a7c15ull;
   z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
@@ -680,7 +695,7 @@ __global__ __launch_bounds__(PD_BLOCK) void k_check_division(int64_t samples, ui
   const uint64_t h1 = mix64(seed ^ (2 * (uint64_t)i)), h2 = mix64(seed ^ (2 * (uint64_t)i + 1));
   double a, b;
   if (i & 1) {
-    a = (h1 & 15) == 0 ? ((h1 >> 4) & 1 ? -0.0 : 0.0) : draw_f64(h1, -552, 100);
+    a = (h1 & 15) == 0 ? 0.0 : draw_f64(h1, -352, 100);
     const float small_f = (float)1.0e-10;
     float bf = (h2 & 15) == 0 ? small_f : fabsf((float)draw_f64(h2, -34, 101));
     if (!(bf >= small_f)) bf = small_f;
@@ -688,9 +703,19 @@ __global__ __launch_bounds__(PD_BLOCK) void k_check_division(int64_t samples, ui
   } else if ((i & 3) == 2) {  // a current bond length over a reference length
     a = (h1 & 15) == 0 ? 0.0 : fabs(draw_f64(h1, -537, 101));
     b = fabs(draw_f64(h2, -400, 99));
-  } else {  // a bond stiffness over a reference length
+  } else if ((i & 15) == 0) {  // a bond stiffness over a reference length
     a = fabs(draw_f64(h1, -300, 299));
     b = fabs(draw_f64(h2, -400, 99));
+  } else if ((i & 15) == 8) {  // (horizon - d0) / 2 over a radius
+    a = (h1 & 15) == 0 ? 0.0 : draw_f64(h1, -453, 299);
+    b = fabs(draw_f64(h2, -300, 299));
+  } else if ((i & 15) == 4) {  // a reference length over a current length
+    a = fabs(draw_f64(h1, -400, 99));
+    b = fabs(draw_f64(h2, -352, 101));
+  } else {  // the square root of a squared bond length
+    const double x = fabs(draw_f64(h1, -704, 203));
+    if (__double_as_longlong(sqrt(x)) != __double_as_longlong(sqrt_nr(x))) atomicAdd(bad, 1ull);
+    return;
   }
   const double ieee = a / b;
   const double shared = div_rcp(a, b, rcp_nr(b));
@@ -857,7 +882,6 @@ __global__ __launch_bounds__(PD_BLOCK) void k_cg_init_fold(CgArgs a, int ntiles)
 // y = A p; partial <y,p>
 template <int U>
 __global__ __launch_bounds__(PD_BLOCK) void k_cg_spmv(CgArgs a) {
-  if (a.mf_gate && *a.A.range_bad == 0) return;  // k_cg_spmv_mf did it
   __shared__ double lds4[4];
   const int ntiles = gridDim.x;
   const int tile = xcd_tile(blockIdx.x, ntiles);
@@ -878,13 +902,16 @@ __global__ __launch_bounds__(PD_BLOCK) void k_cg_spmv(CgArgs a) {
   if (threadIdx.x == 0) a.part[tile] = v;
 }
 
-// The matrix-free CG product in its common case alone: gathers from the
-// record, shared reciprocals (the range flag clear, else it returns at once
-// and k_cg_spmv takes the product), two bonds per trip.  Without the other
-// variants the kernel needs 82 VGPRs instead of 126 (five waves per SIMD
-// instead of four): 570 against 593 us per product at 1M particles.
+// The matrix-free CG product after an assembly: gathers from the record;
+// with the range flag clear (the common case) shared reciprocals and
+// sqrt_nr, four bonds per trip, else IEEE quotients, two bonds per trip
+// (the general path fits in the common path's registers).
+// The row loop waits on a dependent chain (column, then the neighbour's
+// record) each trip, so the bonds per trip, not the waves per SIMD, hide
+// its latency: at 1M particles 511 us per product with four bonds per trip
+// (126 VGPRs, four waves per SIMD) against 562 with two (76 VGPRs, six
+// waves), 592 with three, 569 with six and 557 with eight.
 __global__ __launch_bounds__(PD_BLOCK) void k_cg_spmv_mf(CgArgs a) {
-  if (*a.A.range_bad != 0) return;
   __shared__ double lds4[4];
   const int ntiles = gridDim.x;
   const int tile = xcd_tile(blockIdx.x, ntiles);
@@ -893,7 +920,10 @@ __global__ __launch_bounds__(PD_BLOCK) void k_cg_spmv_mf(CgArgs a) {
   double t = 0.0;
   if (p < a.A.n) {
     double y0, y1, y2;
-    spmv_row_mf<2, false, true, true>(a.A, a.p, p, y0, y1, y2, a.A.rec);
+    if (*a.A.range_bad == 0)  // uniform over the grid
+      spmv_row_mf<4, false, true, true>(a.A, a.p, p, y0, y1, y2, a.A.rec);
+    else
+      spmv_row_mf<2, false, true, false>(a.A, a.p, p, y0, y1, y2, a.A.rec);
     a.y[p] = make_double4(y0, y1, y2, 0.0);
     const double4 pp = a.p[p];
     t = term3(y0, y1, y2, pp.x, pp.y, pp.z);
@@ -1109,13 +1139,10 @@ void launch_cg_spmv(const CgArgs& a, hipStream_t s) {
   if (!a.A.n) return;
   const int g = tiles_of(a.A.n);
   if (a.A.mf) {
-    CgArgs b = a;
-    b.mf_gate = 0;
-    if (a.A.rec && a.A.range_bad) {  // the common case first, the general kernel if needed
+    if (a.A.rec && a.A.range_bad)  // after an assembly: the record and the range flag
       hipLaunchKernelGGL(k_cg_spmv_mf, dim3(g), dim3(PD_BLOCK), 0, s, a);
-      b.mf_gate = 1;
-    }
-    hipLaunchKernelGGL(k_cg_spmv<0>, dim3(g), dim3(PD_BLOCK), 0, s, b);
+    else
+      hipLaunchKernelGGL(k_cg_spmv<0>, dim3(g), dim3(PD_BLOCK), 0, s, a);
     return;
   }
   switch (a.A.unroll) {

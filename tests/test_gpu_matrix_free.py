@@ -100,17 +100,20 @@ def test_shared_reciprocal_division_matches_ieee():
     ctx.close()
 
 
-@pytest.mark.parametrize("case", ["aligned", "tiny_coordinate"])
+@pytest.mark.parametrize("case", ["aligned", "tiny_coordinate", "negative_zero"])
 def test_mf_range_cases_equal_stored(case):
     """An unjittered lattice (coordinate differences exactly zero, quotients
-    through shared reciprocals) and a coordinate of 1e-200 (outside the
+    through shared reciprocals), a coordinate of 1e-200 (outside the
     shared-reciprocal range: the assembly's flag sends every quotient
-    through IEEE division): the matrix-free steps equal the stored ones and
-    the oracle."""
+    through IEEE division) and a coordinate of -0 (a -0 difference, which
+    the shared quotients do not reproduce: the flag is set too): the
+    matrix-free steps equal the stored ones and the oracle."""
     say("case mf range", case)
     x4, bc, vbc, v4 = config0_state(6, 6, 8, jitter=0.0 if case == "aligned" else 0.05)
     if case == "tiny_coordinate":
         x4[0, 0] = 1e-200
+    if case == "negative_zero":
+        x4[0, 0] = -0.0
     mats = [material(0.01)]
     sys_, mf = make_pair(x4, mats, v4=v4, bc=bc, vbc=vbc)
     _, stored = make_pair(x4, mats, v4=v4, bc=bc, vbc=vbc)
