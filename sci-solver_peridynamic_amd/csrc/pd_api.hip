@@ -855,8 +855,8 @@ int run_solve(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st, int resum
         pd::launch_cg_spmv(a, c->stream);
       }
       if (timing) HIPCHK(c, hipEventRecord(c->ev[8 + 2 * (launched + i) + 1], c->stream));
-      pd::launch_cg_update(a, c->stream);
-      pd::launch_cg_dir(a, c->stream);
+      pd::launch_cg_update(a, launched + i, c->stream);
+      pd::launch_cg_dir(a, launched + i, c->stream);
     }
     CHECK_LAUNCH(c, "cg iteration");
     launched += chunk;
@@ -1307,8 +1307,8 @@ int capture_step(pd_ctx* c, const pd_step_params* prm, int K) {
         } else {
           pd::launch_cg_spmv(a, c->stream);
         }
-        pd::launch_cg_update(a, c->stream);
-        pd::launch_cg_dir(a, c->stream);
+        pd::launch_cg_update(a, i, c->stream);
+        pd::launch_cg_dir(a, i, c->stream);
       }
       HIPCHK(c, hipMemcpyAsync(c->h_st, c->st, sizeof(pd_cg_state) * ns, hipMemcpyDeviceToHost,
                                c->stream));

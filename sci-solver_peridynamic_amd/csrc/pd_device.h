@@ -40,11 +40,12 @@ struct pd_cg_state {
   double tol;    // rtol * bnorm
   double rnorm;  // sqrt(rr)
   double rtol;
+  double rz_prev;  // <r, z> the current iteration's update divided by (beta's denominator)
   int32_t iter;
   int32_t done;
   int32_t converged;
   int32_t max_iter;
-  uint32_t arrive;  // reserved
+  int32_t upd_it;  // the iteration whose update ran last (-1: none yet)
   uint32_t pad[3];
 };
 

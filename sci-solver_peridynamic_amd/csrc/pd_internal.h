@@ -340,8 +340,9 @@ void launch_cg_init_store(const CgArgs& a, hipStream_t s);
 // the blocks of the current bond lists written to vals from the positions
 // (a step's deferred blocks when no product wrote them)
 void launch_write_blocks(const SpmvView& A, double* vals, hipStream_t s);
-void launch_cg_update(const CgArgs& a, hipStream_t s);
-void launch_cg_dir(const CgArgs& a, hipStream_t s);
+// iteration `it` (0-based, counted from the solve's start) of the CG loop
+void launch_cg_update(const CgArgs& a, int it, hipStream_t s);
+void launch_cg_dir(const CgArgs& a, int it, hipStream_t s);
 void launch_spmv_plain(const SpmvView& A, const double4* x, double4* y, hipStream_t s);
 // pd_mixed.hip
 void launch_mx_setup(pd_mx_state* st, int32_t nspec, double rtol, int32_t max_iter,
