@@ -40,7 +40,6 @@ struct pd_cg_state {
   double tol;    // rtol * bnorm
   double rnorm;  // sqrt(rr)
   double rtol;
-  double rz_prev;  // <r, z> the current iteration's update divided by (beta's denominator)
   int32_t iter;
   int32_t done;
   int32_t converged;

@@ -791,7 +791,6 @@ __global__ void k_cg_setup(pd_cg_state* states, int nspec, double rtol, int max_
     st->done = 0;
     st->converged = 0;
     st->max_iter = max_iter;
-    st->rz_prev = 0.0;
     st->upd_it = -1;
   }
 }
@@ -991,8 +990,7 @@ __global__ __launch_bounds__(PD_BLOCK) void k_write_blocks(SpmvView A, double* v
 // partials <r,z>, <r,r> into the third and fourth partial rows (the first
 // row is still being folded by other workgroups).  x += alpha p is left to
 // k_cg_dir, which reads p anyway.  The specimen's first workgroup records
-// alpha, the rz it divided (the next beta's denominator) and that the
-// update of iteration `it` ran.
+// alpha and that the update of iteration `it` ran.
 __global__ __launch_bounds__(PD_BLOCK) void k_cg_update(CgArgs a, int it) {
   __shared__ double lds4[4];
   const int ntiles = gridDim.x;
@@ -1024,10 +1022,7 @@ __global__ __launch_bounds__(PD_BLOCK) void k_cg_update(CgArgs a, int it) {
       st->alpha = alpha;
     }
   }
-  if (tile == t0 && threadIdx.x == 0) {
-    st->rz_prev = rz;
-    st->upd_it = it;
-  }
+  if (tile == t0 && threadIdx.x == 0) st->upd_it = it;
   double t_rz = 0.0, t_rr = 0.0;
   if (row) {
     if (fr) {
@@ -1067,79 +1062,52 @@ __global__ __launch_bounds__(PD_BLOCK) void k_cg_alpha(CgArgs a, int ntiles) {
   }
 }
 
-// Specimens above PD_ALPHA_FOLD_TILES tiles: <r,z> and <r,r> of iteration
-// `it` folded once, by one workgroup, into rz and rr ahead of k_cg_dir (whose
-// per-workgroup folds would grow with the square of the tile count).
+// beta = rz' / rz, iteration count and the monitor test ahead of the next
+// iteration (||r|| <= tol converged, iter >= max_iter failed), for every
+// specimen whose update ran in iteration `it`
 __global__ __launch_bounds__(PD_BLOCK) void k_cg_beta(CgArgs a, int ntiles, int it) {
-  pd_cg_state* st = a.st + blockIdx.x;
-  if (st->upd_it != it) return;
+  if (a.st[blockIdx.x].upd_it != it) return;
   __shared__ double lds4[4];
   int t0, nt;
   spec_range(a.batch, ntiles, t0, nt);
-  if (nt <= PD_ALPHA_FOLD_TILES) return;
   const double rz = fold_partials(a.part + 2 * (int64_t)ntiles + t0, nt, lds4);
   const double rr = fold_partials(a.part + 3 * (int64_t)ntiles + t0, nt, lds4);
   if (threadIdx.x == 0) {
+    pd_cg_state* st = a.st + blockIdx.x;
+    const double rz_old = st->rz;
     st->rz = rz;
+    st->beta = rz / rz_old;
+    st->iter = st->iter + 1;
     st->rr = rr;
+    const double rnorm = sqrt(rr);
+    st->rnorm = rnorm;
+    if (rnorm <= st->tol) {
+      st->done = 1;
+      st->converged = 1;
+    } else if (st->iter >= st->max_iter) {
+      st->done = 1;
+    }
   }
 }
 
-// The end of iteration `it`, for every specimen whose update ran in it:
-// every workgroup folds the new <r,z> and <r,r> in the one order; beta =
-// rz' / rz; the iteration count and the monitor test (||r|| <= tol
-// converged, iter >= max_iter failed), which the specimen's first workgroup
-// records; x += alpha p; and, unless the solve ended here, p = z + beta p.
-// Workgroups decide from values no launch-mate writes: the update's record
-// of iteration `it`, alpha and rz_prev, and their own folds.
+// The end of iteration `it` for every specimen whose update ran in it:
+// x += alpha p, and, unless k_cg_beta found the solve ended, p = z + beta p.
 __global__ __launch_bounds__(PD_BLOCK) void k_cg_dir(CgArgs a, int it) {
-  __shared__ double lds4[4];
-  const int ntiles = gridDim.x;
-  const int tile = xcd_tile(blockIdx.x, ntiles);
-  const int s = spec_of(a.batch, tile);
-  pd_cg_state* st = a.st + s;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const pd_cg_state* st = a.st + spec_of(a.batch, tile);
   if (st->upd_it != it) return;  // the solve had ended before this iteration
   const int p = tile * PD_BLOCK + threadIdx.x;
-  const bool row = p < a.A.n;
-  double4 pp = make_double4(0.0, 0.0, 0.0, 0.0), xx = pp, zz = pp;
-  if (row) {
-    pp = a.p[p];
-    xx = a.x[p];
-    zz = a.z[p];
-  }
-  int t0, nt;
-  spec_range_of(a.batch, s, ntiles, t0, nt);
-  double rz, rr;
-  if (nt > PD_ALPHA_FOLD_TILES) {  // k_cg_beta folded them ahead of this launch
-    rz = st->rz;
-    rr = st->rr;
-  } else {
-    rz = fold_partials(a.part + 2 * (int64_t)ntiles + t0, nt, lds4);
-    rr = fold_partials(a.part + 3 * (int64_t)ntiles + t0, nt, lds4);
-  }
-  const double beta = rz / st->rz_prev;
-  const int iter = it + 1;
-  const double rnorm = sqrt(rr);
-  const bool conv = rnorm <= st->tol;
-  const bool done = conv || iter >= st->max_iter;
-  if (tile == t0 && threadIdx.x == 0) {
-    st->rz = rz;
-    st->beta = beta;
-    st->iter = iter;
-    st->rr = rr;
-    st->rnorm = rnorm;
-    if (done) {
-      st->done = 1;
-      st->converged = conv ? 1 : 0;
-    }
-  }
-  if (!row) return;
+  if (p >= a.A.n) return;
+  double4 pp = a.p[p];
+  double4 xx = a.x[p];
   const double alpha = st->alpha;
   xx.x = alpha * pp.x + xx.x;
   xx.y = alpha * pp.y + xx.y;
   xx.z = alpha * pp.z + xx.z;
   a.x[p] = xx;
-  if (done) return;
+  if (st->done) return;
+  const double beta = st->beta;
+  const double4 zz = a.z[p];
   pp.x = zz.x + beta * pp.x;
   pp.y = zz.y + beta * pp.y;
   pp.z = zz.z + beta * pp.z;
@@ -1208,8 +1176,7 @@ void launch_cg_update(const CgArgs& a, int it, hipStream_t s) {
 void launch_cg_dir(const CgArgs& a, int it, hipStream_t s) {
   if (!a.A.n) return;
   const int g = tiles_of(a.A.n);
-  if (g > PD_ALPHA_FOLD_TILES)
-    hipLaunchKernelGGL(k_cg_beta, dim3(a.batch.nspec), dim3(PD_BLOCK), 0, s, a, g, it);
+  hipLaunchKernelGGL(k_cg_beta, dim3(a.batch.nspec), dim3(PD_BLOCK), 0, s, a, g, it);
   hipLaunchKernelGGL(k_cg_dir, dim3(g), dim3(PD_BLOCK), 0, s, a, it);
 }
 
