@@ -205,9 +205,8 @@ __device__ __forceinline__ void mlp_forward(const bf16x8& x, const __bf16* w, co
   // layer 1: K = 16
 #pragma unroll
   for (int nt = 0; nt < 2; ++nt) {
-    bf16x8 b;
-#pragma unroll
-    for (int i = 0; i < 8; ++i) b[i] = w[PC_OFF_W1 + (nt * 32 + r) * PC_NIN + 8 * kh + i];
+    // 16-byte loads: every operand row starts on a multiple of 8 elements
+    const bf16x8 b = *(const bf16x8*)(w + PC_OFF_W1 + (nt * 32 + r) * PC_NIN + 8 * kh);
     const floatx16 c = mfma(x, b, zero16());
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
@@ -221,13 +220,9 @@ __device__ __forceinline__ void mlp_forward(const bf16x8& x, const __bf16* w, co
   floatx16 c0 = zero16(), c1 = zero16();
 #pragma unroll
   for (int ks = 0; ks < 4; ++ks) {
-    bf16x8 av, b0, b1;
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      av[i] = h1[r][ks * 16 + 8 * kh + i];
-      b0[i] = w[PC_OFF_W2 + r * PC_NH + ks * 16 + 8 * kh + i];
-      b1[i] = w[PC_OFF_W2 + (32 + r) * PC_NH + ks * 16 + 8 * kh + i];
-    }
+    const bf16x8 av = *(const bf16x8*)&h1[r][ks * 16 + 8 * kh];
+    const bf16x8 b0 = *(const bf16x8*)(w + PC_OFF_W2 + r * PC_NH + ks * 16 + 8 * kh);
+    const bf16x8 b1 = *(const bf16x8*)(w + PC_OFF_W2 + (32 + r) * PC_NH + ks * 16 + 8 * kh);
     c0 = mfma(av, b0, c0);
     c1 = mfma(av, b1, c1);
   }
@@ -243,12 +238,11 @@ __device__ __forceinline__ void mlp_forward(const bf16x8& x, const __bf16* w, co
   floatx16 c3 = zero16();
 #pragma unroll
   for (int ks = 0; ks < 4; ++ks) {
-    bf16x8 av, bv;
+    const bf16x8 av = *(const bf16x8*)&h2[r][ks * 16 + 8 * kh];
+    bf16x8 bv;
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      av[i] = h2[r][ks * 16 + 8 * kh + i];
-      bv[i] = r < PC_NOUT ? w[PC_OFF_W3 + r * PC_NH + ks * 16 + 8 * kh + i] : (__bf16)0.0f;
-    }
+    for (int i = 0; i < 8; ++i) bv[i] = (__bf16)0.0f;
+    if (r < PC_NOUT) bv = *(const bf16x8*)(w + PC_OFF_W3 + r * PC_NH + ks * 16 + 8 * kh);
     c3 = mfma(av, bv, c3);
   }
   if (r < PC_NOUT) {
@@ -279,11 +273,10 @@ __global__ __launch_bounds__(PD_BLOCK) void k_pc_infer(int n, const float* feat,
   const int r = l & 31;
   const int row0 = blockIdx.x * 128 + w * 32;
   const float* P = net.p;
-  const int pr = row0 + r;
-  float fz[PC_NIN];
-#pragma unroll
-  for (int i = 0; i < PC_NIN; ++i) fz[i] = pr < n ? feat[(int64_t)pr * PC_NIN + i] : 0.0f;
-  const bf16x8 x = load_x(fz, P);
+  // the lane's eight features straight from memory (a row past the end
+  // reads the last row; its outputs are dropped below)
+  const int pr = row0 + r < n ? row0 + r : n - 1;
+  const bf16x8 x = load_x(feat + (int64_t)pr * PC_NIN, P);
   mlp_forward(x, net.w, P, h1[w], h2[w], outs[w]);
   if (l >= 32) return;
   const int p = row0 + l;
