@@ -190,9 +190,11 @@ __global__ __launch_bounds__(PD_BLOCK) void k_pc_features(PcFeatArgs a) {
     }
     f[9] = (float)(dmin / mt.horizon);
   }
-  float* o = a.feat + (int64_t)p * PC_NIN;
+  // four 16-byte stores (a row of features is 64 bytes)
+  float4* o = (float4*)(a.feat + (int64_t)p * PC_NIN);
 #pragma unroll
-  for (int i = 0; i < PC_NIN; ++i) o[i] = f[i];
+  for (int i = 0; i < PC_NIN / 4; ++i)
+    o[i] = make_float4(f[4 * i], f[4 * i + 1], f[4 * i + 2], f[4 * i + 3]);
 }
 
 // One wave evaluates the MLP for 32 rows (its part of a 128-row tile); the
