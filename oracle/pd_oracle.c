@@ -1159,7 +1159,7 @@ void orc_contact(orc_system* sys, const orc_step_params* prm) {
 }
 
 /* core/simulator.cu:297-467 (implicit Euler) / 470-652 (Newmark-beta)
- * without the adaptive-rollback branch, which the driver owns. */
+ * without the adaptive-rollback branch, which the host Simulator owns. */
 int32_t orc_step(orc_system* sys, const orc_step_params* prm,
                  orc_step_stats* st) {
   orc_contact(sys, prm);

@@ -1,7 +1,7 @@
 """Workload-level parity: the shipped params.txt configurations run through the
 engine's own Scene / params surface on the GPU, against the CPU oracle.
 
-Bar (north-star parity contract): bond topology bit-exact; positions within
+Bar (the parity contract): bond topology bit-exact; positions within
 1e-9 relative over 100 FP64 steps; broken-bond counts equal until the first
 step with a bond closer than 1e-6 to its threshold (reported)."""
 import json

@@ -1,7 +1,7 @@
 """GPU parity: the HIP path through the C ABI against the CPU oracle on the
 same seeded inputs.  Integer / index results must match exactly; floating
 point is compared bit for bit where the kernels follow the oracle's order of
-operations, with the north-star tolerance (1e-9 relative) as the bar."""
+operations, with the parity contract's tolerance (1e-9 relative) as the bar."""
 import sys
 
 import numpy as np
