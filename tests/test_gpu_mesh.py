@@ -29,8 +29,11 @@ def say(*a):
     sys.stdout.flush()
 
 
-def test_bunny_steps_match_oracle():
-    say("case bunny 30 steps")
+@pytest.mark.parametrize("mf", [0, 1])
+def test_bunny_steps_match_oracle(mf):
+    """The bunny scene (irregular, boundary-heavy neighbour lists) stepped
+    with stored (mf 0) and recomputed (mf 1) bond blocks."""
+    say("case bunny 30 steps, matrix_free", mf)
     desc = pds.describe(CFG, BUNNY)
     sc = pds.scene(CFG, BUNNY)
     n = desc["num_pd_particle"]
@@ -47,7 +50,7 @@ def test_bunny_steps_match_oracle():
     for k in range(1, 31):
         kw = pds.step_kwargs(desc, k)
         so = S.step(orc.make_params(**kw))
-        sg = ctx.step(pds.make_params(**kw))
+        sg = ctx.step(pds.make_params(**dict(kw, matrix_free=mf)))
         assert (sg.cg_iterations, sg.broken_bonds) == (so.cg_iterations, so.broken_bonds), k
         assert np.array_equal(ctx.download("position")[:, :3], S.x[:, :3]), k
         if k % 10 == 0:
