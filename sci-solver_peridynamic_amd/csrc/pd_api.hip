@@ -521,6 +521,9 @@ pd::MxArgs mx_args(pd_ctx* c, int precond) {
   a.precond = precond;
   a.unroll = c->unroll[1];
   a.A64 = spmv_view(c);
+  // the mixed assembly wrote the record's positions (the residual gathers
+  // positions and x from it)
+  a.A64.rec = c->asm_fresh && c->mat_prec == 1 && c->rec ? c->rec : nullptr;
   return a;
 }
 
@@ -684,7 +687,10 @@ int enqueue_assemble(pd_ctx* c, const pd_step_params* prm,
     c->d0_valid = true;
   }
   const bool defer = defer_blocks(prm);
-  if (prm->matrix_free || defer)
+  // the record: positions and p for the matrix-free FP64 products,
+  // positions and x for the mixed solve's FP64 residual
+  const bool rec = prm->matrix_free || defer || prm->precision == 1;
+  if (rec)
     if (int e = ensure_rec(c)) return e;
   {  // the assembly record, the Dirichlet start values (fuse), the range flag
     pd::AsmPrepArgs pa;
@@ -707,7 +713,7 @@ int enqueue_assemble(pd_ctx* c, const pd_step_params* prm,
     pa.vbc = c->vbc;
     pa.d0_bad = c->d0_bad;
     pa.arec = c->arec;
-    pa.rec = prm->matrix_free || defer ? c->rec : nullptr;
+    pa.rec = rec ? c->rec : nullptr;
     pa.counters = c->counters;
     pd::launch_asm_prep(pa, c->stream);
     CHECK_LAUNCH(c, "assembly record");
@@ -943,6 +949,7 @@ int run_solve_mixed(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st, int
     if (fused) {
       pd::launch_mx_resid_fold(a, c->stream);
     } else {
+      pd::launch_mx_rec_x(a, c->stream);
       if (timing) {
         if (int e = pair(resid_ev)) return e;
         HIPCHK(c, hipEventRecord(c->ev[resid_ev.back()], c->stream));
@@ -1265,8 +1272,11 @@ int capture_step(pd_ctx* c, const pd_step_params* prm, int K) {
     CHECK_LAUNCH(c, "reference bond lengths");
     c->d0_valid = true;
   }
-  if (prm->matrix_free || defer_blocks(prm))  // no allocation inside the capture
+  // no allocation inside the capture
+  if (prm->matrix_free || defer_blocks(prm) || prm->precision == 1)
     if (int e = ensure_rec(c)) return e;
+  if (prm->precision == 1)
+    if (int e = ensure_mixed(c)) return e;
   int e = capture(c, &c->g_solve, [&]() -> int {
     const bool fuse = true;  // precond 0 or 1 (check_params)
     if (int e2 = enqueue_assemble(c, prm, c->consts, fuse)) return e2;
@@ -1276,10 +1286,12 @@ int capture_step(pd_ctx* c, const pd_step_params* prm, int K) {
       const MixedLimits lim = mixed_limits(prm);
       pd::launch_mx_setup(c->mx, ns, prm->rtol, prm->max_iter, lim.inner_rtol, lim.inner_max,
                           lim.max_refine, c->stream);
-      if (fuse)
+      if (fuse) {
         pd::launch_mx_resid_fold(a, c->stream);
-      else
+      } else {
+        pd::launch_mx_rec_x(a, c->stream);
         pd::launch_mx_resid(a, c->stream);
+      }
       for (int i = 0; i < K; ++i) {
         pd::launch_mx_spmv(a, c->stream);
         pd::launch_mx_update(a, c->stream);

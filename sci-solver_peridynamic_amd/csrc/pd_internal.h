@@ -204,7 +204,10 @@ struct MxArgs {
   int32_t precond;
   int32_t unroll;  // slots per trip of the SpMV row loops: 2, 4 or 8
   // the FP64 system of the residual: its bond blocks recomputed from the
-  // positions as the assembly forms them (matrix-free), FP64 diagonal
+  // positions as the assembly forms them (matrix-free), FP64 diagonal.
+  // A64.rec, set between a mixed assembly and the next position change,
+  // holds per particle the position and x side by side (the apply and
+  // launch_mx_rec_x write x)
   SpmvView A64;
 };
 
@@ -348,6 +351,9 @@ void launch_spmv_plain(const SpmvView& A, const double4* x, double4* y, hipStrea
 void launch_mx_setup(pd_mx_state* st, int32_t nspec, double rtol, int32_t max_iter,
                      double inner_rtol, int32_t inner_max, int32_t max_refine, hipStream_t s);
 void launch_mx_resid(const MxArgs& a, hipStream_t s);  // residual + its fold
+// x into the gather record ahead of a residual that no apply preceded
+// (a no-op unless A64.rec is set)
+void launch_mx_rec_x(const MxArgs& a, hipStream_t s);
 // the fold alone, after an assembly that produced the residual partials
 void launch_mx_resid_fold(const MxArgs& a, hipStream_t s);
 void launch_mx_spmv(const MxArgs& a, hipStream_t s);

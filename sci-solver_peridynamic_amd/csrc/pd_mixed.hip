@@ -216,7 +216,10 @@ __global__ void k_mx_setup(pd_mx_state* states, int nspec, double rtol, int max_
 
 // r = S(b - A x) in FP64 (the FP64 system, blocks recomputed); r32 =
 // float(r); z = M32 r32; p = z; d = 0; partials <r,r>, <r32,z>, <r32,r32>,
-// <Sb,Sb>
+// <Sb,Sb>.  After a mixed assembly (A64.rec set: the record holds the
+// positions and x side by side) the neighbours come from the record, four
+// bonds per trip with shared reciprocals when the range flag is clear, two
+// with IEEE quotients when it is set; otherwise from the arrays.
 __global__ __launch_bounds__(PD_BLOCK) void k_mx_resid(MxArgs a) {
   __shared__ double lds4[4];
   const int ntiles = gridDim.x;
@@ -226,7 +229,12 @@ __global__ __launch_bounds__(PD_BLOCK) void k_mx_resid(MxArgs a) {
   double t_rr = 0.0, t_rz = 0.0, t_r32 = 0.0, t_bb = 0.0;
   if (p < a.n) {
     double y0, y1, y2;
-    spmv_row_mf<3, true, false, false>(a.A64, a.x, p, y0, y1, y2);
+    if (!a.A64.rec)
+      spmv_row_mf<3, true, false, false>(a.A64, a.x, p, y0, y1, y2);
+    else if (*a.A64.range_bad == 0)  // uniform over the grid
+      spmv_row_mf<4, true, true, true>(a.A64, a.x, p, y0, y1, y2, a.A64.rec);
+    else
+      spmv_row_mf<2, true, true, false>(a.A64, a.x, p, y0, y1, y2, a.A64.rec);
     const double4 bp = a.b[p];
     const bool fr = a.bc[p] == 0;
     const double r0 = fr ? bp.x - y0 : 0.0;
@@ -427,7 +435,8 @@ __global__ __launch_bounds__(PD_BLOCK) void k_mx_dir(MxArgs a) {
   a.p[p] = pp;
 }
 
-// x += d (FP64)
+// x += d (FP64), and x into the record (A64.rec set) for the residual's
+// neighbour gathers
 __global__ __launch_bounds__(PD_BLOCK) void k_mx_apply(MxArgs a) {
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   if (a.st[spec_of(a.batch, tile)].phase != 2) return;
@@ -439,6 +448,26 @@ __global__ __launch_bounds__(PD_BLOCK) void k_mx_apply(MxArgs a) {
   xx.y = xx.y + (double)dd.y;
   xx.z = xx.z + (double)dd.z;
   a.x[p] = xx;
+  if (a.A64.rec) {
+    double* rp = a.A64.rec + 6 * (int64_t)p;
+    rp[3] = xx.x;
+    rp[4] = xx.y;
+    rp[5] = xx.z;
+  }
+}
+
+// x into the record (A64.rec set) ahead of a residual no apply preceded:
+// the rows of specimens whose residual is due
+__global__ __launch_bounds__(PD_BLOCK) void k_mx_rec_x(MxArgs a) {
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  if (!resid_due(a.st[spec_of(a.batch, tile)].phase)) return;
+  const int p = tile * PD_BLOCK + threadIdx.x;
+  if (p >= a.n) return;
+  const double4 xx = a.x[p];
+  double* rp = a.A64.rec + 6 * (int64_t)p;
+  rp[3] = xx.x;
+  rp[4] = xx.y;
+  rp[5] = xx.z;
 }
 
 template <int U>
@@ -469,6 +498,10 @@ void launch_mx_setup(pd_mx_state* st, int32_t nspec, double rtol, int32_t max_it
                      double inner_rtol, int32_t inner_max, int32_t max_refine, hipStream_t s) {
   hipLaunchKernelGGL(k_mx_setup, dim3(1), dim3(256), 0, s, st, nspec, rtol, max_iter, inner_rtol,
                      inner_max, max_refine);
+}
+void launch_mx_rec_x(const MxArgs& a, hipStream_t s) {
+  if (a.n && a.A64.rec)
+    hipLaunchKernelGGL(k_mx_rec_x, dim3(tiles_of(a.n)), dim3(PD_BLOCK), 0, s, a);
 }
 void launch_mx_resid(const MxArgs& a, hipStream_t s) {
   if (!a.n) return;
