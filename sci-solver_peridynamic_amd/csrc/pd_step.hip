@@ -15,12 +15,15 @@ namespace pd {
 // Fused assembly: core/cg_solver.cuh:200-339 (calculatePDForceDerivative)
 // + core/implicit_euler.cuh:19-86 / core/newmark_beta.cuh:19-86 (fill A, b)
 // + the block-Jacobi inverse of A_pp.  One lane per particle; a wave is one
-// 64-row slice, so every slot read / write is 64 consecutive words.  The
-// next bond's column and neighbour data are fetched while the current bond
-// is computed.  FP64 bond blocks are not written here: the solve's first
-// product recomputes and writes them (writing them here cost more than
-// that recomputation, the stores adding to the arithmetic rather than
-// overlapping it), or every matrix-free product recomputes them.
+// 64-row slice, so every slot read / write is 64 consecutive words.  With
+// shared reciprocals a lane first takes its list three slots per trip
+// without swap-remove, up to the first trip holding a breaking bond; the
+// sequential pass (the next bond's column and neighbour data fetched while
+// the current bond is computed) does the rest.  FP64 bond blocks are not
+// written here: the solve's first product recomputes and writes them
+// (writing them here cost more than that recomputation, the stores adding
+// to the arithmetic rather than overlapping it), or every matrix-free
+// product recomputes them.
 // ---------------------------------------------------------------------------
 // Neighbour q's position, stretch limit and (fuse) start value from the
 // assembly record.
@@ -83,43 +86,22 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
     int gq = -1;
     const double hr = mt.horizon - mt.radius;
     const double yr = SHARED ? rcp_nr(mt.radius) : 0.0;  // 1 / radius for vscale
-    int bnd = 0;
-    // software pipeline: the neighbour data of slot bnd + 1 and the column
-    // of slot bnd + 2 are in flight while slot bnd is computed, so neither
-    // the column load nor the gathers it feeds wait in line
-    int q_n = 0, q_nn = 0;
-    double4 xq_n = make_double4(0, 0, 0, 0);
-    double4 us_n = make_double4(0, 0, 0, 0);  // fused residual: xs of the neighbour
-    double sq_n = 0.0;
-    double l_n = 0.0;  // the slot's reference length
-    if (top >= 0) {
-      q_n = a.col[base * 64 + lane];
-      l_n = a.d0[base * 64 + lane];
-      gather_asm<FUSE>(a, q_n, xq_n, sq_n, us_n);
-    }
-    if (top >= 1) q_nn = a.col[(base + 1) * 64 + lane];
-    while (bnd <= top) {
-      const int64_t slot = base + bnd;
-      const int q = q_n;
-      const double4 xq = xq_n;
-      const double d0 = l_n;
-      const double sqv = sq_n;
-      const double4 u = us_n;
-      if (bnd + 1 <= top) {
-        q_n = q_nn;
-        l_n = a.d0[(slot + 1) * 64 + lane];
-        gather_asm<FUSE>(a, q_n, xq_n, sq_n, us_n);
-      }
-      if (bnd + 2 <= top) q_nn = a.col[(slot + 2) * 64 + lane];
-      double e0, e1, e2, dij;
+    // A bond's geometry, stretch and breaking threshold, cg_solver.cuh:253-264
+    // (yd: SHARED, 1 / d0 for dij / d0 and c / d0); the bond breaks when
+    // drdivd > thr.
+    auto measure = [&](const double4& xq, double sqv, double d0, double& e0, double& e1,
+                       double& e2, double& dij, double& yd, double& drdivd, double& thr) {
       bond_dir<SHARED>(pos_p, xq, e0, e1, e2, dij);
       if constexpr (SHARED) zero_len = zero_len || dij == 0.0;
-      double y0 = 0.0;  // SHARED: 1 / d0 for dij / d0 and c / d0
-      if constexpr (SHARED) y0 = rcp_nr(d0);
-      const double drdivd = (SHARED ? div_rcp(dij, d0, y0) : dij / d0) - 1.0;
+      yd = 0.0;
+      if constexpr (SHARED) yd = rcp_nr(d0);
+      drdivd = (SHARED ? div_rcp(dij, d0, yd) : dij / d0) - 1.0;
       // cg_solver.cuh:264: threshold (float)max(s_p, s_q)
       const float sq_f = (float)sqv;
-      const double thr = (double)(stretch_f > sq_f ? stretch_f : sq_f);
+      thr = (double)(stretch_f > sq_f ? stretch_f : sq_f);
+    };
+    // GAP: the bond nearest its threshold so far (the first one on ties)
+    auto track_gap = [&](int q, double drdivd, double thr) {
       if constexpr (GAP) {
         const double gap = fabs(drdivd - thr);
         if (gap < gmin) {
@@ -127,25 +109,11 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
           gq = q;
         }
       }
-      if (drdivd > thr) {
-        // swap-remove: the last live bond takes this slot, walker stays
-        const int64_t last_slot = (base + top) * 64 + lane;
-        const int last = a.col[last_slot];
-        a.col[slot * 64 + lane] = last;
-        a.col[last_slot] = q;
-        {  // the reference lengths move with their columns
-          const double dl = a.d0[last_slot];
-          a.d0[last_slot] = a.d0[slot * 64 + lane];
-          a.d0[slot * 64 + lane] = dl;
-        }
-        --top;
-        // slot bnd again, now holding `last`: restart the pipeline there
-        q_n = last;
-        l_n = a.d0[slot * 64 + lane];
-        gather_asm<FUSE>(a, last, xq_n, sq_n, us_n);
-        if (bnd + 1 <= top) q_nn = a.col[(slot + 1) * 64 + lane];
-        continue;
-      }
+    };
+    // A live bond's force, Jacobian, diagonal and residual terms,
+    // cg_solver.cuh:277-319; the mixed solver's float block goes to `slot`.
+    auto accumulate = [&](int64_t slot, const double4& u, double d0, double e0, double e1,
+                          double e2, double dij, double yd, double drdivd) {
       double vscale = 1.0;
       if (d0 > hr)
         vscale = 0.5 + (SHARED ? div_rcp((mt.horizon - d0) / 2.0, mt.radius, yr)
@@ -158,7 +126,7 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
       f1 += kk * e1;
       f2 += kk * e2;
       double F00, F01, F02, F11, F12, F22;
-      const double sc = SHARED ? div_rcp(mt.c_bond, d0, y0) : mt.c_bond / d0;
+      const double sc = SHARED ? div_rcp(mt.c_bond, d0, yd) : mt.c_bond / d0;
       bond_jac_sc<SHARED>(e0, e1, e2, dij, d0, sc, F00, F01, F02, F11, F12, F22);
       k00 += F00;
       k01 += F01;
@@ -186,6 +154,107 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
         rb1 += b01 * u.x + b11 * u.y + b12 * u.z;
         rb2 += b02 * u.x + b12 * u.y + b22 * u.z;
       }
+    };
+    int bnd = 0;
+    // With shared reciprocals: first an optimistic pass over the list as
+    // it stands, OPT slots per trip (independent chains for the
+    // scheduler) and no swap-remove.  Breaking bonds are rare (tens per
+    // step among 10^8): a lane stops at the first trip that holds one and
+    // the sequential pass below takes over at that trip's first slot, with
+    // the sums of the slots before it -- those of the sequential pass, which
+    // reaches that slot with the same bonds in the same order.
+    constexpr int OPT = SHARED ? 3 : 0;
+    if constexpr (OPT > 0) {
+      bool brk = false;
+      for (; bnd + OPT - 1 <= top; bnd += OPT) {
+        int qv[OPT];
+        double lv[OPT], sv[OPT];
+        double4 xv[OPT], uv[OPT];
+#pragma unroll
+        for (int k = 0; k < OPT; ++k) {
+          qv[k] = ntl(a.col + (base + bnd + k) * 64 + lane);
+          lv[k] = ntl(a.d0 + (base + bnd + k) * 64 + lane);
+        }
+#pragma unroll
+        for (int k = 0; k < OPT; ++k) gather_asm<FUSE>(a, qv[k], xv[k], sv[k], uv[k]);
+        double e0[OPT], e1[OPT], e2[OPT], dij[OPT], yd[OPT], dr[OPT], th[OPT];
+#pragma unroll
+        for (int k = 0; k < OPT; ++k) {
+          measure(xv[k], sv[k], lv[k], e0[k], e1[k], e2[k], dij[k], yd[k], dr[k], th[k]);
+          brk = brk || dr[k] > th[k];
+        }
+        if (brk) break;
+#pragma unroll
+        for (int k = 0; k < OPT; ++k) {
+          track_gap(qv[k], dr[k], th[k]);
+          accumulate(base + bnd + k, uv[k], lv[k], e0[k], e1[k], e2[k], dij[k], yd[k], dr[k]);
+        }
+      }
+      for (; !brk && bnd <= top; ++bnd) {
+        const int q = ntl(a.col + (base + bnd) * 64 + lane);
+        const double d0 = ntl(a.d0 + (base + bnd) * 64 + lane);
+        double4 xq, u;
+        double sqv;
+        gather_asm<FUSE>(a, q, xq, sqv, u);
+        double e0, e1, e2, dij, yd, drdivd, thr;
+        measure(xq, sqv, d0, e0, e1, e2, dij, yd, drdivd, thr);
+        brk = drdivd > thr;
+        if (brk) break;
+        track_gap(q, drdivd, thr);
+        accumulate(base + bnd, u, d0, e0, e1, e2, dij, yd, drdivd);
+      }
+    }
+    // Sequential pass with swap-remove, in software pipeline: the neighbour
+    // data of slot bnd + 1 and the column of slot bnd + 2 are in flight while
+    // slot bnd is computed, so neither the column load nor the gathers it
+    // feeds wait in line
+    int q_n = 0, q_nn = 0;
+    double4 xq_n = make_double4(0, 0, 0, 0);
+    double4 us_n = make_double4(0, 0, 0, 0);  // fused residual: xs of the neighbour
+    double sq_n = 0.0;
+    double l_n = 0.0;  // the slot's reference length
+    if (bnd <= top) {
+      q_n = a.col[(base + bnd) * 64 + lane];
+      l_n = a.d0[(base + bnd) * 64 + lane];
+      gather_asm<FUSE>(a, q_n, xq_n, sq_n, us_n);
+    }
+    if (bnd + 1 <= top) q_nn = a.col[(base + bnd + 1) * 64 + lane];
+    while (bnd <= top) {
+      const int64_t slot = base + bnd;
+      const int q = q_n;
+      const double4 xq = xq_n;
+      const double d0 = l_n;
+      const double sqv = sq_n;
+      const double4 u = us_n;
+      if (bnd + 1 <= top) {
+        q_n = q_nn;
+        l_n = a.d0[(slot + 1) * 64 + lane];
+        gather_asm<FUSE>(a, q_n, xq_n, sq_n, us_n);
+      }
+      if (bnd + 2 <= top) q_nn = a.col[(slot + 2) * 64 + lane];
+      double e0, e1, e2, dij, yd, drdivd, thr;
+      measure(xq, sqv, d0, e0, e1, e2, dij, yd, drdivd, thr);
+      track_gap(q, drdivd, thr);
+      if (drdivd > thr) {
+        // swap-remove: the last live bond takes this slot, walker stays
+        const int64_t last_slot = (base + top) * 64 + lane;
+        const int last = a.col[last_slot];
+        a.col[slot * 64 + lane] = last;
+        a.col[last_slot] = q;
+        {  // the reference lengths move with their columns
+          const double dl = a.d0[last_slot];
+          a.d0[last_slot] = a.d0[slot * 64 + lane];
+          a.d0[slot * 64 + lane] = dl;
+        }
+        --top;
+        // slot bnd again, now holding `last`: restart the pipeline there
+        q_n = last;
+        l_n = a.d0[slot * 64 + lane];
+        gather_asm<FUSE>(a, last, xq_n, sq_n, us_n);
+        if (bnd + 1 <= top) q_nn = a.col[(slot + 1) * 64 + lane];
+        continue;
+      }
+      accumulate(slot, u, d0, e0, e1, e2, dij, yd, drdivd);
       ++bnd;
     }
     // A_pp = K_pp dt^2 + m I
