@@ -64,12 +64,21 @@ def test_config_run_parity(cfg, steps, precision, mf):
     worst_rel = 0.0
     worst_rel_disp = 0.0
     exact_steps = 0
+    prev_exact = True  # the state entering the step is bit-equal to the oracle's
+    gap_checked = 0
     for k in range(1, steps + 1):
         kw = dict(pds.step_kwargs(desc, k), precision=precision)
         so = S.step(orc.make_params(**kw))
         sg = ctx.step(pds.make_params(**dict(kw, track_gap=1, matrix_free=mf)))
+        if prev_exact:
+            # the same assembly inputs: the same bond nearest its threshold
+            # (the first one in processing order on ties), bit for bit
+            assert (sg.min_gap, sg.gap_p, sg.gap_q) == (so.min_gap, so.gap_p, so.gap_q), k
+            gap_checked += 1
         xg = ctx.download("position")
         d = np.abs(xg[:, :3] - S.x[:, :3]).max()
+        prev_exact = prev_exact and bool(d == 0.0) and np.array_equal(
+            ctx.download("velocity")[:, :3], S.v[:, :3])
         rel = d / np.abs(S.x[:, :3]).max()
         disp = np.abs(S.x[:, :3] - S.x0[:, :3]).max()
         rel_disp = d / disp if disp > 0 else 0.0
@@ -89,7 +98,7 @@ def test_config_run_parity(cfg, steps, precision, mf):
                      broken=(so.broken_bonds, sg.broken_bonds), live=sg.live_bonds, rel=rel,
                      rel_disp=rel_disp, gap=so.min_gap))
     say(dict(cfg=cfg, steps=steps, worst_rel=worst_rel, worst_rel_disp=worst_rel_disp,
-             bitwise_equal_steps=exact_steps, first_near=first_near))
+             bitwise_equal_steps=exact_steps, first_near=first_near, gap_checked=gap_checked))
     ctx.close()
 
 
@@ -232,6 +241,8 @@ def test_full_size_impact_steps_match_oracle():
     worst_rel = 0.0
     broken_total = 0
     mf_steps = 0
+    prev_exact = True  # the state entering the step is bit-equal to the oracle's
+    gap_checked = 0
     for k in range(1, steps + 1):
         kw = pds.step_kwargs(desc, k)
         so = S.step(orc.make_params(**kw))
@@ -240,6 +251,9 @@ def test_full_size_impact_steps_match_oracle():
         sg = ctx.step(pds.make_params(**dict(kw, track_gap=1, matrix_free=2)))
         mf_steps += sg.operator_mf
         assert so.converged and sg.converged
+        if prev_exact:  # the same bond nearest its threshold, bit for bit
+            assert (sg.min_gap, sg.gap_p, sg.gap_q) == (so.min_gap, so.gap_p, so.gap_q), k
+            gap_checked += 1
         if first_near is None and so.min_gap < 1e-6:
             first_near = (k, so.min_gap)
         if first_near is None:
@@ -257,6 +271,8 @@ def test_full_size_impact_steps_match_oracle():
         rel = d / np.abs(S.x[:, :3]).max()
         worst_rel = max(worst_rel, rel)
         exact_steps += int(d == 0.0)
+        prev_exact = prev_exact and bool(d == 0.0) and np.array_equal(
+            ctx.download("velocity")[:, :3], S.v[:, :3])
         assert rel <= REL_TOL, (k, rel)
         if k % 10 == 0 or k in (first_contact, first_break):
             say(dict(step=k, it=(so.cg_iterations, sg.cg_iterations),
@@ -267,5 +283,5 @@ def test_full_size_impact_steps_match_oracle():
     assert 0 < mf_steps < steps, mf_steps
     say(dict(steps=steps, first_contact=first_contact, first_break=first_break,
              broken_total=broken_total, bitwise_equal_steps=exact_steps, worst_rel=worst_rel,
-             first_near=first_near, steps_recomputing_blocks=mf_steps))
+             first_near=first_near, steps_recomputing_blocks=mf_steps, gap_checked=gap_checked))
     ctx.close()
