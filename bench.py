@@ -583,6 +583,9 @@ def main():
             "phase_ms_per_step": {k: v / m["steps"] for k, v in m["phase"].items()},
             "roofline": spmv_roofline(m, "k_mx_spmv (float inner CG SpMV)", read_traffic(1)),
             "resid_spmv_ms_mean": m["resid_ms"] / max(m["resid_n"], 1),
+            # the first inner product, which writes the float blocks the
+            # assembly leaves to it
+            "first_inner_product_with_block_stores_ms": m["store_ms"] / m["steps"],
             "plain_spmv_gbs": m["plain_bytes"] / (m["plain_ms"] * 1e-3) / 1e9,
             "spmv_unroll": m["unroll"],
             "spmv_tune_ms_2_4_8": m["tune_ms"],

@@ -175,7 +175,8 @@ typedef struct pd_step_stats {
   int32_t resid_launches;
   int32_t precond_fallbacks; /* precond 2: 1 if block Jacobi took over */
   /* stored-block FP64 steps: the solve's first product, which also writes
-   * the bond blocks the assembly left to it (not in ms_spmv_total) */
+   * the bond blocks the assembly left to it; mixed steps: the first inner
+   * product, which writes the float blocks (neither in ms_spmv_total) */
   double ms_spmv_store;
   int32_t precond_probe; /* precond 2: 1 if this step's solve was a
                             block-Jacobi probe (learned_probe) */

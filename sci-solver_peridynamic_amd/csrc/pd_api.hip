@@ -164,6 +164,8 @@ struct pd_ctx {
   // product (launch_cg_spmv_store); cleared when that product or
   // ensure_blocks has written them
   bool blocks_deferred = false;
+  // the same for the float blocks of a mixed assembly (launch_mx_spmv_store)
+  bool blocks32_deferred = false;
   // matrix_free = 2 (pd_step): measured times of the two operators, ms per
   // CG product (0: not measured yet); reset when the bonds change
   double op_t_mf = 0.0;     // recomputing product
@@ -446,6 +448,7 @@ int ensure_slots(pd_ctx* c, int64_t slots) {
   dfree(c->d0);
   c->d0_valid = false;
   c->blocks_deferred = false;  // new bond lists: no matrix to keep
+  c->blocks32_deferred = false;
   c->asm_fresh = false;
   if (int e = dalloc(c, &c->col, (size_t)slots * 64)) return e;
   if (int e = dalloc(c, &c->val, (size_t)slots * 64 * 6)) return e;
@@ -662,10 +665,16 @@ bool defer_blocks(const pd_step_params* prm) {
 // Deferred blocks written from the assembly's positions (for callers that
 // read the stored blocks without a solve having run).
 int ensure_blocks(pd_ctx* c) {
-  if (!c->blocks_deferred) return PD_OK;
-  pd::launch_write_blocks(spmv_view(c), c->val, c->stream);
-  CHECK_LAUNCH(c, "write blocks");
-  c->blocks_deferred = false;
+  if (c->blocks_deferred) {
+    pd::launch_write_blocks(spmv_view(c), c->val, c->stream);
+    CHECK_LAUNCH(c, "write blocks");
+    c->blocks_deferred = false;
+  }
+  if (c->blocks32_deferred) {
+    pd::launch_write_blocks32(spmv_view(c), c->val32, c->stream);
+    CHECK_LAUNCH(c, "write float blocks");
+    c->blocks32_deferred = false;
+  }
   return PD_OK;
 }
 
@@ -724,6 +733,7 @@ int enqueue_assemble(pd_ctx* c, const pd_step_params* prm,
   a.rec = prm->matrix_free || defer ? c->rec : nullptr;
   c->mf = prm->matrix_free ? 1 : 0;
   c->blocks_deferred = defer;
+  c->blocks32_deferred = prm->precision == 1;  // launch_mx_spmv_store writes them
   c->asm_dt2 = dt2;
   c->asm_consts = consts;
   a.fuse = fuse ? 1 : 0;
@@ -736,7 +746,6 @@ int enqueue_assemble(pd_ctx* c, const pd_step_params* prm,
   a.d32 = c->d32;
   a.part = c->part;
   a.mixed = prm->precision == 1 ? 1 : 0;
-  a.val32 = c->val32;
   a.diag32 = c->diag32;
   a.minv32 = c->minv32;
   a.n = c->n;
@@ -973,8 +982,9 @@ int run_solve_mixed(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st, int
       all_done = all_done && c->h_mx[k].phase == 3;
     }
   }
-  double spmv_ms = 0.0, resid_ms = 0.0;
+  double spmv_ms = 0.0, resid_ms = 0.0, store_ms = 0.0;
   int spmv_ran = 0, resid_ran = 0;
+  int store_pair = -1;  // the event pair of the launch that wrote the float blocks
   std::vector<int> chunk_spmv;
   while (!all_done) {
     chunk_spmv.clear();
@@ -984,7 +994,13 @@ int run_solve_mixed(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st, int
         chunk_spmv.push_back(spmv_ev.back());
         HIPCHK(c, hipEventRecord(c->ev[spmv_ev.back()], c->stream));
       }
-      pd::launch_mx_spmv(a, c->stream);
+      if (c->blocks32_deferred) {  // the first inner product writes the float blocks
+        pd::launch_mx_spmv_store(a, c->val32, c->stream);
+        c->blocks32_deferred = false;
+        if (timing) store_pair = spmv_ev.back();
+      } else {
+        pd::launch_mx_spmv(a, c->stream);
+      }
       if (timing) HIPCHK(c, hipEventRecord(c->ev[spmv_ev.back() + 1], c->stream));
       pd::launch_mx_update(a, c->stream);
       pd::launch_mx_dir(a, c->stream);
@@ -1018,9 +1034,13 @@ int run_solve_mixed(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st, int
     }
     if (timing) {
       ran = std::min((int)chunk_spmv.size(), ran);
-      for (int k = 0; k < ran; ++k)
+      for (int k = 0; k < ran; ++k) {
+        if (chunk_spmv[k] == store_pair) continue;  // reported on its own
         spmv_ms += elapsed(c->ev[chunk_spmv[k]], c->ev[chunk_spmv[k] + 1]);
-      spmv_ran += ran;
+        ++spmv_ran;
+      }
+      if (store_pair >= 0 && syncs == 1)
+        store_ms = elapsed(c->ev[store_pair], c->ev[store_pair + 1]);
       if (syncs == 1 && !resumed && !fused) {
         resid_ms += elapsed(c->ev[resid_ev[0]], c->ev[resid_ev[0] + 1]);
         ++resid_ran;
@@ -1067,6 +1087,7 @@ int run_solve_mixed(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st, int
       st->spmv_launches = spmv_ran;
       st->ms_resid_total = resid_ms;
       st->resid_launches = resid_ran;
+      st->ms_spmv_store = store_ms;
     }
   }
   return PD_OK;
@@ -1293,7 +1314,12 @@ int capture_step(pd_ctx* c, const pd_step_params* prm, int K) {
         pd::launch_mx_resid(a, c->stream);
       }
       for (int i = 0; i < K; ++i) {
-        pd::launch_mx_spmv(a, c->stream);
+        if (c->blocks32_deferred) {  // the first inner product writes the float blocks
+          pd::launch_mx_spmv_store(a, c->val32, c->stream);
+          c->blocks32_deferred = false;
+        } else {
+          pd::launch_mx_spmv(a, c->stream);
+        }
         pd::launch_mx_update(a, c->stream);
         pd::launch_mx_dir(a, c->stream);
       }
@@ -1734,6 +1760,7 @@ int pd_build_bonds(pd_ctx* c, pd_bond_stats* stats) {
   c->h_cap = counts;
   c->d0_valid = false;
   c->blocks_deferred = false;  // new bond lists: no matrix to keep
+  c->blocks32_deferred = false;
   c->asm_fresh = false;
   reset_operator_times(c);
   if (stats) {
@@ -1791,6 +1818,7 @@ int pd_set_bonds(pd_ctx* c, const int32_t* counts, const int32_t* lists, int32_t
   c->h_cap = cap;
   c->d0_valid = false;
   c->blocks_deferred = false;  // new bond lists: no matrix to keep
+  c->blocks32_deferred = false;
   c->asm_fresh = false;
   reset_operator_times(c);
   return PD_OK;

@@ -25,8 +25,7 @@ struct AsmArgs {
   double* val;
   double* diag;
   double* minv;
-  float* val32;   // mixed precision: float blocks instead of val / minv
-  float* diag32;
+  float* diag32;  // mixed precision: float diagonal and inverse beside the FP64 ones
   float* minv32;
   int32_t mixed;
   const double4* x;
@@ -357,6 +356,13 @@ void launch_mx_rec_x(const MxArgs& a, hipStream_t s);
 // the fold alone, after an assembly that produced the residual partials
 void launch_mx_resid_fold(const MxArgs& a, hipStream_t s);
 void launch_mx_spmv(const MxArgs& a, hipStream_t s);
+// the first inner product after a mixed assembly, which leaves the float
+// blocks: they are recomputed from the positions, written to vals (every
+// row) and multiplied
+void launch_mx_spmv_store(const MxArgs& a, float* vals, hipStream_t s);
+// the float blocks of the current bond lists written to vals from the
+// positions (a step's deferred float blocks when no product wrote them)
+void launch_write_blocks32(const SpmvView& A, float* vals, hipStream_t s);
 void launch_mx_update(const MxArgs& a, hipStream_t s);  // alpha, update, beta
 void launch_mx_dir(const MxArgs& a, hipStream_t s);
 void launch_mx_apply(const MxArgs& a, hipStream_t s);

@@ -158,6 +158,102 @@ __device__ __forceinline__ void row_f32(const MxArgs& A, const float4* __restric
   }
 }
 
+// One bond's float block: the FP64 block recomputed from the positions as
+// the assembly forms it (bond_dir, bond_jac, the time-step scaling),
+// rounded to float -- the values the inner solves multiply.
+template <bool SHARED>
+__device__ __forceinline__ void bond_block32(const double4 pos_p, const double4 xq, double d0,
+                                             double cb, double dt2, float v[6]) {
+  double e0, e1, e2, dij;
+  bond_dir<SHARED, SHARED>(pos_p, xq, e0, e1, e2, dij);
+  double F00, F01, F02, F11, F12, F22;
+  bond_jac<SHARED>(e0, e1, e2, dij, d0, cb, F00, F01, F02, F11, F12, F22);
+  v[0] = (float)((F00 * -1.0) * dt2);
+  v[1] = (float)((F01 * -1.0) * dt2);
+  v[2] = (float)((F02 * -1.0) * dt2);
+  v[3] = (float)((F11 * -1.0) * dt2);
+  v[4] = (float)((F12 * -1.0) * dt2);
+  v[5] = (float)((F22 * -1.0) * dt2);
+}
+
+__device__ __forceinline__ void store_block32(float* vals, int64_t slot, int lane,
+                                              const float v[6]) {
+  float* vo = vals + slot * 6 * 64 + lane;
+#pragma unroll
+  for (int c = 0; c < 6; ++c) __builtin_nontemporal_store(v[c], vo + c * 64);
+}
+
+// Row p of A32 x as row_f32 computes it (the same float products and sums
+// in the same order), the row's float blocks formed on the way
+// (bond_block32) and written to vals.  REC: neighbour positions from the
+// record A64.rec; SHARED: shared reciprocals (range flag clear).  V slots
+// per trip.
+template <int V, bool REC, bool SHARED>
+__device__ __forceinline__ void row_f32_store(const MxArgs& A, float* __restrict__ vals,
+                                              const float4* __restrict__ x, int p, float& y0,
+                                              float& y1, float& y2) {
+  const int s = p >> 6;
+  const int lane = p & 63;
+  const float* dg = A.diag32 + (int64_t)s * 6 * 64 + lane;
+  blk6f(dg[0], dg[64], dg[128], dg[192], dg[256], dg[320], x[p], y0, y1, y2);
+  const int top = A.top[p];
+  if (top < 0) return;
+  const SpmvView& F = A.A64;
+  const int64_t base = A.slice_off[s];
+  const int* __restrict__ cp = A.col + base * 64 + lane;
+  const double* __restrict__ lp = F.d0 + base * 64 + lane;
+  const double4 pos_p = F.pos[p];
+  const double cb = F.mat[F.body[p]].c_bond;
+  const double dt2 = F.consts ? F.consts->dt2 : F.dt2;
+  auto pos_of = [&](int q) -> double4 {
+    if constexpr (REC) {
+      const dbl2v* r = (const dbl2v*)(F.rec + 6 * (int64_t)q);
+      const dbl2v r0 = r[0], r1 = r[1];
+      return make_double4(r0.x, r0.y, r1.x, 0.0);
+    } else {
+      return F.pos[q];
+    }
+  };
+  int bnd = 0;
+  for (; bnd + V - 1 <= top; bnd += V) {
+    int q[V];
+    double l0[V];
+#pragma unroll
+    for (int u = 0; u < V; ++u) {
+      q[u] = ntl(cp + (int64_t)(bnd + u) * 64);
+      l0[u] = ntl(lp + (int64_t)(bnd + u) * 64);
+    }
+    double4 xq[V];
+#pragma unroll
+    for (int u = 0; u < V; ++u) xq[u] = pos_of(q[u]);
+    float t[V][3];
+#pragma unroll
+    for (int u = 0; u < V; ++u) {
+      float v[6];
+      bond_block32<SHARED>(pos_p, xq[u], l0[u], cb, dt2, v);
+      store_block32(vals, base + bnd + u, lane, v);
+      blk6f(v[0], v[1], v[2], v[3], v[4], v[5], x[q[u]], t[u][0], t[u][1], t[u][2]);
+    }
+#pragma unroll
+    for (int u = 0; u < V; ++u) {
+      y0 += t[u][0];
+      y1 += t[u][1];
+      y2 += t[u][2];
+    }
+  }
+  for (; bnd <= top; ++bnd) {
+    const int q = ntl(cp + (int64_t)bnd * 64);
+    float v[6];
+    bond_block32<SHARED>(pos_p, pos_of(q), ntl(lp + (int64_t)bnd * 64), cb, dt2, v);
+    store_block32(vals, base + bnd, lane, v);
+    float t0, t1, t2;
+    blk6f(v[0], v[1], v[2], v[3], v[4], v[5], x[q], t0, t1, t2);
+    y0 += t0;
+    y1 += t1;
+    y2 += t2;
+  }
+}
+
 __device__ __forceinline__ void precond_f(const float* minv, int pc, int p, float r0, float r1,
                                           float r2, float& z0, float& z1, float& z2) {
   if (pc != 0) {  // block Jacobi (1) or the learned block (2)
@@ -322,6 +418,58 @@ __global__ __launch_bounds__(PD_BLOCK) void k_mx_spmv(MxArgs a) {
   }
   const double v = tile_sum(t, lds4);
   if (threadIdx.x == 0) a.part[tile] = v;
+}
+
+// The first inner product of a solve after a mixed assembly, which leaves
+// the float blocks to it: y = A32 p and the <y,p> partial as k_mx_spmv
+// (for the specimens in their inner loop), and every row's float blocks
+// written to vals, converged specimens' too, so the step leaves the whole
+// float matrix.
+__global__ __launch_bounds__(PD_BLOCK) void k_mx_spmv_store(MxArgs a, float* vals) {
+  __shared__ double lds4[4];
+  const int ntiles = gridDim.x;
+  const int tile = xcd_tile(blockIdx.x, ntiles);
+  const bool act = a.st[spec_of(a.batch, tile)].phase == 1;
+  const int p = tile * PD_BLOCK + threadIdx.x;
+  double t = 0.0;
+  if (p < a.n) {
+    float y0, y1, y2;
+    if (!a.A64.rec)
+      row_f32_store<3, false, false>(a, vals, a.p, p, y0, y1, y2);
+    else if (*a.A64.range_bad == 0)  // uniform over the grid
+      row_f32_store<3, true, true>(a, vals, a.p, p, y0, y1, y2);
+    else
+      row_f32_store<3, true, false>(a, vals, a.p, p, y0, y1, y2);
+    if (act) {
+      a.y[p] = make_float4(y0, y1, y2, 0.0f);
+      const float4 pp = a.p[p];
+      t = term3f(y0, y1, y2, pp.x, pp.y, pp.z);
+    }
+  }
+  if (!act) return;  // uniform over the tile
+  const double v = tile_sum(t, lds4);
+  if (threadIdx.x == 0) a.part[tile] = v;
+}
+
+// The float blocks of the current bond lists from the positions (IEEE
+// quotients), for callers that read them without a solve having run.
+__global__ __launch_bounds__(PD_BLOCK) void k_write_blocks32(SpmvView A, float* vals) {
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int p = tile * PD_BLOCK + threadIdx.x;
+  if (p >= A.n) return;
+  const int top = A.top[p];
+  if (top < 0) return;
+  const int s = p >> 6, lane = p & 63;
+  const int64_t base = A.slice_off[s];
+  const double4 pos_p = A.pos[p];
+  const double cb = A.mat[A.body[p]].c_bond;
+  const double dt2 = A.consts ? A.consts->dt2 : A.dt2;
+  for (int bnd = 0; bnd <= top; ++bnd) {
+    const int64_t i = (base + bnd) * 64 + lane;
+    float v[6];
+    bond_block32<false>(pos_p, A.pos[A.col[i]], A.d0[i], cb, dt2, v);
+    store_block32(vals, base + bnd, lane, v);
+  }
 }
 
 // alpha = rz / <y,p>, formed by every workgroup from its specimen's <y,p>
@@ -522,6 +670,13 @@ void launch_mx_spmv(const MxArgs& a, hipStream_t s) {
     case 8: hipLaunchKernelGGL(k_mx_spmv<8>, g, dim3(PD_BLOCK), 0, s, a); break;
     default: hipLaunchKernelGGL(k_mx_spmv<4>, g, dim3(PD_BLOCK), 0, s, a);
   }
+}
+void launch_mx_spmv_store(const MxArgs& a, float* vals, hipStream_t s) {
+  if (a.n)
+    hipLaunchKernelGGL(k_mx_spmv_store, dim3(tiles_of(a.n)), dim3(PD_BLOCK), 0, s, a, vals);
+}
+void launch_write_blocks32(const SpmvView& A, float* vals, hipStream_t s) {
+  if (A.n) hipLaunchKernelGGL(k_write_blocks32, dim3(tiles_of(A.n)), dim3(PD_BLOCK), 0, s, A, vals);
 }
 void launch_mx_update(const MxArgs& a, hipStream_t s) {
   if (!a.n) return;

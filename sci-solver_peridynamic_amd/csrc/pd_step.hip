@@ -40,9 +40,9 @@ __device__ __forceinline__ void gather_asm(const AsmArgs& a, int q, double4& xq,
   }
 }
 
-// MIXED: the bond blocks and the block-Jacobi inverse are written rounded
-// to float (non-temporal stores: they are read back once per inner
-// iteration, never soon enough to gain from L2), the diagonal in both.
+// MIXED: the block-Jacobi inverse is written rounded to float, the
+// diagonal in both precisions, and the start is the mixed solve's; the
+// float bond blocks are left to the solve's first inner product.
 // Neighbour data come from the step's assembly record (position, stretch
 // limit and the solver's start value side by side, k_asm_prep).  SHARED:
 // the quotients by the float-rounded length (bond_dir), by the reference
@@ -111,9 +111,9 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
       }
     };
     // A live bond's force, Jacobian, diagonal and residual terms,
-    // cg_solver.cuh:277-319; the mixed solver's float block goes to `slot`.
-    auto accumulate = [&](int64_t slot, const double4& u, double d0, double e0, double e1,
-                          double e2, double dij, double yd, double drdivd) {
+    // cg_solver.cuh:277-319.
+    auto accumulate = [&](const double4& u, double d0, double e0, double e1, double e2,
+                          double dij, double yd, double drdivd) {
       double vscale = 1.0;
       if (d0 > hr)
         vscale = 0.5 + (SHARED ? div_rcp((mt.horizon - d0) / 2.0, mt.radius, yr)
@@ -137,18 +137,10 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
       const double b00 = (F00 * -1.0) * a.dt2, b01 = (F01 * -1.0) * a.dt2,
                    b02 = (F02 * -1.0) * a.dt2, b11 = (F11 * -1.0) * a.dt2,
                    b12 = (F12 * -1.0) * a.dt2, b22 = (F22 * -1.0) * a.dt2;
-      if constexpr (MIXED) {  // the inner solves' float blocks
-        float* vo = a.val32 + slot * 6 * 64 + lane;
-        __builtin_nontemporal_store((float)b00, vo + 0 * 64);
-        __builtin_nontemporal_store((float)b01, vo + 1 * 64);
-        __builtin_nontemporal_store((float)b02, vo + 2 * 64);
-        __builtin_nontemporal_store((float)b11, vo + 3 * 64);
-        __builtin_nontemporal_store((float)b12, vo + 4 * 64);
-        __builtin_nontemporal_store((float)b22, vo + 5 * 64);
-      }
-      // FP64 blocks are written by the solve's first product (k_cg_spmv_store
-      // or k_cg_init's storing variant) or, matrix-free, recomputed by every
-      // product; here they feed the first residual of the FP64 system
+      // The blocks are written by the solve's first product (FP64:
+      // k_cg_spmv_store or k_cg_init's storing variant; mixed, rounded to
+      // float: k_mx_spmv_store) or, matrix-free, recomputed by every product;
+      // here they feed the first residual of the FP64 system
       if constexpr (FUSE) {
         rb0 += b00 * u.x + b01 * u.y + b02 * u.z;
         rb1 += b01 * u.x + b11 * u.y + b12 * u.z;
@@ -187,7 +179,7 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
 #pragma unroll
         for (int k = 0; k < OPT; ++k) {
           track_gap(qv[k], dr[k], th[k]);
-          accumulate(base + bnd + k, uv[k], lv[k], e0[k], e1[k], e2[k], dij[k], yd[k], dr[k]);
+          accumulate(uv[k], lv[k], e0[k], e1[k], e2[k], dij[k], yd[k], dr[k]);
         }
       }
       for (; !brk && bnd <= top; ++bnd) {
@@ -201,7 +193,7 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
         brk = drdivd > thr;
         if (brk) break;
         track_gap(q, drdivd, thr);
-        accumulate(base + bnd, u, d0, e0, e1, e2, dij, yd, drdivd);
+        accumulate(u, d0, e0, e1, e2, dij, yd, drdivd);
       }
     }
     // Sequential pass with swap-remove, in software pipeline: the neighbour
@@ -254,7 +246,7 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
         if (bnd + 1 <= top) q_nn = a.col[(slot + 1) * 64 + lane];
         continue;
       }
-      accumulate(slot, u, d0, e0, e1, e2, dij, yd, drdivd);
+      accumulate(u, d0, e0, e1, e2, dij, yd, drdivd);
       ++bnd;
     }
     // A_pp = K_pp dt^2 + m I
