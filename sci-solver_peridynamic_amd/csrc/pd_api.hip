@@ -1890,8 +1890,11 @@ int pd_get_matrix(pd_ctx* c, double* diag9, double* vals9, int32_t stride) {
     const int64_t base = c->h_slice_off[s];
     for (int32_t k = 0; k <= top[p]; ++k) {
       double* out = vals9 + 9 * ((size_t)p * stride + k);
-      if (c->mat_prec == 1) {  // float component planes
-        unpack(vv.data() + (size_t)(base + k) * 384 + lane, 64, out);
+      if (c->mat_prec == 1) {  // float: a plane of 4 values per lane, then one of 2
+        const double* sl = vv.data() + (size_t)(base + k) * 384;
+        const double six[6] = {sl[4 * lane], sl[4 * lane + 1], sl[4 * lane + 2],
+                               sl[4 * lane + 3], sl[256 + 2 * lane], sl[256 + 2 * lane + 1]};
+        unpack(six, 1, out);
       } else {  // FP64 planes of pairs: value i at (i / 2) * 128 + 2 lane + i % 2
         const double* sl = vv.data() + (size_t)(base + k) * 384 + 2 * lane;
         const double six[6] = {sl[0], sl[1], sl[128], sl[129], sl[256], sl[257]};

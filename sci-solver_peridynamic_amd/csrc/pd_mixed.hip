@@ -31,6 +31,24 @@ namespace pd {
 
 namespace {
 
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef float f2v __attribute__((ext_vector_type(2)));
+
+// Float bond blocks: slot k's 384 values hold a plane of (a00 a01 a02 a11)
+// per lane, then a plane of (a12 a22) per lane, so a lane reads or writes
+// its block with one 16-byte and one 8-byte access.
+__device__ __forceinline__ void load_block32(const float* __restrict__ slot, int lane,
+                                             float v[6]) {
+  const f4v a = __builtin_nontemporal_load((const f4v*)slot + lane);
+  const f2v b = __builtin_nontemporal_load((const f2v*)(slot + 256) + lane);
+  v[0] = a.x;
+  v[1] = a.y;
+  v[2] = a.z;
+  v[3] = a.w;
+  v[4] = b.x;
+  v[5] = b.y;
+}
+
 
 __device__ __forceinline__ void blk6w(float a00, float a01, float a02, float a11, float a12,
                                       float a22, const double4 x, double& o0, double& o1,
@@ -65,7 +83,7 @@ __device__ __forceinline__ void row_wide(const MxArgs& A, const double4* __restr
   const int64_t base = A.slice_off[s];
   const int top = A.top[p];
   const int* __restrict__ cp = A.col + base * 64 + lane;
-  const float* __restrict__ vp = A.val32 + base * 6 * 64 + lane;
+  const float* __restrict__ vp = A.val32 + base * 6 * 64;
   int bnd = 0;
   for (; bnd + U - 1 <= top; bnd += U) {
     int q[U];
@@ -73,11 +91,7 @@ __device__ __forceinline__ void row_wide(const MxArgs& A, const double4* __restr
 #pragma unroll
     for (int u = 0; u < U; ++u) q[u] = ntl(cp + (int64_t)(bnd + u) * 64);
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const float* vv = vp + (int64_t)(bnd + u) * 384;
-#pragma unroll
-      for (int c = 0; c < 6; ++c) v[u][c] = ntl(vv + c * 64);
-    }
+    for (int u = 0; u < U; ++u) load_block32(vp + (int64_t)(bnd + u) * 384, lane, v[u]);
     double4 xx[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) xx[u] = x[q[u]];
@@ -92,10 +106,10 @@ __device__ __forceinline__ void row_wide(const MxArgs& A, const double4* __restr
   }
   for (; bnd <= top; ++bnd) {
     const int q0 = ntl(cp + (int64_t)bnd * 64);
-    const float* v0 = vp + (int64_t)bnd * 384;
+    float v[6];
+    load_block32(vp + (int64_t)bnd * 384, lane, v);
     double t0, t1, t2;
-    blk6w(ntl(v0), ntl(v0 + 64), ntl(v0 + 128), ntl(v0 + 192), ntl(v0 + 256), ntl(v0 + 320),
-          x[q0], t0, t1, t2);
+    blk6w(v[0], v[1], v[2], v[3], v[4], v[5], x[q0], t0, t1, t2);
     y0 += t0;
     y1 += t1;
     y2 += t2;
@@ -121,7 +135,7 @@ __device__ __forceinline__ void row_f32(const MxArgs& A, const float4* __restric
   const int64_t base = A.slice_off[s];
   const int top = A.top[p];
   const int* __restrict__ cp = A.col + base * 64 + lane;
-  const float* __restrict__ vp = A.val32 + base * 6 * 64 + lane;
+  const float* __restrict__ vp = A.val32 + base * 6 * 64;
   int bnd = 0;
   for (; bnd + U - 1 <= top; bnd += U) {
     int q[U];
@@ -129,11 +143,7 @@ __device__ __forceinline__ void row_f32(const MxArgs& A, const float4* __restric
 #pragma unroll
     for (int u = 0; u < U; ++u) q[u] = ntl(cp + (int64_t)(bnd + u) * 64);
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const float* vv = vp + (int64_t)(bnd + u) * 384;
-#pragma unroll
-      for (int c = 0; c < 6; ++c) v[u][c] = ntl(vv + c * 64);
-    }
+    for (int u = 0; u < U; ++u) load_block32(vp + (int64_t)(bnd + u) * 384, lane, v[u]);
     float4 xx[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) xx[u] = x[q[u]];
@@ -148,10 +158,10 @@ __device__ __forceinline__ void row_f32(const MxArgs& A, const float4* __restric
   }
   for (; bnd <= top; ++bnd) {
     const int q0 = ntl(cp + (int64_t)bnd * 64);
-    const float* v0 = vp + (int64_t)bnd * 384;
+    float v[6];
+    load_block32(vp + (int64_t)bnd * 384, lane, v);
     float t0, t1, t2;
-    blk6f(ntl(v0), ntl(v0 + 64), ntl(v0 + 128), ntl(v0 + 192), ntl(v0 + 256), ntl(v0 + 320),
-          x[q0], t0, t1, t2);
+    blk6f(v[0], v[1], v[2], v[3], v[4], v[5], x[q0], t0, t1, t2);
     y0 += t0;
     y1 += t1;
     y2 += t2;
@@ -176,11 +186,12 @@ __device__ __forceinline__ void bond_block32(const double4 pos_p, const double4 
   v[5] = (float)((F22 * -1.0) * dt2);
 }
 
+// One block into slot `slot` of vals (the layout load_block32 reads).
 __device__ __forceinline__ void store_block32(float* vals, int64_t slot, int lane,
                                               const float v[6]) {
-  float* vo = vals + slot * 6 * 64 + lane;
-#pragma unroll
-  for (int c = 0; c < 6; ++c) __builtin_nontemporal_store(v[c], vo + c * 64);
+  float* vo = vals + slot * 6 * 64;
+  __builtin_nontemporal_store(f4v{v[0], v[1], v[2], v[3]}, (f4v*)vo + lane);
+  __builtin_nontemporal_store(f2v{v[4], v[5]}, (f2v*)(vo + 256) + lane);
 }
 
 // Row p of A32 x as row_f32 computes it (the same float products and sums
