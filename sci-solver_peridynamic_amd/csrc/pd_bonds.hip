@@ -122,33 +122,47 @@ __global__ __launch_bounds__(PD_BLOCK) void k_bbox_bodies_partial(const double4*
   }
 }
 
-// each thread folds the block partials t, t + 256, ... (g <= 4 PD_BLOCK),
-// then every value of the bodies that exist is folded by wave butterflies
-// and across the four waves; the other bodies' entries are left unwritten
+// each thread folds the block partials t, t + 256, ... (all values of a
+// partial are adjacent, so the loads of a block go out together), then
+// every value of the bodies that exist is folded by wave butterflies and
+// across the four waves; the other bodies' entries are left unwritten
 __global__ __launch_bounds__(PD_BLOCK) void k_bbox_bodies_final(const double* part, int g,
                                                                int nbody, double* out) {
-  __shared__ double s[PD_BLOCK / 64][PD_MAX_BODIES * 6];
+  constexpr int NV = PD_MAX_BODIES * 6;
+  __shared__ double s[PD_BLOCK / 64][NV];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  for (int k = 0; k < nbody * 6; ++k) {
-    const bool lo = (k % 6) < 3;
-    double r = lo ? INFINITY : -INFINITY;
-    for (int i = t; i < g; i += PD_BLOCK) {
-      const double v = part[i * PD_MAX_BODIES * 6 + k];
-      r = lo ? fmin(r, v) : fmax(r, v);
-    }
+  const int nv = nbody * 6;
+  double r[NV];
 #pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) {
-      const double o = __shfl_xor(r, m, 64);
-      r = lo ? fmin(r, o) : fmax(r, o);
+  for (int k = 0; k < NV; ++k) r[k] = (k % 6) < 3 ? INFINITY : -INFINITY;
+  for (int i = t; i < g; i += PD_BLOCK) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      if (k < nv) {  // uniform
+        const double v = part[i * NV + k];
+        r[k] = (k % 6) < 3 ? fmin(r[k], v) : fmax(r[k], v);
+      }
     }
-    if (lane == 0) s[wave][k] = r;
+  }
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    if (k < nv) {  // uniform
+      const bool lo = (k % 6) < 3;
+      double x = r[k];
+#pragma unroll
+      for (int m = 32; m >= 1; m >>= 1) {
+        const double o = __shfl_xor(x, m, 64);
+        x = lo ? fmin(x, o) : fmax(x, o);
+      }
+      if (lane == 0) s[wave][k] = x;
+    }
   }
   __syncthreads();
-  if (t < nbody * 6) {
+  if (t < nv) {
     const bool lo = (t % 6) < 3;
-    double r = s[0][t];
-    for (int w = 1; w < PD_BLOCK / 64; ++w) r = lo ? fmin(r, s[w][t]) : fmax(r, s[w][t]);
-    out[t] = r;
+    double x = s[0][t];
+    for (int w = 1; w < PD_BLOCK / 64; ++w) x = lo ? fmin(x, s[w][t]) : fmax(x, s[w][t]);
+    out[t] = x;
   }
 }
 
