@@ -81,14 +81,20 @@ def test_bond_overflow_is_an_error():
     ctx.close()
 
 
-@pytest.mark.parametrize("integrator", [0, 1])
-def test_assemble_bit_exact(integrator):
-    say("case assemble integrator", integrator)
-    sys_, ctx = deformed_pair()
+@pytest.mark.parametrize("integrator,amp", [(0, 0.03), (1, 0.03), (0, 0.3)])
+def test_assemble_bit_exact(integrator, amp):
+    """amp 0.3: displacements of 0.3 spacings break a large share of the
+    bonds, so rows hold several breaking bonds at every position of the
+    assembly's three-slot trips and of its tail."""
+    say("case assemble integrator", integrator, "amp", amp)
+    sys_, ctx = deformed_pair(amp=amp)
     kw = dict(dt=1e-4, clock_scale=0.4, integrator=integrator)
+    live0 = int((sys_.top + 1).sum())
     st_o = sys_.assemble(orc.make_params(**kw))
     st_g = ctx.assemble(pds.make_params(track_gap=1, **kw))
     assert st_o.broken_bonds > 0, "the case must exercise breaking"
+    if amp > 0.1:
+        assert st_o.broken_bonds > live0 // 5, (st_o.broken_bonds, live0)
     assert st_g.broken_bonds == st_o.broken_bonds
     assert st_g.live_bonds == st_o.live_bonds
     assert st_g.min_gap == st_o.min_gap and (st_g.gap_p, st_g.gap_q) == (st_o.gap_p, st_o.gap_q)
@@ -102,6 +108,24 @@ def test_assemble_bit_exact(integrator):
         rs = sys_.row_start[p]
         k = sys_.top[p] + 1
         assert np.array_equal(vals[p, :k], sys_.val[rs:rs + k]), p
+    ctx.close()
+
+
+def test_assemble_ties_exact_lattice():
+    """An unjittered, undisplaced lattice: every bond of the interior has the
+    same stretch, so the nearest-to-threshold bond is decided by the order
+    in which bonds are processed (the first one wins)."""
+    say("case assemble ties")
+    h = 0.01
+    x4 = lattice(9, 8, 7, h, 0.0, 0)
+    sys_, ctx = make_pair(x4, [material(h)])
+    kw = dict(dt=1e-4, clock_scale=0.4)
+    st_o = sys_.assemble(orc.make_params(**kw))
+    st_g = ctx.assemble(pds.make_params(track_gap=1, **kw))
+    assert st_g.min_gap == st_o.min_gap and (st_g.gap_p, st_g.gap_q) == (st_o.gap_p, st_o.gap_q)
+    compare_bonds(sys_, ctx)
+    assert np.array_equal(ctx.download("force")[:, :3], sys_.f[:, :3])
+    assert np.array_equal(ctx.download("rhs")[:, :3], sys_.b[:, :3])
     ctx.close()
 
 
