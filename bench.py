@@ -72,6 +72,25 @@ def read_traffic(precision=0):
     return None
 
 
+def read_counters(key, kernel):
+    """The counter row of `kernel` in a committed rocprof summary (the
+    columns of its utilisation table), as fractions and per-launch counts;
+    None when the summary holds no such row."""
+    try:
+        with open(TRAFFIC_FILES[key]) as f:
+            for line in f:
+                cells = [c.strip() for c in line.strip().strip("|").split("|")]
+                if len(cells) == 12 and cells[0] == f"`pd::{kernel}`":
+                    v = [float(c) for c in cells[1:]]
+                    return {"valu_busy": v[2] / 100.0, "waves_waiting": v[3] / 100.0,
+                            "valu_instr_per_launch": v[4] * 1e6,
+                            "ta_busy": v[7] / 100.0, "td_busy": v[8] / 100.0,
+                            "source": os.path.relpath(TRAFFIC_FILES[key], ROOT)}
+    except (OSError, ValueError):
+        pass
+    return None
+
+
 def new_context(desc, sc):
     n = desc["num_pd_particle"]
     s00 = pds.stretch_init(n, desc["s0"], desc["stretch_seed"])
@@ -361,6 +380,8 @@ def mf_roofline(r):
         "fp64_frac": tflops / FP64_PEAK_TFLOPS,
         "avg_launch_ms": avg_ms,
         "launches": r["spmv_n"],
+        # what holds it below that ceiling, from the committed counter runs
+        "counters": read_counters("mf", "k_cg_spmv_mf"),
     }
 
 

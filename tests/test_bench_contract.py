@@ -39,6 +39,10 @@ def test_matrix_free_roofline_is_hbm_bound():
     assert abs(rf["fp64_frac"] - rf["fp64_tflops"] / 78.6) < 1e-12
     # the traffic comes from the committed counter summary
     assert rf["traffic"] is not None and rf["traffic"] > r["spmv_bytes"]
+    # and so do the utilisation counters that say what limits it
+    c = rf["counters"]
+    assert c is not None and 0.0 < c["valu_busy"] < 1.0 and 0.0 < c["td_busy"] <= 1.0
+    assert 50 < c["valu_instr_per_launch"] * 64 / r["live"] < 300
 
 
 def test_step_rooflines():
