@@ -846,9 +846,11 @@ int run_solve(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st, int resum
   int launched = resumed;
   int syncs = 0;
   int store_at = -1;  // the launch that also wrote the blocks
-  // launch one iteration fewer than the previous solve needed, then single
-  // iterations, then doubling chunks: launches after convergence are rare
-  int chunk = resumed ? 1 : std::max(1, c->last_iters - 1);
+  // launch as many iterations as the previous solve needed (steps in a run
+  // mostly repeat their count, and a launch after convergence returns at
+  // once, cheaper than a host round trip), then single iterations, then
+  // doubling chunks
+  int chunk = resumed ? 1 : std::max(1, c->last_iters);
   int next = 1;
   int singles = 2;
   const bool timing = prm->timing != 0 && !resumed;
