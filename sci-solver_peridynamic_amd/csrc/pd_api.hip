@@ -88,6 +88,7 @@ struct pd_ctx {
   uint32_t* hist = nullptr;
   double* bbox_part = nullptr;
   double* bbox = nullptr;
+  pd::ContactPlan* plan = nullptr;  // the step's contact plan (k_contact_plan)
   int32_t *cell_start = nullptr, *cell_end = nullptr;
   int64_t cell_cap = 0;
   int32_t* counts = nullptr;
@@ -116,6 +117,7 @@ struct pd_ctx {
   double* h_gap = nullptr;
   int32_t* h_gap_pq = nullptr;
   uint32_t* h_zone_count = nullptr;
+  pd::ContactPlan* h_plan = nullptr;
 
   std::vector<hipEvent_t> ev;  // event pool for timing
 
@@ -558,21 +560,21 @@ pd::CgArgs cg_args(pd_ctx* c, int precond) {
 int enqueue_contact(pd_ctx* c, const pd_step_params* prm) {
   c->last_grid = 0;
   const int nb = c->nbody;
-  double bb[PD_MAX_BODIES][6];
-  double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
-  if (int e = read_bboxes(c, bb)) return e;
-  for (int b = 0; b < nb; ++b) {
-    for (int k = 0; k < 3; ++k) {
-      mn[k] = std::min(mn[k], bb[b][k]);
-      mx[k] = std::max(mx[k], bb[b][3 + k]);
-    }
-  }
-  for (int k = 0; k < 3; ++k)
-    if (!(isfinite(mn[k]) && isfinite(mx[k])))
-      return fail(c, PD_ERR_ARG, "non-finite particle position");
   const double cell = prm->contact_cell;
   if (prm->contact_mode == 1) {
     // every pair: the grid over all particles
+    double bb[PD_MAX_BODIES][6];
+    double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    if (int e = read_bboxes(c, bb)) return e;
+    for (int b = 0; b < nb; ++b) {
+      for (int k = 0; k < 3; ++k) {
+        mn[k] = std::min(mn[k], bb[b][k]);
+        mx[k] = std::max(mx[k], bb[b][3 + k]);
+      }
+    }
+    for (int k = 0; k < 3; ++k)
+      if (!(isfinite(mn[k]) && isfinite(mx[k])))
+        return fail(c, PD_ERR_ARG, "non-finite particle position");
     pd::GridView g = {};
     int32_t c0[3] = {0, 0, 0}, dims[3];
     for (int k = 0; k < 3; ++k) {
@@ -586,62 +588,45 @@ int enqueue_contact(pd_ctx* c, const pd_step_params* prm) {
     CHECK_LAUNCH(c, "contact");
     return PD_OK;
   }
-  // a small widening keeps pairs at a distance rounding to Rc inside
+  // The body boxes, the zones (boxes widened by the contact radius, a small
+  // widening keeping pairs at a distance rounding to Rc inside) and the
+  // sub-grid box are formed on the device and read back with the candidate
+  // count, in one host wait.  A candidate of body o lies in o's own box and
+  // in the zone of another body b, so every candidate's cell lies in the
+  // box over those intersections; the sub-grid spans just that box.  Its
+  // cells keep their full-grid coordinates and the cells it leaves out hold
+  // no candidate, so each contact sum runs over the same members in the
+  // same order.
   const double w = prm->contact_radius * (1.0 + 1e-6);
-  pd::ZoneBoxes z;
-  z.nbody = nb;
-  double ulo[3] = {INFINITY, INFINITY, INFINITY}, uhi[3] = {-INFINITY, -INFINITY, -INFINITY};
-  for (int b = 0; b < nb; ++b)
-    for (int k = 0; k < 3; ++k) {
-      z.lo[b][k] = bb[b][k] - w;  // empty bodies keep lo = +inf: no zone
-      z.hi[b][k] = bb[b][3 + k] + w;
-      ulo[k] = std::min(ulo[k], z.lo[b][k]);
-      uhi[k] = std::max(uhi[k], z.hi[b][k]);
-    }
-  // A candidate of body o lies in o's own box and in the zone of another
-  // body b, so every candidate's cell lies in the box over those
-  // intersections; the sub-grid spans just that box.  Its cells keep their
-  // full-grid coordinates and the cells it leaves out hold no candidate, so
-  // each contact sum runs over the same members in the same order.
-  double glo[3] = {INFINITY, INFINITY, INFINITY}, ghi[3] = {-INFINITY, -INFINITY, -INFINITY};
-  for (int b = 0; b < nb; ++b)
-    for (int o = 0; o < nb; ++o) {
-      if (o == b) continue;
-      double lo[3], hi[3];
-      bool meet = true;
-      for (int k = 0; k < 3; ++k) {
-        lo[k] = std::max(z.lo[b][k], bb[o][k]);
-        hi[k] = std::min(z.hi[b][k], bb[o][3 + k]);
-        meet = meet && lo[k] <= hi[k];
-      }
-      if (!meet) continue;
-      for (int k = 0; k < 3; ++k) {
-        glo[k] = std::min(glo[k], lo[k]);
-        ghi[k] = std::max(ghi[k], hi[k]);
-      }
-    }
+  pd::launch_bbox_bodies(c->x, c->body, c->n, c->nbody, c->bbox_part, c->bbox, c->stream);
+  pd::launch_contact_plan(c->bbox, nb, w, c->plan, c->stream);
   const size_t tw = pd::zone_tile_words(c->n);
-  pd::launch_zone_select(c->x, c->body, c->n, z, c->zone_cnt, c->cand, c->stream);
+  pd::launch_zone_select(c->x, c->body, c->n, &c->plan->z, c->zone_cnt, c->cand, c->stream);
   CHECK_LAUNCH(c, "contact candidates");
+  HIPCHK(c, hipMemcpyAsync(c->h_plan, c->plan, sizeof(pd::ContactPlan), hipMemcpyDeviceToHost,
+                           c->stream));
   uint32_t m = 0;
-  if (c->n > 0) {
+  if (c->n > 0)
     HIPCHK(c, hipMemcpyAsync(c->h_zone_count, c->zone_cnt + tw - 1, sizeof(uint32_t),
                              hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    m = *c->h_zone_count;
-  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const pd::ContactPlan& P = *c->h_plan;
+  for (int k = 0; k < 3; ++k)
+    if (!(isfinite(P.mn[k]) && isfinite(P.mx[k])))
+      return fail(c, PD_ERR_ARG, "non-finite particle position");
+  if (c->n > 0) m = *c->h_zone_count;
   pd::GridView g = {};
   int32_t c0[3] = {0, 0, 0}, dims[3] = {1, 1, 1};
   if (m > 0) {
     for (int k = 0; k < 3; ++k) {
-      const bool tight = glo[k] <= ghi[k];  // always, when a candidate exists
-      const double lo = tight ? glo[k] : std::max(ulo[k], mn[k]);
-      const double hi = tight ? ghi[k] : std::min(uhi[k], mx[k]);
-      c0[k] = cell_index(lo, mn[k], cell);
-      dims[k] = cell_index(hi, mn[k], cell) - c0[k] + 1;
+      const bool tight = P.glo[k] <= P.ghi[k];  // always, when a candidate exists
+      const double lo = tight ? P.glo[k] : std::max(P.ulo[k], P.mn[k]);
+      const double hi = tight ? P.ghi[k] : std::min(P.uhi[k], P.mx[k]);
+      c0[k] = cell_index(lo, P.mn[k], cell);
+      dims[k] = cell_index(hi, P.mn[k], cell) - c0[k] + 1;
       if (dims[k] < 1) dims[k] = 1;
     }
-    if (int e = grid_over(c, cell, mn, c0, dims, c->cand, (int32_t)m, &g)) return e;
+    if (int e = grid_over(c, cell, P.mn, c0, dims, c->cand, (int32_t)m, &g)) return e;
   }
   pd::launch_contact(c->n, c->x, c->body, c->act, g, c->cand, (int32_t)m, prm->contact_radius,
                      prm->contact_stiffness, prm->contact_volume, 0, c->fc, c->stream);
@@ -1583,6 +1568,7 @@ int pd_ctx_create(const pd_config* cfg, pd_ctx** out) {
   ALLOC(c->hist, pd::radix_hist_words(c->n));
   ALLOC(c->bbox_part, 1024 * 6 * PD_MAX_BODIES);
   ALLOC(c->bbox, 6 * PD_MAX_BODIES);
+  ALLOC(c->plan, 1);
   ALLOC(c->counts, n);
   ALLOC(c->zone_cnt, pd::zone_tile_words(c->n));
   ALLOC(c->cand, n);
@@ -1597,7 +1583,8 @@ int pd_ctx_create(const pd_config* cfg, pd_ctx** out) {
       hipHostMalloc((void**)&c->h_bbox, 6 * PD_MAX_BODIES * sizeof(double), hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc((void**)&c->h_gap, sizeof(double), hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc((void**)&c->h_gap_pq, 2 * sizeof(int32_t), hipHostMallocDefault) != hipSuccess ||
-      hipHostMalloc((void**)&c->h_zone_count, sizeof(uint32_t), hipHostMallocDefault) != hipSuccess)
+      hipHostMalloc((void**)&c->h_zone_count, sizeof(uint32_t), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&c->h_plan, sizeof(pd::ContactPlan), hipHostMallocDefault) != hipSuccess)
     return bail(fail(c, PD_ERR_HIP, "hipHostMalloc failed"));
   pd_mat_dev hm[PD_MAX_BODIES];
   double hh[PD_MAX_BODIES];
@@ -1629,7 +1616,7 @@ int pd_ctx_destroy(pd_ctx* c) {
   dfree(c->gap_val); dfree(c->gap_q); dfree(c->gpart_v); dfree(c->gpart_p); dfree(c->gpart_q);
   dfree(c->gap_out); dfree(c->gap_pq);
   dfree(c->keys_a); dfree(c->keys_b); dfree(c->idx_a); dfree(c->idx_b); dfree(c->hist);
-  dfree(c->bbox_part); dfree(c->bbox); dfree(c->cell_start); dfree(c->cell_end);
+  dfree(c->bbox_part); dfree(c->bbox); dfree(c->plan); dfree(c->cell_start); dfree(c->cell_end);
   dfree(c->counts); dfree(c->widths); dfree(c->zone_cnt); dfree(c->cand); dfree(c->x_bak); dfree(c->v_bak); dfree(c->top_bak);
   dfree(c->val32); dfree(c->diag32); dfree(c->minv32); dfree(c->r32); dfree(c->z32);
   dfree(c->p32); dfree(c->y32); dfree(c->d32); dfree(c->mx);
@@ -1646,6 +1633,7 @@ int pd_ctx_destroy(pd_ctx* c) {
   if (c->h_gap) (void)hipHostFree(c->h_gap);
   if (c->h_gap_pq) (void)hipHostFree(c->h_gap_pq);
   if (c->h_zone_count) (void)hipHostFree(c->h_zone_count);
+  if (c->h_plan) (void)hipHostFree(c->h_plan);
   for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;

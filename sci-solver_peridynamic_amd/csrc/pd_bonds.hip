@@ -188,18 +188,70 @@ void launch_bbox(const double4* x, const int32_t* body, int32_t n, int32_t body_
 #define CP_ITEMS 4
 #define CP_TILE (PD_BLOCK * CP_ITEMS)
 
-__device__ __forceinline__ bool in_zone(const double4 p, int b, const ZoneBoxes& z) {
-  for (int k = 0; k < z.nbody; ++k) {
+__device__ __forceinline__ bool in_zone(const double4 p, int b, const ZoneBoxes* __restrict__ z) {
+  for (int k = 0; k < z->nbody; ++k) {
     if (k == b) continue;
-    if (p.x >= z.lo[k][0] && p.x <= z.hi[k][0] && p.y >= z.lo[k][1] && p.y <= z.hi[k][1] &&
-        p.z >= z.lo[k][2] && p.z <= z.hi[k][2])
+    if (p.x >= z->lo[k][0] && p.x <= z->hi[k][0] && p.y >= z->lo[k][1] &&
+        p.y <= z->hi[k][1] && p.z >= z->lo[k][2] && p.z <= z->hi[k][2])
       return true;
   }
   return false;
 }
 
+// std::min / std::max as the host forms them (the first operand on ties,
+// so -0 and +0 keep their order)
+__device__ __forceinline__ double first_min(double a, double b) { return b < a ? b : a; }
+__device__ __forceinline__ double first_max(double a, double b) { return a < b ? b : a; }
+
+// The contact plan, one thread: the zones are the body boxes widened by w,
+// and the sub-grid box is the bound of every zone's intersection with
+// another body's box (the order of the host loops it replaces).
+__global__ void k_contact_plan(const double* __restrict__ bb, int nb, double w,
+                               ContactPlan* __restrict__ out) {
+  if (threadIdx.x != 0) return;
+  ContactPlan P;
+  for (int k = 0; k < 3; ++k) {
+    P.mn[k] = P.ulo[k] = P.glo[k] = INFINITY;
+    P.mx[k] = P.uhi[k] = P.ghi[k] = -INFINITY;
+  }
+  for (int b = 0; b < nb; ++b)
+    for (int k = 0; k < 3; ++k) {
+      P.mn[k] = first_min(P.mn[k], bb[6 * b + k]);
+      P.mx[k] = first_max(P.mx[k], bb[6 * b + 3 + k]);
+    }
+  P.z.nbody = nb;
+  for (int b = 0; b < PD_MAX_ZONE_BODIES; ++b)
+    for (int k = 0; k < 3; ++k) {
+      P.z.lo[b][k] = b < nb ? bb[6 * b + k] - w : INFINITY;  // empty bodies: +inf, no zone
+      P.z.hi[b][k] = b < nb ? bb[6 * b + 3 + k] + w : -INFINITY;
+    }
+  for (int b = 0; b < nb; ++b)
+    for (int k = 0; k < 3; ++k) {
+      P.ulo[k] = first_min(P.ulo[k], P.z.lo[b][k]);
+      P.uhi[k] = first_max(P.uhi[k], P.z.hi[b][k]);
+    }
+  for (int b = 0; b < nb; ++b)
+    for (int o = 0; o < nb; ++o) {
+      if (o == b) continue;
+      double lo[3], hi[3];
+      bool meet = true;
+      for (int k = 0; k < 3; ++k) {
+        lo[k] = first_max(P.z.lo[b][k], bb[6 * o + k]);
+        hi[k] = first_min(P.z.hi[b][k], bb[6 * o + 3 + k]);
+        meet = meet && lo[k] <= hi[k];
+      }
+      if (!meet) continue;
+      for (int k = 0; k < 3; ++k) {
+        P.glo[k] = first_min(P.glo[k], lo[k]);
+        P.ghi[k] = first_max(P.ghi[k], hi[k]);
+      }
+    }
+  *out = P;
+}
+
 __global__ __launch_bounds__(PD_BLOCK) void k_zone_count(const double4* x, const int32_t* body,
-                                                        int n, ZoneBoxes z, uint32_t* tile_cnt) {
+                                                        int n, const ZoneBoxes* z,
+                                                        uint32_t* tile_cnt) {
   __shared__ uint32_t c;
   if (threadIdx.x == 0) c = 0;
   __syncthreads();
@@ -240,7 +292,7 @@ __global__ __launch_bounds__(1024) void k_scan_small(uint32_t* v, int g) {
 }
 
 __global__ __launch_bounds__(PD_BLOCK) void k_zone_scatter(const double4* x, const int32_t* body,
-                                                          int n, ZoneBoxes z,
+                                                          int n, const ZoneBoxes* z,
                                                           const uint32_t* tile_off,
                                                           int32_t* out) {
   __shared__ uint32_t wcnt[4];
@@ -264,7 +316,12 @@ __global__ __launch_bounds__(PD_BLOCK) void k_zone_scatter(const double4* x, con
   }
 }
 
-void launch_zone_select(const double4* x, const int32_t* body, int32_t n, const ZoneBoxes& z,
+void launch_contact_plan(const double* bbox, int32_t nbody, double w, ContactPlan* plan,
+                         hipStream_t s) {
+  hipLaunchKernelGGL(k_contact_plan, dim3(1), dim3(64), 0, s, bbox, nbody, w, plan);
+}
+
+void launch_zone_select(const double4* x, const int32_t* body, int32_t n, const ZoneBoxes* z,
                         uint32_t* tile_cnt, int32_t* out, hipStream_t s) {
   const int g = (n + CP_TILE - 1) / CP_TILE;
   if (g == 0) return;

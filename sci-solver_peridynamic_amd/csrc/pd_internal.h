@@ -400,6 +400,15 @@ struct ZoneBoxes {
   double hi[PD_MAX_ZONE_BODIES][3];
 };
 
+// The contact sub-grid plan of a step, formed on the device from the body
+// boxes: the zones, the componentwise bounds of all bodies (mn, mx: the
+// grid origin), of all zones (ulo, uhi) and of the intersections of each
+// zone with another body's box (glo, ghi; glo > ghi when none meet).
+struct ContactPlan {
+  ZoneBoxes z;
+  double mn[3], mx[3], ulo[3], uhi[3], glo[3], ghi[3];
+};
+
 // pd_bonds.hip
 // bounds of the particles of body `body_sel` (all particles if < 0)
 // the bounding boxes of bodies 0 .. nbody - 1: out[b * 6 + k] (lo xyz, hi
@@ -412,8 +421,12 @@ void launch_bbox(const double4* x, const int32_t* body, int32_t n, int32_t body_
 // stable (ascending) list of contact candidates; the count lands in
 // tile_cnt[zone_tile_words(n) - 1]
 void launch_zone_select(const double4* x, const int32_t* body, int32_t n,
-                        const ZoneBoxes& z, uint32_t* tile_cnt, int32_t* out,
+                        const ZoneBoxes* z, uint32_t* tile_cnt, int32_t* out,
                         hipStream_t s);
+// plan from the boxes of bodies 0 .. nbody - 1 (launch_bbox_bodies' out),
+// zones widened by w
+void launch_contact_plan(const double* bbox, int32_t nbody, double w, ContactPlan* plan,
+                         hipStream_t s);
 size_t zone_tile_words(int32_t n);
 // keys of the cells (sub-grid origin cell c0, dims) of particles subset[i]
 // (or i when subset is null); idx[i] receives the particle index
