@@ -598,8 +598,8 @@ int enqueue_contact(pd_ctx* c, const pd_step_params* prm) {
   // no candidate, so each contact sum runs over the same members in the
   // same order.
   const double w = prm->contact_radius * (1.0 + 1e-6);
-  pd::launch_bbox_bodies(c->x, c->body, c->n, c->nbody, c->bbox_part, c->bbox, c->stream);
-  pd::launch_contact_plan(c->bbox, nb, w, c->plan, c->stream);
+  pd::launch_bbox_bodies(c->x, c->body, c->n, c->nbody, c->bbox_part, c->bbox, c->stream,
+                         c->plan, w);
   const size_t tw = pd::zone_tile_words(c->n);
   pd::launch_zone_select(c->x, c->body, c->n, &c->plan->z, c->zone_cnt, c->cand, c->stream);
   CHECK_LAUNCH(c, "contact candidates");
@@ -667,7 +667,7 @@ int ensure_blocks(pd_ctx* c) {
 // and the block-Jacobi inverse the pass has in registers (the next solve is
 // run with fused = true).  With the learned preconditioner (precond 2) the
 // start is the block-Jacobi one; run_solve_pc replaces z = M r and its
-// partial once the learned blocks exist (launch_pc_start), or keeps it for a
+// partial once the learned blocks exist (launch_pc_infer_start), or keeps it for a
 // block-Jacobi probe.
 int enqueue_assemble(pd_ctx* c, const pd_step_params* prm,
                      const pd_step_consts* consts = nullptr, bool fuse = false) {
@@ -1098,7 +1098,12 @@ int enqueue_update(pd_ctx* c, const pd_step_params* prm,
 // The learned blocks M_p of the assembled system into pc_minv (and pc_minv32
 // for the mixed solve); the solve with precond 2 applies them in place of the
 // assembly's block-Jacobi inverse (minv / minv32), which stays intact.
-int apply_learned(pd_ctx* c, const pd_step_params* prm) {
+// The learned blocks M_p of the assembled system (features, then the
+// network).  fused_start: after a fused assembly the same pass also forms
+// the solve's start with them -- z = M r of the residual the assembly
+// formed, p = z, and the <r, z> tile partials in place of the assembly's
+// block-Jacobi ones.
+int apply_learned(pd_ctx* c, const pd_step_params* prm, bool fused_start = false) {
   const size_t npad = (size_t)c->nslices * 64;
   if (!c->pc_feat)
     if (int e = dalloc(c, &c->pc_feat, (size_t)std::max(c->n, 1) * PC_NIN)) return e;
@@ -1123,30 +1128,28 @@ int apply_learned(pd_ctx* c, const pd_step_params* prm) {
   pd::PcNet net;
   net.w = c->pc_w;
   net.p = c->pc_p;
-  pd::launch_pc_infer(c->n, c->pc_feat, net, c->diag, c->act, c->pc_minv,
-                      prm->precision == 1 ? c->pc_minv32 : nullptr, c->stream);
+  float* minv32 = prm->precision == 1 ? c->pc_minv32 : nullptr;
+  if (fused_start) {
+    pd::PcInferStartArgs a;
+    a.n = c->n;
+    a.mixed = prm->precision == 1 ? 1 : 0;
+    a.feat = c->pc_feat;
+    a.net = net;
+    a.diag = c->diag;
+    a.act = c->act;
+    a.minv = c->pc_minv;
+    a.minv32 = minv32;
+    a.r64 = c->r;
+    a.p64 = c->p;
+    a.rec = (c->mf || c->blocks_deferred) ? c->rec : nullptr;
+    a.r32 = c->r32;
+    a.p32 = c->p32;
+    a.part = c->part;
+    pd::launch_pc_infer_start(a, c->stream);
+  } else {
+    pd::launch_pc_infer(c->n, c->pc_feat, net, c->diag, c->act, c->pc_minv, minv32, c->stream);
+  }
   CHECK_LAUNCH(c, "learned preconditioner");
-  return PD_OK;
-}
-
-// The learned preconditioner's start after a fused assembly: z = M r with
-// the learned blocks just written, p = z, and the <r, z> tile partials in
-// place of the assembly's block-Jacobi ones.
-int learned_start(pd_ctx* c, const pd_step_params* prm) {
-  pd::PcStartArgs a;
-  a.n = c->n;
-  a.mixed = prm->precision == 1 ? 1 : 0;
-  a.act = c->act;
-  a.minv = c->pc_minv;
-  a.minv32 = c->pc_minv32;
-  a.r64 = c->r;
-  a.p64 = c->p;
-  a.rec = (c->mf || c->blocks_deferred) ? c->rec : nullptr;
-  a.r32 = c->r32;
-  a.p32 = c->p32;
-  a.part = c->part;
-  pd::launch_pc_start(a, c->stream);
-  CHECK_LAUNCH(c, "learned start");
   return PD_OK;
 }
 
@@ -1185,10 +1188,8 @@ int run_solve_pc(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st,
     }
     ++c->pc_since_probe;
   }
-  if (int e = apply_learned(c, prm)) return e;
+  if (int e = apply_learned(c, prm, fused)) return e;
   c->last_learned = true;
-  if (fused)
-    if (int e = learned_start(c, prm)) return e;
   pd_step_params p2 = *prm;
   const double f = prm->learned_fallback > 0.0 ? prm->learned_fallback : 1.5;
   if (c->bj_iters > 0)

@@ -71,6 +71,53 @@ static const int kBboxBlocks = 256;
 // all-body boxes: four workgroups per CU keep the position stream in flight
 static const int kBodyBboxBlocks = 1024;
 
+// std::min / std::max as the host forms them (the first operand on ties,
+// so -0 and +0 keep their order)
+__device__ __forceinline__ double first_min(double a, double b) { return b < a ? b : a; }
+__device__ __forceinline__ double first_max(double a, double b) { return a < b ? b : a; }
+
+// The contact plan (one thread): the zones are the body boxes widened by
+// w, and the sub-grid box is the bound of every zone's intersection with
+// another body's box, formed in the order of the host loops it replaces.
+__device__ void contact_plan(const double* bb, int nb, double w, ContactPlan& P) {
+  for (int k = 0; k < 3; ++k) {
+    P.mn[k] = P.ulo[k] = P.glo[k] = INFINITY;
+    P.mx[k] = P.uhi[k] = P.ghi[k] = -INFINITY;
+  }
+  for (int b = 0; b < nb; ++b)
+    for (int k = 0; k < 3; ++k) {
+      P.mn[k] = first_min(P.mn[k], bb[6 * b + k]);
+      P.mx[k] = first_max(P.mx[k], bb[6 * b + 3 + k]);
+    }
+  P.z.nbody = nb;
+  for (int b = 0; b < PD_MAX_ZONE_BODIES; ++b)
+    for (int k = 0; k < 3; ++k) {
+      P.z.lo[b][k] = b < nb ? bb[6 * b + k] - w : INFINITY;  // empty bodies: +inf, no zone
+      P.z.hi[b][k] = b < nb ? bb[6 * b + 3 + k] + w : -INFINITY;
+    }
+  for (int b = 0; b < nb; ++b)
+    for (int k = 0; k < 3; ++k) {
+      P.ulo[k] = first_min(P.ulo[k], P.z.lo[b][k]);
+      P.uhi[k] = first_max(P.uhi[k], P.z.hi[b][k]);
+    }
+  for (int b = 0; b < nb; ++b)
+    for (int o = 0; o < nb; ++o) {
+      if (o == b) continue;
+      double lo[3], hi[3];
+      bool meet = true;
+      for (int k = 0; k < 3; ++k) {
+        lo[k] = first_max(P.z.lo[b][k], bb[6 * o + k]);
+        hi[k] = first_min(P.z.hi[b][k], bb[6 * o + 3 + k]);
+        meet = meet && lo[k] <= hi[k];
+      }
+      if (!meet) continue;
+      for (int k = 0; k < 3; ++k) {
+        P.glo[k] = first_min(P.glo[k], lo[k]);
+        P.ghi[k] = first_max(P.ghi[k], hi[k]);
+      }
+    }
+}
+
 // Bounding boxes of all bodies in one pass: per block (lo xyz, hi xyz) for
 // each body b < PD_MAX_BODIES at part[(block * PD_MAX_BODIES + b) * 6 + k].
 // min / max are exact, so the result does not depend on the order.
@@ -122,56 +169,56 @@ __global__ __launch_bounds__(PD_BLOCK) void k_bbox_bodies_partial(const double4*
   }
 }
 
-// each thread folds the block partials t, t + 256, ... (all values of a
-// partial are adjacent, so the loads of a block go out together), then
-// every value of the bodies that exist is folded by wave butterflies and
-// across the four waves; the other bodies' entries are left unwritten
+// each thread folds the block partials t, t + 256, ... (g <= 4 PD_BLOCK),
+// then every value of the bodies that exist is folded by wave butterflies
+// and across the four waves; the other bodies' entries are left unwritten.
+// plan non-null: thread 0 then forms the contact plan from the boxes
+// (contact_plan, zones widened by w).
 __global__ __launch_bounds__(PD_BLOCK) void k_bbox_bodies_final(const double* part, int g,
-                                                               int nbody, double* out) {
-  constexpr int NV = PD_MAX_BODIES * 6;
-  __shared__ double s[PD_BLOCK / 64][NV];
+                                                               int nbody, double* out,
+                                                               ContactPlan* plan, double w) {
+  __shared__ double s[PD_BLOCK / 64][PD_MAX_BODIES * 6];
+  __shared__ double box[PD_MAX_BODIES * 6];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const int nv = nbody * 6;
-  double r[NV];
-#pragma unroll
-  for (int k = 0; k < NV; ++k) r[k] = (k % 6) < 3 ? INFINITY : -INFINITY;
-  for (int i = t; i < g; i += PD_BLOCK) {
-#pragma unroll
-    for (int k = 0; k < NV; ++k) {
-      if (k < nv) {  // uniform
-        const double v = part[i * NV + k];
-        r[k] = (k % 6) < 3 ? fmin(r[k], v) : fmax(r[k], v);
-      }
+  for (int k = 0; k < nbody * 6; ++k) {
+    const bool lo = (k % 6) < 3;
+    double r = lo ? INFINITY : -INFINITY;
+    for (int i = t; i < g; i += PD_BLOCK) {
+      const double v = part[i * PD_MAX_BODIES * 6 + k];
+      r = lo ? fmin(r, v) : fmax(r, v);
     }
-  }
 #pragma unroll
-  for (int k = 0; k < NV; ++k) {
-    if (k < nv) {  // uniform
-      const bool lo = (k % 6) < 3;
-      double x = r[k];
-#pragma unroll
-      for (int m = 32; m >= 1; m >>= 1) {
-        const double o = __shfl_xor(x, m, 64);
-        x = lo ? fmin(x, o) : fmax(x, o);
-      }
-      if (lane == 0) s[wave][k] = x;
+    for (int m = 32; m >= 1; m >>= 1) {
+      const double o = __shfl_xor(r, m, 64);
+      r = lo ? fmin(r, o) : fmax(r, o);
     }
+    if (lane == 0) s[wave][k] = r;
   }
   __syncthreads();
-  if (t < nv) {
+  if (t < nbody * 6) {
     const bool lo = (t % 6) < 3;
-    double x = s[0][t];
-    for (int w = 1; w < PD_BLOCK / 64; ++w) x = lo ? fmin(x, s[w][t]) : fmax(x, s[w][t]);
-    out[t] = x;
+    double r = s[0][t];
+    for (int w = 1; w < PD_BLOCK / 64; ++w) r = lo ? fmin(r, s[w][t]) : fmax(r, s[w][t]);
+    out[t] = r;
+    box[t] = r;
   }
+  if (!plan) return;  // uniform
+  __shared__ ContactPlan sp;
+  __syncthreads();
+  if (t == 0) contact_plan(box, nbody, w, sp);
+  __syncthreads();
+  static_assert(sizeof(ContactPlan) % 8 == 0, "the plan is copied in 8-byte words");
+  const uint64_t* src = (const uint64_t*)&sp;
+  uint64_t* dst = (uint64_t*)plan;
+  for (int i = t; i < (int)(sizeof(ContactPlan) / 8); i += PD_BLOCK) dst[i] = src[i];
 }
 
 void launch_bbox_bodies(const double4* x, const int32_t* body, int32_t n, int32_t nbody,
-                        double* part, double* out, hipStream_t s) {
+                        double* part, double* out, hipStream_t s, ContactPlan* plan, double w) {
   hipLaunchKernelGGL(k_bbox_bodies_partial, dim3(kBodyBboxBlocks), dim3(PD_BLOCK), 0, s, x,
                      body, n, nbody, part);
   hipLaunchKernelGGL(k_bbox_bodies_final, dim3(1), dim3(PD_BLOCK), 0, s, part, kBodyBboxBlocks,
-                     nbody, out);
+                     nbody, out, plan, w);
 }
 
 void launch_bbox(const double4* x, const int32_t* body, int32_t n, int32_t body_sel,
@@ -196,57 +243,6 @@ __device__ __forceinline__ bool in_zone(const double4 p, int b, const ZoneBoxes*
       return true;
   }
   return false;
-}
-
-// std::min / std::max as the host forms them (the first operand on ties,
-// so -0 and +0 keep their order)
-__device__ __forceinline__ double first_min(double a, double b) { return b < a ? b : a; }
-__device__ __forceinline__ double first_max(double a, double b) { return a < b ? b : a; }
-
-// The contact plan, one thread: the zones are the body boxes widened by w,
-// and the sub-grid box is the bound of every zone's intersection with
-// another body's box (the order of the host loops it replaces).
-__global__ void k_contact_plan(const double* __restrict__ bb, int nb, double w,
-                               ContactPlan* __restrict__ out) {
-  if (threadIdx.x != 0) return;
-  ContactPlan P;
-  for (int k = 0; k < 3; ++k) {
-    P.mn[k] = P.ulo[k] = P.glo[k] = INFINITY;
-    P.mx[k] = P.uhi[k] = P.ghi[k] = -INFINITY;
-  }
-  for (int b = 0; b < nb; ++b)
-    for (int k = 0; k < 3; ++k) {
-      P.mn[k] = first_min(P.mn[k], bb[6 * b + k]);
-      P.mx[k] = first_max(P.mx[k], bb[6 * b + 3 + k]);
-    }
-  P.z.nbody = nb;
-  for (int b = 0; b < PD_MAX_ZONE_BODIES; ++b)
-    for (int k = 0; k < 3; ++k) {
-      P.z.lo[b][k] = b < nb ? bb[6 * b + k] - w : INFINITY;  // empty bodies: +inf, no zone
-      P.z.hi[b][k] = b < nb ? bb[6 * b + 3 + k] + w : -INFINITY;
-    }
-  for (int b = 0; b < nb; ++b)
-    for (int k = 0; k < 3; ++k) {
-      P.ulo[k] = first_min(P.ulo[k], P.z.lo[b][k]);
-      P.uhi[k] = first_max(P.uhi[k], P.z.hi[b][k]);
-    }
-  for (int b = 0; b < nb; ++b)
-    for (int o = 0; o < nb; ++o) {
-      if (o == b) continue;
-      double lo[3], hi[3];
-      bool meet = true;
-      for (int k = 0; k < 3; ++k) {
-        lo[k] = first_max(P.z.lo[b][k], bb[6 * o + k]);
-        hi[k] = first_min(P.z.hi[b][k], bb[6 * o + 3 + k]);
-        meet = meet && lo[k] <= hi[k];
-      }
-      if (!meet) continue;
-      for (int k = 0; k < 3; ++k) {
-        P.glo[k] = first_min(P.glo[k], lo[k]);
-        P.ghi[k] = first_max(P.ghi[k], hi[k]);
-      }
-    }
-  *out = P;
 }
 
 __global__ __launch_bounds__(PD_BLOCK) void k_zone_count(const double4* x, const int32_t* body,
@@ -314,11 +310,6 @@ __global__ __launch_bounds__(PD_BLOCK) void k_zone_scatter(const double4* x, con
     __syncthreads();
     if (threadIdx.x == 0) run += wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
   }
-}
-
-void launch_contact_plan(const double* bbox, int32_t nbody, double w, ContactPlan* plan,
-                         hipStream_t s) {
-  hipLaunchKernelGGL(k_contact_plan, dim3(1), dim3(64), 0, s, bbox, nbody, w, plan);
 }
 
 void launch_zone_select(const double4* x, const int32_t* body, int32_t n, const ZoneBoxes* z,

@@ -255,14 +255,18 @@ void launch_pc_features(const PcFeatArgs& a, hipStream_t s);
 // minv32 is non-null, its float copy
 void launch_pc_infer(int32_t n, const float* feat, const PcNet& net, const double* diag,
                      const int32_t* act, double* minv, float* minv32, hipStream_t s);
-// The solve's start with the learned blocks after an assembly that formed
-// the first residual (fuse): z = M r, p = z, the <r, z> tile partials.
-struct PcStartArgs {
+// launch_pc_infer and the solve's start with the learned blocks in one
+// pass, after an assembly that formed the first residual (fuse): z = M r,
+// p = z, the <r, z> tile partials
+struct PcInferStartArgs {
   int32_t n;
   int32_t mixed;
+  const float* feat;
+  PcNet net;
+  const double* diag;
   const int32_t* act;
-  const double* minv;    // FP64 planes (mixed 0)
-  const float* minv32;   // float planes (mixed 1)
+  double* minv;          // the learned blocks (FP64)
+  float* minv32;         // and rounded to float (mixed 1)
   const double4* r64;
   double4* p64;
   double* rec;           // FP64: the gather record's p part, when in use
@@ -270,7 +274,7 @@ struct PcStartArgs {
   float4* p32;
   double* part;          // the solve's tile partials
 };
-void launch_pc_start(const PcStartArgs& a, hipStream_t s);
+void launch_pc_infer_start(const PcInferStartArgs& a, hipStream_t s);
 
 // training samples: feat[NIN], D (6, float), v = (A e)_p (3), e_p (3)
 #define PC_SAMPLE 28
@@ -414,8 +418,10 @@ struct ContactPlan {
 // the bounding boxes of bodies 0 .. nbody - 1: out[b * 6 + k] (lo xyz, hi
 // xyz; +inf / -inf for bodies without particles); part holds
 // 1024 * PD_MAX_BODIES * 6
+// plan non-null: the contact plan of the boxes too, zones widened by w
 void launch_bbox_bodies(const double4* x, const int32_t* body, int32_t n, int32_t nbody,
-                        double* part, double* out, hipStream_t s);
+                        double* part, double* out, hipStream_t s,
+                        ContactPlan* plan = nullptr, double w = 0.0);
 void launch_bbox(const double4* x, const int32_t* body, int32_t n, int32_t body_sel,
                  double* part, double* out6, hipStream_t s);
 // stable (ascending) list of contact candidates; the count lands in
@@ -423,10 +429,7 @@ void launch_bbox(const double4* x, const int32_t* body, int32_t n, int32_t body_
 void launch_zone_select(const double4* x, const int32_t* body, int32_t n,
                         const ZoneBoxes* z, uint32_t* tile_cnt, int32_t* out,
                         hipStream_t s);
-// plan from the boxes of bodies 0 .. nbody - 1 (launch_bbox_bodies' out),
-// zones widened by w
-void launch_contact_plan(const double* bbox, int32_t nbody, double w, ContactPlan* plan,
-                         hipStream_t s);
+
 size_t zone_tile_words(int32_t n);
 // keys of the cells (sub-grid origin cell c0, dims) of particles subset[i]
 // (or i when subset is null); idx[i] receives the particle index

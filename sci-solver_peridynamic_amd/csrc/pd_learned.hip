@@ -265,6 +265,38 @@ __device__ __forceinline__ bf16x8 load_x(const float* f, const float* P) {
   return x;
 }
 
+// M_p of row p from the network's outputs o (the lane's row), written to
+// minv (and, rounded, to minv32 when non-null); m returns the FP64 values.
+__device__ __forceinline__ void learned_row(int p, const float* o, const float* P,
+                                            const double* diag, double* minv, float* minv32,
+                                            double m[6]) {
+  const int s = p >> 6, lane = p & 63;
+  const double* dg = diag + (int64_t)s * 6 * 64 + lane;
+  const double d[6] = {dg[0], dg[64], dg[128], dg[192], dg[256], dg[320]};
+  // the block algebra in float, as in training (the network is a BF16
+  // approximation and any SPD block serves; the solve takes M as given)
+  const float df[6] = {(float)d[0], (float)d[1], (float)d[2], (float)d[3], (float)d[4],
+                       (float)d[5]};
+  float ci[6], L[6], mf6[6];
+  bool cl[3];
+  if (chol_inv(df, ci)) {
+    make_l(o, L, cl);
+    learned_block(L, P[PC_OFF_EPS], ci, mf6);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) m[c] = (double)mf6[c];
+  } else {  // a block without a Cholesky factor keeps block Jacobi
+    bj_inverse(d, m);
+  }
+  double* mo = minv + (int64_t)s * 6 * 64 + lane;
+#pragma unroll
+  for (int c = 0; c < 6; ++c) mo[c * 64] = m[c];
+  if (minv32) {
+    float* mf = minv32 + (int64_t)s * 6 * 64 + lane;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) mf[c * 64] = (float)m[c];
+  }
+}
+
 __global__ __launch_bounds__(PD_BLOCK) void k_pc_infer(int n, const float* feat, PcNet net,
                                                       const double* diag, const int32_t* act,
                                                       double* minv, float* minv32) {
@@ -283,53 +315,44 @@ __global__ __launch_bounds__(PD_BLOCK) void k_pc_infer(int n, const float* feat,
   if (l >= 32) return;
   const int p = row0 + l;
   if (p >= n || act[p] == PD_INVISIBLE) return;
-  const int s = p >> 6, lane = p & 63;
-  const double* dg = diag + (int64_t)s * 6 * 64 + lane;
-  const double d[6] = {dg[0], dg[64], dg[128], dg[192], dg[256], dg[320]};
-  // the block algebra in float, as in training (the network is a BF16
-  // approximation and any SPD block serves; the solve takes M as given)
-  const float df[6] = {(float)d[0], (float)d[1], (float)d[2], (float)d[3], (float)d[4],
-                       (float)d[5]};
-  float ci[6], L[6], mf6[6];
   double m[6];
-  bool cl[3];
-  if (chol_inv(df, ci)) {
-    make_l(outs[w][l], L, cl);
-    learned_block(L, P[PC_OFF_EPS], ci, mf6);
-#pragma unroll
-    for (int c = 0; c < 6; ++c) m[c] = (double)mf6[c];
-  } else {  // a block without a Cholesky factor keeps block Jacobi
-    bj_inverse(d, m);
-  }
-  double* mo = minv + (int64_t)s * 6 * 64 + lane;
-#pragma unroll
-  for (int c = 0; c < 6; ++c) mo[c * 64] = m[c];
-  if (minv32) {
-    float* mf = minv32 + (int64_t)s * 6 * 64 + lane;
-#pragma unroll
-    for (int c = 0; c < 6; ++c) mf[c * 64] = (float)m[c];
-  }
+  learned_row(p, outs[w][l], P, diag, minv, minv32, m);
 }
 
-
-// The solver's start after a fused assembly when the learned blocks replace
-// block Jacobi: z = M r of the residual the assembly formed, p = z (FP64: and
-// the p part of the gather record), and the <r, z> tile partial in the slot
-// the assembly keeps for it (FP64: part[tile]; mixed, on the float vectors:
-// part[ntiles + tile]).  The arithmetic is the assembly's block-Jacobi start
-// with M_p in place of the block-Jacobi inverse.
+// k_pc_infer and the learned solve's start in one pass, after a fused
+// assembly: a workgroup takes the 256 rows of one solver tile (each wave
+// two MLP passes of 32 rows), every lane forms the M_p of its row and then
+// z = M_p r (mixed: with M_p rounded to float, on the float residual),
+// p = z (FP64: and the gather record's p part) and its term of the <r, z>
+// tile partial, folded by the solver's tile tree into the slot the
+// assembly's block-Jacobi start used (FP64: part[tile]; mixed:
+// part[ntiles + tile]).
 template <bool MIXED>
-__global__ __launch_bounds__(PD_BLOCK) void k_pc_start(PcStartArgs a) {
+__global__ __launch_bounds__(PD_BLOCK) void k_pc_infer_start(PcInferStartArgs a) {
+  __shared__ __bf16 h1[4][32][kH];
+  __shared__ __bf16 h2[4][32][kH];
+  __shared__ float outs[4][64][PC_NOUT];
   __shared__ double lds4[4];
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int r = l & 31;
   const int tile = blockIdx.x;
-  const int p = tile * PD_BLOCK + threadIdx.x;
+  const int row0 = tile * PD_BLOCK + w * 64;
+  const int n = a.n;
+  const float* P = a.net.p;
+#pragma unroll
+  for (int half = 0; half < 2; ++half) {
+    const int pr = row0 + 32 * half + r < n ? row0 + 32 * half + r : n - 1;
+    const bf16x8 x = load_x(a.feat + (int64_t)pr * PC_NIN, P);
+    mlp_forward(x, a.net.w, P, h1[w], h2[w], outs[w] + 32 * half);
+  }
+  const int p = row0 + l;
   double t = 0.0;
-  if (p < a.n && a.act[p] != PD_INVISIBLE) {
-    const int s = p >> 6, lane = p & 63;
-    if constexpr (MIXED) {
-      const float* mo = a.minv32 + (int64_t)s * 6 * 64 + lane;
-      const float m0 = mo[0], m1 = mo[64], m2 = mo[128], m3 = mo[192], m4 = mo[256],
-                  m5 = mo[320];
+  if (p < n && a.act[p] != PD_INVISIBLE) {
+    double m[6];
+    learned_row(p, outs[w][l], P, a.diag, a.minv, MIXED ? a.minv32 : nullptr, m);
+    if constexpr (MIXED) {  // z = M32 r32 with the float blocks just written
+      const float m0 = (float)m[0], m1 = (float)m[1], m2 = (float)m[2], m3 = (float)m[3],
+                  m4 = (float)m[4], m5 = (float)m[5];
       const float4 h = a.r32[p];
       const float z0 = m0 * h.x + m1 * h.y + m2 * h.z;
       const float z1 = m1 * h.x + m3 * h.y + m4 * h.z;
@@ -337,13 +360,10 @@ __global__ __launch_bounds__(PD_BLOCK) void k_pc_start(PcStartArgs a) {
       a.p32[p] = make_float4(z0, z1, z2, 0.0f);
       t = ((double)h.x * (double)z0 + (double)h.y * (double)z1) + (double)h.z * (double)z2;
     } else {
-      const double* mo = a.minv + (int64_t)s * 6 * 64 + lane;
-      const double m0 = mo[0], m1 = mo[64], m2 = mo[128], m3 = mo[192], m4 = mo[256],
-                   m5 = mo[320];
-      const double4 r = a.r64[p];
-      const double z0 = m0 * r.x + m1 * r.y + m2 * r.z;
-      const double z1 = m1 * r.x + m3 * r.y + m4 * r.z;
-      const double z2 = m2 * r.x + m4 * r.y + m5 * r.z;
+      const double4 rr = a.r64[p];
+      const double z0 = m[0] * rr.x + m[1] * rr.y + m[2] * rr.z;
+      const double z1 = m[1] * rr.x + m[3] * rr.y + m[4] * rr.z;
+      const double z2 = m[2] * rr.x + m[4] * rr.y + m[5] * rr.z;
       a.p64[p] = make_double4(z0, z1, z2, 0.0);
       if (a.rec) {
         double* rp = a.rec + 6 * (int64_t)p;
@@ -351,12 +371,13 @@ __global__ __launch_bounds__(PD_BLOCK) void k_pc_start(PcStartArgs a) {
         rp[4] = z1;
         rp[5] = z2;
       }
-      t = (r.x * z0 + r.y * z1) + r.z * z2;
+      t = (rr.x * z0 + rr.y * z1) + rr.z * z2;
     }
   }
   const double v = tile_sum(t, lds4);
   if (threadIdx.x == 0) a.part[(MIXED ? (int64_t)gridDim.x : 0) + tile] = v;
 }
+
 
 // random-sign probe vectors
 __global__ __launch_bounds__(PD_BLOCK) void k_pc_probe(int n, uint64_t seed, double4* e) {
@@ -692,12 +713,12 @@ void launch_pc_infer(int32_t n, const float* feat, const PcNet& net, const doubl
   if (n) hipLaunchKernelGGL(k_pc_infer, dim3((n + 127) / 128), dim3(PD_BLOCK), 0, s, n, feat, net,
                             diag, act, minv, minv32);
 }
-void launch_pc_start(const PcStartArgs& a, hipStream_t s) {
+void launch_pc_infer_start(const PcInferStartArgs& a, hipStream_t s) {
   if (!a.n) return;
   if (a.mixed)
-    hipLaunchKernelGGL(k_pc_start<true>, dim3(blocks_of(a.n)), dim3(PD_BLOCK), 0, s, a);
+    hipLaunchKernelGGL(k_pc_infer_start<true>, dim3(blocks_of(a.n)), dim3(PD_BLOCK), 0, s, a);
   else
-    hipLaunchKernelGGL(k_pc_start<false>, dim3(blocks_of(a.n)), dim3(PD_BLOCK), 0, s, a);
+    hipLaunchKernelGGL(k_pc_infer_start<false>, dim3(blocks_of(a.n)), dim3(PD_BLOCK), 0, s, a);
 }
 void launch_pc_probe(int32_t n, uint64_t seed, double4* e, hipStream_t s) {
   if (n) hipLaunchKernelGGL(k_pc_probe, dim3(blocks_of(n)), dim3(PD_BLOCK), 0, s, n, seed, e);
