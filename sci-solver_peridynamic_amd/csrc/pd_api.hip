@@ -601,7 +601,8 @@ int enqueue_contact(pd_ctx* c, const pd_step_params* prm) {
   pd::launch_bbox_bodies(c->x, c->body, c->n, c->nbody, c->bbox_part, c->bbox, c->stream,
                          c->plan, w);
   const size_t tw = pd::zone_tile_words(c->n);
-  pd::launch_zone_select(c->x, c->body, c->n, &c->plan->z, c->zone_cnt, c->cand, c->stream);
+  pd::launch_zone_select(c->x, c->body, c->n, &c->plan->z, c->zone_cnt, c->cand, c->stream,
+                         c->fc);
   CHECK_LAUNCH(c, "contact candidates");
   HIPCHK(c, hipMemcpyAsync(c->h_plan, c->plan, sizeof(pd::ContactPlan), hipMemcpyDeviceToHost,
                            c->stream));
@@ -629,7 +630,7 @@ int enqueue_contact(pd_ctx* c, const pd_step_params* prm) {
     if (int e = grid_over(c, cell, P.mn, c0, dims, c->cand, (int32_t)m, &g)) return e;
   }
   pd::launch_contact(c->n, c->x, c->body, c->act, g, c->cand, (int32_t)m, prm->contact_radius,
-                     prm->contact_stiffness, prm->contact_volume, 0, c->fc, c->stream);
+                     prm->contact_stiffness, prm->contact_volume, 0, c->fc, c->stream, true);
   CHECK_LAUNCH(c, "contact");
   return PD_OK;
 }
@@ -817,13 +818,14 @@ int run_solve(pd_ctx* c, const pd_step_params* prm, pd_step_stats* st, int resum
   const pd::CgArgs a = cg_args(c, prm->precond);
   const int ns = c->nspec;
   if (!resumed) {
-    pd::launch_cg_setup(c->st, ns, prm->rtol, prm->max_iter, c->stream);
-    if (fused) {
-      pd::launch_cg_init_fold(a, c->stream);
+    if (fused) {  // the fold sets the solver state up too
+      pd::launch_cg_init_fold(a, prm->rtol, prm->max_iter, c->stream);
     } else if (c->blocks_deferred) {  // the start's product writes the blocks
+      pd::launch_cg_setup(c->st, ns, prm->rtol, prm->max_iter, c->stream);
       pd::launch_cg_init_store(a, c->stream);
       c->blocks_deferred = false;
     } else {
+      pd::launch_cg_setup(c->st, ns, prm->rtol, prm->max_iter, c->stream);
       pd::launch_cg_init(a, c->stream);
     }
     CHECK_LAUNCH(c, "cg init");
@@ -1317,13 +1319,14 @@ int capture_step(pd_ctx* c, const pd_step_params* prm, int K) {
                                c->stream));
     } else {
       const pd::CgArgs a = cg_args(c, prm->precond);
-      pd::launch_cg_setup(c->st, ns, prm->rtol, prm->max_iter, c->stream);
-      if (fuse) {
-        pd::launch_cg_init_fold(a, c->stream);
+      if (fuse) {  // the fold sets the solver state up too
+        pd::launch_cg_init_fold(a, prm->rtol, prm->max_iter, c->stream);
       } else if (c->blocks_deferred) {
+        pd::launch_cg_setup(c->st, ns, prm->rtol, prm->max_iter, c->stream);
         pd::launch_cg_init_store(a, c->stream);
         c->blocks_deferred = false;
       } else {
+        pd::launch_cg_setup(c->st, ns, prm->rtol, prm->max_iter, c->stream);
         pd::launch_cg_init(a, c->stream);
       }
       for (int i = 0; i < K; ++i) {

@@ -247,14 +247,17 @@ __device__ __forceinline__ bool in_zone(const double4 p, int b, const ZoneBoxes*
 
 __global__ __launch_bounds__(PD_BLOCK) void k_zone_count(const double4* x, const int32_t* body,
                                                         int n, const ZoneBoxes* z,
-                                                        uint32_t* tile_cnt) {
+                                                        uint32_t* tile_cnt, double4* fc) {
   __shared__ uint32_t c;
   if (threadIdx.x == 0) c = 0;
   __syncthreads();
   uint32_t mine = 0;
   for (int j = 0; j < CP_ITEMS; ++j) {
     const int i = blockIdx.x * CP_TILE + j * PD_BLOCK + threadIdx.x;
-    if (i < n && in_zone(x[i], body[i], z)) ++mine;
+    if (i < n) {
+      if (fc) fc[i] = make_double4(0.0, 0.0, 0.0, 0.0);
+      if (in_zone(x[i], body[i], z)) ++mine;
+    }
   }
   if (mine) atomicAdd(&c, mine);
   __syncthreads();
@@ -313,10 +316,10 @@ __global__ __launch_bounds__(PD_BLOCK) void k_zone_scatter(const double4* x, con
 }
 
 void launch_zone_select(const double4* x, const int32_t* body, int32_t n, const ZoneBoxes* z,
-                        uint32_t* tile_cnt, int32_t* out, hipStream_t s) {
+                        uint32_t* tile_cnt, int32_t* out, hipStream_t s, double4* fc) {
   const int g = (n + CP_TILE - 1) / CP_TILE;
   if (g == 0) return;
-  hipLaunchKernelGGL(k_zone_count, dim3(g), dim3(PD_BLOCK), 0, s, x, body, n, z, tile_cnt);
+  hipLaunchKernelGGL(k_zone_count, dim3(g), dim3(PD_BLOCK), 0, s, x, body, n, z, tile_cnt, fc);
   hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(1024), 0, s, tile_cnt, g);
   hipLaunchKernelGGL(k_zone_scatter, dim3(g), dim3(PD_BLOCK), 0, s, x, body, n, z, tile_cnt, out);
 }

@@ -333,8 +333,9 @@ void launch_gap_reduce(const double* gap_val, const int32_t* gap_q, int32_t n,
 void launch_cg_setup(pd_cg_state* st, int32_t nspec, double rtol, int32_t max_iter,
                      hipStream_t s);
 void launch_cg_init(const CgArgs& a, hipStream_t s);
-// the fold alone, after an assembly that produced the start's partials
-void launch_cg_init_fold(const CgArgs& a, hipStream_t s);
+// the solver state set up (launch_cg_setup's) and the fold of the start's
+// partials, after an assembly that produced them
+void launch_cg_init_fold(const CgArgs& a, double rtol, int32_t max_iter, hipStream_t s);
 void launch_cg_spmv(const CgArgs& a, hipStream_t s);
 // the first CG product of a step whose assembly did not write the blocks:
 // y = A p with the blocks recomputed and written to a.vals (for every row,
@@ -389,11 +390,12 @@ void launch_check_division(int64_t samples, uint64_t seed, unsigned long long* b
 void launch_bond_d0(int32_t n, const int64_t* slice_off, const int32_t* cap, const int32_t* col,
                     const double4* x0, double* d0, unsigned int* bad, hipStream_t s);
 // contact forces of the m candidates cand[0..m) (particles 0..m when cand
-// is null); fc of every other particle is set to zero
+// is null); fc of every other particle is set to zero (fc_zeroed: the zone
+// select already cleared it)
 void launch_contact(int32_t n, const double4* x, const int32_t* body,
                     const int32_t* act, const GridView& g, const int32_t* cand,
                     int32_t m, double Rc, double kc, double vol, int all_pairs,
-                    double4* fc, hipStream_t s);
+                    double4* fc, hipStream_t s, bool fc_zeroed = false);
 void launch_reset_counters(pd_step_counters* c, hipStream_t s);
 
 // Per-body boxes, already widened by the contact radius: particle p is a
@@ -426,9 +428,10 @@ void launch_bbox(const double4* x, const int32_t* body, int32_t n, int32_t body_
                  double* part, double* out6, hipStream_t s);
 // stable (ascending) list of contact candidates; the count lands in
 // tile_cnt[zone_tile_words(n) - 1]
+// (fc non-null: every particle's contact force is cleared on the way)
 void launch_zone_select(const double4* x, const int32_t* body, int32_t n,
                         const ZoneBoxes* z, uint32_t* tile_cnt, int32_t* out,
-                        hipStream_t s);
+                        hipStream_t s, double4* fc = nullptr);
 
 size_t zone_tile_words(int32_t n);
 // keys of the cells (sub-grid origin cell c0, dims) of particles subset[i]

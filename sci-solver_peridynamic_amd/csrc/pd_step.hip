@@ -913,7 +913,11 @@ __global__ __launch_bounds__(PD_BLOCK) void k_cg_init(CgArgs a) {
 }
 
 // one workgroup per specimen
-__global__ __launch_bounds__(PD_BLOCK) void k_cg_init_fold(CgArgs a, int ntiles) {
+// SETUP: the solver state is set up here (k_cg_setup's values, with rtol
+// and max_iter), else k_cg_setup ran before and they are read from it.
+template <bool SETUP>
+__global__ __launch_bounds__(PD_BLOCK) void k_cg_init_fold(CgArgs a, int ntiles, double rtol,
+                                                          int max_iter) {
   __shared__ double lds4[4];
   int t0, nt;
   spec_range(a.batch, ntiles, t0, nt);
@@ -922,8 +926,21 @@ __global__ __launch_bounds__(PD_BLOCK) void k_cg_init_fold(CgArgs a, int ntiles)
   const double bb = fold_partials(a.part + 2 * (int64_t)ntiles + t0, nt, lds4);
   if (threadIdx.x == 0) {
     pd_cg_state* st = a.st + blockIdx.x;
+    if constexpr (SETUP) {  // the state's start (k_cg_setup's)
+      st->yp = 0.0;
+      st->alpha = 0.0;
+      st->beta = 0.0;
+      st->rtol = rtol;
+      st->done = 0;
+      st->converged = 0;
+      st->max_iter = max_iter;
+      st->upd_it = -1;
+    } else {
+      rtol = st->rtol;
+      max_iter = st->max_iter;
+    }
     const double bnorm = sqrt(bb);
-    const double tol = st->rtol * bnorm;
+    const double tol = rtol * bnorm;
     const double rnorm = sqrt(rr);
     st->rz = rz;
     st->rr = rr;
@@ -934,7 +951,7 @@ __global__ __launch_bounds__(PD_BLOCK) void k_cg_init_fold(CgArgs a, int ntiles)
     if (rnorm <= tol) {
       st->done = 1;
       st->converged = 1;
-    } else if (st->max_iter <= 0) {
+    } else if (max_iter <= 0) {
       st->done = 1;
     }
   }
@@ -1190,18 +1207,22 @@ void launch_cg_init(const CgArgs& a, hipStream_t s) {
       default: hipLaunchKernelGGL(k_cg_init<4>, dim3(g), dim3(PD_BLOCK), 0, s, a);
     }
   }
-  hipLaunchKernelGGL(k_cg_init_fold, dim3(a.batch.nspec), dim3(PD_BLOCK), 0, s, a, g);
+  hipLaunchKernelGGL(k_cg_init_fold<false>, dim3(a.batch.nspec), dim3(PD_BLOCK), 0, s, a, g,
+                     0.0, 0);
 }
 void launch_cg_init_store(const CgArgs& a, hipStream_t s) {
   if (!a.A.n) return;
   const int g = tiles_of(a.A.n);
   hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cg_init<0, true>), dim3(g), dim3(PD_BLOCK), 0, s, a);
-  hipLaunchKernelGGL(k_cg_init_fold, dim3(a.batch.nspec), dim3(PD_BLOCK), 0, s, a, g);
+  hipLaunchKernelGGL(k_cg_init_fold<false>, dim3(a.batch.nspec), dim3(PD_BLOCK), 0, s, a, g,
+                     0.0, 0);
 }
-void launch_cg_init_fold(const CgArgs& a, hipStream_t s) {
+void launch_cg_init_fold(const CgArgs& a, double rtol, int32_t max_iter, hipStream_t s) {
   if (a.A.n)
-    hipLaunchKernelGGL(k_cg_init_fold, dim3(a.batch.nspec), dim3(PD_BLOCK), 0, s, a,
-                       tiles_of(a.A.n));
+    hipLaunchKernelGGL(k_cg_init_fold<true>, dim3(a.batch.nspec), dim3(PD_BLOCK), 0, s, a,
+                       tiles_of(a.A.n), rtol, max_iter);
+  else
+    launch_cg_setup(a.st, a.batch.nspec, rtol, max_iter, s);
 }
 void launch_cg_spmv(const CgArgs& a, hipStream_t s) {
   if (!a.A.n) return;
@@ -1476,9 +1497,9 @@ __global__ __launch_bounds__(PD_BLOCK) void k_contact(ContactArgs a) {
 
 void launch_contact(int32_t n, const double4* x, const int32_t* body, const int32_t* act,
                     const GridView& g, const int32_t* cand, int32_t m, double Rc, double kc,
-                    double vol, int all_pairs, double4* fc, hipStream_t s) {
+                    double vol, int all_pairs, double4* fc, hipStream_t s, bool fc_zeroed) {
   const int nb = (n + PD_BLOCK - 1) / PD_BLOCK;
-  if (nb) hipLaunchKernelGGL(k_zero4, dim3(nb), dim3(PD_BLOCK), 0, s, fc, n);
+  if (nb && !fc_zeroed) hipLaunchKernelGGL(k_zero4, dim3(nb), dim3(PD_BLOCK), 0, s, fc, n);
   if (m <= 0) return;
   ContactArgs a;
   a.n = n;
