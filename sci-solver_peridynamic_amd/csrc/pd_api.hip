@@ -726,7 +726,7 @@ int enqueue_assemble(pd_ctx* c, const pd_step_params* prm,
   a.pc = prm->precond != 0 ? 1 : 0;  // precond 2: the block-Jacobi start first
   a.xs = c->sol;
   a.r64 = c->r;
-  a.p64 = c->p;
+  a.p64 = prm->matrix_free ? nullptr : c->p;  // matrix-free: p lives in the record alone
   a.r32 = c->r32;
   a.p32 = c->p32;
   a.d32 = c->d32;
@@ -1142,7 +1142,7 @@ int apply_learned(pd_ctx* c, const pd_step_params* prm, bool fused_start = false
     a.minv = c->pc_minv;
     a.minv32 = minv32;
     a.r64 = c->r;
-    a.p64 = c->p;
+    a.p64 = c->mf ? nullptr : c->p;  // matrix-free: p lives in the record alone
     a.rec = (c->mf || c->blocks_deferred) ? c->rec : nullptr;
     a.r32 = c->r32;
     a.p32 = c->p32;

@@ -196,6 +196,13 @@ __device__ __forceinline__ void store_block(double* vals, int64_t slot, int lane
   __builtin_nontemporal_store(vpair{b12, b22}, vo + 128);
 }
 
+// The second half of particle q's record (the vector the record carries).
+__device__ __forceinline__ double4 rec_vec(const double* __restrict__ rec, int q) {
+  const dbl2v* r = (const dbl2v*)(rec + 6 * (int64_t)q);
+  const dbl2v r1 = r[1], r2 = r[2];
+  return make_double4(r1.y, r2.x, r2.y, 0.0);
+}
+
 template <bool REC>
 __device__ __forceinline__ void gather_mf(const SpmvView& A, const double* __restrict__ rec,
                                           const double4* __restrict__ x, int q, double4& xq,
@@ -220,12 +227,14 @@ __device__ __forceinline__ void spmv_row_mf(const SpmvView& A, const double4* __
   const int s = p >> 6;
   const int lane = p & 63;
   const double* dg = A.diag + (int64_t)s * 6 * 64 + lane;
+  // REC: the row's own value from its record too (x may not hold it)
   if constexpr (DIAG_LAST) {
     y0 = 0.0;
     y1 = 0.0;
     y2 = 0.0;
   } else {
-    blk6(dg[0], dg[64], dg[128], dg[192], dg[256], dg[320], x[p], y0, y1, y2);
+    blk6(dg[0], dg[64], dg[128], dg[192], dg[256], dg[320], REC ? rec_vec(rec, p) : x[p], y0,
+         y1, y2);
   }
   const int top = A.top[p];
   if (top >= 0) {
@@ -290,7 +299,8 @@ __device__ __forceinline__ void spmv_row_mf(const SpmvView& A, const double4* __
   }
   if constexpr (DIAG_LAST) {
     double d0, d1, d2;
-    blk6(dg[0], dg[64], dg[128], dg[192], dg[256], dg[320], x[p], d0, d1, d2);
+    blk6(dg[0], dg[64], dg[128], dg[192], dg[256], dg[320], REC ? rec_vec(rec, p) : x[p], d0,
+         d1, d2);
     y0 = d0 + y0;
     y1 = d1 + y1;
     y2 = d2 + y2;

@@ -364,7 +364,7 @@ __global__ __launch_bounds__(PD_BLOCK) void k_pc_infer_start(PcInferStartArgs a)
       const double z0 = m[0] * rr.x + m[1] * rr.y + m[2] * rr.z;
       const double z1 = m[1] * rr.x + m[3] * rr.y + m[4] * rr.z;
       const double z2 = m[2] * rr.x + m[4] * rr.y + m[5] * rr.z;
-      a.p64[p] = make_double4(z0, z1, z2, 0.0);
+      if (a.p64) a.p64[p] = make_double4(z0, z1, z2, 0.0);  // null: matrix-free
       if (a.rec) {
         double* rp = a.rec + 6 * (int64_t)p;
         rp[3] = z0;

@@ -370,7 +370,7 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
           z2 = m64[2] * r0 + m64[4] * r1 + m64[5] * r2;
         }
         a.r64[p] = make_double4(r0, r1, r2, 0.0);
-        a.p64[p] = make_double4(z0, z1, z2, 0.0);
+        if (a.p64) a.p64[p] = make_double4(z0, z1, z2, 0.0);  // null: matrix-free
         if (a.rec) {
           double* rp = a.rec + 6 * (int64_t)p;
           rp[3] = z0;
@@ -413,7 +413,13 @@ __global__ __launch_bounds__(PD_BLOCK) void k_assemble(AsmArgs a) {
       } else {
         const double4 z4 = make_double4(0.0, 0.0, 0.0, 0.0);
         a.r64[p] = z4;
-        a.p64[p] = z4;
+        if (a.p64) a.p64[p] = z4;
+        if (a.rec) {
+          double* rp = a.rec + 6 * (int64_t)p;
+          rp[3] = 0.0;
+          rp[4] = 0.0;
+          rp[5] = 0.0;
+        }
       }
     }
   }
@@ -1003,7 +1009,7 @@ __global__ __launch_bounds__(PD_BLOCK) void k_cg_spmv_mf(CgArgs a) {
     else
       spmv_row_mf<2, false, true, false>(a.A, a.p, p, y0, y1, y2, a.A.rec);
     a.y[p] = make_double4(y0, y1, y2, 0.0);
-    const double4 pp = a.p[p];
+    const double4 pp = rec_vec(a.A.rec, p);  // p lives in the record alone
     t = term3(y0, y1, y2, pp.x, pp.y, pp.z);
   }
   const double v = tile_sum(t, lds4);
@@ -1176,7 +1182,8 @@ __global__ __launch_bounds__(PD_BLOCK) void k_cg_dir(CgArgs a, int it) {
   if (st->upd_it != it) return;  // the solve had ended before this iteration
   const int p = tile * PD_BLOCK + threadIdx.x;
   if (p >= a.A.n) return;
-  double4 pp = a.p[p];
+  // matrix-free (A.rec set): p lives in the gather record alone
+  double4 pp = a.A.rec ? rec_vec(a.A.rec, p) : a.p[p];
   double4 xx = a.x[p];
   const double alpha = st->alpha;
   xx.x = alpha * pp.x + xx.x;
@@ -1189,8 +1196,10 @@ __global__ __launch_bounds__(PD_BLOCK) void k_cg_dir(CgArgs a, int it) {
   pp.x = zz.x + beta * pp.x;
   pp.y = zz.y + beta * pp.y;
   pp.z = zz.z + beta * pp.z;
-  a.p[p] = pp;
-  rec_p(a.A.rec, p, pp.x, pp.y, pp.z);
+  if (a.A.rec)
+    rec_p(a.A.rec, p, pp.x, pp.y, pp.z);
+  else
+    a.p[p] = pp;
 }
 
 static inline int tiles_of(int n) { return (n + PD_BLOCK - 1) / PD_BLOCK; }
