@@ -57,7 +57,8 @@ struct AsmArgs {
   // and its tile partials in part (Dirichlet rows of xs must hold their
   // prescribed values already: k_asm_prep).  z = r (pc = 0) or
   // the block-Jacobi inverse just formed times r (pc = 1).
-  //   FP64:  r64 = r, p64 = z; partials <r,z>, <r,r>, <Sb,Sb>
+  //   FP64:  r64 = r, p64 = z (p64 null: a matrix-free solve, whose p lives
+  //          in the record rec alone); partials <r,z>, <r,r>, <Sb,Sb>
   //   mixed: r32 = float(r), p32 = z, d32 = 0 (A32 with float bond blocks);
   //          partials <r,r>, <r32,z>, <r32,r32>, <Sb,Sb>
   int32_t fuse;
@@ -268,7 +269,7 @@ struct PcInferStartArgs {
   double* minv;          // the learned blocks (FP64)
   float* minv32;         // and rounded to float (mixed 1)
   const double4* r64;
-  double4* p64;
+  double4* p64;          // null: a matrix-free solve (p in rec alone)
   double* rec;           // FP64: the gather record's p part, when in use
   const float4* r32;
   float4* p32;
