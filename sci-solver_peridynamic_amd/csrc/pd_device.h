@@ -11,8 +11,10 @@
 //     values (a00 a01 a02 a11 a12 a22) are stored.  FP64: in three planes
 //     of pairs, so a lane reads 16 bytes per load: value c of slot k of row
 //     p at ((slice_off[s] + k) * 3 + c / 2) * 128 + 2 * lane + c % 2.
-//     Float copy (val32): in component planes, value c at
-//     ((slice_off[s] + k) * 6 + c) * 64 + lane;
+//     Float copy (val32): 384 floats per slot row, a plane of quads
+//     (a00 a01 a02 a11) then a plane of pairs (a12 a22), so a lane reads
+//     one 16-byte and one 8-byte word: at (slice_off[s] + k) * 384, the
+//     quad at 4 * lane and the pair at 256 + 2 * lane;
 //   * diagonal and block-Jacobi inverse blocks: (s * 6 + c) * 64 + lane;
 //   * mixed precision keeps float copies in the same positions (val32,
 //     diag32, minv32) and float4 vectors (16 B) for the inner solve.
