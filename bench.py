@@ -84,6 +84,7 @@ def read_counters(key, kernel):
                     v = [float(c) for c in cells[1:]]
                     return {"valu_busy": v[2] / 100.0, "waves_waiting": v[3] / 100.0,
                             "valu_instr_per_launch": v[4] * 1e6,
+                            "fp64_transcendental_per_launch": v[5] * 1e6,
                             "ta_busy": v[7] / 100.0, "td_busy": v[8] / 100.0,
                             "source": os.path.relpath(TRAFFIC_FILES[key], ROOT)}
     except (OSError, ValueError):
@@ -350,6 +351,9 @@ def mf_step_roofline(r):
 MF_FLOPS_BOND = 69.0
 MF_FLOPS_ROW = 20.0
 FP64_PEAK_TFLOPS = 78.6
+# MI355X: 256 CUs of four SIMDs, 2.4 GHz peak engine clock
+SIMDS = 1024
+CLOCK_HZ = 2.4e9
 
 
 def mf_roofline(r):
@@ -359,6 +363,11 @@ def mf_roofline(r):
     gbs = r["spmv_bytes"] / (avg_ms * 1e-3) / 1e9
     intensity = flops / r["spmv_bytes"]
     ridge = FP64_PEAK_TFLOPS * 1e3 / HBM_PEAK_GBS
+    cnt = read_counters("mf", "k_cg_spmv_mf")
+    floor_ms = None
+    if cnt is not None:
+        cycles = 4.0 * cnt["valu_instr_per_launch"] + 12.0 * cnt["fp64_transcendental_per_launch"]
+        floor_ms = cycles / (SIMDS * CLOCK_HZ) * 1e3
     return {
         "kernel": "k_cg_spmv_mf (matrix-free CG SpMV: blocks recomputed from positions)",
         # its FP64 operations per algorithmic byte lie below the ridge of
@@ -381,7 +390,12 @@ def mf_roofline(r):
         "avg_launch_ms": avg_ms,
         "launches": r["spmv_n"],
         # what holds it below that ceiling, from the committed counter runs
-        "counters": read_counters("mf", "k_cg_spmv_mf"),
+        "counters": cnt,
+        # the time its VALU instructions alone need: a wave64 instruction
+        # issues over four cycles of a SIMD, an FP64 reciprocal or square
+        # root over sixteen, on 1,024 SIMDs at 2.4 GHz
+        "valu_issue_floor_ms": floor_ms,
+        "valu_issue_frac": floor_ms / avg_ms if floor_ms is not None else None,
     }
 
 

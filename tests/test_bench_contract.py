@@ -43,6 +43,14 @@ def test_matrix_free_roofline_is_hbm_bound():
     c = rf["counters"]
     assert c is not None and 0.0 < c["valu_busy"] < 1.0 and 0.0 < c["td_busy"] <= 1.0
     assert 50 < c["valu_instr_per_launch"] * 64 / r["live"] < 300
+    # the time its VALU instructions alone need (four cycles per wave64
+    # instruction, sixteen per FP64 reciprocal or square root, 1,024 SIMDs
+    # at 2.4 GHz) is below the time it takes
+    want = (4 * c["valu_instr_per_launch"] + 12 * c["fp64_transcendental_per_launch"]) \
+        / (1024 * 2.4e9) * 1e3
+    assert abs(rf["valu_issue_floor_ms"] - want) < 1e-12
+    assert abs(rf["valu_issue_frac"] - want / 0.5114) < 1e-12
+    assert 0.3 < rf["valu_issue_frac"] < 1.0
 
 
 def test_step_rooflines():
